@@ -1,0 +1,219 @@
+/*
+ * libspecann: C ABI of the MI355X (gfx950) SpecAnn alpha-sweep hot path.
+ *
+ * Every function takes plain device pointers and sizes plus the HIP stream to launch on, returns 0
+ * or a negative SA_E_* status, never throws, and allocates no device memory: the caller owns every
+ * buffer (the reference's kernels likewise run on caller-owned CuPy arrays). Functions are stream
+ * ordered. Reductions carry their partial sums in the caller's work buffer (sa_work_bytes), whose
+ * first bytes are the sa_state block; two calls that share one work buffer must not overlap.
+ *
+ * Precision suffixes: _f32 = FP32 couplings and vectors (the reference's precision 32 with
+ * fp32_spmv), _f64 = FP64 everywhere (the reference's SpecAnn-64). Lanczos coefficients, the Ritz
+ * solve, dots and norms are FP64 in both.
+ *
+ * Reference citations are relative to the Vgueca/SpecAnn source tree.
+ */
+#ifndef SPECANN_H
+#define SPECANN_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SA_OK 0
+#define SA_E_ARG (-1)
+#define SA_E_LAUNCH (-2)
+#define SA_E_BREAKDOWN (-3)
+#define SA_E_OOM (-4)
+#define SA_E_NONINTEGER (-5)
+
+/* Lanczos status values (sa_state.status). */
+#define SA_RUNNING 0
+#define SA_CONVERGED 1
+#define SA_MAXITER 2
+#define SA_RESTART 3
+#define SA_REJECTED 4
+#define SA_BROKEN (-1)
+
+#define SA_MAX_NCV 128
+#define SA_RED_GRID 512        /* workgroups of the reduction kernels */
+#define SA_RED_COLS 8          /* basis vectors per dot-product chunk */
+
+/* Symmetric CSR of the couplings, device pointers. nnz < 2^31 (int32 indices). */
+typedef struct sa_csr {
+  int64_t n;
+  int64_t nnz;
+  const int32_t* indptr;    /* n + 1 */
+  const int32_t* indices;   /* nnz, sorted per row */
+  const void* vals;         /* nnz normalised couplings J / c: float (_f32) or double (_f64) */
+  const int32_t* jint;      /* nnz integer couplings J (local search, exact energies); may be NULL */
+  const int32_t* tile_rows; /* ntiles + 1: first row whose first stored entry is >= t * tile_nnz */
+  int64_t ntiles;
+  int32_t tile_nnz;         /* stored entries per SpMV tile (power of two, 512..8192) */
+  int32_t lanes_per_row;    /* 0: choose per tile from its mean row length; else 1..64 */
+} sa_csr;
+
+/* Device-side state of one eigensolve; lives at the start of the work buffer. */
+typedef struct sa_state {
+  int32_t status;      /* SA_RUNNING, ... */
+  int32_t j;           /* Lanczos steps done in the current cycle */
+  int32_t steps;       /* operator applications since sa_lanczos_begin */
+  int32_t cycles;      /* restarts */
+  int32_t ncv;
+  int32_t maxiter;
+  int32_t gate;        /* 1 while step 0 runs on a predicted start vector */
+  int32_t flags;
+  double theta;        /* largest Ritz value of N (the reference's mu = rho - theta) */
+  double res;          /* residual estimate |beta_j * s_j| */
+  double first_beta;   /* beta_0 of a gated start: the predictor residual */
+  double tol;
+  double bound;        /* breakdown bound on |alpha|, |beta| (10 x (rho + g) is applied) */
+  double pred_tol;
+  double r[8];         /* results of scalar reductions */
+  uint64_t hash;       /* spin signature of the last rounding */
+  int64_t i64[4];
+  double a[SA_MAX_NCV];
+  double b[SA_MAX_NCV];
+  double sigma[SA_MAX_NCV + 1];
+  double h[SA_MAX_NCV + 1];
+  double ritz[SA_MAX_NCV];
+  unsigned int counters[16];
+} sa_state;
+
+int sa_version(void);
+/* Copies the last error message of this thread; returns its length. */
+int sa_last_error(char* buf, size_t len);
+/* Bytes of the work buffer for a problem of n rows (zero-fill it once before first use). */
+size_t sa_work_bytes(int64_t n);
+/* sizeof(sa_state), for bindings that mirror the struct. */
+size_t sa_state_bytes(void);
+
+/* ---- Set-up: values, degrees, normalisation (models/spectral_annealing.py:270-288) ---------- */
+/* vals_out = (T) raw (the host's float64 data), jint = (int32) raw; *nonint_dev (device int32)
+ * is set to 1 if some value is not an integer in [-2^30, 2^30]. */
+int sa_prepare_values_f32(int64_t nnz, const double* raw, float* vals_out, int32_t* jint, int32_t* nonint_dev,
+                          hipStream_t stream);
+int sa_prepare_values_f64(int64_t nnz, const double* raw, double* vals_out, int32_t* jint, int32_t* nonint_dev,
+                          hipStream_t stream);
+/* deg_i = sum_j |vals_ij| (accumulated in FP64, rounded to T); state.r[0] = max_i deg_i. */
+int sa_abs_degree_f32(const sa_csr* A, float* deg, void* work, hipStream_t stream);
+int sa_abs_degree_f64(const sa_csr* A, double* deg, void* work, hipStream_t stream);
+/* vals /= c in place, d = deg / c (models/spectral_annealing.py:282-288). */
+int sa_normalize_f32(int64_t nnz, float* vals, int64_t n, const float* deg, float* d, double c, hipStream_t stream);
+int sa_normalize_f64(int64_t nnz, double* vals, int64_t n, const double* deg, double* d, double c,
+                     hipStream_t stream);
+/* tile_rows[t] = first row r with indptr[r] >= t * tile_nnz (t = 0..ntiles; tile_rows[ntiles] = n). */
+int sa_csr_tiles(int64_t n, const int32_t* indptr, int32_t tile_nnz, int32_t* tile_rows, int64_t ntiles,
+                 hipStream_t stream);
+
+/* ---- Per-alpha scaling (models/spectral_annealing.py:316, :34) ------------------------------ */
+/* s_i = 1 / sqrt(max(d_i^alpha, clamp)); state.r[1] = sum_i d_i^alpha (degree_scale, :326);
+ * state.r[2] = max_i d_i^(1 - alpha) (the Gershgorin bound g, :579). */
+int sa_alpha_scaling_f32(int64_t n, const float* d, double alpha, double clamp, float* s, void* work,
+                         hipStream_t stream);
+int sa_alpha_scaling_f64(int64_t n, const double* d, double alpha, double clamp, double* s, void* work,
+                         hipStream_t stream);
+
+/* ---- Fused scaled operator (NAlphaOperator._matvec / SRhoOperator._matvec, :38-41, :52-53) -- */
+/* y = a * x + b * s (.) (J/c) (s (.) x). a = rho, b = -1 is S_rho; a = 0, b = 1 is N_alpha.
+ * z (n, caller scratch) receives s (.) x, so the SpMV gathers one vector. */
+int sa_spmv_scaled_f32(const sa_csr* A, const float* s, const float* x, double a, double b, float* z, float* y,
+                       hipStream_t stream);
+int sa_spmv_scaled_f64(const sa_csr* A, const double* s, const double* x, double a, double b, double* z, double* y,
+                       hipStream_t stream);
+
+/* ---- Warm-started Lanczos on N_alpha (utils/eigsh.py:10-17, models/spectral_annealing.py:671-677)
+ * Basis V is (ncv + 1) x n, row-major, leading dimension n. Z, W are n-vectors of scratch. */
+int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, double bound, double pred_tol,
+                     hipStream_t stream);
+/* Start vector V[0] = x (.) max(d, 1e-12)^pexp (the diagonal predictor, utils/predictors.py:6-26, with
+ * pexp = delta_alpha / 2; pexp = 0 copies x) or cx * x + cy * y when y != NULL (the secant predictor,
+ * :29-56). Z = s (.) V[0]; sigma_0 = 1 / ||V[0]||; state.r[3] = ||V[0]||. gate = 1 marks the vector as
+ * predicted: step 0 then also reports beta_0 (the predictor residual, spectral_annealing.py:605-612)
+ * and sets SA_REJECTED if it exceeds pred_tol. only_if >= 0 makes the call a no-op unless
+ * state.status == only_if (device-side gating, so restarts need no host round trip). */
+int sa_lanczos_start_f32(void* work, int64_t n, const float* s, const float* x, const float* d, double pexp,
+                         double cx, const float* y, double cy, float* V, float* Z, int32_t gate, int32_t only_if,
+                         hipStream_t stream);
+int sa_lanczos_start_f64(void* work, int64_t n, const double* s, const double* x, const double* d, double pexp,
+                         double cx, const double* y, double cy, double* V, double* Z, int32_t gate, int32_t only_if,
+                         hipStream_t stream);
+/* One Lanczos step with full (classical Gram-Schmidt) reorthogonalisation and a Ritz solve of the
+ * tridiagonal; no-op unless state.status == SA_RUNNING. */
+int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, float* Z, float* W, hipStream_t stream);
+int sa_lanczos_step_f64(const sa_csr* A, void* work, const double* s, double* V, double* Z, double* W,
+                        hipStream_t stream);
+/* Ritz vector y = sum_m ritz_m v_m of the current cycle; state.r[4] = ||y||, state.r[5] = y . y_last
+ * (y_last may be NULL). only_if as in sa_lanczos_start. */
+int sa_lanczos_ritz_f32(void* work, int64_t n, const float* V, const float* y_last, float* y, int32_t only_if,
+                        hipStream_t stream);
+int sa_lanczos_ritz_f64(void* work, int64_t n, const double* V, const double* y_last, double* y, int32_t only_if,
+                        hipStream_t stream);
+
+/* ---- Vector helpers ------------------------------------------------------------------------- */
+/* state.r[slot] = ||a*x + b*y||_2 (y may be NULL); state.r[slot+1] = x . y (0 when y is NULL). */
+int sa_norm_dot_f32(void* work, int64_t n, const float* x, const float* y, double a, double b, int32_t slot,
+                    hipStream_t stream);
+int sa_norm_dot_f64(void* work, int64_t n, const double* x, const double* y, double a, double b, int32_t slot,
+                    hipStream_t stream);
+/* out_i = scale * x_i (in T). */
+int sa_scale_copy_f32(int64_t n, const float* x, double scale, float* out, hipStream_t stream);
+int sa_scale_copy_f64(int64_t n, const double* x, double scale, double* out, hipStream_t stream);
+/* Counter-based standard normal vector (Box-Muller on SplitMix64 streams (seed, stream_id, i)). */
+int sa_fill_normal_f32(int64_t n, uint64_t seed, uint64_t stream_id, float* out, hipStream_t stream);
+int sa_fill_normal_f64(int64_t n, uint64_t seed, uint64_t stream_id, double* out, hipStream_t stream);
+
+/* ---- Rounding and signature (models/spectral_annealing.py:330-345, :557-561) ---------------- */
+/* y_last = scale * y (scale = sign / ||y||, the sign alignment and normalisation of :653-660 and
+ * :331-340); bits word w, bit l = 1 iff scale * y[64 w + l] >= 0 (spin +1, :345); state.hash =
+ * order-aware 64-bit signature of the bits. y_last may be NULL (no copy). */
+int sa_round_f32(void* work, int64_t n, const float* y, double scale, float* y_last, uint64_t* bits,
+                 hipStream_t stream);
+int sa_round_f64(void* work, int64_t n, const double* y, double scale, double* y_last, uint64_t* bits,
+                 hipStream_t stream);
+/* state.i64[0] = number of differing words of two bit vectors (0: identical candidates). */
+int sa_bits_compare(void* work, int64_t nwords, const uint64_t* a, const uint64_t* b, hipStream_t stream);
+
+/* ---- Energies (models/spectral_annealing.py:925-976, exact for integer couplings) ----------- */
+/* e2[c] = sum_i s_i sum_j J_ij s_j (int64, exact) for k candidate bit vectors (candidate-major,
+ * nwords words each); the energy is -e2 / 2. */
+int sa_energy_bits_i64(const sa_csr* A, int32_t k, const uint64_t* cand_bits, int64_t nwords, int64_t* e2,
+                       hipStream_t stream);
+
+/* ---- Batched deterministic local search (flip_refinement, :1039-1112; gpu_final_sweep :983-1036)
+ * Candidates are node-major: spins int8 [n][kp], fields int32 [n][kp], kp = 64 * groups. */
+/* node-major spin bits sb[i * groups + g] bit l = candidate 64 g + l, from candidate-major bits. */
+int sa_bits_transpose(int64_t n, int32_t k, const uint64_t* cand_bits, int64_t nwords, int32_t groups,
+                      uint64_t* sb, hipStream_t stream);
+/* spins, fields = J s, e2[c] = sum_i s_i h_i (int64 atomics; zero e2 first). */
+int sa_ls_init(const sa_csr* A, int32_t k, int32_t groups, const uint64_t* sb, int8_t* spins, int32_t* fields,
+               int64_t* e2, hipStream_t stream);
+/* One colour class of one pass: node i of the class (selected if u53(mix3(seed, pass, i)) < frac)
+ * flips candidate c iff s_ic * h_ic < 0, then adds 2 J_ij s_ic(new) to h_jc of every neighbour j.
+ * flags[c] = 1 for candidates with a flip. class_nodes holds the class's nodes. */
+int sa_ls_class_pass(const sa_csr* A, int32_t k, int32_t groups, const int32_t* class_nodes, int64_t count,
+                     uint64_t seed, int64_t pass, double frac, int8_t* spins, int32_t* fields, int32_t* flags,
+                     hipStream_t stream);
+/* e2[c] = sum_i s_ic h_ic from maintained fields (zero e2 first). */
+int sa_ls_energy(int64_t n, int32_t k, int32_t groups, const int8_t* spins, const int32_t* fields, int64_t* e2,
+                 hipStream_t stream);
+/* out = spins[:, c] as +-1 in T. */
+int sa_ls_extract_f32(int64_t n, int32_t groups, int32_t c, const int8_t* spins, float* out, hipStream_t stream);
+int sa_ls_extract_f64(int64_t n, int32_t groups, int32_t c, const int8_t* spins, double* out, hipStream_t stream);
+
+/* ---- Colouring (flip order of the local search; cf. local_search_guards/common.py:136-159) --- */
+/* One round of greedy colouring in the order (degree, hash(seed, i), i) descending: an uncoloured
+ * node all of whose higher-priority neighbours are coloured takes the smallest colour none of them
+ * has. The result equals the sequential greedy colouring in that order. state.i64[1] += nodes
+ * still uncoloured after the round. */
+int sa_colour_round(const sa_csr* A, uint64_t seed, int32_t* colour, void* work, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPECANN_H */

@@ -1,0 +1,185 @@
+"""ctypes binding of libspecann (include/specann.h).
+
+The solver has no CPU fallback: if the library is missing or no GPU is visible, the calls below
+raise. Status codes map onto the exception types the reference's solver loop distinguishes
+(models/spectral_annealing.py:70-77, utils/failures.py:15-25): SA_E_OOM is a MemoryError, so the
+run is tagged "out of memory"; SA_E_BREAKDOWN is an EigensolverBreakdown (numerical, the alpha is
+skipped).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().with_name("libspecann.so")
+
+SA_OK = 0
+SA_E_ARG = -1
+SA_E_LAUNCH = -2
+SA_E_BREAKDOWN = -3
+SA_E_OOM = -4
+SA_E_NONINTEGER = -5
+
+SA_RUNNING = 0
+SA_CONVERGED = 1
+SA_MAXITER = 2
+SA_RESTART = 3
+SA_REJECTED = 4
+SA_BROKEN = -1
+
+SA_MAX_NCV = 128
+
+
+class SpecAnnLibraryError(RuntimeError):
+    """libspecann is missing or a call failed for a non-numerical reason."""
+
+
+class EigensolverBreakdown(RuntimeError):
+    """The eigensolver produced an unusable result (models/spectral_annealing.py:70-71)."""
+
+
+class DeviceOutOfMemory(MemoryError):
+    """A HIP allocation or launch ran out of device memory."""
+
+
+class SaCsr(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int64),
+        ("nnz", c_int64),
+        ("indptr", c_void_p),
+        ("indices", c_void_p),
+        ("vals", c_void_p),
+        ("jint", c_void_p),
+        ("tile_rows", c_void_p),
+        ("ntiles", c_int64),
+        ("tile_nnz", c_int32),
+        ("lanes_per_row", c_int32),
+    ]
+
+
+class SaState(ctypes.Structure):
+    _fields_ = [
+        ("status", c_int32),
+        ("j", c_int32),
+        ("steps", c_int32),
+        ("cycles", c_int32),
+        ("ncv", c_int32),
+        ("maxiter", c_int32),
+        ("gate", c_int32),
+        ("flags", c_int32),
+        ("theta", c_double),
+        ("res", c_double),
+        ("first_beta", c_double),
+        ("tol", c_double),
+        ("bound", c_double),
+        ("pred_tol", c_double),
+        ("r", c_double * 8),
+        ("hash", c_uint64),
+        ("i64", c_int64 * 4),
+        ("a", c_double * SA_MAX_NCV),
+        ("b", c_double * SA_MAX_NCV),
+        ("sigma", c_double * (SA_MAX_NCV + 1)),
+        ("h", c_double * (SA_MAX_NCV + 1)),
+        ("ritz", c_double * SA_MAX_NCV),
+        ("counters", c_uint32 * 16),
+    ]
+
+
+# Bytes of the leading part of SaState the host reads after each batch of launches.
+STATE_HEAD_BYTES = SaState.a.offset
+
+_P = c_void_p
+_SIGNATURES = {
+    "sa_version": (c_int32, []),
+    "sa_last_error": (c_int32, [ctypes.c_char_p, c_size_t]),
+    "sa_work_bytes": (c_size_t, [c_int64]),
+    "sa_state_bytes": (c_size_t, []),
+    "sa_prepare_values_f32": (c_int32, [c_int64, _P, _P, _P, _P, _P]),
+    "sa_prepare_values_f64": (c_int32, [c_int64, _P, _P, _P, _P, _P]),
+    "sa_abs_degree_f32": (c_int32, [POINTER(SaCsr), _P, _P, _P]),
+    "sa_abs_degree_f64": (c_int32, [POINTER(SaCsr), _P, _P, _P]),
+    "sa_normalize_f32": (c_int32, [c_int64, _P, c_int64, _P, _P, c_double, _P]),
+    "sa_normalize_f64": (c_int32, [c_int64, _P, c_int64, _P, _P, c_double, _P]),
+    "sa_csr_tiles": (c_int32, [c_int64, _P, c_int32, _P, c_int64, _P]),
+    "sa_alpha_scaling_f32": (c_int32, [c_int64, _P, c_double, c_double, _P, _P, _P]),
+    "sa_alpha_scaling_f64": (c_int32, [c_int64, _P, c_double, c_double, _P, _P, _P]),
+    "sa_spmv_scaled_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, c_double, _P, _P, _P]),
+    "sa_spmv_scaled_f64": (c_int32, [POINTER(SaCsr), _P, _P, c_double, c_double, _P, _P, _P]),
+    "sa_lanczos_begin": (c_int32, [_P, c_int32, c_int32, c_double, c_double, c_double, _P]),
+    "sa_lanczos_start_f32": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, _P, c_double, _P, _P, c_int32,
+                                       c_int32, _P]),
+    "sa_lanczos_start_f64": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, _P, c_double, _P, _P, c_int32,
+                                       c_int32, _P]),
+    "sa_lanczos_step_f32": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P, _P, _P]),
+    "sa_lanczos_step_f64": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P, _P, _P]),
+    "sa_lanczos_ritz_f32": (c_int32, [_P, c_int64, _P, _P, _P, c_int32, _P]),
+    "sa_lanczos_ritz_f64": (c_int32, [_P, c_int64, _P, _P, _P, c_int32, _P]),
+    "sa_norm_dot_f32": (c_int32, [_P, c_int64, _P, _P, c_double, c_double, c_int32, _P]),
+    "sa_norm_dot_f64": (c_int32, [_P, c_int64, _P, _P, c_double, c_double, c_int32, _P]),
+    "sa_scale_copy_f32": (c_int32, [c_int64, _P, c_double, _P, _P]),
+    "sa_scale_copy_f64": (c_int32, [c_int64, _P, c_double, _P, _P]),
+    "sa_fill_normal_f32": (c_int32, [c_int64, c_uint64, c_uint64, _P, _P]),
+    "sa_fill_normal_f64": (c_int32, [c_int64, c_uint64, c_uint64, _P, _P]),
+    "sa_round_f32": (c_int32, [_P, c_int64, _P, c_double, _P, _P, _P]),
+    "sa_round_f64": (c_int32, [_P, c_int64, _P, c_double, _P, _P, _P]),
+    "sa_bits_compare": (c_int32, [_P, c_int64, _P, _P, _P]),
+    "sa_energy_bits_i64": (c_int32, [POINTER(SaCsr), c_int32, _P, c_int64, _P, _P]),
+    "sa_bits_transpose": (c_int32, [c_int64, c_int32, _P, c_int64, c_int32, _P, _P]),
+    "sa_ls_init": (c_int32, [POINTER(SaCsr), c_int32, c_int32, _P, _P, _P, _P, _P]),
+    "sa_ls_class_pass": (c_int32, [POINTER(SaCsr), c_int32, c_int32, _P, c_int64, c_uint64, c_int64, c_double, _P,
+                                   _P, _P, _P]),
+    "sa_ls_energy": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P, _P]),
+    "sa_ls_extract_f32": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P]),
+    "sa_ls_extract_f64": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P]),
+    "sa_colour_round": (c_int32, [POINTER(SaCsr), c_uint64, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def load(path: Path | str | None = None) -> ctypes.CDLL:
+    """Load libspecann once; raise SpecAnnLibraryError if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise SpecAnnLibraryError(
+            f"{p} is missing: build the HIP library first (python -m specann_amd.build); "
+            "the solver has no CPU fallback")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.sa_state_bytes() != ctypes.sizeof(SaState):
+        raise SpecAnnLibraryError(
+            f"sa_state layout mismatch: library {lib.sa_state_bytes()} bytes, binding {ctypes.sizeof(SaState)}")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def exported_names() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def last_error() -> str:
+    lib = load()
+    buf = ctypes.create_string_buffer(1024)
+    lib.sa_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def call(name: str, *args) -> int:
+    """Call an entry point; map a negative status to an exception."""
+    rc = getattr(load(), name)(*args)
+    if rc >= 0:
+        return rc
+    msg = f"{name} failed ({rc}): {last_error()}"
+    if rc == SA_E_OOM:
+        raise DeviceOutOfMemory(msg)
+    if rc == SA_E_BREAKDOWN:
+        raise EigensolverBreakdown(msg)
+    raise SpecAnnLibraryError(msg)

@@ -1,0 +1,127 @@
+// Shared device helpers of libspecann: error reporting, counter-based hashing, wavefront and
+// workgroup reductions. Written for gfx950 (CDNA4): 64-wide wavefronts, 256-thread workgroups.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/specann.h"
+
+#define SA_BLOCK 256
+#define SA_WAVES_PER_BLOCK (SA_BLOCK / 64)
+
+namespace sa {
+
+// ---------------------------------------------------------------------------------------------
+// Error reporting: every entry point returns a status and leaves a message for sa_last_error.
+// ---------------------------------------------------------------------------------------------
+void set_error(const std::string& msg);
+int check_launch(const char* what);
+
+#define SA_REQUIRE(cond, msg)                                 \
+  do {                                                        \
+    if (!(cond)) {                                            \
+      ::sa::set_error(std::string(__func__) + ": " + (msg));  \
+      return SA_E_ARG;                                        \
+    }                                                         \
+  } while (0)
+
+// ---------------------------------------------------------------------------------------------
+// Counter-based hashing (SplitMix64 finaliser). The host oracle restates these bit for bit.
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__host__ __device__ inline uint64_t mix3(uint64_t a, uint64_t b, uint64_t c) {
+  return splitmix64(splitmix64(splitmix64(a) ^ b) ^ c);
+}
+
+// 2^-53 * (53 random bits): uniform double in [0, 1).
+__host__ __device__ inline double u53(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+
+// ---------------------------------------------------------------------------------------------
+// Reductions. Wave reductions use cross-lane shuffles; the workgroup reduction combines the four
+// wave results in a fixed order, so every sum here is deterministic for a fixed launch shape.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename T>
+__device__ inline T wave_max(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    T u = __shfl_xor(v, o, 64);
+    v = u > v ? u : v;
+  }
+  return v;
+}
+
+// Sum over a 256-thread workgroup; the result is valid in thread 0. `red` holds 4 values.
+template <typename T>
+__device__ inline T block_sum(T v, T* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  T r = 0;
+  if (threadIdx.x == 0) r = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();
+  return r;
+}
+
+// "Last block done" handshake: every block stores its partial, then thread 0 bumps the counter;
+// the block that sees gridDim-1 is the last one and may read all partials. It resets the counter.
+__device__ inline bool last_block_arrive(unsigned int* counter, unsigned int total, bool* flag_lds) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int prev = atomicAdd(counter, 1u);
+    *flag_lds = (prev == total - 1);
+  }
+  __syncthreads();
+  bool last = *flag_lds;
+  if (last) {
+    __threadfence();
+    if (threadIdx.x == 0) atomicExch(counter, 0u);
+  }
+  return last;
+}
+
+// Coherent load of a value another workgroup wrote in this kernel (after the handshake above):
+// an agent-scope atomic load, which cannot be served from a stale per-CU cache line.
+template <typename T>
+__device__ inline T load_coherent(const T* p) {
+  static_assert(sizeof(T) == 8 || sizeof(T) == 4, "4- or 8-byte values only");
+  if constexpr (sizeof(T) == 8) {
+    unsigned long long v = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_bit_cast(T, v);
+  } else {
+    unsigned int v = __hip_atomic_load(reinterpret_cast<const unsigned int*>(p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_bit_cast(T, v);
+  }
+}
+
+template <typename T>
+__device__ inline T stream_load(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+inline int grid_for(int64_t work, int per_block, int cap) {
+  int64_t g = (work + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+}  // namespace sa

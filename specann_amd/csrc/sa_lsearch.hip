@@ -1,0 +1,293 @@
+// Exact energies, the batched deterministic local search and the greedy colouring for gfx950.
+//
+// Local search layout (node-major, all candidates of a node side by side):
+//   spins  int8  [n][kp]   kp = 64 * groups, candidate c in lane c % 64 of group c / 64
+//   fields int32 [n][kp]   h_ic = sum_j J_ij s_jc, kept exact by integer updates
+// A wavefront handles one (node, group): the node's row is read once (wave-uniform loads) for 64
+// candidates, and the field updates of one neighbour are one coalesced 256-byte atomic.
+//
+// The flip order is a sequence of colour classes (independent sets). Within a class no two nodes
+// are adjacent, so flipping all improving nodes of the class at once equals flipping them one at a
+// time in any order: the parallel pass is a sequential single-flip descent, bit for bit.
+#include "sa_reduce.hpp"
+
+namespace sa {
+
+__device__ inline int32_t uniform(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ---------------------------------------------------------------------------------------------
+// Exact energies of candidate bit vectors: e2[c] = sum_i s_i sum_j J_ij s_j.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SA_BLOCK) void energy_bits_kernel(int64_t n, const int32_t* __restrict__ indptr,
+                                                               const int32_t* __restrict__ indices,
+                                                               const int32_t* __restrict__ jint,
+                                                               const uint64_t* __restrict__ cand, int64_t nwords,
+                                                               int64_t* __restrict__ e2) {
+  __shared__ int64_t red[SA_WAVES_PER_BLOCK];
+  const int c = blockIdx.y;
+  const uint64_t* __restrict__ bits = cand + (int64_t)c * nwords;
+  int64_t acc = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t h = 0;
+    for (int32_t e = indptr[i]; e < indptr[i + 1]; ++e) {
+      const int32_t j = stream_load(&indices[e]);
+      const int32_t sj = ((bits[j >> 6] >> (j & 63)) & 1ull) ? 1 : -1;
+      h += (int64_t)stream_load(&jint[e]) * sj;
+    }
+    const int64_t si = ((bits[i >> 6] >> (i & 63)) & 1ull) ? 1 : -1;
+    acc += si * h;
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0 && acc != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&e2[c]), (unsigned long long)acc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Candidate-major bits -> node-major bits (word i*groups+g, bit l = candidate 64 g + l).
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SA_BLOCK) void bits_transpose_kernel(int64_t n, int32_t k,
+                                                                  const uint64_t* __restrict__ cand, int64_t nwords,
+                                                                  int32_t groups, uint64_t* __restrict__ sb) {
+  const int64_t total = n * groups;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = q / groups;
+    const int g = (int)(q % groups);
+    uint64_t word = 0;
+    for (int l = 0; l < 64; ++l) {
+      const int c = g * 64 + l;
+      if (c >= k) break;
+      word |= ((cand[(int64_t)c * nwords + (i >> 6)] >> (i & 63)) & 1ull) << l;
+    }
+    sb[q] = word;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fields and raw energies of the start candidates. blockIdx.y = group.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SA_BLOCK) void ls_init_kernel(int64_t n, const int32_t* __restrict__ indptr,
+                                                           const int32_t* __restrict__ indices,
+                                                           const int32_t* __restrict__ jint, int32_t k,
+                                                           int32_t groups, const uint64_t* __restrict__ sb,
+                                                           int8_t* __restrict__ spins, int32_t* __restrict__ fields,
+                                                           int64_t* __restrict__ e2) {
+  __shared__ int64_t red[SA_WAVES_PER_BLOCK][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = blockIdx.y;
+  const int c = g * 64 + lane;
+  const int64_t kp = (int64_t)groups * 64;
+  const int64_t nw = (int64_t)gridDim.x * SA_WAVES_PER_BLOCK;
+  int64_t acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * SA_WAVES_PER_BLOCK + wave; i < n; i += nw) {
+    const int32_t e0 = uniform(indptr[i]), e1 = uniform(indptr[i + 1]);
+    int32_t h = 0;
+    for (int32_t e = e0; e < e1; ++e) {
+      const int32_t j = uniform(indices[e]);
+      const int32_t jv = uniform(jint[e]);
+      const uint64_t w = sb[(int64_t)j * groups + g];
+      h += ((w >> lane) & 1ull) ? jv : -jv;
+    }
+    const uint64_t wi = sb[i * groups + g];
+    const int32_t si = ((wi >> lane) & 1ull) ? 1 : -1;
+    spins[i * kp + c] = (int8_t)si;
+    fields[i * kp + c] = h;
+    acc += (int64_t)si * h;
+  }
+  red[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0 && c < k) {
+    const int64_t v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&e2[c]), (unsigned long long)v);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// One colour class of one pass. blockIdx.y = group; a wave per class node.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SA_BLOCK) void ls_class_kernel(const int32_t* __restrict__ indptr,
+                                                            const int32_t* __restrict__ indices,
+                                                            const int32_t* __restrict__ jint, int32_t k,
+                                                            int32_t groups, const int32_t* __restrict__ nodes,
+                                                            int64_t count, uint64_t seed, uint64_t pass,
+                                                            uint64_t thresh, int32_t all, int8_t* __restrict__ spins,
+                                                            int32_t* __restrict__ fields, int32_t* __restrict__ flags) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = blockIdx.y;
+  const int c = g * 64 + lane;
+  const int64_t kp = (int64_t)groups * 64;
+  const int64_t nw = (int64_t)gridDim.x * SA_WAVES_PER_BLOCK;
+  bool any_flip = false;
+  for (int64_t u = (int64_t)blockIdx.x * SA_WAVES_PER_BLOCK + wave; u < count; u += nw) {
+    const int32_t i = uniform(nodes[u]);
+    if (!all && (mix3(seed, pass, (uint64_t)i) >> 11) >= thresh) continue;
+    const int64_t off = (int64_t)i * kp + c;
+    const int32_t s = spins[off];
+    const int32_t h = fields[off];
+    const bool flip = (c < k) && (s * h < 0);
+    if (!__any(flip)) continue;
+    const int32_t snew = -s;
+    if (flip) spins[off] = (int8_t)snew;
+    any_flip |= flip;
+    const int32_t e0 = uniform(indptr[i]), e1 = uniform(indptr[i + 1]);
+    for (int32_t e = e0; e < e1; ++e) {
+      const int32_t j = uniform(indices[e]);
+      const int32_t jv = uniform(jint[e]);
+      if (flip) atomicAdd(&fields[(int64_t)j * kp + c], 2 * jv * snew);
+    }
+  }
+  if (any_flip) flags[c] = 1;
+}
+
+__global__ __launch_bounds__(SA_BLOCK) void ls_energy_kernel(int64_t n, int32_t k, int32_t groups,
+                                                             const int8_t* __restrict__ spins,
+                                                             const int32_t* __restrict__ fields,
+                                                             int64_t* __restrict__ e2) {
+  __shared__ int64_t red[SA_WAVES_PER_BLOCK][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = blockIdx.y;
+  const int c = g * 64 + lane;
+  const int64_t kp = (int64_t)groups * 64;
+  const int64_t nw = (int64_t)gridDim.x * SA_WAVES_PER_BLOCK;
+  int64_t acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * SA_WAVES_PER_BLOCK + wave; i < n; i += nw)
+    acc += (int64_t)spins[i * kp + c] * (int64_t)fields[i * kp + c];
+  red[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0 && c < k) {
+    const int64_t v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&e2[c]), (unsigned long long)v);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(SA_BLOCK) void ls_extract_kernel(int64_t n, int32_t groups, int32_t c,
+                                                              const int8_t* __restrict__ spins, T* __restrict__ out) {
+  const int64_t kp = (int64_t)groups * 64;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (T)spins[i * kp + c];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Greedy colouring round (see the header comment of sa_colour_round).
+// ---------------------------------------------------------------------------------------------
+__device__ inline bool higher(int32_t dj, uint64_t hj, int32_t j, int32_t di, uint64_t hi, int32_t i) {
+  if (dj != di) return dj > di;
+  if (hj != hi) return hj > hi;
+  return j > i;
+}
+
+__global__ __launch_bounds__(SA_BLOCK) void colour_round_kernel(int64_t n, const int32_t* __restrict__ indptr,
+                                                                const int32_t* __restrict__ indices, uint64_t seed,
+                                                                int32_t* colour, void* work) {
+  uint64_t left = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (__hip_atomic_load(&colour[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= 0) continue;
+    const int32_t e0 = indptr[i], e1 = indptr[i + 1];
+    const int32_t di = e1 - e0;
+    const uint64_t hi = mix3(seed, 0xC0102ull, (uint64_t)i);
+    bool ready = true;
+    for (int32_t e = e0; e < e1 && ready; ++e) {
+      const int32_t j = indices[e];
+      if (j == (int32_t)i) continue;
+      const int32_t cj = __hip_atomic_load(&colour[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cj < 0 && higher(indptr[j + 1] - indptr[j], mix3(seed, 0xC0102ull, (uint64_t)j), j, di, hi, (int32_t)i))
+        ready = false;
+    }
+    if (!ready) {
+      ++left;
+      continue;
+    }
+    int32_t col = -1;
+    for (int32_t base = 0; col < 0; base += 64) {
+      uint64_t used = 0;
+      for (int32_t e = e0; e < e1; ++e) {
+        const int32_t j = indices[e];
+        if (j == (int32_t)i) continue;
+        const int32_t cj = __hip_atomic_load(&colour[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cj >= base && cj < base + 64) used |= 1ull << (cj - base);
+      }
+      if (~used) col = base + __builtin_ctzll(~used);
+    }
+    __hip_atomic_store(&colour[i], col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  uint64_t out;
+  if (grid_reduce_u64(left, work, &out) && threadIdx.x == 0) state_of(work)->i64[1] = (int64_t)out;
+}
+
+}  // namespace sa
+
+using namespace sa;
+
+extern "C" {
+
+int sa_energy_bits_i64(const sa_csr* A, int32_t k, const uint64_t* cand_bits, int64_t nwords, int64_t* e2,
+                       hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->jint && cand_bits && e2 && k >= 1 && k <= 65535, "bad arguments");
+  SA_REQUIRE(nwords * 64 >= A->n, "nwords too small for n");
+  hipLaunchKernelGGL(energy_bits_kernel, dim3(grid_for(A->n, SA_BLOCK * 4, 2048), k), dim3(SA_BLOCK), 0, stream,
+                     A->n, A->indptr, A->indices, A->jint, cand_bits, nwords, e2);
+  return check_launch("energy_bits_i64");
+}
+
+int sa_bits_transpose(int64_t n, int32_t k, const uint64_t* cand_bits, int64_t nwords, int32_t groups, uint64_t* sb,
+                      hipStream_t stream) {
+  SA_REQUIRE(n > 0 && k >= 1 && groups >= 1 && k <= 64 * groups && cand_bits && sb, "bad arguments");
+  SA_REQUIRE(nwords * 64 >= n, "nwords too small for n");
+  hipLaunchKernelGGL(bits_transpose_kernel, dim3(grid_for(n * groups, SA_BLOCK, 4096)), dim3(SA_BLOCK), 0, stream, n,
+                     k, cand_bits, nwords, groups, sb);
+  return check_launch("bits_transpose");
+}
+
+int sa_ls_init(const sa_csr* A, int32_t k, int32_t groups, const uint64_t* sb, int8_t* spins, int32_t* fields,
+               int64_t* e2, hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->jint && sb && spins && fields && e2, "bad arguments");
+  SA_REQUIRE(k >= 1 && groups >= 1 && k <= 64 * groups && groups <= 1024, "bad candidate count");
+  hipLaunchKernelGGL(ls_init_kernel, dim3(grid_for(A->n, SA_WAVES_PER_BLOCK * 8, 4096), groups), dim3(SA_BLOCK), 0,
+                     stream, A->n, A->indptr, A->indices, A->jint, k, groups, sb, spins, fields, e2);
+  return check_launch("ls_init");
+}
+
+int sa_ls_class_pass(const sa_csr* A, int32_t k, int32_t groups, const int32_t* class_nodes, int64_t count,
+                     uint64_t seed, int64_t pass, double frac, int8_t* spins, int32_t* fields, int32_t* flags,
+                     hipStream_t stream) {
+  SA_REQUIRE(A && A->jint && class_nodes && spins && fields && flags && pass >= 0, "bad arguments");
+  SA_REQUIRE(k >= 1 && groups >= 1 && k <= 64 * groups && groups <= 1024, "bad candidate count");
+  SA_REQUIRE(frac > 0.0 && frac <= 1.0, "frac must be in (0, 1]");
+  if (count <= 0) return SA_OK;
+  const int32_t all = frac >= 1.0 ? 1 : 0;
+  const uint64_t thresh = (uint64_t)(frac * 9007199254740992.0);
+  hipLaunchKernelGGL(ls_class_kernel, dim3(grid_for(count, SA_WAVES_PER_BLOCK * 2, 8192), groups), dim3(SA_BLOCK), 0,
+                     stream, A->indptr, A->indices, A->jint, k, groups, class_nodes, count, seed, (uint64_t)pass,
+                     thresh, all, spins, fields, flags);
+  return check_launch("ls_class_pass");
+}
+
+int sa_ls_energy(int64_t n, int32_t k, int32_t groups, const int8_t* spins, const int32_t* fields, int64_t* e2,
+                 hipStream_t stream) {
+  SA_REQUIRE(n > 0 && spins && fields && e2 && k >= 1 && groups >= 1 && k <= 64 * groups, "bad arguments");
+  hipLaunchKernelGGL(ls_energy_kernel, dim3(grid_for(n, SA_WAVES_PER_BLOCK * 16, 2048), groups), dim3(SA_BLOCK), 0,
+                     stream, n, k, groups, spins, fields, e2);
+  return check_launch("ls_energy");
+}
+
+int sa_ls_extract_f32(int64_t n, int32_t groups, int32_t c, const int8_t* spins, float* out, hipStream_t stream) {
+  SA_REQUIRE(n > 0 && spins && out && c >= 0 && c < 64 * groups, "bad arguments");
+  hipLaunchKernelGGL(ls_extract_kernel<float>, dim3(grid_for(n, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0, stream, n,
+                     groups, c, spins, out);
+  return check_launch("ls_extract_f32");
+}
+
+int sa_ls_extract_f64(int64_t n, int32_t groups, int32_t c, const int8_t* spins, double* out, hipStream_t stream) {
+  SA_REQUIRE(n > 0 && spins && out && c >= 0 && c < 64 * groups, "bad arguments");
+  hipLaunchKernelGGL(ls_extract_kernel<double>, dim3(grid_for(n, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0, stream, n,
+                     groups, c, spins, out);
+  return check_launch("ls_extract_f64");
+}
+
+int sa_colour_round(const sa_csr* A, uint64_t seed, int32_t* colour, void* work, hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && colour && work, "bad arguments");
+  hipLaunchKernelGGL(colour_round_kernel, dim3(grid_for(A->n, SA_BLOCK * 4, SA_RED_GRID)), dim3(SA_BLOCK), 0, stream,
+                     A->n, A->indptr, A->indices, seed, colour, work);
+  return check_launch("colour_round");
+}
+
+}  // extern "C"

@@ -1,0 +1,33 @@
+// Fused scaled CSR SpMV: y_r = a * x_r + b * s_r * sum_k J_rk g_k, where g = s (.) x is gathered.
+#pragma once
+
+#include "sa_common.hpp"
+
+namespace sa {
+
+template <typename T>
+struct SpmvParams {
+  int64_t n;
+  const int32_t* indptr;
+  const int32_t* indices;
+  const T* vals;
+  const int32_t* tile_rows;
+  int64_t ntiles;
+  int32_t lanes_fixed;
+  const T* g;                 // gathered vector (s (.) x)
+  const T* s;                 // row scale of the epilogue; NULL means 1
+  const T* x;                 // diagonal-term vector; NULL when a == 0
+  double a, b;
+  // NULL, or the Lanczos state: the launch then runs only while status == SA_RUNNING and b is
+  // multiplied by sigma[j], the scale of the raw basis vector being applied.
+  const sa_state* lz;
+  T* y;
+};
+
+template <typename T>
+int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream);
+
+template <typename T>
+int launch_prescale(int64_t n, const T* s, const T* x, T* z, hipStream_t stream);
+
+}  // namespace sa

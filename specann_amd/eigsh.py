@@ -1,0 +1,166 @@
+"""Warm-started Lanczos eigensolver on the GPU: the counterpart of utils/eigsh.py:10-17.
+
+The reference calls CuPy's thick-restart ``eigsh(S_rho, k=1, which='SA', tol, maxiter, v0, ncv)``
+(models/spectral_annealing.py:671-677). With k = 1 a thick restart keeps only the Ritz vector, so
+it spans the same Krylov space as an explicit restart from it; this solver restarts explicitly
+from the Ritz vector every ``ncv`` steps. It differs from CuPy in three deliberate ways:
+
+* the recurrence runs on N_alpha (the reference's S_rho = rho I - N has the same Krylov space and
+  T_S = rho I - T_N), which keeps the FP32 basis orthogonal; mu = rho - theta is reported;
+* the stopping test |beta_j s_j| <= tol is applied after every step, not once per restart;
+* ``iterations`` counts operator applications (CuPy's counter is ncv + (ncv - 1) r).
+
+Every step is three HIP launches (fused scaled SpMV, projections, Gram-Schmidt + FP64 Ritz solve);
+all launches read the solve's status on the device and do nothing once it stopped, so the host
+queues several steps, and the restart pair, between reads of the status.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (SA_BROKEN, SA_CONVERGED, SA_MAXITER, SA_REJECTED, SA_RESTART, SA_RUNNING, EigensolverBreakdown,
+                   call)
+from .device import DeviceProblem, ptr, stream
+
+TERMINAL = (SA_CONVERGED, SA_MAXITER, SA_BROKEN, SA_REJECTED)
+
+
+@dataclass
+class LanczosResult:
+    status: int
+    theta: float          # largest Ritz value of N
+    res: float            # residual estimate |beta s_last|
+    steps: int            # operator applications
+    cycles: int           # restarts
+    first_beta: float     # beta_0 of a gated (predicted) start: the predictor residual
+    start_norm: float
+
+
+def default_ncv(n: int, ncv: int | None) -> int:
+    """CuPy's default for k = 1 is min(max(2k, k + 32), n - 1) = min(33, n - 1)."""
+    v = min(33, n - 1) if ncv is None else int(ncv)
+    return max(2, min(v, _lib.SA_MAX_NCV, n))
+
+
+class LanczosSolver:
+    """Buffers and launch sequence of the warm-started Lanczos for one DeviceProblem."""
+
+    def __init__(self, prob: DeviceProblem, ncv: int | None = None, chunk: int = 8):
+        self.prob = prob
+        self.n = prob.n
+        self.ncv = default_ncv(prob.n, ncv)
+        self.chunk = max(1, int(chunk))
+        dev, dt = prob.device, prob.dtype
+        self.V = torch.empty((self.ncv + 1, self.n), dtype=dt, device=dev)
+        self.Z = torch.empty(self.n, dtype=dt, device=dev)
+        self.W = torch.empty(self.n, dtype=dt, device=dev)
+        self.Y = torch.empty(self.n, dtype=dt, device=dev)
+        self.sfx = prob.sfx
+        self.work = prob.work
+
+    # -- launch helpers ------------------------------------------------------------------------
+    def begin(self, tol: float, maxiter: int, bound: float, pred_tol: float = float("inf")) -> None:
+        call("sa_lanczos_begin", self.work.ptr, self.ncv, int(maxiter), float(tol), float(bound), float(pred_tol),
+             stream())
+
+    def start(self, s: torch.Tensor | None, x: torch.Tensor, *, d: torch.Tensor | None = None, pexp: float = 0.0,
+              cx: float = 1.0, y: torch.Tensor | None = None, cy: float = 0.0, gate: int = 0,
+              only_if: int = -1) -> None:
+        call(f"sa_lanczos_start_{self.sfx}", self.work.ptr, self.n, ptr(s), ptr(x), ptr(d), float(pexp), float(cx),
+             ptr(y), float(cy), ptr(self.V), ptr(self.Z), int(gate), int(only_if), stream())
+
+    def _step(self, s: torch.Tensor | None) -> None:
+        call(f"sa_lanczos_step_{self.sfx}", self.prob.csr_ref, self.work.ptr, ptr(s), ptr(self.V), ptr(self.Z),
+             ptr(self.W), stream())
+
+    def ritz(self, y_last: torch.Tensor | None = None, only_if: int = -1) -> None:
+        call(f"sa_lanczos_ritz_{self.sfx}", self.work.ptr, self.n, ptr(self.V), ptr(y_last), ptr(self.Y),
+             int(only_if), stream())
+
+    # -- the solve ------------------------------------------------------------------------------
+    def run(self, s: torch.Tensor | None, maxiter: int, first_check: int | None = None) -> LanczosResult:
+        """Iterate from the started vector until converged, stopped at maxiter, rejected or broken."""
+        j = 0
+        launched = 0
+        check = first_check or self.chunk
+        limit = int(maxiter) + 4 * self.chunk + self.ncv + 8
+        while True:
+            for _ in range(check):
+                self._step(s)
+                launched += 1
+                j += 1
+                if j == self.ncv:
+                    # basis full: restart from the Ritz vector, on the device, only if still running
+                    self.ritz(None, only_if=SA_RESTART)
+                    self.start(s, self.Y, only_if=SA_RESTART)
+                    j = 0
+            st = self.work.head()
+            if st.status in TERMINAL:
+                return LanczosResult(st.status, st.theta, st.res, st.steps, st.cycles, st.first_beta, st.r[3])
+            if launched > limit:
+                raise EigensolverBreakdown(f"Lanczos did not stop after {launched} steps (status {st.status})")
+            check = self.chunk
+
+    def final_vector(self, y_last: torch.Tensor | None) -> tuple[float, float]:
+        """Ritz vector of the last cycle into Y; returns (||Y||, Y . y_last)."""
+        self.ritz(y_last, only_if=-1)
+        st = self.work.head()
+        return float(st.r[4]), float(st.r[5])
+
+
+class ScaledOperator:
+    """S_rho = rho I - N_alpha (or N_alpha itself with rho = None), N = s (.) (J/c) (.) s.
+
+    The counterpart of SRhoOperator / NAlphaOperator (models/spectral_annealing.py:29-53).
+    """
+
+    def __init__(self, prob: DeviceProblem, s: torch.Tensor | None, rho: float | None):
+        self.prob, self.s, self.rho = prob, s, rho
+        self.shape = (prob.n, prob.n)
+        self.dtype = prob.dtype
+        self._z = torch.empty(prob.n, dtype=prob.dtype, device=prob.device)
+
+    def matvec(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        out = torch.empty_like(x) if out is None else out
+        if self.rho is None:
+            return self.prob.matvec(self.s, x, 0.0, 1.0, out, self._z)
+        return self.prob.matvec(self.s, x, self.rho, -1.0, out, self._z)
+
+    __call__ = matvec
+
+
+def eigsh(A: ScaledOperator, k: int = 1, which: str = "SA", tol: float = 0.0, maxiter: int | None = None,
+          v0: torch.Tensor | None = None, ncv: int | None = None, seed: int = 0):
+    """Leading eigenpair on the GPU with the reference wrapper's return convention (w, v, iterations).
+
+    For S_rho (A.rho set) ``which='SA'`` gives mu = rho - lambda_max(N); for N (A.rho None) ``which='LA'``
+    gives lambda_max(N). v is an (n, 1) device tensor of unit norm; v0 = None starts from a seeded
+    normal vector.
+    """
+    if k != 1:
+        raise ValueError("only k = 1 is supported (the reference calls eigsh with k = 1)")
+    expect = "SA" if A.rho is not None else "LA"
+    if which != expect:
+        raise ValueError(f"which must be {expect!r} for this operator")
+    prob = A.prob
+    solver = LanczosSolver(prob, ncv)
+    maxiter = int(maxiter) if maxiter is not None else 10 * prob.n
+    tol = float(tol) if tol and tol > 0 else float(np.finfo(np.float32 if prob.dtype == torch.float32
+                                                            else np.float64).eps)
+    x = v0 if v0 is not None else prob.fill_normal(seed, 0, torch.empty(prob.n, dtype=prob.dtype,
+                                                                        device=prob.device))
+    rho = A.rho if A.rho is not None else 0.0
+    solver.begin(tol, maxiter, bound=abs(rho) + 2.0)
+    solver.start(A.s, x)
+    res = solver.run(A.s, maxiter)
+    if res.status == SA_BROKEN:
+        raise EigensolverBreakdown("Lanczos breakdown: non-finite or oversized tridiagonal entry")
+    nrm, _ = solver.final_vector(None)
+    v = (solver.Y / nrm).reshape(-1, 1)
+    w = np.array([rho - res.theta if A.rho is not None else res.theta])
+    return w, v, res.steps

@@ -1,0 +1,183 @@
+"""Batched deterministic local search on the GPU: the counterpart of flip_refinement and
+gpu_final_sweep (models/spectral_annealing.py:983-1112).
+
+The reference flips spins in place from many warps at once, so its result depends on thread timing
+(README.md:204-208). Here a pass visits the colour classes of a greedy colouring (independent sets)
+one after another; inside a class every selected node with s_i h_i < 0 flips at once, which is the
+same as flipping them one at a time because no two of them are adjacent. The outcome is therefore
+the sequential single-flip descent in that order, bit for bit, for every candidate, and the energy
+never increases. Fields h = J s are kept exactly in int32 and energies in int64.
+
+The termination schedule is the reference's: main passes over a `subset_fraction` share of the
+nodes until every candidate has had a pass without a flip (at most `max_iterations`), then
+`num_rounds` rounds of at most `final_sweep_iters` passes over a `final_sweep_fraction` share,
+each round ending at a pass without flips. Node selection and class order are counter-based hashes
+of (seed, pass), so the CPU oracle reproduces them exactly.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ._lib import call
+from .device import DeviceProblem, ptr, stream
+
+ORDER_SALT = 0x0DE5C1A55C0FFEE1
+FINAL_PASS_BASE = 1 << 40
+_M = (1 << 64) - 1
+
+
+def _splitmix64(x: int) -> int:
+    z = (x + 0x9E3779B97F4A7C15) & _M
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M
+    return z ^ (z >> 31)
+
+
+def _mix3(a: int, b: int, c: int) -> int:
+    return _splitmix64(_splitmix64(_splitmix64(a & _M) ^ (b & _M)) ^ (c & _M))
+
+
+def class_order(seed: int, q: int, num_classes: int) -> list[int]:
+    """Order in which pass q visits the colour classes."""
+    keys = [_mix3(seed ^ ORDER_SALT, q, c) for c in range(num_classes)]
+    return sorted(range(num_classes), key=lambda c: (keys[c], c))
+
+
+@dataclass
+class Colouring:
+    colours: torch.Tensor      # int32 [n]
+    class_nodes: torch.Tensor  # int32 [n], nodes grouped by colour, ascending within a class
+    offsets: list              # class q = class_nodes[offsets[q]:offsets[q+1]]
+    rounds: int
+
+    @property
+    def num_classes(self) -> int:
+        return len(self.offsets) - 1
+
+
+def colour_graph(prob: DeviceProblem, seed: int) -> Colouring:
+    """Greedy colouring on the GPU (descending degree, then hash, then index), one round per launch."""
+    n = prob.n
+    colours = torch.full((n,), -1, dtype=torch.int32, device=prob.device)
+    rounds = 0
+    while True:
+        call("sa_colour_round", prob.csr_ref, int(seed) & _M, ptr(colours), prob.work.ptr, stream())
+        rounds += 1
+        left = int(prob.work.head().i64[1])
+        if left == 0:
+            break
+        if rounds > n + 1:
+            raise RuntimeError("colouring did not finish")
+    num = int(colours.max().item()) + 1
+    class_nodes = torch.argsort(colours, stable=True).to(torch.int32)
+    counts = torch.bincount(colours, minlength=num).cpu().numpy()
+    offsets = [0] + np.cumsum(counts).tolist()
+    return Colouring(colours, class_nodes, offsets, rounds)
+
+
+@dataclass
+class LocalSearchState:
+    spins: torch.Tensor        # int8 [n][kp]
+    fields: torch.Tensor       # int32 [n][kp]
+    groups: int
+    k: int
+    passes: int                # main passes (the reference's local_search_iterations)
+    total_passes: int
+    e2_start: np.ndarray       # int64 [k]: sum_i s_i h_i of the start candidates (energy = -e2/2)
+    e2_final: np.ndarray
+
+    @property
+    def energies_start(self) -> np.ndarray:
+        return -(self.e2_start // 2)
+
+    @property
+    def energies_final(self) -> np.ndarray:
+        return -(self.e2_final // 2)
+
+
+def candidates_to_node_major(prob: DeviceProblem, cand_bits: torch.Tensor, k: int) -> tuple[torch.Tensor, int]:
+    """Candidate-major spin bits (k x nwords uint64) -> node-major bits (n x groups)."""
+    groups = (k + 63) // 64
+    nwords = cand_bits.shape[1]
+    sb = torch.empty((prob.n, groups), dtype=torch.int64, device=prob.device)
+    call("sa_bits_transpose", prob.n, k, ptr(cand_bits), nwords, groups, ptr(sb), stream())
+    return sb, groups
+
+
+def init_fields(prob: DeviceProblem, cand_bits: torch.Tensor, k: int):
+    """Spins, exact fields and the start energies of k candidates."""
+    if prob.jint is None:
+        raise NotImplementedError("the exact local search needs integer couplings")
+    sb, groups = candidates_to_node_major(prob, cand_bits, k)
+    kp = 64 * groups
+    spins = torch.empty((prob.n, kp), dtype=torch.int8, device=prob.device)
+    fields = torch.empty((prob.n, kp), dtype=torch.int32, device=prob.device)
+    e2 = torch.zeros(kp, dtype=torch.int64, device=prob.device)
+    call("sa_ls_init", prob.csr_ref, k, groups, ptr(sb), ptr(spins), ptr(fields), ptr(e2), stream())
+    return spins, fields, groups, e2
+
+
+def flip_refinement(prob: DeviceProblem, cand_bits: torch.Tensor, k: int, max_iterations: int = 1000,
+                    final_sweep_fraction: float = 1.0, final_sweep_iters: int = 25, num_rounds: int = 25,
+                    subset_fraction: float = 0.9, seed: int = 0, colouring: Colouring | None = None
+                    ) -> LocalSearchState:
+    """Local search of k candidates given as spin bits (flip_refinement, :1039-1112)."""
+    seed = int(seed) & _M
+    spins, fields, groups, e2 = init_fields(prob, cand_bits, k)
+    e2_start = e2[:k].cpu().numpy().copy()
+    col = colouring or colour_graph(prob, seed)
+    C = col.num_classes
+    flags = torch.zeros(64 * groups, dtype=torch.int32, device=prob.device)
+    q = [0]
+
+    def run_pass(sel_id: int, frac: float) -> np.ndarray:
+        flags.zero_()
+        for cls in class_order(seed, q[0], C):
+            lo, hi = col.offsets[cls], col.offsets[cls + 1]
+            if hi <= lo:
+                continue
+            call("sa_ls_class_pass", prob.csr_ref, k, groups, ctypes.c_void_p(col.class_nodes.data_ptr() + 4 * lo),
+                 hi - lo, seed, int(sel_id), float(frac), ptr(spins), ptr(fields), ptr(flags), stream())
+        q[0] += 1
+        return flags[:k].cpu().numpy() != 0
+
+    active = np.ones(k, dtype=bool)
+    passes = 0
+    for it in range(int(max_iterations)):
+        passes += 1
+        active &= run_pass(it, subset_fraction)
+        if not active.any():
+            break
+    fixed_point = False
+    for r in range(int(num_rounds)):
+        if fixed_point:
+            break
+        for _ in range(int(final_sweep_iters)):
+            if not run_pass(FINAL_PASS_BASE + r, final_sweep_fraction).any():
+                # a pass over every node without a flip is a fixed point of all later passes
+                fixed_point = final_sweep_fraction >= 1.0
+                break
+    e2.zero_()
+    call("sa_ls_energy", prob.n, k, groups, ptr(spins), ptr(fields), ptr(e2), stream())
+    e2_final = e2[:k].cpu().numpy().copy()
+    return LocalSearchState(spins, fields, groups, k, passes, q[0], e2_start, e2_final)
+
+
+def extract(prob: DeviceProblem, state: LocalSearchState, c: int) -> torch.Tensor:
+    """Candidate c of the local-search buffer as a +-1 vector in the problem's dtype."""
+    out = torch.empty(prob.n, dtype=prob.dtype, device=prob.device)
+    call(f"sa_ls_extract_{prob.sfx}", prob.n, state.groups, int(c), ptr(state.spins), ptr(out), stream())
+    return out
+
+
+def energies_of_bits(prob: DeviceProblem, cand_bits: torch.Tensor, k: int) -> np.ndarray:
+    """Exact energies -1/2 s^T J s of k candidate bit vectors (int64)."""
+    if prob.jint is None:
+        raise NotImplementedError("exact energies need integer couplings")
+    e2 = torch.zeros(k, dtype=torch.int64, device=prob.device)
+    call("sa_energy_bits_i64", prob.csr_ref, k, ptr(cand_bits), cand_bits.shape[1], ptr(e2), stream())
+    return -(e2.cpu().numpy() // 2)

@@ -1,0 +1,169 @@
+"""Kernel-level parity of libspecann against the CPU oracle (GPU)."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from oracle import specann_oracle as O
+from specann_amd import graphs
+from specann_amd.device import DeviceProblem, ptr, stream
+from specann_amd._lib import call
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.finfo(np.float32).eps
+F64 = np.finfo(np.float64).eps
+
+
+def star(n):
+    rows = np.zeros(n - 1, dtype=np.int64)
+    cols = np.arange(1, n, dtype=np.int64)
+    vals = np.where(np.arange(n - 1) % 3 == 0, 1.0, -1.0)
+    return graphs.symmetric_csr(n, rows, cols, vals)
+
+
+CASES = {
+    "er_2k_pm1": lambda: graphs.erdos_renyi(2000, 40 / 1999, seed=1, kind="pm1"),
+    "ba_20k_neg": lambda: graphs.barabasi_albert(20000, 3, seed=2, kind="neg"),
+    "star_9k": lambda: star(9000),
+    "er_300_sparse": lambda: graphs.erdos_renyi(300, 2 / 299, seed=3, kind="pm1"),
+}
+
+
+def _dt(torch_dtype):
+    return np.float32 if torch_dtype == torch.float32 else np.float64
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("tile", [512, 2048])
+def test_scaled_spmv_matches_oracle(gpu, case, tdtype, tile):
+    J = CASES[case]()
+    print(f"spmv case={case} dtype={tdtype} tile={tile} n={J.shape[0]} nnz={J.nnz}", flush=True)
+    prob = DeviceProblem(J, tdtype, tile_nnz=tile)
+    dt = _dt(tdtype)
+    Jn, d, c = O.normalize(J, dt)
+    assert prob.c == pytest.approx(c, rel=0, abs=0)
+    np.testing.assert_array_equal(prob.d.cpu().numpy(), d)
+    alpha = 0.37
+    s_ref, dsum, g = O.alpha_scaling(d, alpha)
+    s = torch.empty(prob.n, dtype=tdtype, device=gpu)
+    dsum_g, g_g = prob.alpha_scaling(alpha, 1e-12, s)
+    np.testing.assert_allclose(s.cpu().numpy(), s_ref, rtol=4 * np.finfo(dt).eps)
+    assert dsum_g == pytest.approx(dsum, rel=1e-5 if dt == np.float32 else 1e-12)
+    assert g_g == pytest.approx(g, rel=1e-6 if dt == np.float32 else 1e-14)
+    x = O.normal_vector(prob.n, 5, 0).astype(dt)
+    xt = torch.from_numpy(x).to(gpu)
+    y = torch.empty_like(xt)
+    z = torch.empty_like(xt)
+    rho = 1.001
+    prob.matvec(s, xt, rho, -1.0, y, z)
+    s_dev = s.cpu().numpy()
+    ref = O.s_matvec(Jn, s_dev, x, rho)
+    # error bound: a few ulps of the magnitudes involved (FP64 row sums of exact products)
+    mag = rho * np.abs(x) + s_dev.astype(np.float64) * (abs(Jn).astype(np.float64) @ np.abs(s_dev * x))
+    eps = np.finfo(dt).eps
+    err = np.abs(y.cpu().numpy().astype(np.float64) - ref)
+    assert np.all(err <= 6 * eps * mag + 1e-300), f"max excess {np.max(err / (mag + 1e-300))}"
+
+
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+def test_normal_and_rounding_bits(gpu, tdtype):
+    n = 10007
+    prob = DeviceProblem(graphs.erdos_renyi(n, 8 / (n - 1), seed=4, kind="neg"), tdtype)
+    v = torch.empty(n, dtype=tdtype, device=gpu)
+    prob.fill_normal(11, 3, v)
+    ref = O.normal_vector(n, 11, 3)
+    np.testing.assert_allclose(v.cpu().numpy().astype(np.float64), ref, rtol=2 * np.finfo(_dt(tdtype)).eps,
+                               atol=1e-6 if tdtype == torch.float32 else 1e-14)
+    nwords = (n + 63) // 64
+    bits = torch.empty(nwords, dtype=torch.int64, device=gpu)
+    ylast = torch.empty_like(v)
+    call(f"sa_round_{prob.sfx}", prob.work.ptr, n, ptr(v), -0.5, ptr(ylast), ptr(bits), stream())
+    h = prob.work.head().hash
+    yl = ylast.cpu().numpy()
+    np.testing.assert_array_equal(yl, (-0.5 * v.cpu().numpy().astype(np.float64)).astype(_dt(tdtype)))
+    spins = O.round_spins(yl)
+    ref_bits = O.spin_bits(spins)
+    np.testing.assert_array_equal(bits.cpu().numpy().view(np.uint64), ref_bits)
+    assert h == O.bits_signature(ref_bits)
+    call("sa_bits_compare", prob.work.ptr, nwords, ptr(bits), ptr(bits), stream())
+    assert prob.work.head().i64[0] == 0
+
+
+@pytest.mark.parametrize("case", ["er_2k_pm1", "ba_20k_neg"])
+def test_energy_bits_exact(gpu, case):
+    J = CASES[case]()
+    prob = DeviceProblem(J, torch.float32)
+    Jint = O.integer_couplings(J)
+    k = 5
+    cands = np.stack([O.round_spins(O.normal_vector(prob.n, 7, i)) for i in range(k)])
+    bits = torch.from_numpy(np.stack([O.spin_bits(c) for c in cands]).view(np.int64)).to(gpu)
+    e2 = torch.zeros(k, dtype=torch.int64, device=gpu)
+    call("sa_energy_bits_i64", prob.csr_ref, k, ptr(bits), bits.shape[1], ptr(e2), stream())
+    got = -(e2.cpu().numpy() // 2)
+    ref = np.array([O.energy_int64(Jint, c) for c in cands])
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("case", ["er_2k_pm1", "ba_20k_neg", "star_9k"])
+def test_colouring_equals_sequential_greedy(gpu, case):
+    from specann_amd.local_search import colour_graph
+    J = CASES[case]()
+    prob = DeviceProblem(J, torch.float32)
+    col = colour_graph(prob, seed=42)
+    ref = O.greedy_colouring(J.indptr, J.indices, 42)
+    np.testing.assert_array_equal(col.colours.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("case,k", [("er_2k_pm1", 3), ("ba_20k_neg", 70), ("er_300_sparse", 1)])
+def test_local_search_bit_exact(gpu, case, k):
+    from specann_amd.local_search import flip_refinement
+    J = CASES[case]()
+    print(f"ls case={case} k={k}", flush=True)
+    prob = DeviceProblem(J, torch.float32)
+    Jint = O.integer_couplings(J)
+    cands = np.stack([O.round_spins(O.normal_vector(prob.n, 9, i)) for i in range(k)])
+    bits = torch.from_numpy(np.stack([O.spin_bits(c) for c in cands]).view(np.int64)).to(gpu)
+    st = flip_refinement(prob, bits, k, max_iterations=1000, seed=123)
+    ref = O.local_search(Jint, cands, 123, max_iterations=1000)
+    got = st.spins[:, :k].cpu().numpy().T
+    np.testing.assert_array_equal(got, ref.spins)
+    assert st.passes == ref.passes
+    np.testing.assert_array_equal(st.e2_start, ref.e2_start)
+    np.testing.assert_array_equal(st.e2_final, ref.e2_final)
+    # monotone descent and exact final energies
+    assert np.all(st.e2_final >= st.e2_start)
+    for c in range(k):
+        assert -st.e2_final[c] // 2 == O.energy_int64(Jint, ref.spins[c])
+
+
+@pytest.mark.parametrize("case", ["er_2k_pm1", "ba_20k_neg"])
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+def test_lanczos_leading_eigenpair(gpu, case, tdtype):
+    from specann_amd.eigsh import LanczosSolver
+    from specann_amd._lib import SA_CONVERGED
+    J = CASES[case]()
+    prob = DeviceProblem(J, tdtype)
+    dt = _dt(tdtype)
+    Jn, d, c = O.normalize(J, dt)
+    alpha = 0.5
+    s = torch.empty(prob.n, dtype=tdtype, device=gpu)
+    prob.alpha_scaling(alpha, 1e-12, s)
+    sv = s.cpu().numpy()
+    lam_ref, y_ref = O.leading_eigpair(Jn, sv)
+    tol = 1e-6 if tdtype == torch.float32 else 1e-10
+    solver = LanczosSolver(prob, ncv=None)
+    x0 = prob.fill_normal(1, 0, torch.empty(prob.n, dtype=tdtype, device=gpu))
+    solver.begin(tol, 5000, bound=3.0)
+    solver.start(s, x0)
+    res = solver.run(s, 5000)
+    print(f"lanczos case={case} dtype={tdtype} steps={res.steps} status={res.status} res={res.res:.3e}")
+    assert res.status == SA_CONVERGED
+    assert abs(res.theta - lam_ref) <= 10 * tol
+    nrm, _ = solver.final_vector(None)
+    y = solver.Y.cpu().numpy().astype(np.float64) / nrm
+    true_res = np.linalg.norm(O.n_matvec(Jn, sv, y) - res.theta * y)
+    assert true_res <= (1e-5 if tdtype == torch.float32 else 1e-9)
