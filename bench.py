@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark of the SpecAnn alpha-sweep on one MI355X.
+
+A step is one full sweep of the hot path on the workload's graph: 128 alphas (warm-started
+Lanczos per alpha, rounding, deduplication) plus the batched local search of the distinct
+candidates and the exact energies. The graph is generated and uploaded (J/c, degrees) before the
+timed region, so every timed step starts with its inputs resident in HBM.
+
+Reported (one JSON line): value = seconds per FP32 sweep (diagonal predictor, the SpecAnn run);
+the FP64 form (SpecAnn-64) is timed once beside it, with the best-energy gap between the two; the
+roofline object gives the fused scaled SpMV's algorithmic bandwidth (8 nnz + 4 (n+1) + 12 n bytes
+per launch in FP32) against the 8 TB/s HBM peak, timed with HIP events on the launch stream; the
+cpu_baseline object times the reference's CPU operator (scipy CSR, S_rho in FP32) on a bounded
+sample and scales it to the sweep's operator applications.
+
+    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--workload er22|ba20|er1k]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "alpha-sweep sec at n=2^22 FP32 vs FP64; energy gap; SpMV % of HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "er22": dict(kind="er", n=1 << 22, deg=40, coupling="pm1", seed=42,
+                 text="Erdos-Renyi n=2^22, mean degree 40, J uniform in {-1,+1} (seed 42)"),
+    "ba20": dict(kind="ba", n=1 << 20, m=20, coupling="pm1", seed=42,
+                 text="signed Barabasi-Albert n=2^20, m=20, J uniform in {-1,+1} (seed 42)"),
+    "er1k": dict(kind="er", n=1024, deg=40, coupling="neg", seed=42,
+                 text="Erdos-Renyi n=1024, p=40/1023, J=-1 (seed 42)"),
+}
+
+
+def make_graph(w):
+    from specann_amd import graphs
+    if w["kind"] == "er":
+        return graphs.erdos_renyi(w["n"], w["deg"] / (w["n"] - 1), seed=w["seed"], kind=w["coupling"])
+    return graphs.barabasi_albert(w["n"], w["m"], seed=w["seed"], kind=w["coupling"])
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spmv_roofline(prob, reps=30):
+    """Average duration of the fused scaled SpMV kernel (the Lanczos step's launch), HIP events."""
+    import torch
+    from specann_amd._lib import call
+    from specann_amd.device import ptr, stream
+    n, nnz = prob.n, prob.nnz
+    s = torch.empty(n, dtype=prob.dtype, device=prob.device)
+    prob.alpha_scaling(0.5, 1e-12, s)
+    g = torch.empty(n, dtype=prob.dtype, device=prob.device)
+    prob.fill_normal(7, 0, g)
+    y = torch.empty_like(g)
+    fn = f"sa_spmv_gather_{prob.sfx}"
+    for _ in range(3):
+        call(fn, prob.csr_ref, ptr(s), ptr(g), 0.0, None, 1.0, ptr(y), stream())
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        call(fn, prob.csr_ref, ptr(s), ptr(g), 0.0, None, 1.0, ptr(y), stream())
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / reps
+    esz = 4 if prob.sfx == "f32" else 8
+    alg = (4 + esz) * nnz + 4 * (n + 1) + 3 * esz * n
+    return sec, alg
+
+
+def cpu_operator_sample(J, seconds=12.0):
+    """Seconds per application of the reference's CPU operator S_rho = rho I - N (scipy, FP32)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import specann_oracle as O
+    Jn, d, c = O.normalize(J, np.float32)
+    s, _, _ = O.alpha_scaling(d, 0.5)
+    x = O.normal_vector(J.shape[0], 1, 0).astype(np.float32)
+    rho = np.float32(1.001)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        y = rho * x - s * (Jn @ (s * x))
+        reps += 1
+        if time.perf_counter() - t0 >= seconds and reps >= 2:
+            break
+    del y
+    return (time.perf_counter() - t0) / reps, reps
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--workload", default="er22", choices=sorted(WORKLOADS))
+    ap.add_argument("--alphas", type=int, default=128)
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--spmv-only", action="store_true", help="only the SpMV timing loop (for profiling)")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from specann_amd.device import DeviceProblem, require_gpu
+    from specann_amd.spectral_annealing import SpectralAnnealing
+    require_gpu()
+    w = WORKLOADS[args.workload]
+    t0 = time.time()
+    J = make_graph(w)
+    log(f"graph {args.workload}: n={J.shape[0]} nnz={J.nnz} generated in {time.time() - t0:.1f}s")
+    prob32 = DeviceProblem(J, torch.float32)
+    torch.cuda.synchronize()
+    log(f"uploaded FP32 problem, tile_nnz={prob32.tile_nnz}")
+
+    spmv_sec, spmv_bytes = spmv_roofline(prob32)
+    achieved = spmv_bytes / spmv_sec / 1e9
+    log(f"spmv f32: {spmv_sec * 1e6:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
+    if args.spmv_only:
+        print(json.dumps({"spmv_us": spmv_sec * 1e6, "spmv_gbs": achieved}))
+        return 0
+
+    def solver(precision):
+        return SpectralAnnealing("bench", "synthetic", random_seed=42, precision=precision, verbose=False,
+                                 num_alphas=args.alphas)
+
+    for i in range(args.warmup):
+        out = solver(32).run_on_device(prob32)
+        log(f"warmup {i}: {out.sweep_time + out.post_sweep_time:.2f}s")
+    torch.cuda.synchronize()
+    times, outs = [], []
+    for i in range(args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = solver(32).run_on_device(prob32)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        outs.append(out)
+        log(f"step {i}: {times[-1]:.3f}s (sweep {out.sweep_time:.3f}s, local search {out.post_sweep_time:.3f}s, "
+            f"{out.matvecs} operator applications, {out.num_candidates} candidates, best {out.best_energy})")
+    sec32 = float(np.mean(times))
+    e32 = outs[-1].best_energy
+    matvecs = outs[-1].matvecs
+    conv = float(np.mean([bool(r['eigsh_converged']) for r in outs[-1].rows]))
+
+    sec64 = e64 = gap = None
+    if not args.skip_fp64:
+        del prob32
+        torch.cuda.empty_cache()
+        prob64 = DeviceProblem(J, torch.float64)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out64 = solver(64).run_on_device(prob64)
+        torch.cuda.synchronize()
+        sec64 = time.perf_counter() - t
+        e64 = out64.best_energy
+        gap = abs(e32 - e64) / abs(e64) if e64 else None
+        log(f"fp64 sweep: {sec64:.3f}s, best {e64}, {out64.matvecs} operator applications")
+        del prob64
+
+    cpu = None
+    if not args.skip_cpu:
+        cores = len(os.sched_getaffinity(0))
+        per_op, reps = cpu_operator_sample(J)
+        cpu = {"value": per_op * matvecs, "unit": "s (extrapolated)", "cores": 1, "kind": "port",
+               "sample": (f"{reps} applications of the reference's CPU operator S_rho = rho I - N "
+                          f"(scipy CSR, FP32, single-threaded matvec; {cores} cores visible) on the same graph, "
+                          f"{per_op:.3f} s each, times the {matvecs} operator applications of the GPU FP32 sweep; "
+                          f"the CPU local search is not included")}
+        log(f"cpu operator: {per_op:.3f}s per application ({reps} reps)")
+
+    line = {
+        "metric": METRIC, "value": sec32, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec32 * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w["text"], "alphas": args.alphas, "predictor": "diagonal",
+                   "local_search": "batched deterministic", "precision": "FP32 vectors, FP64 Lanczos scalars"},
+        "fp64_sweep_s": sec64, "fp32_vs_fp64_speedup": (sec64 / sec32) if sec64 else None,
+        "best_energy_fp32": e32, "best_energy_fp64": e64, "energy_gap": gap,
+        "operator_applications": matvecs, "converged_fraction": conv,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "spmv_tile_kernel<float> (fused scaled CSR SpMV)",
+                     "bytes_per_launch": spmv_bytes, "us_per_launch": spmv_sec * 1e6},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -126,6 +126,13 @@ int sa_spmv_scaled_f32(const sa_csr* A, const float* s, const float* x, double a
 int sa_spmv_scaled_f64(const sa_csr* A, const double* s, const double* x, double a, double b, double* z, double* y,
                        hipStream_t stream);
 
+/* The bare fused kernel the Lanczos step launches: y = a * x + b * s (.) (J/c) g, with the gather vector
+ * g = s (.) x prescaled by the caller (x may be NULL when a == 0). */
+int sa_spmv_gather_f32(const sa_csr* A, const float* s, const float* g, double a, const float* x, double b, float* y,
+                       hipStream_t stream);
+int sa_spmv_gather_f64(const sa_csr* A, const double* s, const double* g, double a, const double* x, double b,
+                       double* y, hipStream_t stream);
+
 /* ---- Warm-started Lanczos on N_alpha (utils/eigsh.py:10-17, models/spectral_annealing.py:671-677)
  * Basis V is (ncv + 1) x n, row-major, leading dimension n. Z, W are n-vectors of scratch. */
 int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, double bound, double pred_tol,

@@ -142,9 +142,42 @@ static int spmv_scaled(const sa_csr* A, const T* s, const T* x, double a, double
   return launch_spmv<T>(p, A->tile_nnz, stream);
 }
 
+template <typename T>
+static int spmv_gather(const sa_csr* A, const T* s, const T* g, double a, const T* x, double b, T* y,
+                       hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && A->vals && A->tile_rows && g && y, "bad arguments");
+  SA_REQUIRE(a == 0.0 || x, "x is required when a != 0");
+  SpmvParams<T> p;
+  p.n = A->n;
+  p.indptr = A->indptr;
+  p.indices = A->indices;
+  p.vals = (const T*)A->vals;
+  p.tile_rows = A->tile_rows;
+  p.ntiles = A->ntiles;
+  p.lanes_fixed = A->lanes_per_row;
+  p.g = g;
+  p.s = s;
+  p.x = (a != 0.0) ? x : nullptr;
+  p.a = a;
+  p.b = b;
+  p.lz = nullptr;
+  p.y = y;
+  return launch_spmv<T>(p, A->tile_nnz, stream);
+}
+
 }  // namespace sa
 
 extern "C" {
+
+int sa_spmv_gather_f32(const sa_csr* A, const float* s, const float* g, double a, const float* x, double b, float* y,
+                       hipStream_t stream) {
+  return sa::spmv_gather<float>(A, s, g, a, x, b, y, stream);
+}
+
+int sa_spmv_gather_f64(const sa_csr* A, const double* s, const double* g, double a, const double* x, double b,
+                       double* y, hipStream_t stream) {
+  return sa::spmv_gather<double>(A, s, g, a, x, b, y, stream);
+}
 
 int sa_spmv_scaled_f32(const sa_csr* A, const float* s, const float* x, double a, double b, float* z, float* y,
                        hipStream_t stream) {

@@ -29,8 +29,11 @@ def couplings(m: int, kind: str, seed: int) -> np.ndarray:
 
 def symmetric_csr(n: int, rows: np.ndarray, cols: np.ndarray, vals: np.ndarray) -> sp.csr_matrix:
     """J = U + U^T for an upper-triangle edge list (rows < cols, no duplicates)."""
-    order = np.lexsort((cols, rows))
-    rows, cols, vals = rows[order], cols[order], vals[order]
+    key = rows.astype(np.int64) * n + cols
+    if key.size > 1 and not np.all(key[1:] > key[:-1]):
+        order = np.argsort(key, kind="stable")
+        rows, cols, vals = rows[order], cols[order], vals[order]
+    del key
     indptr = np.zeros(n + 1, dtype=np.int64)
     np.cumsum(np.bincount(rows, minlength=n), out=indptr[1:])
     upper = sp.csr_matrix((vals, cols.astype(np.int32), indptr), shape=(n, n))
