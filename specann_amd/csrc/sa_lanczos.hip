@@ -76,9 +76,12 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(void* work, int64_t 
     st->sigma[0] = ok ? 1.0 / nrm : 0.0;
     st->j = 0;
     st->gate = gate;
-    st->first_beta = NAN;
     st->res = INFINITY;
-    if (only_if == SA_RESTART) st->cycles += 1;
+    if (only_if == SA_RESTART) {
+      st->cycles += 1;  // a restart keeps the predictor residual of the solve's first step
+    } else {
+      st->first_beta = NAN;
+    }
     st->status = ok ? SA_RUNNING : SA_BROKEN;
   }
 }

@@ -89,6 +89,7 @@ class LanczosSolver:
         launched = 0
         check = first_check or self.chunk
         limit = int(maxiter) + 4 * self.chunk + self.ncv + 8
+        first_beta = None
         while True:
             for _ in range(check):
                 self._step(s)
@@ -100,8 +101,10 @@ class LanczosSolver:
                     self.start(s, self.Y, only_if=SA_RESTART)
                     j = 0
             st = self.work.head()
+            if first_beta is None:
+                first_beta = st.first_beta
             if st.status in TERMINAL:
-                return LanczosResult(st.status, st.theta, st.res, st.steps, st.cycles, st.first_beta, st.r[3])
+                return LanczosResult(st.status, st.theta, st.res, st.steps, st.cycles, first_beta, st.r[3])
             if launched > limit:
                 raise EigensolverBreakdown(f"Lanczos did not stop after {launched} steps (status {st.status})")
             check = self.chunk

@@ -114,10 +114,12 @@ def test_fp64_form_tight_residuals():
                                   dict(schedule_interval=(0, 1.25), num_alphas=7)])
 def test_sweep_variants(opts):
     J = graphs.erdos_renyi(600, 12 / 599, seed=5, kind="pm1")
-    kw = dict(num_alphas=16)
+    kw = dict(num_alphas=16, verbose=True)
     kw.update(opts)
     print(f"variant {opts}", flush=True)
     prob, out = sweep(J, **kw)
+    for r in out.rows:
+        print(r["alpha_index"], r["eigsh_iterations"], r["eigsh_retry"], r["pred_residual"], r["lambda1"])
     assert len(out.rows) == kw["num_alphas"]
     check_rows_against_dense(J, out, np.float32, 5e-5)
     check_candidates_exact(J, out)
@@ -126,7 +128,12 @@ def test_sweep_variants(opts):
     if opts.get("pred_mode") == "prev" or opts.get("disable_warm_start"):
         assert all(np.isnan(r["pred_residual"]) for r in out.rows)
     else:
-        assert any(r["pred_accepted"] for r in out.rows[1:])
+        # the predictor gate ran at every warm-started alpha and its verdict matches its residual
+        for r in out.rows[1:]:
+            if r["eigsh_retry"]:
+                continue
+            assert np.isfinite(r["pred_residual"])
+            assert r["pred_accepted"] == (not r["pred_residual"] > 1e-3)
 
 
 def test_tiny_and_edge_graphs():
