@@ -119,8 +119,9 @@ def main(argv=None) -> int:
     J = make_graph(w)
     log(f"graph {args.workload}: n={J.shape[0]} nnz={J.nnz} generated in {time.time() - t0:.1f}s")
     prob32 = DeviceProblem(J, torch.float32)
+    tune32 = prob32.autotune()
     torch.cuda.synchronize()
-    log(f"uploaded FP32 problem, tile_nnz={prob32.tile_nnz}")
+    log(f"uploaded FP32 problem; autotuned SpMV: {tune32['choice']} of {tune32['tried']}")
 
     spmv_sec, spmv_bytes = spmv_roofline(prob32)
     achieved = spmv_bytes / spmv_sec / 1e9
@@ -157,6 +158,8 @@ def main(argv=None) -> int:
         del prob32
         torch.cuda.empty_cache()
         prob64 = DeviceProblem(J, torch.float64)
+        tune64 = prob64.autotune()
+        log(f"FP64 autotuned SpMV: {tune64['choice']}")
         torch.cuda.synchronize()
         t = time.perf_counter()
         out64 = solver(64).run_on_device(prob64)
@@ -183,13 +186,15 @@ def main(argv=None) -> int:
         "ms_per_step": sec32 * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": w["text"], "alphas": args.alphas, "predictor": "diagonal",
-                   "local_search": "batched deterministic", "precision": "FP32 vectors, FP64 Lanczos scalars"},
+                   "local_search": "batched deterministic", "precision": "FP32 vectors, FP64 Lanczos scalars",
+                   "spmv_layout": tune32["choice"]},
         "fp64_sweep_s": sec64, "fp32_vs_fp64_speedup": (sec64 / sec32) if sec64 else None,
         "best_energy_fp32": e32, "best_energy_fp64": e64, "energy_gap": gap,
         "operator_applications": matvecs, "converged_fraction": conv,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "spmv_tile_kernel<float> (fused scaled CSR SpMV)",
+                     "kernel": ("spmv_panel_kernel<float> + combine (XCD-panel fused scaled SpMV)"
+                                if tune32["choice"]["npanels"] else "spmv_tile_kernel<float> (fused scaled CSR SpMV)"),
                      "bytes_per_launch": spmv_bytes, "us_per_launch": spmv_sec * 1e6},
         "cpu_baseline": cpu,
     }

@@ -53,8 +53,22 @@ typedef struct sa_csr {
   const int32_t* jint;      /* nnz integer couplings J (local search, exact energies); may be NULL */
   const int32_t* tile_rows; /* ntiles + 1: first row whose first stored entry is >= t * tile_nnz */
   int64_t ntiles;
-  int32_t tile_nnz;         /* stored entries per SpMV tile (power of two, 512..8192) */
+  int32_t tile_nnz;         /* stored entries per SpMV tile: 512, 1024, 2048 or 4096 */
   int32_t lanes_per_row;    /* 0: choose per tile from its mean row length; else 1..64 */
+  /* Optional XCD-panel layout (npanels = 0: plain CSR). The same couplings regrouped by column
+   * panel p = [pbound[p], pbound[p+1]); workgroups reach the 8 XCDs round-robin, and workgroup w
+   * works on panel (w / (8 * pmax_tiles)) * 8 + w % 8, so each XCD gathers only from its panel's
+   * slice of the vector, which stays in that XCD's L2. Per-panel row sums land in `partial` and
+   * are combined (in a fixed order) by the consumer. */
+  int32_t npanels;          /* 0, or a multiple of 8 up to 64 */
+  int32_t pad0;
+  const int32_t* pindptr;   /* npanels x (n + 1): offsets into pindices / pvals */
+  const int32_t* pindices;  /* nnz */
+  const void* pvals;        /* nnz, T */
+  const int32_t* ptile_rows;/* per panel: ntiles_p + 1 tile rows, concatenated */
+  const int64_t* ptile_off; /* npanels + 1 offsets into ptile_rows */
+  int64_t pmax_tiles;       /* max over panels of ntiles_p */
+  void* partial;            /* npanels x n scratch of T */
 } sa_csr;
 
 /* Device-side state of one eigensolve; lives at the start of the work buffer. */
@@ -109,6 +123,19 @@ int sa_normalize_f64(int64_t nnz, double* vals, int64_t n, const double* deg, do
 /* tile_rows[t] = first row r with indptr[r] >= t * tile_nnz (t = 0..ntiles; tile_rows[ntiles] = n). */
 int sa_csr_tiles(int64_t n, const int32_t* indptr, int32_t tile_nnz, int32_t* tile_rows, int64_t ntiles,
                  hipStream_t stream);
+
+/* XCD panels from the plain CSR (sorted columns). pbound: npanels + 1 column boundaries (device).
+ * counts (npanels x n int32, device) receives per-panel row counts; after the caller's exclusive scan
+ * of counts into A->pindptr (npanels x (n + 1), absolute offsets), sa_panel_scatter copies the entries.
+ * sa_panel_tiles fills the per-panel tile rows (tile t of panel p: rows whose first panel entry is in
+ * [pindptr[p][0] + t tile_nnz, ...)). */
+int sa_panel_count(const sa_csr* A, int32_t npanels, const int32_t* pbound, int32_t* counts, hipStream_t stream);
+int sa_panel_scatter_f32(const sa_csr* A, int32_t npanels, const int32_t* pbound, int32_t* pindptr, int32_t* pindices,
+                         float* pvals, hipStream_t stream);
+int sa_panel_scatter_f64(const sa_csr* A, int32_t npanels, const int32_t* pbound, int32_t* pindptr, int32_t* pindices,
+                         double* pvals, hipStream_t stream);
+int sa_panel_tiles(int64_t n, const int32_t* pindptr_p, int32_t tile_nnz, int32_t* tile_rows, int64_t ntiles,
+                   hipStream_t stream);
 
 /* ---- Per-alpha scaling (models/spectral_annealing.py:316, :34) ------------------------------ */
 /* s_i = 1 / sqrt(max(d_i^alpha, clamp)); state.r[1] = sum_i d_i^alpha (degree_scale, :326);

@@ -55,6 +55,15 @@ class SaCsr(ctypes.Structure):
         ("ntiles", c_int64),
         ("tile_nnz", c_int32),
         ("lanes_per_row", c_int32),
+        ("npanels", c_int32),
+        ("pad0", c_int32),
+        ("pindptr", c_void_p),
+        ("pindices", c_void_p),
+        ("pvals", c_void_p),
+        ("ptile_rows", c_void_p),
+        ("ptile_off", c_void_p),
+        ("pmax_tiles", c_int64),
+        ("partial", c_void_p),
     ]
 
 
@@ -102,6 +111,10 @@ _SIGNATURES = {
     "sa_normalize_f32": (c_int32, [c_int64, _P, c_int64, _P, _P, c_double, _P]),
     "sa_normalize_f64": (c_int32, [c_int64, _P, c_int64, _P, _P, c_double, _P]),
     "sa_csr_tiles": (c_int32, [c_int64, _P, c_int32, _P, c_int64, _P]),
+    "sa_panel_count": (c_int32, [POINTER(SaCsr), c_int32, _P, _P, _P]),
+    "sa_panel_scatter_f32": (c_int32, [POINTER(SaCsr), c_int32, _P, _P, _P, _P, _P]),
+    "sa_panel_scatter_f64": (c_int32, [POINTER(SaCsr), c_int32, _P, _P, _P, _P, _P]),
+    "sa_panel_tiles": (c_int32, [c_int64, _P, c_int32, _P, c_int64, _P]),
     "sa_alpha_scaling_f32": (c_int32, [c_int64, _P, c_double, c_double, _P, _P, _P]),
     "sa_alpha_scaling_f64": (c_int32, [c_int64, _P, c_double, c_double, _P, _P, _P]),
     "sa_spmv_scaled_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, c_double, _P, _P, _P]),

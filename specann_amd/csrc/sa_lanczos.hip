@@ -86,6 +86,21 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(void* work, int64_t 
   }
 }
 
+// W = sigma_j s (.) sum_p partial[p] (XCD-panel layout; panel order fixed).
+template <typename T>
+__global__ __launch_bounds__(SA_BLOCK) void lz_combine_kernel(void* work, int64_t n, int32_t npanels,
+                                                              const T* __restrict__ partial, const T* __restrict__ s,
+                                                              T* __restrict__ W) {
+  const sa_state* st = state_of(work);
+  if (st->status != SA_RUNNING) return;
+  const double sigma = st->sigma[st->j];
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int p = 0; p < npanels; ++p) acc += (double)partial[(int64_t)p * n + r];
+    W[r] = (T)(sigma * (s ? (double)s[r] : 1.0) * acc);
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(void* work, int64_t n, const T* __restrict__ V,
                                                            const T* __restrict__ W) {
@@ -417,23 +432,19 @@ template <typename T>
 static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, T* Z, T* W, hipStream_t stream) {
   SA_REQUIRE(A && work && V && Z && W && A->tile_rows, "bad arguments");
   const int64_t n = A->n;
-  SpmvParams<T> p;
-  p.n = n;
-  p.indptr = A->indptr;
-  p.indices = A->indices;
-  p.vals = (const T*)A->vals;
-  p.tile_rows = A->tile_rows;
-  p.ntiles = A->ntiles;
-  p.lanes_fixed = A->lanes_per_row;
+  SpmvParams<T> p = params_from<T>(A);
   p.g = Z;
   p.s = s;
-  p.x = nullptr;
-  p.a = 0.0;
-  p.b = 1.0;
   p.lz = reinterpret_cast<const sa_state*>(work);
   p.y = W;
   int rc = launch_spmv<T>(p, A->tile_nnz, stream);
   if (rc) return rc;
+  if (A->npanels > 0) {
+    hipLaunchKernelGGL(lz_combine_kernel<T>, dim3(grid_for(n, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0, stream, work,
+                       n, A->npanels, (const T*)A->partial, s, W);
+    rc = check_launch("lanczos_combine");
+    if (rc) return rc;
+  }
   const int G = red_grid(n);
   hipLaunchKernelGGL(lz_dots_kernel<T>, dim3(G, (SA_MAX_NCV + SA_RED_COLS - 1) / SA_RED_COLS), dim3(SA_BLOCK), 0,
                      stream, work, n, (const T*)V, (const T*)W);

@@ -1,22 +1,129 @@
 // Fused scaled CSR SpMV for gfx950.
 //
-// The matrix is cut into tiles of tile_nnz stored entries; workgroup t owns the rows whose first
-// entry lies in tile t. Phase 1 streams the tile's column indices and values (non-temporal loads,
-// fully coalesced), gathers g at the column and keeps the exact FP64 product in LDS. Phase 2 sums
-// each row's products with P lanes per row (P from the tile's row count) and a shuffle reduction,
-// then applies the epilogue a x_r + b s_r sum. A row that runs past the end of its tile (only the
-// last row of a tile can) is summed by the whole workgroup. All sums run in a fixed order, so the
-// result is bitwise reproducible; row sums are FP64 even in the FP32 form.
+// The matrix is cut into tiles of tile_nnz stored entries; a workgroup owns the rows whose first
+// entry lies in its tile. Phase 1 streams the tile's column indices and values (non-temporal,
+// coalesced), eight entries per thread with all eight gathers of g in flight at once, and keeps the
+// exact FP64 products in LDS. Phase 2 sums each row's products with P lanes per row (P from the
+// tile's row count) and a shuffle reduction. A row that runs past the end of its tile (only the last
+// row of a tile can) is summed by the whole workgroup. Sums run in a fixed order, so results are
+// bitwise reproducible; row sums are FP64 even in the FP32 form.
+//
+// Random gathers are the bound of this kernel on graphs without locality (a 4-byte gather costs a
+// cache-line access). Workgroups reach the 8 XCDs round-robin, so in the XCD-panel layout each XCD
+// only gathers from its own column panel, which then stays resident in that XCD's 4 MB L2; the
+// per-panel row sums are combined by the consumer in panel order.
 #include "sa_spmv.hpp"
+
+#include "sa_reduce.hpp"
 
 namespace sa {
 
+constexpr int kUnroll = 8;
+
 template <typename T>
-__device__ inline void spmv_store(const SpmvParams<T>& p, int32_t r, double acc, double bb) {
-  const double sr = p.s ? (double)p.s[r] : 1.0;
-  double v = bb * sr * acc;
-  if (p.x) v += p.a * (double)p.x[r];
-  p.y[r] = (T)v;
+struct PlainStore {
+  const T* s;
+  const T* x;
+  double a, bb;
+  T* y;
+  __device__ void operator()(int32_t r, double acc) const {
+    const double sr = s ? (double)s[r] : 1.0;
+    double v = bb * sr * acc;
+    if (x) v += a * (double)x[r];
+    y[r] = (T)v;
+  }
+};
+
+template <typename T>
+struct PanelStore {
+  T* part;  // partial + p * n
+  __device__ void operator()(int32_t r, double acc) const { part[r] = (T)acc; }
+};
+
+// Products of entries [e0, e0 + cnt) into prod[0, cnt), eight gathers in flight per thread.
+template <typename T>
+__device__ inline void stage_products(const int32_t* __restrict__ indices, const T* __restrict__ vals,
+                                      const T* __restrict__ g, int32_t e0, int32_t cnt, double* prod) {
+  for (int32_t base = 0; base < cnt; base += SA_BLOCK * kUnroll) {
+    int32_t c[kUnroll];
+    T v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int32_t e = base + u * SA_BLOCK + (int32_t)threadIdx.x;
+      c[u] = e < cnt ? stream_load(&indices[e0 + e]) : 0;
+      v[u] = e < cnt ? stream_load(&vals[e0 + e]) : (T)0;
+    }
+    T gv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int32_t e = base + u * SA_BLOCK + (int32_t)threadIdx.x;
+      gv[u] = e < cnt ? g[c[u]] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int32_t e = base + u * SA_BLOCK + (int32_t)threadIdx.x;
+      if (e < cnt) prod[e] = (double)v[u] * (double)gv[u];
+    }
+  }
+}
+
+// One tile: rows [r0, r1) of the CSR (indptr, indices, vals); store(r, row sum) for each row.
+template <typename T, int TILE, class Store>
+__device__ inline void tile_body(const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                                 const T* __restrict__ vals, const T* __restrict__ g, int32_t r0, int32_t r1,
+                                 int lanes_fixed, double* prod, double* red, const Store& store) {
+  const int32_t e0 = indptr[r0];
+  const int32_t e_last = indptr[r1 - 1];
+  const int32_t e1 = indptr[r1];
+  const bool long_last = (e1 - e0) > TILE;
+  const int32_t rs = long_last ? r1 - 1 : r1;
+  const int32_t es = long_last ? e_last : e1;
+
+  stage_products<T>(indices, vals, g, e0, es - e0, prod);
+  __syncthreads();
+
+  const int32_t nrows = rs - r0;
+  if (nrows > 0) {
+    int P = lanes_fixed;
+    if (P <= 0) {
+      const int q = SA_BLOCK / nrows;
+      P = q >= 64 ? 64 : q >= 32 ? 32 : q >= 16 ? 16 : q >= 8 ? 8 : q >= 4 ? 4 : q >= 2 ? 2 : 1;
+    }
+    const int groups = SA_BLOCK / P;
+    const int gid = threadIdx.x / P, lig = threadIdx.x % P;
+    for (int32_t rb = r0; rb < rs; rb += groups) {
+      const int32_t r = rb + gid;
+      double acc = 0.0;
+      if (r < rs) {
+        const int32_t b0 = indptr[r] - e0, b1 = indptr[r + 1] - e0;
+        for (int32_t e = b0 + lig; e < b1; e += P) acc += prod[e];
+      }
+      for (int o = P >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      if (lig == 0 && r < rs) store(r, acc);
+    }
+  }
+
+  if (long_last) {
+    const int32_t r = r1 - 1;
+    double acc = 0.0;
+    for (int32_t base = e_last; base < e1; base += SA_BLOCK * kUnroll) {
+      int32_t c[kUnroll];
+      T v[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int32_t e = base + u * SA_BLOCK + (int32_t)threadIdx.x;
+        c[u] = e < e1 ? stream_load(&indices[e]) : 0;
+        v[u] = e < e1 ? stream_load(&vals[e]) : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int32_t e = base + u * SA_BLOCK + (int32_t)threadIdx.x;
+        if (e < e1) acc += (double)v[u] * (double)g[c[u]];
+      }
+    }
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) store(r, acc);
+  }
 }
 
 template <typename T, int TILE>
@@ -31,53 +138,42 @@ __global__ __launch_bounds__(SA_BLOCK) void spmv_tile_kernel(SpmvParams<T> p) {
   const int64_t t = blockIdx.x;
   const int32_t r0 = p.tile_rows[t], r1 = p.tile_rows[t + 1];
   if (r0 >= r1) return;
-  const int32_t e0 = p.indptr[r0];
-  const int32_t e_last = p.indptr[r1 - 1];
-  const int32_t e1 = p.indptr[r1];
-  const bool long_last = (e1 - e0) > TILE;
-  const int32_t rs = long_last ? r1 - 1 : r1;
-  const int32_t es = long_last ? e_last : e1;
+  PlainStore<T> st{p.s, p.x, p.a, bb, p.y};
+  tile_body<T, TILE>(p.indptr, p.indices, p.vals, p.g, r0, r1, p.lanes_fixed, prod, red, st);
+}
 
-  // Phase 1: products of the short rows into LDS.
-  for (int32_t e = e0 + (int32_t)threadIdx.x; e < es; e += SA_BLOCK) {
-    const int32_t c = stream_load(&p.indices[e]);
-    const T v = stream_load(&p.vals[e]);
-    prod[e - e0] = (double)v * (double)p.g[c];
-  }
-  __syncthreads();
+// Workgroup w -> XCD w % 8; panel (w / (8 maxT)) * 8 + w % 8, tile (w / 8) % maxT.
+template <typename T, int TILE>
+__global__ __launch_bounds__(SA_BLOCK) void spmv_panel_kernel(SpmvParams<T> p) {
+  __shared__ double prod[TILE];
+  __shared__ double red[SA_WAVES_PER_BLOCK];
+  if (p.lz && p.lz->status != SA_RUNNING) return;
+  const int64_t w = blockIdx.x;
+  const int64_t q = w >> 3;
+  const int panel = (int)((q / p.pmax_tiles) * 8 + (w & 7));
+  const int64_t t = q % p.pmax_tiles;
+  if (panel >= p.npanels) return;
+  const int64_t toff = p.ptile_off[panel];
+  const int64_t nt = p.ptile_off[panel + 1] - toff - 1;
+  if (t >= nt) return;
+  const int32_t r0 = p.ptile_rows[toff + t], r1 = p.ptile_rows[toff + t + 1];
+  if (r0 >= r1) return;
+  PanelStore<T> st{p.partial + (int64_t)panel * p.n};
+  tile_body<T, TILE>(p.pindptr + (int64_t)panel * (p.n + 1), p.pindices, p.pvals, p.g, r0, r1, p.lanes_fixed,
+                     prod, red, st);
+}
 
-  // Phase 2: segmented row sums, P lanes per row.
-  const int32_t nrows = rs - r0;
-  if (nrows > 0) {
-    int P = p.lanes_fixed;
-    if (P <= 0) {
-      const int q = SA_BLOCK / nrows;
-      P = q >= 64 ? 64 : q >= 32 ? 32 : q >= 16 ? 16 : q >= 8 ? 8 : q >= 4 ? 4 : q >= 2 ? 2 : 1;
-    }
-    const int groups = SA_BLOCK / P;
-    const int gid = threadIdx.x / P, lig = threadIdx.x % P;
-    for (int32_t rb = r0; rb < rs; rb += groups) {
-      const int32_t r = rb + gid;
-      double acc = 0.0;
-      if (r < rs) {
-        const int32_t b0 = p.indptr[r] - e0, b1 = p.indptr[r + 1] - e0;
-        for (int32_t e = b0 + lig; e < b1; e += P) acc += prod[e];
-      }
-      for (int o = P >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-      if (lig == 0 && r < rs) spmv_store(p, r, acc, bb);
-    }
-  }
-
-  // A row longer than what is left of its tile: the whole workgroup strides over it.
-  if (long_last) {
-    const int32_t r = r1 - 1;
+// y_r = a x_r + b s_r sum_p partial[p][r] (panel order fixed).
+template <typename T>
+__global__ __launch_bounds__(SA_BLOCK) void combine_kernel(int64_t n, int32_t npanels, const T* __restrict__ partial,
+                                                           const T* __restrict__ s, const T* __restrict__ x,
+                                                           double a, double b, T* __restrict__ y) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
-    for (int32_t e = e_last + (int32_t)threadIdx.x; e < e1; e += SA_BLOCK) {
-      const int32_t c = stream_load(&p.indices[e]);
-      acc += (double)stream_load(&p.vals[e]) * (double)p.g[c];
-    }
-    acc = block_sum(acc, red);
-    if (threadIdx.x == 0) spmv_store(p, r, acc, bb);
+    for (int p = 0; p < npanels; ++p) acc += (double)partial[(int64_t)p * n + r];
+    double v = b * (s ? (double)s[r] : 1.0) * acc;
+    if (x) v += a * (double)x[r];
+    y[r] = (T)v;
   }
 }
 
@@ -88,13 +184,136 @@ __global__ __launch_bounds__(SA_BLOCK) void prescale_kernel(int64_t n, const T* 
     z[r] = s ? s[r] * x[r] : x[r];
 }
 
+// ---- panel layout construction --------------------------------------------------------------
+__global__ __launch_bounds__(SA_BLOCK) void panel_count_kernel(int64_t n, const int32_t* __restrict__ indptr,
+                                                               const int32_t* __restrict__ indices, int32_t npanels,
+                                                               const int32_t* __restrict__ pbound,
+                                                               int32_t* __restrict__ counts) {
+  __shared__ int32_t bnd[65];
+  for (int i = threadIdx.x; i <= npanels; i += blockDim.x) bnd[i] = pbound[i];
+  __syncthreads();
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    int p = 0, run = 0;
+    for (int32_t e = indptr[r]; e < indptr[r + 1]; ++e) {
+      const int32_t c = indices[e];
+      while (p < npanels - 1 && c >= bnd[p + 1]) {
+        counts[(int64_t)p * n + r] = run;
+        run = 0;
+        ++p;
+      }
+      ++run;
+    }
+    for (; p < npanels; ++p) {
+      counts[(int64_t)p * n + r] = run;
+      run = 0;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(SA_BLOCK) void panel_scatter_kernel(int64_t n, const int32_t* __restrict__ indptr,
+                                                                 const int32_t* __restrict__ indices,
+                                                                 const T* __restrict__ vals, int32_t npanels,
+                                                                 const int32_t* __restrict__ pbound,
+                                                                 const int32_t* __restrict__ pindptr,
+                                                                 int32_t* __restrict__ pindices,
+                                                                 T* __restrict__ pvals) {
+  __shared__ int32_t bnd[65];
+  for (int i = threadIdx.x; i <= npanels; i += blockDim.x) bnd[i] = pbound[i];
+  __syncthreads();
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    int p = 0;
+    int32_t dst = pindptr[r];
+    for (int32_t e = indptr[r]; e < indptr[r + 1]; ++e) {
+      const int32_t c = indices[e];
+      while (p < npanels - 1 && c >= bnd[p + 1]) {
+        ++p;
+        dst = pindptr[(int64_t)p * (n + 1) + r];
+      }
+      pindices[dst] = c;
+      pvals[dst] = vals[e];
+      ++dst;
+    }
+  }
+}
+
+__global__ __launch_bounds__(SA_BLOCK) void panel_tiles_kernel(int64_t n, const int32_t* __restrict__ indptr,
+                                                               int32_t tile_nnz, int32_t* __restrict__ tile_rows,
+                                                               int64_t ntiles) {
+  const int64_t base = indptr[0];
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= ntiles; t += (int64_t)gridDim.x * blockDim.x) {
+    if (t == ntiles) {
+      tile_rows[t] = (int32_t)n;
+      continue;
+    }
+    const int64_t target = base + t * (int64_t)tile_nnz;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if ((int64_t)indptr[mid] >= target) hi = mid; else lo = mid + 1;
+    }
+    tile_rows[t] = (int32_t)lo;
+  }
+}
+
+// ---- launchers ------------------------------------------------------------------------------
+template <typename T>
+SpmvParams<T> params_from(const sa_csr* A) {
+  SpmvParams<T> p;
+  p.n = A->n;
+  p.indptr = A->indptr;
+  p.indices = A->indices;
+  p.vals = (const T*)A->vals;
+  p.tile_rows = A->tile_rows;
+  p.ntiles = A->ntiles;
+  p.lanes_fixed = A->lanes_per_row;
+  p.g = nullptr;
+  p.s = nullptr;
+  p.x = nullptr;
+  p.a = 0.0;
+  p.b = 1.0;
+  p.lz = nullptr;
+  p.y = nullptr;
+  p.npanels = A->npanels;
+  p.pindptr = A->pindptr;
+  p.pindices = A->pindices;
+  p.pvals = (const T*)A->pvals;
+  p.ptile_rows = A->ptile_rows;
+  p.ptile_off = A->ptile_off;
+  p.pmax_tiles = A->pmax_tiles;
+  p.partial = (T*)A->partial;
+  return p;
+}
+
 template <typename T>
 int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream) {
+  const dim3 block(SA_BLOCK);
+  if (p.npanels > 0) {
+    if (p.npanels % 8 != 0 || p.npanels > 64 || p.pmax_tiles < 1 || !p.pindptr || !p.pindices || !p.pvals ||
+        !p.ptile_rows || !p.ptile_off || !p.partial) {
+      set_error("spmv: incomplete XCD panel layout");
+      return SA_E_ARG;
+    }
+    const int64_t nblocks = (int64_t)p.npanels * p.pmax_tiles;
+    if (nblocks >= (1ll << 31)) {
+      set_error("spmv: panel grid too large");
+      return SA_E_ARG;
+    }
+    const dim3 grid((unsigned)nblocks);
+    switch (tile_nnz) {
+      case 512: hipLaunchKernelGGL((spmv_panel_kernel<T, 512>), grid, block, 0, stream, p); break;
+      case 1024: hipLaunchKernelGGL((spmv_panel_kernel<T, 1024>), grid, block, 0, stream, p); break;
+      case 2048: hipLaunchKernelGGL((spmv_panel_kernel<T, 2048>), grid, block, 0, stream, p); break;
+      case 4096: hipLaunchKernelGGL((spmv_panel_kernel<T, 4096>), grid, block, 0, stream, p); break;
+      default: set_error("spmv: tile_nnz must be 512, 1024, 2048 or 4096"); return SA_E_ARG;
+    }
+    return check_launch("spmv_panel");
+  }
   if (p.ntiles < 1 || p.ntiles >= (1ll << 31)) {
     set_error("spmv: ntiles out of range");
     return SA_E_ARG;
   }
-  const dim3 grid((unsigned)p.ntiles), block(SA_BLOCK);
+  const dim3 grid((unsigned)p.ntiles);
   switch (tile_nnz) {
     case 512: hipLaunchKernelGGL((spmv_tile_kernel<T, 512>), grid, block, 0, stream, p); break;
     case 1024: hipLaunchKernelGGL((spmv_tile_kernel<T, 1024>), grid, block, 0, stream, p); break;
@@ -112,10 +331,30 @@ int launch_prescale(int64_t n, const T* s, const T* x, T* z, hipStream_t stream)
   return check_launch("prescale");
 }
 
+template SpmvParams<float> params_from<float>(const sa_csr*);
+template SpmvParams<double> params_from<double>(const sa_csr*);
 template int launch_spmv<float>(const SpmvParams<float>&, int32_t, hipStream_t);
 template int launch_spmv<double>(const SpmvParams<double>&, int32_t, hipStream_t);
 template int launch_prescale<float>(int64_t, const float*, const float*, float*, hipStream_t);
 template int launch_prescale<double>(int64_t, const double*, const double*, double*, hipStream_t);
+
+// y = a x + b s (.) (J g): the tile kernel, or the panel kernel plus the combine.
+template <typename T>
+static int spmv_apply(const sa_csr* A, const T* s, const T* g, double a, const T* x, double b, T* y,
+                      hipStream_t stream) {
+  SpmvParams<T> p = params_from<T>(A);
+  p.g = g;
+  p.s = s;
+  p.x = (a != 0.0) ? x : nullptr;
+  p.a = a;
+  p.b = b;
+  p.y = y;
+  int rc = launch_spmv<T>(p, A->tile_nnz, stream);
+  if (rc || A->npanels == 0) return rc;
+  hipLaunchKernelGGL(combine_kernel<T>, dim3(grid_for(A->n, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0, stream, A->n,
+                     A->npanels, (const T*)A->partial, s, p.x, a, b, y);
+  return check_launch("spmv_combine");
+}
 
 template <typename T>
 static int spmv_scaled(const sa_csr* A, const T* s, const T* x, double a, double b, T* z, T* y, hipStream_t stream) {
@@ -124,22 +363,7 @@ static int spmv_scaled(const sa_csr* A, const T* s, const T* x, double a, double
   SA_REQUIRE(A->nnz >= 0 && A->nnz < (1ll << 31), "nnz must fit int32 indices");
   int rc = launch_prescale<T>(A->n, s, x, z, stream);
   if (rc) return rc;
-  SpmvParams<T> p;
-  p.n = A->n;
-  p.indptr = A->indptr;
-  p.indices = A->indices;
-  p.vals = (const T*)A->vals;
-  p.tile_rows = A->tile_rows;
-  p.ntiles = A->ntiles;
-  p.lanes_fixed = A->lanes_per_row;
-  p.g = z;
-  p.s = s;
-  p.x = (a != 0.0) ? x : nullptr;
-  p.a = a;
-  p.b = b;
-  p.lz = nullptr;
-  p.y = y;
-  return launch_spmv<T>(p, A->tile_nnz, stream);
+  return spmv_apply<T>(A, s, z, a, x, b, y, stream);
 }
 
 template <typename T>
@@ -147,27 +371,33 @@ static int spmv_gather(const sa_csr* A, const T* s, const T* g, double a, const 
                        hipStream_t stream) {
   SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && A->vals && A->tile_rows && g && y, "bad arguments");
   SA_REQUIRE(a == 0.0 || x, "x is required when a != 0");
-  SpmvParams<T> p;
-  p.n = A->n;
-  p.indptr = A->indptr;
-  p.indices = A->indices;
-  p.vals = (const T*)A->vals;
-  p.tile_rows = A->tile_rows;
-  p.ntiles = A->ntiles;
-  p.lanes_fixed = A->lanes_per_row;
-  p.g = g;
-  p.s = s;
-  p.x = (a != 0.0) ? x : nullptr;
-  p.a = a;
-  p.b = b;
-  p.lz = nullptr;
-  p.y = y;
-  return launch_spmv<T>(p, A->tile_nnz, stream);
+  return spmv_apply<T>(A, s, g, a, x, b, y, stream);
+}
+
+template <typename T>
+static int panel_scatter(const sa_csr* A, int32_t npanels, const int32_t* pbound, int32_t* pindptr,
+                         int32_t* pindices, T* pvals, hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && A->vals && pbound && pindptr && pindices && pvals,
+             "bad arguments");
+  SA_REQUIRE(npanels >= 1 && npanels <= 64, "npanels must be in 1..64");
+  hipLaunchKernelGGL(panel_scatter_kernel<T>, dim3(grid_for(A->n, SA_BLOCK, 8192)), dim3(SA_BLOCK), 0, stream, A->n,
+                     A->indptr, A->indices, (const T*)A->vals, npanels, pbound, pindptr, pindices, pvals);
+  return check_launch("panel_scatter");
 }
 
 }  // namespace sa
 
 extern "C" {
+
+int sa_spmv_scaled_f32(const sa_csr* A, const float* s, const float* x, double a, double b, float* z, float* y,
+                       hipStream_t stream) {
+  return sa::spmv_scaled<float>(A, s, x, a, b, z, y, stream);
+}
+
+int sa_spmv_scaled_f64(const sa_csr* A, const double* s, const double* x, double a, double b, double* z, double* y,
+                       hipStream_t stream) {
+  return sa::spmv_scaled<double>(A, s, x, a, b, z, y, stream);
+}
 
 int sa_spmv_gather_f32(const sa_csr* A, const float* s, const float* g, double a, const float* x, double b, float* y,
                        hipStream_t stream) {
@@ -179,14 +409,32 @@ int sa_spmv_gather_f64(const sa_csr* A, const double* s, const double* g, double
   return sa::spmv_gather<double>(A, s, g, a, x, b, y, stream);
 }
 
-int sa_spmv_scaled_f32(const sa_csr* A, const float* s, const float* x, double a, double b, float* z, float* y,
-                       hipStream_t stream) {
-  return sa::spmv_scaled<float>(A, s, x, a, b, z, y, stream);
+int sa_panel_count(const sa_csr* A, int32_t npanels, const int32_t* pbound, int32_t* counts, hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && pbound && counts, "bad arguments");
+  SA_REQUIRE(npanels >= 1 && npanels <= 64, "npanels must be in 1..64");
+  hipLaunchKernelGGL(sa::panel_count_kernel, dim3(sa::grid_for(A->n, SA_BLOCK, 8192)), dim3(SA_BLOCK), 0, stream,
+                     A->n, A->indptr, A->indices, npanels, pbound, counts);
+  return sa::check_launch("panel_count");
 }
 
-int sa_spmv_scaled_f64(const sa_csr* A, const double* s, const double* x, double a, double b, double* z, double* y,
-                       hipStream_t stream) {
-  return sa::spmv_scaled<double>(A, s, x, a, b, z, y, stream);
+int sa_panel_scatter_f32(const sa_csr* A, int32_t npanels, const int32_t* pbound, int32_t* pindptr, int32_t* pindices,
+                         float* pvals, hipStream_t stream) {
+  return sa::panel_scatter<float>(A, npanels, pbound, pindptr, pindices, pvals, stream);
+}
+
+int sa_panel_scatter_f64(const sa_csr* A, int32_t npanels, const int32_t* pbound, int32_t* pindptr, int32_t* pindices,
+                         double* pvals, hipStream_t stream) {
+  return sa::panel_scatter<double>(A, npanels, pbound, pindptr, pindices, pvals, stream);
+}
+
+int sa_panel_tiles(int64_t n, const int32_t* pindptr_p, int32_t tile_nnz, int32_t* tile_rows, int64_t ntiles,
+                   hipStream_t stream) {
+  SA_REQUIRE(n > 0 && pindptr_p && tile_rows && ntiles >= 1, "bad arguments");
+  SA_REQUIRE(tile_nnz == 512 || tile_nnz == 1024 || tile_nnz == 2048 || tile_nnz == 4096,
+             "tile_nnz must be 512, 1024, 2048 or 4096");
+  hipLaunchKernelGGL(sa::panel_tiles_kernel, dim3(sa::grid_for(ntiles + 1, SA_BLOCK, 4096)), dim3(SA_BLOCK), 0,
+                     stream, n, pindptr_p, tile_nnz, tile_rows, ntiles);
+  return sa::check_launch("panel_tiles");
 }
 
 }  // extern "C"

@@ -22,7 +22,19 @@ struct SpmvParams {
   // multiplied by sigma[j], the scale of the raw basis vector being applied.
   const sa_state* lz;
   T* y;
+  // XCD panel layout (npanels > 0): partial row sums per panel go to partial[p * n + r].
+  int32_t npanels;
+  const int32_t* pindptr;
+  const int32_t* pindices;
+  const T* pvals;
+  const int32_t* ptile_rows;
+  const int64_t* ptile_off;
+  int64_t pmax_tiles;
+  T* partial;
 };
+
+template <typename T>
+SpmvParams<T> params_from(const sa_csr* A);
 
 template <typename T>
 int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream);

@@ -86,6 +86,9 @@ class DeviceProblem:
                  lanes_per_row: int = 0, device: torch.device | None = None):
         device = device or require_gpu()
         J = sp.csr_matrix(J)
+        if not J.has_sorted_indices:
+            J = J.copy()
+            J.sort_indices()
         if J.shape[0] != J.shape[1]:
             raise ValueError("J must be square")
         n, nnz = J.shape[0], J.nnz
@@ -128,6 +131,121 @@ class DeviceProblem:
         self.d = torch.empty(n, dtype=dtype, device=device)
         call(f"sa_normalize_{sfx}", nnz, ptr(self.vals), n, ptr(self.deg), ptr(self.d), self.c, s)
         self.sfx = sfx
+        self.row_counts = np.diff(J.indptr).astype(np.int64)   # entries per row = per column (symmetric J)
+        self.npanels = 0
+        self.tuning = None
+
+    # -- XCD panel layout ----------------------------------------------------------------------
+    def panel_bounds(self, npanels: int) -> np.ndarray:
+        """Column boundaries of npanels panels holding about nnz / npanels entries each."""
+        cum = np.cumsum(self.row_counts)
+        targets = (np.arange(1, npanels) * (self.nnz / npanels))
+        inner = np.searchsorted(cum, targets, side="right") + 1
+        b = np.concatenate([[0], np.clip(inner, 0, self.n), [self.n]]).astype(np.int64)
+        return np.maximum.accumulate(b)
+
+    def auto_panels(self, budget_bytes: float = 2.5 * 2**20) -> int:
+        """Smallest multiple of 8 panels (at most 64) whose widest panel's slice of the gathered
+        vector fits the per-XCD L2 budget; 0 if even 64 do not."""
+        esz = 4 if self.dtype == torch.float32 else 8
+        for P in (8, 16, 32, 64):
+            if np.max(np.diff(self.panel_bounds(P))) * esz <= budget_bytes:
+                return P
+        return 0
+
+    def build_panels(self, npanels: int) -> None:
+        """Regroup the couplings by column panel (sa_panel_count / scan / sa_panel_scatter)."""
+        if npanels == 0:
+            self.npanels = 0
+            self.csr.npanels = 0
+            return
+        if npanels % 8 or not 8 <= npanels <= 64:
+            raise ValueError("npanels must be a multiple of 8 in [8, 64]")
+        n, dev, s = self.n, self.device, stream()
+        bounds = self.panel_bounds(npanels)
+        self.pbound = torch.from_numpy(bounds.astype(np.int32)).to(dev)
+        counts = torch.zeros((npanels, n), dtype=torch.int32, device=dev)
+        call("sa_panel_count", self.csr_ref, npanels, ptr(self.pbound), ptr(counts), s)
+        csum = torch.cumsum(counts.to(torch.int64), dim=1)
+        totals = csum[:, -1]
+        starts = torch.cumsum(totals, dim=0) - totals
+        pind = torch.empty((npanels, n + 1), dtype=torch.int64, device=dev)
+        pind[:, 0] = starts
+        pind[:, 1:] = csum + starts[:, None]
+        self.pindptr = pind.to(torch.int32)
+        del counts, csum, pind
+        self.pindices = torch.empty(self.nnz, dtype=torch.int32, device=dev)
+        self.pvals = torch.empty(self.nnz, dtype=self.dtype, device=dev)
+        call(f"sa_panel_scatter_{self.sfx}", self.csr_ref, npanels, ptr(self.pbound), ptr(self.pindptr),
+             ptr(self.pindices), ptr(self.pvals), s)
+        totals = totals.cpu().numpy()
+        ntiles = [max(1, math.ceil(int(t) / self.tile_nnz)) for t in totals]
+        off = np.zeros(npanels + 1, dtype=np.int64)
+        off[1:] = np.cumsum(np.array(ntiles) + 1)
+        self.ptile_rows = torch.empty(int(off[-1]), dtype=torch.int32, device=dev)
+        for p in range(npanels):
+            call("sa_panel_tiles", n, ctypes.c_void_p(self.pindptr.data_ptr() + 4 * p * (n + 1)), self.tile_nnz,
+                 ctypes.c_void_p(self.ptile_rows.data_ptr() + 4 * int(off[p])), ntiles[p], s)
+        self.ptile_off = torch.from_numpy(off).to(dev)
+        self.partial = torch.empty((npanels, n), dtype=self.dtype, device=dev)
+        self.npanels = npanels
+        c = self.csr
+        c.npanels = npanels
+        c.pindptr = self.pindptr.data_ptr()
+        c.pindices = self.pindices.data_ptr()
+        c.pvals = self.pvals.data_ptr()
+        c.ptile_rows = self.ptile_rows.data_ptr()
+        c.ptile_off = self.ptile_off.data_ptr()
+        c.pmax_tiles = int(max(ntiles))
+        c.partial = self.partial.data_ptr()
+
+    def time_spmv(self, reps: int = 5) -> float:
+        """Average seconds of one fused SpMV application (the Lanczos step's launch, plus the panel
+        combine when panels are on), measured with HIP events on the current stream."""
+        s = torch.empty(self.n, dtype=self.dtype, device=self.device)
+        self.alpha_scaling(0.5, 1e-12, s)
+        g = self.fill_normal(11, 0, torch.empty(self.n, dtype=self.dtype, device=self.device))
+        y = torch.empty_like(g)
+        fn = f"sa_spmv_gather_{self.sfx}"
+        call(fn, self.csr_ref, ptr(s), ptr(g), 0.0, None, 1.0, ptr(y), stream())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            call(fn, self.csr_ref, ptr(s), ptr(g), 0.0, None, 1.0, ptr(y), stream())
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    def autotune(self, reps: int = 4, verbose: bool = False) -> dict:
+        """Pick the SpMV layout (plain tiles or XCD panels) and tile size by timing them on this
+        matrix; small problems keep the plain layout."""
+        configs = [(0, t) for t in (1024, 2048, 4096)]
+        P = self.auto_panels() if self.n >= (1 << 17) else 0
+        if P:
+            configs += [(P, t) for t in (1024, 2048)]
+        best, results = None, []
+        for panels, tile in configs:
+            self.set_tiling(tile, self.csr.lanes_per_row)
+            self.build_panels(panels) if panels else self.build_panels(0)
+            sec = self.time_spmv(reps)
+            results.append({"npanels": panels, "tile_nnz": tile, "seconds": sec})
+            if verbose:
+                print(f"[autotune] npanels={panels} tile_nnz={tile}: {sec * 1e6:.1f} us", flush=True)
+            if best is None or sec < best["seconds"]:
+                best = results[-1]
+        self.set_tiling(best["tile_nnz"], self.csr.lanes_per_row)
+        if best["npanels"] != self.npanels or best["npanels"]:
+            self.build_panels(best["npanels"])
+        self._free_unused_panels()
+        self.tuning = {"choice": best, "tried": results}
+        return self.tuning
+
+    def _free_unused_panels(self) -> None:
+        if self.npanels == 0:
+            for name in ("pbound", "pindptr", "pindices", "pvals", "ptile_rows", "ptile_off", "partial"):
+                if hasattr(self, name):
+                    delattr(self, name)
+            torch.cuda.empty_cache()
 
     def set_tiling(self, tile_nnz: int, lanes_per_row: int = 0) -> None:
         """Re-cut the SpMV tiles (used by the autotuner)."""

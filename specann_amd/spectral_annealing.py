@@ -97,7 +97,7 @@ class SpectralAnnealing:
                  schedule: str = "sqrt", num_alphas: int = 128, schedule_interval=(0.0, 1.0),
                  eigsh_ncv: int = None, fp32_spmv: bool = None, ls_policy: str = "default",
                  ls_batch: bool = True, verbose: bool = True, tile_nnz: int = None, lanes_per_row: int = 0,
-                 **kwargs):
+                 autotune: bool = True, **kwargs):
         self.instance_name = instance_name
         self.dataset = dataset
         self.random_seed = random_seed
@@ -144,6 +144,7 @@ class SpectralAnnealing:
         self.solver_name = run_name if run_name else "spectral_annealing_pred"
         self.tile_nnz = tile_nnz
         self.lanes_per_row = lanes_per_row
+        self.autotune = bool(autotune) and tile_nnz is None
         self.verbose = verbose
         if verbose:
             print(f"Initialized {self.solver_name} solver. Results will be saved under '{self.solver_name}'.")
@@ -181,6 +182,8 @@ class SpectralAnnealing:
                 "use ls_policy 'default' (the deterministic batched local search)")
         start_time = time.time()
         prob = DeviceProblem(J, self.tdtype, tile_nnz=self.tile_nnz, lanes_per_row=self.lanes_per_row)
+        if self.autotune and prob.n >= (1 << 17):
+            prob.autotune()
         out = self.run_on_device(prob, start_time=start_time)
         total_time = time.time() - start_time
         self._sweep_time = out.sweep_time
