@@ -42,6 +42,7 @@ extern "C" {
 #define SA_MAX_NCV 128
 #define SA_RED_GRID 512        /* workgroups of the reduction kernels */
 #define SA_RED_COLS 8          /* basis vectors per dot-product chunk */
+#define SA_MAX_KEEP 16         /* Ritz vectors a thick restart may keep */
 
 /* Symmetric CSR of the couplings, device pointers. nnz < 2^31 (int32 indices). */
 typedef struct sa_csr {
@@ -80,7 +81,9 @@ typedef struct sa_state {
   int32_t ncv;
   int32_t maxiter;
   int32_t gate;        /* 1 while step 0 runs on a predicted start vector */
-  int32_t flags;
+  int32_t kept;        /* Ritz vectors kept by the last thick restart (rows 0..kept-1 of the basis) */
+  int32_t keep;        /* Ritz vectors a thick restart keeps (1: restart from the Ritz vector alone) */
+  int32_t pad1;
   double theta;        /* largest Ritz value of N (the reference's mu = rho - theta) */
   double res;          /* residual estimate |beta_j * s_j| */
   double first_beta;   /* beta_0 of a gated start: the predictor residual */
@@ -95,6 +98,7 @@ typedef struct sa_state {
   double sigma[SA_MAX_NCV + 1];
   double h[SA_MAX_NCV + 1];
   double ritz[SA_MAX_NCV];
+  double arrow[SA_MAX_NCV]; /* couplings of the kept Ritz vectors to basis row `kept` */
   unsigned int counters[16];
 } sa_state;
 
@@ -163,7 +167,7 @@ int sa_spmv_gather_f64(const sa_csr* A, const double* s, const double* g, double
 /* ---- Warm-started Lanczos on N_alpha (utils/eigsh.py:10-17, models/spectral_annealing.py:671-677)
  * Basis V is (ncv + 1) x n, row-major, leading dimension n. Z, W are n-vectors of scratch. */
 int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, double bound, double pred_tol,
-                     hipStream_t stream);
+                     int32_t keep, hipStream_t stream);
 /* Start vector V[0] = x (.) max(d, 1e-12)^pexp (the diagonal predictor, utils/predictors.py:6-26, with
  * pexp = delta_alpha / 2; pexp = 0 copies x) or cx * x + cy * y when y != NULL (the secant predictor,
  * :29-56). Z = s (.) V[0]; sigma_0 = 1 / ||V[0]||; state.r[3] = ||V[0]||. gate = 1 marks the vector as
@@ -181,6 +185,13 @@ int sa_lanczos_start_f64(void* work, int64_t n, const double* s, const double* x
 int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, float* Z, float* W, hipStream_t stream);
 int sa_lanczos_step_f64(const sa_csr* A, void* work, const double* s, double* V, double* Z, double* W,
                         hipStream_t stream);
+/* Thick restart (gated on SA_RESTART): keep the `keep` largest Ritz pairs of the full basis. The basis
+ * is rotated in place: rows 0..keep-1 become the Ritz vectors, row keep the last Lanczos vector, Z its
+ * scaled copy, and the projected matrix becomes diag(theta) bordered by the arrow couplings. With
+ * keep = 1 this is the restart from the Ritz vector alone (the k = 1 thick restart of the reference's
+ * eigsh). */
+int sa_lanczos_thick_restart_f32(void* work, int64_t n, const float* s, float* V, float* Z, hipStream_t stream);
+int sa_lanczos_thick_restart_f64(void* work, int64_t n, const double* s, double* V, double* Z, hipStream_t stream);
 /* Ritz vector y = sum_m ritz_m v_m of the current cycle; state.r[4] = ||y||, state.r[5] = y . y_last
  * (y_last may be NULL). only_if as in sa_lanczos_start. */
 int sa_lanczos_ritz_f32(void* work, int64_t n, const float* V, const float* y_last, float* y, int32_t only_if,

@@ -29,6 +29,7 @@ SA_REJECTED = 4
 SA_BROKEN = -1
 
 SA_MAX_NCV = 128
+SA_MAX_KEEP = 16
 
 
 class SpecAnnLibraryError(RuntimeError):
@@ -76,7 +77,9 @@ class SaState(ctypes.Structure):
         ("ncv", c_int32),
         ("maxiter", c_int32),
         ("gate", c_int32),
-        ("flags", c_int32),
+        ("kept", c_int32),
+        ("keep", c_int32),
+        ("pad1", c_int32),
         ("theta", c_double),
         ("res", c_double),
         ("first_beta", c_double),
@@ -91,6 +94,7 @@ class SaState(ctypes.Structure):
         ("sigma", c_double * (SA_MAX_NCV + 1)),
         ("h", c_double * (SA_MAX_NCV + 1)),
         ("ritz", c_double * SA_MAX_NCV),
+        ("arrow", c_double * SA_MAX_NCV),
         ("counters", c_uint32 * 16),
     ]
 
@@ -121,7 +125,9 @@ _SIGNATURES = {
     "sa_spmv_scaled_f64": (c_int32, [POINTER(SaCsr), _P, _P, c_double, c_double, _P, _P, _P]),
     "sa_spmv_gather_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, _P, c_double, _P, _P]),
     "sa_spmv_gather_f64": (c_int32, [POINTER(SaCsr), _P, _P, c_double, _P, c_double, _P, _P]),
-    "sa_lanczos_begin": (c_int32, [_P, c_int32, c_int32, c_double, c_double, c_double, _P]),
+    "sa_lanczos_begin": (c_int32, [_P, c_int32, c_int32, c_double, c_double, c_double, c_int32, _P]),
+    "sa_lanczos_thick_restart_f32": (c_int32, [_P, c_int64, _P, _P, _P, _P]),
+    "sa_lanczos_thick_restart_f64": (c_int32, [_P, c_int64, _P, _P, _P, _P]),
     "sa_lanczos_start_f32": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, _P, c_double, _P, _P, c_int32,
                                        c_int32, _P]),
     "sa_lanczos_start_f64": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, _P, c_double, _P, _P, c_int32,

@@ -23,7 +23,7 @@
 namespace sa {
 
 __global__ void lz_begin_kernel(void* work, int32_t ncv, int32_t maxiter, double tol, double bound,
-                                double pred_tol) {
+                                double pred_tol, int32_t keep) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   sa_state* st = state_of(work);
   st->status = SA_RUNNING;
@@ -33,7 +33,8 @@ __global__ void lz_begin_kernel(void* work, int32_t ncv, int32_t maxiter, double
   st->ncv = ncv;
   st->maxiter = maxiter;
   st->gate = 0;
-  st->flags = 0;
+  st->kept = 0;
+  st->keep = keep;
   st->theta = 0.0;
   st->res = INFINITY;
   st->first_beta = NAN;
@@ -75,6 +76,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(void* work, int64_t 
     const bool ok = nrm > 0.0 && isfinite(nrm);
     st->sigma[0] = ok ? 1.0 / nrm : 0.0;
     st->j = 0;
+    st->kept = 0;
     st->gate = gate;
     st->res = INFINITY;
     if (only_if == SA_RESTART) {
@@ -117,11 +119,25 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(void* work, int64_t n
   double acc[SA_RED_COLS];
 #pragma unroll
   for (int c = 0; c < SA_RED_COLS; ++c) acc[c] = 0.0;
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    const double w = (double)W[r];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  // two rows per iteration: 18 independent loads in flight per thread
+  for (; r + stride < n; r += 2 * stride) {
+    const double w0 = (double)W[r], w1 = (double)W[r + stride];
+    T v0[SA_RED_COLS], v1[SA_RED_COLS];
+#pragma unroll
+    for (int c = 0; c < SA_RED_COLS; ++c) {
+      v0[c] = c < mc ? V[(int64_t)(m0 + c) * n + r] : (T)0;
+      v1[c] = c < mc ? V[(int64_t)(m0 + c) * n + r + stride] : (T)0;
+    }
+#pragma unroll
+    for (int c = 0; c < SA_RED_COLS; ++c) acc[c] += (double)v0[c] * w0 + (double)v1[c] * w1;
+  }
+  if (r < n) {
+    const double w0 = (double)W[r];
 #pragma unroll
     for (int c = 0; c < SA_RED_COLS; ++c)
-      if (c < mc) acc[c] += (double)V[(int64_t)(m0 + c) * n + r] * w;
+      if (c < mc) acc[c] += (double)V[(int64_t)(m0 + c) * n + r] * w0;
   }
   double* part = partials_of(work);
   const int64_t G = gridDim.x;
@@ -131,54 +147,172 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(void* work, int64_t n
     if (threadIdx.x == 0) part[((int64_t)ch * G + blockIdx.x) * SA_RED_COLS + c] = bs;
   }
   if (!last_block_arrive(&st->counters[0], (unsigned)(G * nch), &last)) return;
-  for (int m = 0; m <= j; ++m) {
-    const int chm = m / SA_RED_COLS, cm = m % SA_RED_COLS;
+  // All basis vectors at once: a 32-lane group per vector, each lane summing a strided share of
+  // the block partials in block order, then a fixed shuffle tree.
+  const int lane32 = threadIdx.x & 31, grp = threadIdx.x >> 5;
+  for (int mb = 0; mb <= j; mb += SA_BLOCK / 32) {
+    const int m = mb + grp;
     double x = 0.0;
-    for (int64_t b = threadIdx.x; b < G; b += blockDim.x)
-      x += load_coherent(&part[((int64_t)chm * G + b) * SA_RED_COLS + cm]);
-    x = block_sum(x, red);
-    if (threadIdx.x == 0) st->h[m] = st->sigma[m] * x;
+    if (m <= j) {
+      const int chm = m / SA_RED_COLS, cm = m % SA_RED_COLS;
+      for (int64_t b = lane32; b < G; b += 32) x += load_coherent(&part[((int64_t)chm * G + b) * SA_RED_COLS + cm]);
+    }
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) x += __shfl_xor(x, o, 32);
+    if (lane32 == 0 && m <= j) st->h[m] = st->sigma[m] * x;
   }
+  __syncthreads();
   if (threadIdx.x == 0) st->a[j] = st->h[j];
 }
 
-// Sturm count: number of eigenvalues of the symmetric tridiagonal (a, b) below x.
-__device__ inline int sturm_count(int m, const double* a, const double* b2, double x, double pivmin) {
-  double q = a[0] - x;
+// ---------------------------------------------------------------------------------------------
+// The projected matrix of the current cycle. Rows 0..k-1 hold the Ritz values kept by the last thick
+// restart (diagonal a[i]) and couple only to row k, through arrow[i]; rows k..m-1 are tridiagonal
+// (a, b), b[i] coupling rows i and i+1. With k = 0 it is the plain Lanczos tridiagonal.
+// ---------------------------------------------------------------------------------------------
+struct Proj {
+  int m, k;
+  const double* a;
+  const double* b;
+  const double* ar;
+  const double* b2;
+  const double* ar2;
+};
+
+// Number of eigenvalues below x: the inertia of T - x I from its LDL^T factorisation in the row order
+// 0..m-1 (the kept rows first; their Schur complement only touches the pivot of row k).
+__device__ inline int sturm_count(const Proj& T, double x, double pivmin) {
+  int cnt = 0;
+  double acc = 0.0;
+  for (int i = 0; i < T.k; ++i) {
+    double q = T.a[i] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0.0;
+    acc += T.ar2[i] / q;
+  }
+  double q = (T.a[T.k] - x) - acc;
   if (fabs(q) < pivmin) q = -pivmin;
-  int cnt = q < 0.0;
-  for (int i = 1; i < m; ++i) {
-    q = (a[i] - x) - b2[i - 1] / q;
+  cnt += q < 0.0;
+  for (int i = T.k + 1; i < T.m; ++i) {
+    q = (T.a[i] - x) - T.b2[i - 1] / q;
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
   }
   return cnt;
 }
 
-// Largest eigenpair of the m x m tridiagonal (a, b) held in LDS. All threads of the workgroup take
-// part; on return theta is valid in every thread and the unit eigenvector is in vec (LDS).
-__device__ double ritz_largest(int m, const double* a, const double* b, double* b2, double* work5, int* piv,
-                               double* vec) {
-  __shared__ double sh[4];
-  __shared__ unsigned long long masks[SA_WAVES_PER_BLOCK];
+// Gershgorin radius of row i.
+__device__ inline double gersh_radius(const Proj& T, int i) {
+  if (i < T.k) return fabs(T.ar[i]);
+  double r = 0.0;
+  if (i == T.k) {
+    for (int u = 0; u < T.k; ++u) r += fabs(T.ar[u]);
+  } else {
+    r += fabs(T.b[i - 1]);
+  }
+  if (i < T.m - 1) r += fabs(T.b[i]);
+  return r;
+}
+
+// Inverse iteration for the eigenvector of theta (one thread). The kept rows are eliminated first
+// (s_i = (r_i - ar_i s_k) / (a_i - theta)), which leaves a tridiagonal system on rows k..m-1 with a
+// modified first pivot; that system is solved by LU with partial pivoting (row interchanges create
+// a second superdiagonal). Scratch: d, dl, du, du2, den, rhs (m doubles each) and piv (m ints).
+__device__ void inverse_iteration(const Proj& T, double theta, double tiny, int seed, double* d, double* dl,
+                                  double* du, double* du2, double* den, double* rhs, int* piv, double* vec) {
+  const int m = T.m, k = T.k, mt = m - k;
+  double corr = 0.0;
+  for (int i = 0; i < k; ++i) {
+    double q = T.a[i] - theta;
+    if (fabs(q) < tiny) q = q < 0.0 ? -tiny : tiny;
+    den[i] = q;
+    corr += T.ar2[i] / q;
+  }
+  for (int t = 0; t < mt; ++t) {
+    d[t] = T.a[k + t] - theta;
+    if (t < mt - 1) {
+      dl[t] = T.b[k + t];
+      du[t] = T.b[k + t];
+      du2[t] = 0.0;
+    }
+    piv[t] = t;
+  }
+  d[0] -= corr;
+  for (int t = 0; t < mt - 1; ++t) {
+    if (fabs(d[t]) >= fabs(dl[t])) {
+      if (d[t] == 0.0) d[t] = tiny;
+      const double f = dl[t] / d[t];
+      dl[t] = f;
+      d[t + 1] -= f * du[t];
+      du2[t] = 0.0;
+      piv[t] = t;
+    } else {
+      const double f = d[t] / dl[t];
+      d[t] = dl[t];
+      dl[t] = f;
+      const double tmp = du[t];
+      du[t] = d[t + 1];
+      d[t + 1] = tmp - f * d[t + 1];
+      if (t < mt - 2) {
+        du2[t] = du[t + 1];
+        du[t + 1] = -f * du[t + 1];
+      }
+      piv[t] = t + 1;
+    }
+  }
+  if (d[mt - 1] == 0.0) d[mt - 1] = tiny;
+  for (int i = 0; i < m; ++i) vec[i] = 1.0 + 0.01 * (double)((i + 3 * seed) % 7) + 0.001 * (double)seed;
+  for (int it = 0; it < 3; ++it) {
+    for (int t = 0; t < mt; ++t) rhs[t] = vec[k + t];
+    for (int i = 0; i < k; ++i) rhs[0] -= T.ar[i] * vec[i] / den[i];
+    for (int t = 0; t < mt - 1; ++t) {
+      if (piv[t] == t) {
+        rhs[t + 1] -= dl[t] * rhs[t];
+      } else {
+        const double tmp = rhs[t];
+        rhs[t] = rhs[t + 1];
+        rhs[t + 1] = tmp - dl[t] * rhs[t];
+      }
+    }
+    rhs[mt - 1] /= d[mt - 1];
+    if (mt > 1) rhs[mt - 2] = (rhs[mt - 2] - du[mt - 2] * rhs[mt - 1]) / d[mt - 2];
+    for (int t = mt - 3; t >= 0; --t) rhs[t] = (rhs[t] - du[t] * rhs[t + 1] - du2[t] * rhs[t + 2]) / d[t];
+    for (int i = 0; i < k; ++i) vec[i] = (vec[i] - T.ar[i] * rhs[0]) / den[i];
+    for (int t = 0; t < mt; ++t) vec[k + t] = rhs[t];
+    double mx = 0.0;
+    for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(vec[i]));
+    if (!(mx > 0.0) || !isfinite(mx)) break;
+    double nn = 0.0;
+    for (int i = 0; i < m; ++i) {
+      vec[i] /= mx;
+      nn += vec[i] * vec[i];
+    }
+    nn = sqrt(nn);
+    for (int i = 0; i < m; ++i) vec[i] /= nn;
+  }
+}
+
+// Gershgorin interval [lo, hi], pivot floor and norm estimate of T over the whole workgroup; b2 and
+// ar2 (LDS) receive the squares.
+__device__ void proj_bounds(const Proj& T, double* b2, double* ar2, double* out4) {
+  __shared__ double wlo[SA_WAVES_PER_BLOCK], whi[SA_WAVES_PER_BLOCK], wb2[SA_WAVES_PER_BLOCK];
   const int t = threadIdx.x;
-  // Gershgorin interval and the pivot floor.
   double glo = INFINITY, ghi = -INFINITY, mb2 = 0.0;
-  for (int i = t; i < m; i += blockDim.x) {
-    const double bl = i > 0 ? fabs(b[i - 1]) : 0.0;
-    const double br = i < m - 1 ? fabs(b[i]) : 0.0;
-    glo = fmin(glo, a[i] - bl - br);
-    ghi = fmax(ghi, a[i] + bl + br);
-    if (i < m - 1) {
-      b2[i] = b[i] * b[i];
+  for (int i = t; i < T.m; i += blockDim.x) {
+    const double r = gersh_radius(T, i);
+    glo = fmin(glo, T.a[i] - r);
+    ghi = fmax(ghi, T.a[i] + r);
+    if (i < T.k) {
+      ar2[i] = T.ar[i] * T.ar[i];
+      mb2 = fmax(mb2, ar2[i]);
+    } else if (i < T.m - 1) {
+      b2[i] = T.b[i] * T.b[i];
       mb2 = fmax(mb2, b2[i]);
     }
   }
   glo = -wave_max(-glo);
   ghi = wave_max(ghi);
   mb2 = wave_max(mb2);
-  if ((t & 63) == 0) masks[t >> 6] = 0;
-  __shared__ double wlo[SA_WAVES_PER_BLOCK], whi[SA_WAVES_PER_BLOCK], wb2[SA_WAVES_PER_BLOCK];
   if ((t & 63) == 0) {
     wlo[t >> 6] = glo;
     whi[t >> 6] = ghi;
@@ -194,21 +328,27 @@ __device__ double ritz_largest(int m, const double* a, const double* b, double* 
     }
     const double tn = fmax(fabs(L), fabs(H));
     const double pad = 4.0 * 2.220446049250313e-16 * tn + 1e-300;
-    sh[0] = L - pad;
-    sh[1] = H + pad;
-    sh[2] = 1e-290 * fmax(1.0, B);
-    sh[3] = tn;
+    out4[0] = L - pad;
+    out4[1] = H + pad;
+    out4[2] = 1e-290 * fmax(1.0, B);
+    out4[3] = tn;
   }
   __syncthreads();
-  double lo = sh[0], hi = sh[1];
-  const double pivmin = sh[2], tnorm = sh[3];
+}
+
+// Largest eigenvalue of T by multisection: every thread evaluates one Sturm count per round.
+__device__ double largest_eigenvalue(const Proj& T, const double* bounds) {
+  __shared__ unsigned long long masks[SA_WAVES_PER_BLOCK];
+  const int t = threadIdx.x;
+  double lo = bounds[0], hi = bounds[1];
+  const double pivmin = bounds[2];
   const int nsect = blockDim.x + 1;
   for (int round = 0; round < 24; ++round) {
     const double width = hi - lo;
     if (width <= 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + 2.0 * pivmin) break;
     const double x = lo + width * (double)(t + 1) / (double)nsect;
-    const int c = sturm_count(m, a, b2, x, pivmin);
-    const unsigned long long mk = __ballot(c >= m);
+    const int c = sturm_count(T, x, pivmin);
+    const unsigned long long mk = __ballot(c >= T.m);
     if ((t & 63) == 0) masks[t >> 6] = mk;
     __syncthreads();
     int first = -1;
@@ -228,82 +368,7 @@ __device__ double ritz_largest(int m, const double* a, const double* b, double* 
       hi = nhi;
     }
   }
-  const double theta = 0.5 * (lo + hi);
-
-  // Inverse iteration for the eigenvector (thread 0; m <= SA_MAX_NCV).
-  if (t == 0) {
-    double* d = work5;
-    double* dl = work5 + SA_MAX_NCV;
-    double* du = work5 + 2 * SA_MAX_NCV;
-    double* du2 = work5 + 3 * SA_MAX_NCV;
-    const double tiny = 2.220446049250313e-16 * fmax(tnorm, 1e-300);
-    if (m == 1) {
-      vec[0] = 1.0;
-    } else {
-      for (int i = 0; i < m; ++i) {
-        d[i] = a[i] - theta;
-        if (i < m - 1) {
-          dl[i] = b[i];
-          du[i] = b[i];
-          du2[i] = 0.0;
-        }
-        piv[i] = i;
-      }
-      // LU with partial pivoting of the tridiagonal (row interchanges create a second superdiagonal).
-      for (int i = 0; i < m - 1; ++i) {
-        if (fabs(d[i]) >= fabs(dl[i])) {
-          if (d[i] == 0.0) d[i] = tiny;
-          const double f = dl[i] / d[i];
-          dl[i] = f;
-          d[i + 1] -= f * du[i];
-          du2[i] = 0.0;
-          piv[i] = i;
-        } else {
-          const double f = d[i] / dl[i];
-          d[i] = dl[i];
-          dl[i] = f;
-          const double tmp = du[i];
-          du[i] = d[i + 1];
-          d[i + 1] = tmp - f * d[i + 1];
-          if (i < m - 2) {
-            du2[i] = du[i + 1];
-            du[i + 1] = -f * du[i + 1];
-          }
-          piv[i] = i + 1;
-        }
-      }
-      if (d[m - 1] == 0.0) d[m - 1] = tiny;
-      for (int i = 0; i < m; ++i) vec[i] = 1.0 + 0.01 * (double)(i % 7);
-      for (int it = 0; it < 3; ++it) {
-        // forward: apply P and L
-        for (int i = 0; i < m - 1; ++i) {
-          if (piv[i] == i) {
-            vec[i + 1] -= dl[i] * vec[i];
-          } else {
-            const double tmp = vec[i];
-            vec[i] = vec[i + 1];
-            vec[i + 1] = tmp - dl[i] * vec[i];
-          }
-        }
-        // backward: U with two superdiagonals
-        vec[m - 1] /= d[m - 1];
-        vec[m - 2] = (vec[m - 2] - du[m - 2] * vec[m - 1]) / d[m - 2];
-        for (int i = m - 3; i >= 0; --i) vec[i] = (vec[i] - du[i] * vec[i + 1] - du2[i] * vec[i + 2]) / d[i];
-        double mx = 0.0;
-        for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(vec[i]));
-        if (!(mx > 0.0) || !isfinite(mx)) break;
-        double nn = 0.0;
-        for (int i = 0; i < m; ++i) {
-          vec[i] /= mx;
-          nn += vec[i] * vec[i];
-        }
-        nn = sqrt(nn);
-        for (int i = 0; i < m; ++i) vec[i] /= nn;
-      }
-    }
-  }
-  __syncthreads();
-  return theta;
+  return 0.5 * (lo + hi);
 }
 
 template <typename T>
@@ -321,8 +386,18 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
   T* __restrict__ vn = V + (int64_t)(j + 1) * n;
   double acc = 0.0;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    double w = (double)W[r];
-    for (int m = 0; m <= j; ++m) w -= coef[m] * (double)V[(int64_t)m * n + r];
+    double w0 = (double)W[r], w1 = 0.0, w2 = 0.0, w3 = 0.0;
+    int m = 0;
+    for (; m + 3 <= j; m += 4) {
+      const T a0 = V[(int64_t)m * n + r], a1 = V[(int64_t)(m + 1) * n + r];
+      const T a2 = V[(int64_t)(m + 2) * n + r], a3 = V[(int64_t)(m + 3) * n + r];
+      w0 -= coef[m] * (double)a0;
+      w1 -= coef[m + 1] * (double)a1;
+      w2 -= coef[m + 2] * (double)a2;
+      w3 -= coef[m + 3] * (double)a3;
+    }
+    for (; m <= j; ++m) w0 -= coef[m] * (double)V[(int64_t)m * n + r];
+    const double w = (w0 + w1) + (w2 + w3);
     const T wt = (T)w;
     vn[r] = wt;
     Z[r] = s ? s[r] * wt : wt;
@@ -340,13 +415,15 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
   __syncthreads();
   const double beta = beta_sh;
 
-  __shared__ double la[SA_MAX_NCV], lb[SA_MAX_NCV], lb2[SA_MAX_NCV], lvec[SA_MAX_NCV];
-  __shared__ double lwork[4 * SA_MAX_NCV];
+  __shared__ double la[SA_MAX_NCV], lb[SA_MAX_NCV], lb2[SA_MAX_NCV], lar[SA_MAX_NCV], lar2[SA_MAX_NCV];
+  __shared__ double lvec[SA_MAX_NCV], lwork[6 * SA_MAX_NCV], lbounds[4];
   __shared__ int lpiv[SA_MAX_NCV];
   const int m = j + 1;
+  const int k = st->kept;
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
     la[i] = st->a[i];
     lb[i] = i < j ? st->b[i] : 0.0;
+    lar[i] = i < k ? st->arrow[i] : 0.0;
   }
   __syncthreads();
   const double aj = la[j];
@@ -356,10 +433,18 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
   bool solved = false;
   bool all_finite = finite && !big;
   if (all_finite) {
-    for (int i = 0; i < j; ++i) all_finite = all_finite && isfinite(lb[i]);
+    for (int i = 0; i < j; ++i) all_finite = all_finite && isfinite(lb[i]) && isfinite(lar[i]);
   }
   if (all_finite) {
-    theta = ritz_largest(m, la, lb, lb2, lwork, lpiv, lvec);
+    const Proj P{m, k, la, lb, lar, lb2, lar2};
+    proj_bounds(P, lb2, lar2, lbounds);
+    theta = largest_eigenvalue(P, lbounds);
+    if (threadIdx.x == 0) {
+      const double tiny = 2.220446049250313e-16 * fmax(lbounds[3], 1e-300);
+      inverse_iteration(P, theta, tiny, 0, lwork, lwork + SA_MAX_NCV, lwork + 2 * SA_MAX_NCV, lwork + 3 * SA_MAX_NCV,
+                        lwork + 4 * SA_MAX_NCV, lwork + 5 * SA_MAX_NCV, lpiv, lvec);
+    }
+    __syncthreads();
     solved = true;
   }
   if (threadIdx.x == 0) {
@@ -391,6 +476,121 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
   }
 }
 
+// Thick restart, part 1 (one workgroup): the `keep` largest eigenpairs of the projected matrix.
+// Thread t finds eigenvalue m-1-t by bisection on the Sturm count and its vector by inverse
+// iteration; thread 0 then orthonormalises the vectors in order (modified Gram-Schmidt) and writes
+// the new projected matrix: diag(theta) bordered by arrow_i = beta_{m-1} s_i[m-1].
+__global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(void* work) {
+  __shared__ double la[SA_MAX_NCV], lb[SA_MAX_NCV], lb2[SA_MAX_NCV], lar[SA_MAX_NCV], lar2[SA_MAX_NCV];
+  __shared__ double lbounds[4], ltheta[SA_MAX_KEEP];
+  sa_state* st = state_of(work);
+  if (st->status != SA_RESTART) return;
+  const int m = st->j;
+  const int k = st->kept;
+  int keep = st->keep;
+  if (keep > SA_MAX_KEEP) keep = SA_MAX_KEEP;
+  if (keep > m - 1) keep = m - 1;
+  if (keep < 1) keep = 1;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    la[i] = st->a[i];
+    lb[i] = i < m - 1 ? st->b[i] : 0.0;
+    lar[i] = i < k ? st->arrow[i] : 0.0;
+  }
+  __syncthreads();
+  const Proj T{m, k, la, lb, lar, lb2, lar2};
+  proj_bounds(T, lb2, lar2, lbounds);
+  double* S = restart_of(work);                       // S[t * SA_MAX_NCV + p]
+  double* scratch = S + (int64_t)SA_MAX_KEEP * SA_MAX_NCV;
+  const int t = threadIdx.x;
+  if (t < keep) {
+    const int idx = m - 1 - t;
+    double lo = lbounds[0], hi = lbounds[1];
+    const double pivmin = lbounds[2];
+    for (int it = 0; it < 200; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      if (!(mid > lo && mid < hi)) break;
+      if (sturm_count(T, mid, pivmin) <= idx) lo = mid; else hi = mid;
+    }
+    const double theta = 0.5 * (lo + hi);
+    ltheta[t] = theta;
+    double* base = scratch + (int64_t)t * 8 * SA_MAX_NCV;
+    const double tiny = 2.220446049250313e-16 * fmax(lbounds[3], 1e-300);
+    inverse_iteration(T, theta, tiny, t, base, base + SA_MAX_NCV, base + 2 * SA_MAX_NCV, base + 3 * SA_MAX_NCV,
+                      base + 4 * SA_MAX_NCV, base + 5 * SA_MAX_NCV, reinterpret_cast<int*>(base + 6 * SA_MAX_NCV),
+                      S + (int64_t)t * SA_MAX_NCV);
+  }
+  __threadfence_block();
+  __syncthreads();
+  if (t == 0) {
+    int kk = 0;
+    for (int u = 0; u < keep; ++u) {
+      double* v = S + (int64_t)u * SA_MAX_NCV;
+      for (int w = 0; w < kk; ++w) {
+        const double* q = S + (int64_t)w * SA_MAX_NCV;
+        double dot = 0.0;
+        for (int p = 0; p < m; ++p) dot += q[p] * v[p];
+        for (int p = 0; p < m; ++p) v[p] -= dot * q[p];
+      }
+      double nn = 0.0;
+      for (int p = 0; p < m; ++p) nn += v[p] * v[p];
+      nn = sqrt(nn);
+      if (!(nn > 1e-8) || !isfinite(nn)) continue;     // a vector lost in the span of the earlier ones
+      double* dst = S + (int64_t)kk * SA_MAX_NCV;
+      for (int p = 0; p < m; ++p) dst[p] = v[p] / nn;
+      st->a[kk] = ltheta[u];
+      ++kk;
+    }
+    const double beta_last = st->b[m - 1];   // beta of the last step: couples v_{m-1} to v_m
+    for (int u = 0; u < kk; ++u) st->arrow[u] = beta_last * S[(int64_t)u * SA_MAX_NCV + (m - 1)];
+    st->kept = kk;
+  }
+}
+
+// Thick restart, part 2: rotate the basis in place. Each thread reads all of its row's basis entries
+// before writing any, so rows 0..kept-1 can take the Ritz vectors and row kept the last Lanczos vector.
+template <typename T>
+__global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(void* work, int64_t n, const T* __restrict__ s,
+                                                                    T* __restrict__ V, T* __restrict__ Z) {
+  __shared__ double coef[SA_MAX_KEEP * SA_MAX_NCV];
+  __shared__ bool last;
+  sa_state* st = state_of(work);
+  if (st->status != SA_RESTART) return;
+  const int m = st->j;
+  const int kk = st->kept;
+  const double* S = restart_of(work);
+  for (int q = threadIdx.x; q < kk * m; q += blockDim.x) {
+    const int u = q / m, p = q % m;
+    coef[u * SA_MAX_NCV + p] = S[(int64_t)u * SA_MAX_NCV + p] * st->sigma[p];
+  }
+  __syncthreads();
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double acc[SA_MAX_KEEP];
+#pragma unroll
+    for (int u = 0; u < SA_MAX_KEEP; ++u) acc[u] = 0.0;
+    for (int p = 0; p < m; ++p) {
+      const double vp = (double)V[(int64_t)p * n + r];
+#pragma unroll
+      for (int u = 0; u < SA_MAX_KEEP; ++u)
+        if (u < kk) acc[u] += coef[u * SA_MAX_NCV + p] * vp;
+    }
+    const T lastv = V[(int64_t)m * n + r];
+#pragma unroll
+    for (int u = 0; u < SA_MAX_KEEP; ++u)
+      if (u < kk) V[(int64_t)u * n + r] = (T)acc[u];
+    V[(int64_t)kk * n + r] = lastv;
+    Z[r] = s ? s[r] * lastv : lastv;
+  }
+  if (!last_block_arrive(&st->counters[0], gridDim.x, &last)) return;
+  if (threadIdx.x == 0) {
+    const double sig_last = st->sigma[m];
+    for (int u = 0; u < kk; ++u) st->sigma[u] = 1.0;
+    st->sigma[kk] = sig_last;
+    st->j = kk;
+    st->cycles += 1;
+    st->status = SA_RUNNING;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void lz_ritz_kernel(void* work, int64_t n, const T* __restrict__ V,
                                                            const T* __restrict__ y_last, T* __restrict__ y,
@@ -418,6 +618,9 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_ritz_kernel(void* work, int64_t n
 }
 
 static inline int red_grid(int64_t n) { return grid_for(n, SA_BLOCK * 4, SA_RED_GRID); }
+
+// The Gram-Schmidt update reduces a single column, so it can afford more workgroups.
+constexpr int SA_ORTH_GRID = 2048;
 
 template <typename T>
 static int lanczos_start(void* work, int64_t n, const T* s, const T* x, const T* d, double pexp, double cx,
@@ -450,8 +653,20 @@ static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, T* Z, T* 
                      stream, work, n, (const T*)V, (const T*)W);
   rc = check_launch("lanczos_dots");
   if (rc) return rc;
-  hipLaunchKernelGGL(lz_orth_kernel<T>, dim3(G), dim3(SA_BLOCK), 0, stream, work, n, s, V, Z, (const T*)W);
+  hipLaunchKernelGGL(lz_orth_kernel<T>, dim3(grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID)), dim3(SA_BLOCK), 0, stream, work,
+                     n, s, V, Z, (const T*)W);
   return check_launch("lanczos_orth");
+}
+
+template <typename T>
+static int lanczos_thick_restart(void* work, int64_t n, const T* s, T* V, T* Z, hipStream_t stream) {
+  SA_REQUIRE(work && n > 0 && V && Z, "bad arguments");
+  hipLaunchKernelGGL(lz_thick_prepare_kernel, dim3(1), dim3(SA_BLOCK), 0, stream, work);
+  int rc = check_launch("lanczos_thick_prepare");
+  if (rc) return rc;
+  hipLaunchKernelGGL(lz_thick_vectors_kernel<T>, dim3(grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID)), dim3(SA_BLOCK), 0,
+                     stream, work, n, s, V, Z);
+  return check_launch("lanczos_thick_vectors");
 }
 
 template <typename T>
@@ -470,9 +685,10 @@ using namespace sa;
 extern "C" {
 
 int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, double bound, double pred_tol,
-                     hipStream_t stream) {
+                     int32_t keep, hipStream_t stream) {
   SA_REQUIRE(work && ncv >= 2 && ncv <= SA_MAX_NCV && maxiter >= 1, "bad arguments (2 <= ncv <= 128)");
-  hipLaunchKernelGGL(lz_begin_kernel, dim3(1), dim3(64), 0, stream, work, ncv, maxiter, tol, bound, pred_tol);
+  SA_REQUIRE(keep >= 1 && keep <= SA_MAX_KEEP && keep < ncv, "keep must be in [1, min(16, ncv - 1)]");
+  hipLaunchKernelGGL(lz_begin_kernel, dim3(1), dim3(64), 0, stream, work, ncv, maxiter, tol, bound, pred_tol, keep);
   return check_launch("lanczos_begin");
 }
 
@@ -496,6 +712,14 @@ int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, f
 int sa_lanczos_step_f64(const sa_csr* A, void* work, const double* s, double* V, double* Z, double* W,
                         hipStream_t stream) {
   return lanczos_step<double>(A, work, s, V, Z, W, stream);
+}
+
+int sa_lanczos_thick_restart_f32(void* work, int64_t n, const float* s, float* V, float* Z, hipStream_t stream) {
+  return lanczos_thick_restart<float>(work, n, s, V, Z, stream);
+}
+
+int sa_lanczos_thick_restart_f64(void* work, int64_t n, const double* s, double* V, double* Z, hipStream_t stream) {
+  return lanczos_thick_restart<double>(work, n, s, V, Z, stream);
 }
 
 int sa_lanczos_ritz_f32(void* work, int64_t n, const float* V, const float* y_last, float* y, int32_t only_if,

@@ -19,7 +19,13 @@ __device__ inline double* partials_of(void* work) {
 }
 
 // Number of doubles in the partials area.
-constexpr int64_t kPartials = (int64_t)SA_RED_GRID * (SA_MAX_NCV + 8);
+constexpr int64_t kPartials = (int64_t)2048 * (SA_MAX_NCV + 8);
+
+// Thick-restart scratch after the partials: the kept Ritz coefficient vectors (SA_MAX_KEEP x
+// SA_MAX_NCV) and per-thread inverse-iteration scratch (7 vectors and a pivot array each).
+constexpr int64_t kRestart = (int64_t)SA_MAX_KEEP * SA_MAX_NCV * 9;
+
+__device__ inline double* restart_of(void* work) { return partials_of(work) + kPartials; }
 
 enum RedOp { RED_SUM = 0, RED_MAX = 1 };
 

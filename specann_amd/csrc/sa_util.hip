@@ -232,7 +232,7 @@ size_t sa_state_bytes(void) { return sizeof(sa_state); }
 
 size_t sa_work_bytes(int64_t n) {
   (void)n;
-  return state_bytes_padded() + (size_t)kPartials * sizeof(double);
+  return state_bytes_padded() + (size_t)(kPartials + kRestart) * sizeof(double);
 }
 
 #define SA_GRID(nwork) grid_for((nwork), SA_BLOCK * 4, 4096)

@@ -50,11 +50,14 @@ def default_ncv(n: int, ncv: int | None) -> int:
 class LanczosSolver:
     """Buffers and launch sequence of the warm-started Lanczos for one DeviceProblem."""
 
-    def __init__(self, prob: DeviceProblem, ncv: int | None = None, chunk: int = 8):
+    def __init__(self, prob: DeviceProblem, ncv: int | None = None, chunk: int = 8, keep: int | None = None):
         self.prob = prob
         self.n = prob.n
         self.ncv = default_ncv(prob.n, ncv)
         self.chunk = max(1, int(chunk))
+        # Ritz vectors a thick restart keeps: half the basis (at most 16); 1 = restart from the Ritz
+        # vector alone, the reference eigsh's k = 1 thick restart.
+        self.keep = max(1, min(_lib.SA_MAX_KEEP, self.ncv // 2 if keep is None else int(keep), self.ncv - 1))
         dev, dt = prob.device, prob.dtype
         self.V = torch.empty((self.ncv + 1, self.n), dtype=dt, device=dev)
         self.Z = torch.empty(self.n, dtype=dt, device=dev)
@@ -66,7 +69,7 @@ class LanczosSolver:
     # -- launch helpers ------------------------------------------------------------------------
     def begin(self, tol: float, maxiter: int, bound: float, pred_tol: float = float("inf")) -> None:
         call("sa_lanczos_begin", self.work.ptr, self.ncv, int(maxiter), float(tol), float(bound), float(pred_tol),
-             stream())
+             self.keep, stream())
 
     def start(self, s: torch.Tensor | None, x: torch.Tensor, *, d: torch.Tensor | None = None, pexp: float = 0.0,
               cx: float = 1.0, y: torch.Tensor | None = None, cy: float = 0.0, gate: int = 0,
@@ -77,6 +80,10 @@ class LanczosSolver:
     def _step(self, s: torch.Tensor | None) -> None:
         call(f"sa_lanczos_step_{self.sfx}", self.prob.csr_ref, self.work.ptr, ptr(s), ptr(self.V), ptr(self.Z),
              ptr(self.W), stream())
+
+    def thick_restart(self, s: torch.Tensor | None) -> None:
+        call(f"sa_lanczos_thick_restart_{self.sfx}", self.work.ptr, self.n, ptr(s), ptr(self.V), ptr(self.Z),
+             stream())
 
     def ritz(self, y_last: torch.Tensor | None = None, only_if: int = -1) -> None:
         call(f"sa_lanczos_ritz_{self.sfx}", self.work.ptr, self.n, ptr(self.V), ptr(y_last), ptr(self.Y),
@@ -95,16 +102,20 @@ class LanczosSolver:
                 self._step(s)
                 launched += 1
                 j += 1
-                if j == self.ncv:
-                    # basis full: restart from the Ritz vector, on the device, only if still running
-                    self.ritz(None, only_if=SA_RESTART)
-                    self.start(s, self.Y, only_if=SA_RESTART)
-                    j = 0
+                if j >= self.ncv:
+                    # basis full: thick restart on the device (a no-op unless the solve is at SA_RESTART)
+                    self.thick_restart(s)
+                    j = self.keep
             st = self.work.head()
             if first_beta is None:
                 first_beta = st.first_beta
             if st.status in TERMINAL:
                 return LanczosResult(st.status, st.theta, st.res, st.steps, st.cycles, first_beta, st.r[3])
+            if st.status == SA_RESTART:
+                # the restart came earlier than the host expected (a kept vector was dropped)
+                self.thick_restart(s)
+                st = self.work.head()
+            j = st.j
             if launched > limit:
                 raise EigensolverBreakdown(f"Lanczos did not stop after {launched} steps (status {st.status})")
             check = self.chunk

@@ -189,3 +189,32 @@ def test_autotune_keeps_results(gpu):
     y = torch.empty_like(s)
     prob.matvec(s, x, 0.0, 1.0, y, z)
     torch.testing.assert_close(y, ref, rtol=2e-6, atol=2e-7)
+
+
+@pytest.mark.parametrize("ncv,keep", [(6, 1), (8, 4), (12, 5), (33, 16)])
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+def test_lanczos_thick_restart_converges(gpu, ncv, keep, tdtype):
+    """Many restarts with every keep count reach the dense leading eigenpair."""
+    from specann_amd.eigsh import LanczosSolver
+    from specann_amd._lib import SA_CONVERGED
+    J = CASES["er_2k_pm1"]()
+    prob = DeviceProblem(J, tdtype)
+    dt = _dt(tdtype)
+    Jn, d, c = O.normalize(J, dt)
+    s = torch.empty(prob.n, dtype=tdtype, device=gpu)
+    prob.alpha_scaling(0.8, 1e-12, s)
+    sv = s.cpu().numpy()
+    lam_ref, y_ref = O.leading_eigpair(Jn, sv)
+    tol = 1e-6 if tdtype == torch.float32 else 1e-9
+    solver = LanczosSolver(prob, ncv=ncv, keep=keep)
+    x0 = prob.fill_normal(2, 0, torch.empty(prob.n, dtype=tdtype, device=gpu))
+    solver.begin(tol, 20000, bound=3.0)
+    solver.start(s, x0)
+    res = solver.run(s, 20000)
+    print(f"thick ncv={ncv} keep={keep} dtype={tdtype}: steps={res.steps} restarts={res.cycles} res={res.res:.2e}")
+    assert res.status == SA_CONVERGED
+    assert abs(res.theta - lam_ref) <= 10 * tol
+    nrm, _ = solver.final_vector(None)
+    y = solver.Y.cpu().numpy().astype(np.float64) / nrm
+    assert np.linalg.norm(O.n_matvec(Jn, sv, y) - res.theta * y) <= (2e-5 if tdtype == torch.float32 else 1e-8)
+    assert abs(abs(y @ y_ref) - 1) < 1e-3
