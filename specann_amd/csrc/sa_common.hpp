@@ -117,6 +117,15 @@ __device__ inline T stream_load(const T* p) {
   return __builtin_nontemporal_load(p);
 }
 
+// Rows [lo, hi) owned by this workgroup in the basis-streaming kernels: one contiguous range per
+// workgroup (a multiple of 64 rows), so that the waves of a workgroup walk through the same pages of
+// every basis vector instead of striding the whole vector.
+__device__ inline void block_rows(int64_t n, int64_t& lo, int64_t& hi) {
+  const int64_t per = (((n + gridDim.x - 1) / gridDim.x) + 63) & ~(int64_t)63;
+  lo = (int64_t)blockIdx.x * per;
+  hi = lo + per < n ? lo + per : n;
+}
+
 inline int grid_for(int64_t work, int per_block, int cap) {
   int64_t g = (work + per_block - 1) / per_block;
   if (g < 1) g = 1;

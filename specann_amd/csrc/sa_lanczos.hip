@@ -53,7 +53,9 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(void* work, int64_t 
   if (only_if >= 0 && st->status != only_if) return;
   double acc[1] = {0.0};
   const T clampv = (T)1e-12;
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+  int64_t lo, hi;
+  block_rows(n, lo, hi);
+  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
     T v;
     if (y) {
       v = (T)(cx * (double)x[r] + cy * (double)y[r]);
@@ -112,41 +114,42 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(void* work, int64_t n
   if (st->status != SA_RUNNING) return;
   const int j = st->j;
   const int nch = j / SA_RED_COLS + 1;
-  const int ch = blockIdx.y;
-  if (ch >= nch) return;
-  const int m0 = ch * SA_RED_COLS;
-  const int mc = min(SA_RED_COLS, j + 1 - m0);
-  double acc[SA_RED_COLS];
-#pragma unroll
-  for (int c = 0; c < SA_RED_COLS; ++c) acc[c] = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  // two rows per iteration: 18 independent loads in flight per thread
-  for (; r + stride < n; r += 2 * stride) {
-    const double w0 = (double)W[r], w1 = (double)W[r + stride];
-    T v0[SA_RED_COLS], v1[SA_RED_COLS];
-#pragma unroll
-    for (int c = 0; c < SA_RED_COLS; ++c) {
-      v0[c] = c < mc ? V[(int64_t)(m0 + c) * n + r] : (T)0;
-      v1[c] = c < mc ? V[(int64_t)(m0 + c) * n + r + stride] : (T)0;
-    }
-#pragma unroll
-    for (int c = 0; c < SA_RED_COLS; ++c) acc[c] += (double)v0[c] * w0 + (double)v1[c] * w1;
-  }
-  if (r < n) {
-    const double w0 = (double)W[r];
-#pragma unroll
-    for (int c = 0; c < SA_RED_COLS; ++c)
-      if (c < mc) acc[c] += (double)V[(int64_t)(m0 + c) * n + r] * w0;
-  }
   double* part = partials_of(work);
   const int64_t G = gridDim.x;
+  int64_t lo, hi;
+  block_rows(n, lo, hi);
+  // All chunks of 8 basis vectors over this workgroup's rows; W is re-read per chunk from cache.
+  for (int ch = 0; ch < nch; ++ch) {
+    const int m0 = ch * SA_RED_COLS;
+    const int mc = min(SA_RED_COLS, j + 1 - m0);
+    double acc[SA_RED_COLS];
 #pragma unroll
-  for (int c = 0; c < SA_RED_COLS; ++c) {
-    const double bs = block_sum(acc[c], red);
-    if (threadIdx.x == 0) part[((int64_t)ch * G + blockIdx.x) * SA_RED_COLS + c] = bs;
+    for (int c = 0; c < SA_RED_COLS; ++c) acc[c] = 0.0;
+    int64_t r = lo + threadIdx.x;
+    for (; r + SA_BLOCK < hi; r += 2 * SA_BLOCK) {
+      const double w0 = (double)W[r], w1 = (double)W[r + SA_BLOCK];
+      T v0[SA_RED_COLS], v1[SA_RED_COLS];
+#pragma unroll
+      for (int c = 0; c < SA_RED_COLS; ++c) {
+        v0[c] = c < mc ? V[(int64_t)(m0 + c) * n + r] : (T)0;
+        v1[c] = c < mc ? V[(int64_t)(m0 + c) * n + r + SA_BLOCK] : (T)0;
+      }
+#pragma unroll
+      for (int c = 0; c < SA_RED_COLS; ++c) acc[c] += (double)v0[c] * w0 + (double)v1[c] * w1;
+    }
+    if (r < hi) {
+      const double w0 = (double)W[r];
+#pragma unroll
+      for (int c = 0; c < SA_RED_COLS; ++c)
+        if (c < mc) acc[c] += (double)V[(int64_t)(m0 + c) * n + r] * w0;
+    }
+#pragma unroll
+    for (int c = 0; c < SA_RED_COLS; ++c) {
+      const double bs = block_sum(acc[c], red);
+      if (threadIdx.x == 0) part[((int64_t)ch * G + blockIdx.x) * SA_RED_COLS + c] = bs;
+    }
   }
-  if (!last_block_arrive(&st->counters[0], (unsigned)(G * nch), &last)) return;
+  if (!last_block_arrive(&st->counters[0], (unsigned)G, &last)) return;
   // All basis vectors at once: a 32-lane group per vector, each lane summing a strided share of
   // the block partials in block order, then a fixed shuffle tree.
   const int lane32 = threadIdx.x & 31, grp = threadIdx.x >> 5;
@@ -385,19 +388,23 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
   __syncthreads();
   T* __restrict__ vn = V + (int64_t)(j + 1) * n;
   double acc = 0.0;
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    double w0 = (double)W[r], w1 = 0.0, w2 = 0.0, w3 = 0.0;
+  int64_t lo, hi;
+  block_rows(n, lo, hi);
+  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+    double wv[8];
+    wv[0] = (double)W[r];
+#pragma unroll
+    for (int u = 1; u < 8; ++u) wv[u] = 0.0;
     int m = 0;
-    for (; m + 3 <= j; m += 4) {
-      const T a0 = V[(int64_t)m * n + r], a1 = V[(int64_t)(m + 1) * n + r];
-      const T a2 = V[(int64_t)(m + 2) * n + r], a3 = V[(int64_t)(m + 3) * n + r];
-      w0 -= coef[m] * (double)a0;
-      w1 -= coef[m + 1] * (double)a1;
-      w2 -= coef[m + 2] * (double)a2;
-      w3 -= coef[m + 3] * (double)a3;
+    for (; m + 7 <= j; m += 8) {
+      T a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = V[(int64_t)(m + u) * n + r];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) wv[u] -= coef[m + u] * (double)a[u];
     }
-    for (; m <= j; ++m) w0 -= coef[m] * (double)V[(int64_t)m * n + r];
-    const double w = (w0 + w1) + (w2 + w3);
+    for (; m <= j; ++m) wv[0] -= coef[m] * (double)V[(int64_t)m * n + r];
+    const double w = ((wv[0] + wv[1]) + (wv[2] + wv[3])) + ((wv[4] + wv[5]) + (wv[6] + wv[7]));
     const T wt = (T)w;
     vn[r] = wt;
     Z[r] = s ? s[r] * wt : wt;
@@ -477,12 +484,17 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
 }
 
 // Thick restart, part 1 (one workgroup): the `keep` largest eigenpairs of the projected matrix.
-// Thread t finds eigenvalue m-1-t by bisection on the Sturm count and its vector by inverse
-// iteration; thread 0 then orthonormalises the vectors in order (modified Gram-Schmidt) and writes
-// the new projected matrix: diag(theta) bordered by arrow_i = beta_{m-1} s_i[m-1].
+// Eigenvalue m-1-u is refined by the 16 lanes of group u with a 17-section of its bracket per round
+// (Sturm counts); thread u then finds its vector by inverse iteration, and thread 0 orthonormalises
+// the vectors in order (modified Gram-Schmidt) and writes the new projected matrix: diag(theta)
+// bordered by arrow_u = beta_{m-1} s_u[m-1]. Scratch lives in LDS while m <= kPrepLds.
+constexpr int kPrepLds = 48;
+
 __global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(void* work) {
   __shared__ double la[SA_MAX_NCV], lb[SA_MAX_NCV], lb2[SA_MAX_NCV], lar[SA_MAX_NCV], lar2[SA_MAX_NCV];
   __shared__ double lbounds[4], ltheta[SA_MAX_KEEP];
+  __shared__ double lscr[SA_MAX_KEEP * 8 * kPrepLds];
+  __shared__ double lvec[SA_MAX_KEEP * kPrepLds];
   sa_state* st = state_of(work);
   if (st->status != SA_RESTART) return;
   const int m = st->j;
@@ -497,36 +509,49 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(void* work) 
     lar[i] = i < k ? st->arrow[i] : 0.0;
   }
   __syncthreads();
-  const Proj T{m, k, la, lb, lar, lb2, lar2};
-  proj_bounds(T, lb2, lar2, lbounds);
-  double* S = restart_of(work);                       // S[t * SA_MAX_NCV + p]
-  double* scratch = S + (int64_t)SA_MAX_KEEP * SA_MAX_NCV;
+  const Proj P{m, k, la, lb, lar, lb2, lar2};
+  proj_bounds(P, lb2, lar2, lbounds);
   const int t = threadIdx.x;
-  if (t < keep) {
-    const int idx = m - 1 - t;
+  const int grp = t >> 4, lig = t & 15;
+  {
+    const int idx = m - 1 - (grp < keep ? grp : 0);
     double lo = lbounds[0], hi = lbounds[1];
     const double pivmin = lbounds[2];
-    for (int it = 0; it < 200; ++it) {
-      const double mid = 0.5 * (lo + hi);
-      if (!(mid > lo && mid < hi)) break;
-      if (sturm_count(T, mid, pivmin) <= idx) lo = mid; else hi = mid;
+    for (int round = 0; round < 15; ++round) {
+      const double width = hi - lo;
+      const double x = lo + width * (double)(lig + 1) / 17.0;
+      const bool above = sturm_count(P, x, pivmin) > idx;   // x lies above eigenvalue idx
+      const unsigned long long mk = __ballot(above);
+      const unsigned seg = (unsigned)((mk >> (16 * (grp & 3))) & 0xFFFFull);
+      const int first = seg ? __builtin_ctz(seg) : 16;
+      const double nhi = first < 16 ? lo + width * (double)(first + 1) / 17.0 : hi;
+      const double nlo = first == 0 ? lo : lo + width * (double)first / 17.0;
+      lo = nlo;
+      hi = nhi;
     }
-    const double theta = 0.5 * (lo + hi);
-    ltheta[t] = theta;
-    double* base = scratch + (int64_t)t * 8 * SA_MAX_NCV;
+    if (lig == 0 && grp < keep) ltheta[grp] = 0.5 * (lo + hi);
+  }
+  __syncthreads();
+  double* S = restart_of(work);                              // S[u * SA_MAX_NCV + p], read by part 2
+  double* gscr = S + (int64_t)SA_MAX_KEEP * SA_MAX_NCV;
+  const bool in_lds = m <= kPrepLds;
+  const int vstride = in_lds ? kPrepLds : SA_MAX_NCV;
+  double* vecs = in_lds ? lvec : S;
+  if (t < keep) {
+    double* base = in_lds ? lscr + t * 8 * kPrepLds : gscr + (int64_t)t * 8 * SA_MAX_NCV;
+    const int sc = in_lds ? kPrepLds : SA_MAX_NCV;
     const double tiny = 2.220446049250313e-16 * fmax(lbounds[3], 1e-300);
-    inverse_iteration(T, theta, tiny, t, base, base + SA_MAX_NCV, base + 2 * SA_MAX_NCV, base + 3 * SA_MAX_NCV,
-                      base + 4 * SA_MAX_NCV, base + 5 * SA_MAX_NCV, reinterpret_cast<int*>(base + 6 * SA_MAX_NCV),
-                      S + (int64_t)t * SA_MAX_NCV);
+    inverse_iteration(P, ltheta[t], tiny, t, base, base + sc, base + 2 * sc, base + 3 * sc, base + 4 * sc,
+                      base + 5 * sc, reinterpret_cast<int*>(base + 6 * sc), vecs + (int64_t)t * vstride);
   }
   __threadfence_block();
   __syncthreads();
   if (t == 0) {
     int kk = 0;
     for (int u = 0; u < keep; ++u) {
-      double* v = S + (int64_t)u * SA_MAX_NCV;
+      double* v = vecs + (int64_t)u * vstride;
       for (int w = 0; w < kk; ++w) {
-        const double* q = S + (int64_t)w * SA_MAX_NCV;
+        const double* q = vecs + (int64_t)w * vstride;
         double dot = 0.0;
         for (int p = 0; p < m; ++p) dot += q[p] * v[p];
         for (int p = 0; p < m; ++p) v[p] -= dot * q[p];
@@ -535,14 +560,21 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(void* work) 
       for (int p = 0; p < m; ++p) nn += v[p] * v[p];
       nn = sqrt(nn);
       if (!(nn > 1e-8) || !isfinite(nn)) continue;     // a vector lost in the span of the earlier ones
-      double* dst = S + (int64_t)kk * SA_MAX_NCV;
+      double* dst = vecs + (int64_t)kk * vstride;
       for (int p = 0; p < m; ++p) dst[p] = v[p] / nn;
       st->a[kk] = ltheta[u];
       ++kk;
     }
     const double beta_last = st->b[m - 1];   // beta of the last step: couples v_{m-1} to v_m
-    for (int u = 0; u < kk; ++u) st->arrow[u] = beta_last * S[(int64_t)u * SA_MAX_NCV + (m - 1)];
+    for (int u = 0; u < kk; ++u) st->arrow[u] = beta_last * vecs[(int64_t)u * vstride + (m - 1)];
     st->kept = kk;
+  }
+  __syncthreads();
+  if (in_lds) {
+    for (int q = t; q < keep * m; q += blockDim.x) {
+      const int u = q / m, p = q % m;
+      S[(int64_t)u * SA_MAX_NCV + p] = lvec[u * kPrepLds + p];
+    }
   }
 }
 
@@ -563,11 +595,26 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(void* work, 
     coef[u * SA_MAX_NCV + p] = S[(int64_t)u * SA_MAX_NCV + p] * st->sigma[p];
   }
   __syncthreads();
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+  int64_t lo, hi;
+  block_rows(n, lo, hi);
+  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
     double acc[SA_MAX_KEEP];
 #pragma unroll
     for (int u = 0; u < SA_MAX_KEEP; ++u) acc[u] = 0.0;
-    for (int p = 0; p < m; ++p) {
+    int p = 0;
+    for (; p + 3 < m; p += 4) {
+      T v4[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v4[q] = V[(int64_t)(p + q) * n + r];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double vp = (double)v4[q];
+#pragma unroll
+        for (int u = 0; u < SA_MAX_KEEP; ++u)
+          if (u < kk) acc[u] += coef[u * SA_MAX_NCV + p + q] * vp;
+      }
+    }
+    for (; p < m; ++p) {
       const double vp = (double)V[(int64_t)p * n + r];
 #pragma unroll
       for (int u = 0; u < SA_MAX_KEEP; ++u)
@@ -602,9 +649,17 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_ritz_kernel(void* work, int64_t n
   for (int i = threadIdx.x; i < m; i += blockDim.x) coef[i] = st->ritz[i] * st->sigma[i];
   __syncthreads();
   double acc[2] = {0.0, 0.0};
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    double v = 0.0;
-    for (int i = 0; i < m; ++i) v += coef[i] * (double)V[(int64_t)i * n + r];
+  int64_t lo, hi;
+  block_rows(n, lo, hi);
+  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+    double v4[4] = {0.0, 0.0, 0.0, 0.0};
+    int i = 0;
+    for (; i + 3 < m; i += 4) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v4[q] += coef[i + q] * (double)V[(int64_t)(i + q) * n + r];
+    }
+    for (; i < m; ++i) v4[0] += coef[i] * (double)V[(int64_t)i * n + r];
+    const double v = (v4[0] + v4[1]) + (v4[2] + v4[3]);
     const T vt = (T)v;
     y[r] = vt;
     acc[0] += (double)vt * (double)vt;
@@ -619,8 +674,10 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_ritz_kernel(void* work, int64_t n
 
 static inline int red_grid(int64_t n) { return grid_for(n, SA_BLOCK * 4, SA_RED_GRID); }
 
-// The Gram-Schmidt update reduces a single column, so it can afford more workgroups.
+// The Gram-Schmidt update reduces a single column, so it can afford more workgroups; the projections
+// keep (j+1) partials per workgroup, so their grid is capped by the partials area (2048 x 136).
 constexpr int SA_ORTH_GRID = 2048;
+constexpr int SA_DOTS_GRID = 1024;
 
 template <typename T>
 static int lanczos_start(void* work, int64_t n, const T* s, const T* x, const T* d, double pexp, double cx,
@@ -649,8 +706,8 @@ static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, T* Z, T* 
     if (rc) return rc;
   }
   const int G = red_grid(n);
-  hipLaunchKernelGGL(lz_dots_kernel<T>, dim3(G, (SA_MAX_NCV + SA_RED_COLS - 1) / SA_RED_COLS), dim3(SA_BLOCK), 0,
-                     stream, work, n, (const T*)V, (const T*)W);
+  hipLaunchKernelGGL(lz_dots_kernel<T>, dim3(grid_for(n, SA_BLOCK * 4, SA_DOTS_GRID)), dim3(SA_BLOCK), 0, stream,
+                     work, n, (const T*)V, (const T*)W);
   rc = check_launch("lanczos_dots");
   if (rc) return rc;
   hipLaunchKernelGGL(lz_orth_kernel<T>, dim3(grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID)), dim3(SA_BLOCK), 0, stream, work,
