@@ -69,7 +69,8 @@ typedef struct sa_csr {
   const int32_t* ptile_rows;/* per panel: ntiles_p + 1 tile rows, concatenated */
   const int64_t* ptile_off; /* npanels + 1 offsets into ptile_rows */
   int64_t pmax_tiles;       /* max over panels of ntiles_p */
-  void* partial;            /* npanels x n scratch of T */
+  void* partial;            /* npanels rows of n entries of T, leading dimension pld */
+  int64_t pld;
 } sa_csr;
 
 /* Device-side state of one eigensolve; lives at the start of the work buffer. */
@@ -165,7 +166,9 @@ int sa_spmv_gather_f64(const sa_csr* A, const double* s, const double* g, double
                        double* y, hipStream_t stream);
 
 /* ---- Warm-started Lanczos on N_alpha (utils/eigsh.py:10-17, models/spectral_annealing.py:671-677)
- * Basis V is (ncv + 1) x n, row-major, leading dimension n. Z, W are n-vectors of scratch. */
+ * Basis V is (ncv + 1) rows of n entries, row-major with leading dimension ldv >= n (pad ldv so that
+ * ldv * sizeof(T) is not a multiple of a large power of two: with n = 2^22 unpadded rows start on the
+ * same HBM channels and the basis streams camp on them). Z, W are n-vectors of scratch. */
 int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, double bound, double pred_tol,
                      int32_t keep, hipStream_t stream);
 /* Start vector V[0] = x (.) max(d, 1e-12)^pexp (the diagonal predictor, utils/predictors.py:6-26, with
@@ -175,29 +178,32 @@ int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, doubl
  * and sets SA_REJECTED if it exceeds pred_tol. only_if >= 0 makes the call a no-op unless
  * state.status == only_if (device-side gating, so restarts need no host round trip). */
 int sa_lanczos_start_f32(void* work, int64_t n, const float* s, const float* x, const float* d, double pexp,
-                         double cx, const float* y, double cy, float* V, float* Z, int32_t gate, int32_t only_if,
-                         hipStream_t stream);
+                         double cx, const float* y, double cy, float* V, int64_t ldv, float* Z, int32_t gate,
+                         int32_t only_if, hipStream_t stream);
 int sa_lanczos_start_f64(void* work, int64_t n, const double* s, const double* x, const double* d, double pexp,
-                         double cx, const double* y, double cy, double* V, double* Z, int32_t gate, int32_t only_if,
-                         hipStream_t stream);
+                         double cx, const double* y, double cy, double* V, int64_t ldv, double* Z, int32_t gate,
+                         int32_t only_if, hipStream_t stream);
 /* One Lanczos step with full (classical Gram-Schmidt) reorthogonalisation and a Ritz solve of the
  * tridiagonal; no-op unless state.status == SA_RUNNING. */
-int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, float* Z, float* W, hipStream_t stream);
-int sa_lanczos_step_f64(const sa_csr* A, void* work, const double* s, double* V, double* Z, double* W,
+int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, int64_t ldv, float* Z, float* W,
+                        hipStream_t stream);
+int sa_lanczos_step_f64(const sa_csr* A, void* work, const double* s, double* V, int64_t ldv, double* Z, double* W,
                         hipStream_t stream);
 /* Thick restart (gated on SA_RESTART): keep the `keep` largest Ritz pairs of the full basis. The basis
  * is rotated in place: rows 0..keep-1 become the Ritz vectors, row keep the last Lanczos vector, Z its
  * scaled copy, and the projected matrix becomes diag(theta) bordered by the arrow couplings. With
  * keep = 1 this is the restart from the Ritz vector alone (the k = 1 thick restart of the reference's
  * eigsh). */
-int sa_lanczos_thick_restart_f32(void* work, int64_t n, const float* s, float* V, float* Z, hipStream_t stream);
-int sa_lanczos_thick_restart_f64(void* work, int64_t n, const double* s, double* V, double* Z, hipStream_t stream);
+int sa_lanczos_thick_restart_f32(void* work, int64_t n, const float* s, float* V, int64_t ldv, float* Z,
+                                 hipStream_t stream);
+int sa_lanczos_thick_restart_f64(void* work, int64_t n, const double* s, double* V, int64_t ldv, double* Z,
+                                 hipStream_t stream);
 /* Ritz vector y = sum_m ritz_m v_m of the current cycle; state.r[4] = ||y||, state.r[5] = y . y_last
  * (y_last may be NULL). only_if as in sa_lanczos_start. */
-int sa_lanczos_ritz_f32(void* work, int64_t n, const float* V, const float* y_last, float* y, int32_t only_if,
-                        hipStream_t stream);
-int sa_lanczos_ritz_f64(void* work, int64_t n, const double* V, const double* y_last, double* y, int32_t only_if,
-                        hipStream_t stream);
+int sa_lanczos_ritz_f32(void* work, int64_t n, const float* V, int64_t ldv, const float* y_last, float* y,
+                        int32_t only_if, hipStream_t stream);
+int sa_lanczos_ritz_f64(void* work, int64_t n, const double* V, int64_t ldv, const double* y_last, double* y,
+                        int32_t only_if, hipStream_t stream);
 
 /* ---- Vector helpers ------------------------------------------------------------------------- */
 /* state.r[slot] = ||a*x + b*y||_2 (y may be NULL); state.r[slot+1] = x . y (0 when y is NULL). */

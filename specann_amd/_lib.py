@@ -32,6 +32,13 @@ SA_MAX_NCV = 128
 SA_MAX_KEEP = 16
 
 
+def padded_ld(n: int, elem_bytes: int) -> int:
+    """Leading dimension >= n whose byte stride is n rounded up to 4 KiB plus 4352 bytes (17 x 256):
+    rows of a multi-row buffer then start on different HBM channels whatever n is."""
+    per4k = 4096 // elem_bytes
+    return ((n + per4k - 1) // per4k) * per4k + 4352 // elem_bytes
+
+
 class SpecAnnLibraryError(RuntimeError):
     """libspecann is missing or a call failed for a non-numerical reason."""
 
@@ -65,6 +72,7 @@ class SaCsr(ctypes.Structure):
         ("ptile_off", c_void_p),
         ("pmax_tiles", c_int64),
         ("partial", c_void_p),
+        ("pld", c_int64),
     ]
 
 
@@ -126,16 +134,16 @@ _SIGNATURES = {
     "sa_spmv_gather_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, _P, c_double, _P, _P]),
     "sa_spmv_gather_f64": (c_int32, [POINTER(SaCsr), _P, _P, c_double, _P, c_double, _P, _P]),
     "sa_lanczos_begin": (c_int32, [_P, c_int32, c_int32, c_double, c_double, c_double, c_int32, _P]),
-    "sa_lanczos_thick_restart_f32": (c_int32, [_P, c_int64, _P, _P, _P, _P]),
-    "sa_lanczos_thick_restart_f64": (c_int32, [_P, c_int64, _P, _P, _P, _P]),
-    "sa_lanczos_start_f32": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, _P, c_double, _P, _P, c_int32,
-                                       c_int32, _P]),
-    "sa_lanczos_start_f64": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, _P, c_double, _P, _P, c_int32,
-                                       c_int32, _P]),
-    "sa_lanczos_step_f32": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P, _P, _P]),
-    "sa_lanczos_step_f64": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P, _P, _P]),
-    "sa_lanczos_ritz_f32": (c_int32, [_P, c_int64, _P, _P, _P, c_int32, _P]),
-    "sa_lanczos_ritz_f64": (c_int32, [_P, c_int64, _P, _P, _P, c_int32, _P]),
+    "sa_lanczos_thick_restart_f32": (c_int32, [_P, c_int64, _P, _P, c_int64, _P, _P]),
+    "sa_lanczos_thick_restart_f64": (c_int32, [_P, c_int64, _P, _P, c_int64, _P, _P]),
+    "sa_lanczos_start_f32": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, _P, c_double, _P, c_int64, _P,
+                                       c_int32, c_int32, _P]),
+    "sa_lanczos_start_f64": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, _P, c_double, _P, c_int64, _P,
+                                       c_int32, c_int32, _P]),
+    "sa_lanczos_step_f32": (c_int32, [POINTER(SaCsr), _P, _P, _P, c_int64, _P, _P, _P]),
+    "sa_lanczos_step_f64": (c_int32, [POINTER(SaCsr), _P, _P, _P, c_int64, _P, _P, _P]),
+    "sa_lanczos_ritz_f32": (c_int32, [_P, c_int64, _P, c_int64, _P, _P, c_int32, _P]),
+    "sa_lanczos_ritz_f64": (c_int32, [_P, c_int64, _P, c_int64, _P, _P, c_int32, _P]),
     "sa_norm_dot_f32": (c_int32, [_P, c_int64, _P, _P, c_double, c_double, c_int32, _P]),
     "sa_norm_dot_f64": (c_int32, [_P, c_int64, _P, _P, c_double, c_double, c_int32, _P]),
     "sa_scale_copy_f32": (c_int32, [c_int64, _P, c_double, _P, _P]),

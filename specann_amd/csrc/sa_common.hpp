@@ -78,39 +78,19 @@ __device__ inline T block_sum(T v, T* red) {
   return r;
 }
 
-// "Last block done" handshake: every block stores its partial, then thread 0 bumps the counter;
-// the block that sees gridDim-1 is the last one and may read all partials. It resets the counter.
-__device__ inline bool last_block_arrive(unsigned int* counter, unsigned int total, bool* flag_lds) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int prev = atomicAdd(counter, 1u);
-    *flag_lds = (prev == total - 1);
-  }
-  __syncthreads();
-  bool last = *flag_lds;
-  if (last) {
-    __threadfence();
-    if (threadIdx.x == 0) atomicExch(counter, 0u);
-  }
-  return last;
-}
-
-// Coherent load of a value another workgroup wrote in this kernel (after the handshake above):
-// an agent-scope atomic load, which cannot be served from a stale per-CU cache line.
+// 16-byte vector of T for the basis-streaming kernels (4 floats or 2 doubles per lane).
 template <typename T>
-__device__ inline T load_coherent(const T* p) {
-  static_assert(sizeof(T) == 8 || sizeof(T) == 4, "4- or 8-byte values only");
-  if constexpr (sizeof(T) == 8) {
-    unsigned long long v = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-    return __builtin_bit_cast(T, v);
-  } else {
-    unsigned int v = __hip_atomic_load(reinterpret_cast<const unsigned int*>(p), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-    return __builtin_bit_cast(T, v);
-  }
-}
+struct vec16;
+template <>
+struct vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static constexpr int L = 4;
+};
+template <>
+struct vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+  static constexpr int L = 2;
+};
 
 template <typename T>
 __device__ inline T stream_load(const T* p) {
@@ -125,6 +105,8 @@ __device__ inline void block_rows(int64_t n, int64_t& lo, int64_t& hi) {
   lo = (int64_t)blockIdx.x * per;
   hi = lo + per < n ? lo + per : n;
 }
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 inline int grid_for(int64_t work, int per_block, int cap) {
   int64_t g = (work + per_block - 1) / per_block;

@@ -10,11 +10,12 @@
 //   2. projections h_m = v_m . W for every basis vector of the cycle (FP64 partial sums, 8 vectors
 //      per chunk, last-workgroup reduction);
 //   3. classical Gram-Schmidt W - sum h_m v_m into V[j+1] (raw; sigma_{j+1} = 1/beta_j), Z for the
-//      next step, beta_j = ||.||, then, in the last workgroup, the FP64 Ritz solve of the
-//      tridiagonal T_{j+1}: largest eigenvalue by 257-section Sturm bisection over the whole
-//      workgroup, its eigenvector by inverse iteration with a pivoted tridiagonal LU, and the
+//      next step, beta_j = ||.||, then, in the last workgroup, the FP64 Ritz solve of the projected
+//      matrix: largest eigenvalue by 257-section Sturm bisection over the whole workgroup, the last
+//      component s_j of its eigenvector from the derivative of the last LDL^T pivot, and the
 //      residual estimate beta_j |s_j|, the stopping test the reference's eigsh applies per restart
-//      (here it is applied every step).
+//      (here it is applied every step). The full eigenvector (inverse iteration with a pivoted
+//      tridiagonal LU) is computed only in the step that stops the solve.
 // Every launch reads the status in the work buffer first and does nothing unless the solve is
 // running, so the host may queue several steps (and the restart pair) without a round trip.
 #include "sa_reduce.hpp"
@@ -49,7 +50,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(void* work, int64_t 
                                                             T pexp, double cx, const T* __restrict__ y, double cy,
                                                             T* __restrict__ V, T* __restrict__ Z, int32_t gate,
                                                             int32_t only_if) {
-  sa_state* st = state_of(work);
+  const sa_state* st = state_of(work);
   if (only_if >= 0 && st->status != only_if) return;
   double acc[1] = {0.0};
   const T clampv = (T)1e-12;
@@ -71,9 +72,16 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(void* work, int64_t 
     Z[r] = s ? s[r] * v : v;
     acc[0] += (double)v * (double)v;
   }
-  double out[1];
-  if (grid_reduce<1, RED_SUM>(acc, work, out) && threadIdx.x == 0) {
-    const double nrm = sqrt(out[0]);
+  block_partials<1, RED_SUM>(acc, work);
+}
+
+// ||x0||, sigma_0 and the fresh state of a started solve (phase 2 of the norm of lz_start).
+__global__ __launch_bounds__(SA_BLOCK) void lz_start_finish_kernel(void* work, int G, int32_t gate, int32_t only_if) {
+  sa_state* st = state_of(work);
+  if (only_if >= 0 && st->status != only_if) return;
+  const double tot = sum_partials(partials_of(work), G, 1, 0);
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt(tot);
     st->r[3] = nrm;
     const bool ok = nrm > 0.0 && isfinite(nrm);
     st->sigma[0] = ok ? 1.0 / nrm : 0.0;
@@ -93,72 +101,98 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(void* work, int64_t 
 // W = sigma_j s (.) sum_p partial[p] (XCD-panel layout; panel order fixed).
 template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void lz_combine_kernel(void* work, int64_t n, int32_t npanels,
-                                                              const T* __restrict__ partial, const T* __restrict__ s,
-                                                              T* __restrict__ W) {
+                                                              const T* __restrict__ partial, int64_t pld,
+                                                              const T* __restrict__ s, T* __restrict__ W) {
   const sa_state* st = state_of(work);
   if (st->status != SA_RUNNING) return;
   const double sigma = st->sigma[st->j];
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
-    for (int p = 0; p < npanels; ++p) acc += (double)partial[(int64_t)p * n + r];
+    for (int p = 0; p < npanels; ++p) acc += (double)partial[(int64_t)p * pld + r];
     W[r] = (T)(sigma * (s ? (double)s[r] : 1.0) * acc);
   }
 }
 
+// h_m = sigma_m v_m . W for every basis vector of the cycle. 16-byte loads, chunks of 8 vectors per
+// pass over the workgroup's rows; per-workgroup partials are stored column-major (part[m * G + b]) and
+// the last workgroup sums them in workgroup order, a 32-lane group per column.
 template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(void* work, int64_t n, const T* __restrict__ V,
-                                                           const T* __restrict__ W) {
-  __shared__ double red[SA_WAVES_PER_BLOCK];
-  __shared__ bool last;
-  sa_state* st = state_of(work);
+                                                           int64_t ldv, const T* __restrict__ W) {
+  using VT = typename vec16<T>::type;
+  constexpr int L = vec16<T>::L;
+  __shared__ double red[2][SA_WAVES_PER_BLOCK][SA_RED_COLS];
+  const sa_state* st = state_of(work);
   if (st->status != SA_RUNNING) return;
   const int j = st->j;
   const int nch = j / SA_RED_COLS + 1;
   double* part = partials_of(work);
-  const int64_t G = gridDim.x;
+  const int G = (int)gridDim.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int64_t lo, hi;
   block_rows(n, lo, hi);
-  // All chunks of 8 basis vectors over this workgroup's rows; W is re-read per chunk from cache.
+  const int64_t nv = hi > lo ? (hi - lo) / L : 0;
   for (int ch = 0; ch < nch; ++ch) {
     const int m0 = ch * SA_RED_COLS;
     const int mc = min(SA_RED_COLS, j + 1 - m0);
     double acc[SA_RED_COLS];
 #pragma unroll
     for (int c = 0; c < SA_RED_COLS; ++c) acc[c] = 0.0;
-    int64_t r = lo + threadIdx.x;
-    for (; r + SA_BLOCK < hi; r += 2 * SA_BLOCK) {
-      const double w0 = (double)W[r], w1 = (double)W[r + SA_BLOCK];
-      T v0[SA_RED_COLS], v1[SA_RED_COLS];
-#pragma unroll
-      for (int c = 0; c < SA_RED_COLS; ++c) {
-        v0[c] = c < mc ? V[(int64_t)(m0 + c) * n + r] : (T)0;
-        v1[c] = c < mc ? V[(int64_t)(m0 + c) * n + r + SA_BLOCK] : (T)0;
-      }
-#pragma unroll
-      for (int c = 0; c < SA_RED_COLS; ++c) acc[c] += (double)v0[c] * w0 + (double)v1[c] * w1;
-    }
-    if (r < hi) {
-      const double w0 = (double)W[r];
+    for (int64_t g = threadIdx.x; g < nv; g += SA_BLOCK) {
+      const int64_t r = lo + g * L;
+      const VT w = *reinterpret_cast<const VT*>(&W[r]);
+      VT v[SA_RED_COLS];
 #pragma unroll
       for (int c = 0; c < SA_RED_COLS; ++c)
-        if (c < mc) acc[c] += (double)V[(int64_t)(m0 + c) * n + r] * w0;
+        v[c] = c < mc ? *reinterpret_cast<const VT*>(&V[(int64_t)(m0 + c) * ldv + r]) : VT{};
+#pragma unroll
+      for (int c = 0; c < SA_RED_COLS; ++c) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) acc[c] += (double)v[c][l] * (double)w[l];
+      }
+    }
+    for (int64_t r = lo + nv * L + threadIdx.x; r < hi; r += SA_BLOCK) {
+      const double w = (double)W[r];
+#pragma unroll
+      for (int c = 0; c < SA_RED_COLS; ++c)
+        if (c < mc) acc[c] += (double)V[(int64_t)(m0 + c) * ldv + r] * w;
     }
 #pragma unroll
     for (int c = 0; c < SA_RED_COLS; ++c) {
-      const double bs = block_sum(acc[c], red);
-      if (threadIdx.x == 0) part[((int64_t)ch * G + blockIdx.x) * SA_RED_COLS + c] = bs;
+      const double x = wave_sum(acc[c]);
+      if (lane == 0) red[ch & 1][wave][c] = x;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < mc) {
+      const int c = threadIdx.x;
+      part[(int64_t)(m0 + c) * G + blockIdx.x] =
+          (red[ch & 1][0][c] + red[ch & 1][1][c]) + (red[ch & 1][2][c] + red[ch & 1][3][c]);
     }
   }
-  if (!last_block_arrive(&st->counters[0], (unsigned)G, &last)) return;
-  // All basis vectors at once: a 32-lane group per vector, each lane summing a strided share of
-  // the block partials in block order, then a fixed shuffle tree.
+}
+
+// Phase 2 of the projections: column m summed over the G workgroups by a 32-lane group, eight loads
+// in flight per lane, in workgroup order per lane and a fixed shuffle tree across lanes.
+__global__ __launch_bounds__(SA_BLOCK) void lz_dots_finish_kernel(void* work, int G) {
+  sa_state* st = state_of(work);
+  if (st->status != SA_RUNNING) return;
+  const int j = st->j;
+  const double* part = partials_of(work);
   const int lane32 = threadIdx.x & 31, grp = threadIdx.x >> 5;
   for (int mb = 0; mb <= j; mb += SA_BLOCK / 32) {
     const int m = mb + grp;
     double x = 0.0;
     if (m <= j) {
-      const int chm = m / SA_RED_COLS, cm = m % SA_RED_COLS;
-      for (int64_t b = lane32; b < G; b += 32) x += load_coherent(&part[((int64_t)chm * G + b) * SA_RED_COLS + cm]);
+      const double* pc = part + (int64_t)m * G;
+      int b = lane32;
+      for (; b + 32 * 7 < G; b += 32 * 8) {
+        double q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) q[u] = pc[b + 32 * u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x += q[u];
+      }
+      for (; b < G; b += 32) x += pc[b];
     }
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) x += __shfl_xor(x, o, 32);
@@ -217,81 +251,120 @@ __device__ inline double gersh_radius(const Proj& T, int i) {
   return r;
 }
 
+// s_{m-1}^2 of the unit eigenvector of the eigenvalue x, from the derivative of the last pivot of the
+// LDL^T factorisation in sturm_count's order: 1 / D_last(y) = sum_l s_l^2 / (theta_l - y) near an
+// eigenvalue theta gives D_last'(theta) = -1 / s^2. D' follows D through the same recurrence:
+// D_i' = -1 + b_{i-1}^2 D_{i-1}' / D_{i-1}^2 (kept rows: D_k' = -1 - sum ar_i^2 / (a_i - x)^2), so
+// D' <= -1 and the result lies in (0, 1]. At the largest eigenvalue every earlier pivot is negative
+// (the leading blocks lie below it), so the recurrence meets no small pivots unless T splits.
+__device__ inline double last_component_sq(const Proj& T, double x, double pivmin) {
+  double acc = 0.0, dacc = 0.0;
+  for (int i = 0; i < T.k; ++i) {
+    double q = T.a[i] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    const double iq = 1.0 / q;
+    const double t = T.ar2[i] * iq;
+    acc += t;
+    dacc += t * iq;
+  }
+  double q = (T.a[T.k] - x) - acc;
+  double dq = -1.0 - dacc;
+  for (int i = T.k + 1; i < T.m; ++i) {
+    if (fabs(q) < pivmin) q = -pivmin;
+    const double iq = 1.0 / q;
+    const double t = T.b2[i - 1] * iq;
+    dq = -1.0 + t * iq * dq;
+    q = (T.a[i] - x) - t;
+  }
+  return -1.0 / dq;
+}
+
 // Inverse iteration for the eigenvector of theta (one thread). The kept rows are eliminated first
 // (s_i = (r_i - ar_i s_k) / (a_i - theta)), which leaves a tridiagonal system on rows k..m-1 with a
 // modified first pivot; that system is solved by LU with partial pivoting (row interchanges create
-// a second superdiagonal). Scratch: d, dl, du, du2, den, rhs (m doubles each) and piv (m ints).
-__device__ void inverse_iteration(const Proj& T, double theta, double tiny, int seed, double* d, double* dl,
-                                  double* du, double* du2, double* den, double* rhs, int* piv, double* vec) {
+// a second superdiagonal). The recurrences carry their running values in registers and the pivots
+// are stored as reciprocals, so the three solves have no divisions and no LDS round trips on their
+// dependency chains. Scratch: d, dl, du, du2, den, rhs (m doubles each) and piv (m ints).
+__device__ void inverse_iteration(const Proj& T, double theta, double tiny, int seed, double* __restrict__ d,
+                                  double* __restrict__ dl, double* __restrict__ du, double* __restrict__ du2,
+                                  double* __restrict__ den, double* __restrict__ rhs, int* __restrict__ piv,
+                                  double* __restrict__ vec) {
   const int m = T.m, k = T.k, mt = m - k;
   double corr = 0.0;
   for (int i = 0; i < k; ++i) {
     double q = T.a[i] - theta;
     if (fabs(q) < tiny) q = q < 0.0 ? -tiny : tiny;
-    den[i] = q;
-    corr += T.ar2[i] / q;
+    const double iq = 1.0 / q;
+    den[i] = iq;
+    corr += T.ar2[i] * iq;
   }
-  for (int t = 0; t < mt; ++t) {
-    d[t] = T.a[k + t] - theta;
-    if (t < mt - 1) {
-      dl[t] = T.b[k + t];
-      du[t] = T.b[k + t];
-      du2[t] = 0.0;
-    }
-    piv[t] = t;
-  }
-  d[0] -= corr;
+  // LU of the tridiagonal part; dc/uc are the current row's diagonal and superdiagonal.
+  double dc = (T.a[k] - theta) - corr;
+  double uc = mt > 1 ? T.b[k] : 0.0;
   for (int t = 0; t < mt - 1; ++t) {
-    if (fabs(d[t]) >= fabs(dl[t])) {
-      if (d[t] == 0.0) d[t] = tiny;
-      const double f = dl[t] / d[t];
+    const double l = T.b[k + t];
+    double dn = T.a[k + t + 1] - theta;
+    double un = t < mt - 2 ? T.b[k + t + 1] : 0.0;
+    if (fabs(dc) >= fabs(l)) {
+      if (dc == 0.0) dc = tiny;
+      const double f = l / dc;
+      d[t] = dc;
       dl[t] = f;
-      d[t + 1] -= f * du[t];
+      du[t] = uc;
       du2[t] = 0.0;
       piv[t] = t;
+      dn -= f * uc;
     } else {
-      const double f = d[t] / dl[t];
-      d[t] = dl[t];
+      const double f = dc / l;
+      d[t] = l;
       dl[t] = f;
-      const double tmp = du[t];
-      du[t] = d[t + 1];
-      d[t + 1] = tmp - f * d[t + 1];
-      if (t < mt - 2) {
-        du2[t] = du[t + 1];
-        du[t + 1] = -f * du[t + 1];
-      }
+      du[t] = dn;
+      du2[t] = un;
       piv[t] = t + 1;
+      dn = uc - f * dn;
+      un = -f * un;
     }
+    dc = dn;
+    uc = un;
   }
-  if (d[mt - 1] == 0.0) d[mt - 1] = tiny;
+  if (dc == 0.0) dc = tiny;
+  d[mt - 1] = dc;
+  for (int t = 0; t < mt; ++t) d[t] = 1.0 / d[t];
   for (int i = 0; i < m; ++i) vec[i] = 1.0 + 0.01 * (double)((i + 3 * seed) % 7) + 0.001 * (double)seed;
   for (int it = 0; it < 3; ++it) {
-    for (int t = 0; t < mt; ++t) rhs[t] = vec[k + t];
-    for (int i = 0; i < k; ++i) rhs[0] -= T.ar[i] * vec[i] / den[i];
+    double cur = vec[k];
+    for (int i = 0; i < k; ++i) cur -= T.ar[i] * vec[i] * den[i];
     for (int t = 0; t < mt - 1; ++t) {
+      const double nxt = vec[k + t + 1];
       if (piv[t] == t) {
-        rhs[t + 1] -= dl[t] * rhs[t];
+        rhs[t] = cur;
+        cur = nxt - dl[t] * cur;
       } else {
-        const double tmp = rhs[t];
-        rhs[t] = rhs[t + 1];
-        rhs[t + 1] = tmp - dl[t] * rhs[t];
+        rhs[t] = nxt;
+        cur = cur - dl[t] * nxt;
       }
     }
-    rhs[mt - 1] /= d[mt - 1];
-    if (mt > 1) rhs[mt - 2] = (rhs[mt - 2] - du[mt - 2] * rhs[mt - 1]) / d[mt - 2];
-    for (int t = mt - 3; t >= 0; --t) rhs[t] = (rhs[t] - du[t] * rhs[t + 1] - du2[t] * rhs[t + 2]) / d[t];
-    for (int i = 0; i < k; ++i) vec[i] = (vec[i] - T.ar[i] * rhs[0]) / den[i];
-    for (int t = 0; t < mt; ++t) vec[k + t] = rhs[t];
+    double x1 = cur * d[mt - 1], x2 = 0.0;
+    vec[k + mt - 1] = x1;
+    for (int t = mt - 2; t >= 0; --t) {
+      const double x0 = (rhs[t] - du[t] * x1 - du2[t] * x2) * d[t];
+      vec[k + t] = x0;
+      x2 = x1;
+      x1 = x0;
+    }
+    for (int i = 0; i < k; ++i) vec[i] = (vec[i] - T.ar[i] * x1) * den[i];
     double mx = 0.0;
     for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(vec[i]));
     if (!(mx > 0.0) || !isfinite(mx)) break;
+    const double imx = 1.0 / mx;
     double nn = 0.0;
     for (int i = 0; i < m; ++i) {
-      vec[i] /= mx;
-      nn += vec[i] * vec[i];
+      const double v = vec[i] * imx;
+      vec[i] = v;
+      nn += v * v;
     }
-    nn = sqrt(nn);
-    for (int i = 0; i < m; ++i) vec[i] /= nn;
+    const double inn = 1.0 / sqrt(nn);
+    for (int i = 0; i < m; ++i) vec[i] *= inn;
   }
 }
 
@@ -376,17 +449,16 @@ __device__ double largest_eigenvalue(const Proj& T, const double* bounds) {
 
 template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n, const T* __restrict__ s,
-                                                           T* __restrict__ V, T* __restrict__ Z,
+                                                           T* __restrict__ V, int64_t ldv, T* __restrict__ Z,
                                                            const T* __restrict__ W) {
   __shared__ double coef[SA_MAX_NCV + 1];
   __shared__ double red[SA_WAVES_PER_BLOCK];
-  __shared__ bool last;
-  sa_state* st = state_of(work);
+  const sa_state* st = state_of(work);
   if (st->status != SA_RUNNING) return;
   const int j = st->j;
   for (int m = threadIdx.x; m <= j; m += blockDim.x) coef[m] = st->h[m] * st->sigma[m];
   __syncthreads();
-  T* __restrict__ vn = V + (int64_t)(j + 1) * n;
+  T* __restrict__ vn = V + (int64_t)(j + 1) * ldv;
   double acc = 0.0;
   int64_t lo, hi;
   block_rows(n, lo, hi);
@@ -399,11 +471,11 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
     for (; m + 7 <= j; m += 8) {
       T a[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] = V[(int64_t)(m + u) * n + r];
+      for (int u = 0; u < 8; ++u) a[u] = V[(int64_t)(m + u) * ldv + r];
 #pragma unroll
       for (int u = 0; u < 8; ++u) wv[u] -= coef[m + u] * (double)a[u];
     }
-    for (; m <= j; ++m) wv[0] -= coef[m] * (double)V[(int64_t)m * n + r];
+    for (; m <= j; ++m) wv[0] -= coef[m] * (double)V[(int64_t)m * ldv + r];
     const double w = ((wv[0] + wv[1]) + (wv[2] + wv[3])) + ((wv[4] + wv[5]) + (wv[6] + wv[7]));
     const T wt = (T)w;
     vn[r] = wt;
@@ -413,14 +485,15 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
   double* part = partials_of(work);
   const double bs = block_sum(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  if (!last_block_arrive(&st->counters[0], gridDim.x, &last)) return;
-  double tot = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) tot += load_coherent(&part[b]);
-  tot = block_sum(tot, red);
-  __shared__ double beta_sh;
-  if (threadIdx.x == 0) beta_sh = sqrt(tot);
-  __syncthreads();
-  const double beta = beta_sh;
+}
+
+// Phase 2 of the Gram-Schmidt step: beta_j from the G partial squared norms, then the FP64 Ritz solve
+// of the projected matrix and the stopping test, in one workgroup.
+__global__ __launch_bounds__(SA_BLOCK) void lz_orth_finish_kernel(void* work, int G) {
+  sa_state* st = state_of(work);
+  if (st->status != SA_RUNNING) return;
+  const int j = st->j;
+  const double beta = sqrt(sum_partials(partials_of(work), G, 1, 0));
 
   __shared__ double la[SA_MAX_NCV], lb[SA_MAX_NCV], lb2[SA_MAX_NCV], lar[SA_MAX_NCV], lar2[SA_MAX_NCV];
   __shared__ double lvec[SA_MAX_NCV], lwork[6 * SA_MAX_NCV], lbounds[4];
@@ -442,16 +515,10 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
   if (all_finite) {
     for (int i = 0; i < j; ++i) all_finite = all_finite && isfinite(lb[i]) && isfinite(lar[i]);
   }
+  const Proj P{m, k, la, lb, lar, lb2, lar2};
   if (all_finite) {
-    const Proj P{m, k, la, lb, lar, lb2, lar2};
     proj_bounds(P, lb2, lar2, lbounds);
     theta = largest_eigenvalue(P, lbounds);
-    if (threadIdx.x == 0) {
-      const double tiny = 2.220446049250313e-16 * fmax(lbounds[3], 1e-300);
-      inverse_iteration(P, theta, tiny, 0, lwork, lwork + SA_MAX_NCV, lwork + 2 * SA_MAX_NCV, lwork + 3 * SA_MAX_NCV,
-                        lwork + 4 * SA_MAX_NCV, lwork + 5 * SA_MAX_NCV, lpiv, lvec);
-    }
-    __syncthreads();
     solved = true;
   }
   if (threadIdx.x == 0) {
@@ -460,8 +527,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
     if (!solved) {
       st->status = SA_BROKEN;
     } else {
-      for (int i = 0; i < m; ++i) st->ritz[i] = lvec[i];
-      const double res = beta * fabs(lvec[m - 1]);
+      const double res = beta * sqrt(last_component_sq(P, theta, lbounds[2]));
       st->theta = theta;
       st->res = res;
       int status = SA_RUNNING;
@@ -475,6 +541,13 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
         if (res <= st->tol) status = SA_CONVERGED;
         else if (st->steps >= st->maxiter) status = SA_MAXITER;
         else if (m >= st->ncv) status = SA_RESTART;
+      }
+      if (status == SA_CONVERGED || status == SA_MAXITER) {
+        // the solve stops here: its Ritz vector's coefficients, for the final combination
+        const double tiny = 2.220446049250313e-16 * fmax(lbounds[3], 1e-300);
+        inverse_iteration(P, theta, tiny, 0, lwork, lwork + SA_MAX_NCV, lwork + 2 * SA_MAX_NCV,
+                          lwork + 3 * SA_MAX_NCV, lwork + 4 * SA_MAX_NCV, lwork + 5 * SA_MAX_NCV, lpiv, lvec);
+        for (int i = 0; i < m; ++i) st->ritz[i] = lvec[i];
       }
       st->sigma[j + 1] = beta > 0.0 ? 1.0 / beta : 0.0;
       st->j = m;
@@ -546,28 +619,40 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(void* work) 
   }
   __threadfence_block();
   __syncthreads();
-  if (t == 0) {
+  // Modified Gram-Schmidt of the vectors in order, by wave 0 with two components per lane (m <= 128);
+  // the butterfly sums leave the same value in every lane, so kk stays uniform.
+  __shared__ int lkk;
+  if (t < 64) {
+    const int p0 = t, p1 = t + 64;
     int kk = 0;
     for (int u = 0; u < keep; ++u) {
-      double* v = vecs + (int64_t)u * vstride;
+      const double* v = vecs + (int64_t)u * vstride;
+      double v0 = p0 < m ? v[p0] : 0.0, v1 = p1 < m ? v[p1] : 0.0;
       for (int w = 0; w < kk; ++w) {
         const double* q = vecs + (int64_t)w * vstride;
-        double dot = 0.0;
-        for (int p = 0; p < m; ++p) dot += q[p] * v[p];
-        for (int p = 0; p < m; ++p) v[p] -= dot * q[p];
+        const double q0 = p0 < m ? q[p0] : 0.0, q1 = p1 < m ? q[p1] : 0.0;
+        const double dot = wave_sum(q0 * v0 + q1 * v1);
+        v0 -= dot * q0;
+        v1 -= dot * q1;
       }
-      double nn = 0.0;
-      for (int p = 0; p < m; ++p) nn += v[p] * v[p];
-      nn = sqrt(nn);
+      const double nn = sqrt(wave_sum(v0 * v0 + v1 * v1));
       if (!(nn > 1e-8) || !isfinite(nn)) continue;     // a vector lost in the span of the earlier ones
+      const double inv = 1.0 / nn;
       double* dst = vecs + (int64_t)kk * vstride;
-      for (int p = 0; p < m; ++p) dst[p] = v[p] / nn;
-      st->a[kk] = ltheta[u];
+      if (p0 < m) dst[p0] = v0 * inv;
+      if (p1 < m) dst[p1] = v1 * inv;
+      if (t == 0) st->a[kk] = ltheta[u];
       ++kk;
     }
+    if (t == 0) lkk = kk;
+  }
+  __threadfence_block();
+  __syncthreads();
+  {
+    const int kk = lkk;
     const double beta_last = st->b[m - 1];   // beta of the last step: couples v_{m-1} to v_m
-    for (int u = 0; u < kk; ++u) st->arrow[u] = beta_last * vecs[(int64_t)u * vstride + (m - 1)];
-    st->kept = kk;
+    for (int u = t; u < kk; u += blockDim.x) st->arrow[u] = beta_last * vecs[(int64_t)u * vstride + (m - 1)];
+    if (t == 0) st->kept = kk;
   }
   __syncthreads();
   if (in_lds) {
@@ -578,70 +663,119 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(void* work) 
   }
 }
 
-// Thick restart, part 2: rotate the basis in place. Each thread reads all of its row's basis entries
+// Thick restart, part 2: rotate the basis in place. Each thread reads all of its rows' basis entries
 // before writing any, so rows 0..kept-1 can take the Ritz vectors and row kept the last Lanczos vector.
+// A thread owns 16 bytes of rows (4 floats or 2 doubles) and keeps four basis loads in flight; the
+// coefficients are stored basis-row-major in LDS so one broadcast read serves a whole row group.
 template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(void* work, int64_t n, const T* __restrict__ s,
-                                                                    T* __restrict__ V, T* __restrict__ Z) {
-  __shared__ double coef[SA_MAX_KEEP * SA_MAX_NCV];
-  __shared__ bool last;
-  sa_state* st = state_of(work);
+                                                                    T* __restrict__ V, int64_t ldv,
+                                                                    T* __restrict__ Z) {
+  using VT = typename vec16<T>::type;
+  constexpr int L = vec16<T>::L;
+  __shared__ double coef[SA_MAX_NCV * SA_MAX_KEEP];   // coef[p * SA_MAX_KEEP + u]
+  const sa_state* st = state_of(work);
   if (st->status != SA_RESTART) return;
   const int m = st->j;
   const int kk = st->kept;
   const double* S = restart_of(work);
   for (int q = threadIdx.x; q < kk * m; q += blockDim.x) {
     const int u = q / m, p = q % m;
-    coef[u * SA_MAX_NCV + p] = S[(int64_t)u * SA_MAX_NCV + p] * st->sigma[p];
+    coef[p * SA_MAX_KEEP + u] = S[(int64_t)u * SA_MAX_NCV + p] * st->sigma[p];
   }
   __syncthreads();
   int64_t lo, hi;
   block_rows(n, lo, hi);
-  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
-    double acc[SA_MAX_KEEP];
+  const int64_t nv = hi > lo ? (hi - lo) / L : 0;
+  for (int64_t g = threadIdx.x; g < nv; g += blockDim.x) {
+    const int64_t r = lo + g * L;
+    double acc[SA_MAX_KEEP][L];
 #pragma unroll
-    for (int u = 0; u < SA_MAX_KEEP; ++u) acc[u] = 0.0;
+    for (int u = 0; u < SA_MAX_KEEP; ++u) {
+#pragma unroll
+      for (int l = 0; l < L; ++l) acc[u][l] = 0.0;
+    }
     int p = 0;
     for (; p + 3 < m; p += 4) {
-      T v4[4];
+      VT v[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v4[q] = V[(int64_t)(p + q) * n + r];
+      for (int q = 0; q < 4; ++q) v[q] = *reinterpret_cast<const VT*>(&V[(int64_t)(p + q) * ldv + r]);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double vp = (double)v4[q];
 #pragma unroll
-        for (int u = 0; u < SA_MAX_KEEP; ++u)
-          if (u < kk) acc[u] += coef[u * SA_MAX_NCV + p + q] * vp;
+        for (int u = 0; u < SA_MAX_KEEP; ++u) {
+          if (u < kk) {
+            const double c = coef[(p + q) * SA_MAX_KEEP + u];
+#pragma unroll
+            for (int l = 0; l < L; ++l) acc[u][l] += c * (double)v[q][l];
+          }
+        }
       }
     }
     for (; p < m; ++p) {
-      const double vp = (double)V[(int64_t)p * n + r];
+      const VT v = *reinterpret_cast<const VT*>(&V[(int64_t)p * ldv + r]);
+#pragma unroll
+      for (int u = 0; u < SA_MAX_KEEP; ++u) {
+        if (u < kk) {
+          const double c = coef[p * SA_MAX_KEEP + u];
+#pragma unroll
+          for (int l = 0; l < L; ++l) acc[u][l] += c * (double)v[l];
+        }
+      }
+    }
+    const VT lastv = *reinterpret_cast<const VT*>(&V[(int64_t)m * ldv + r]);
+#pragma unroll
+    for (int u = 0; u < SA_MAX_KEEP; ++u) {
+      if (u < kk) {
+        VT o;
+#pragma unroll
+        for (int l = 0; l < L; ++l) o[l] = (T)acc[u][l];
+        *reinterpret_cast<VT*>(&V[(int64_t)u * ldv + r]) = o;
+      }
+    }
+    *reinterpret_cast<VT*>(&V[(int64_t)kk * ldv + r]) = lastv;
+    VT z = lastv;
+    if (s) z = *reinterpret_cast<const VT*>(&s[r]) * lastv;
+    *reinterpret_cast<VT*>(&Z[r]) = z;
+  }
+  for (int64_t r = lo + nv * L + threadIdx.x; r < hi; r += blockDim.x) {
+    double acc[SA_MAX_KEEP];
+#pragma unroll
+    for (int u = 0; u < SA_MAX_KEEP; ++u) acc[u] = 0.0;
+    for (int p = 0; p < m; ++p) {
+      const double vp = (double)V[(int64_t)p * ldv + r];
 #pragma unroll
       for (int u = 0; u < SA_MAX_KEEP; ++u)
-        if (u < kk) acc[u] += coef[u * SA_MAX_NCV + p] * vp;
+        if (u < kk) acc[u] += coef[p * SA_MAX_KEEP + u] * vp;
     }
-    const T lastv = V[(int64_t)m * n + r];
+    const T lastv = V[(int64_t)m * ldv + r];
 #pragma unroll
     for (int u = 0; u < SA_MAX_KEEP; ++u)
-      if (u < kk) V[(int64_t)u * n + r] = (T)acc[u];
-    V[(int64_t)kk * n + r] = lastv;
+      if (u < kk) V[(int64_t)u * ldv + r] = (T)acc[u];
+    V[(int64_t)kk * ldv + r] = lastv;
     Z[r] = s ? s[r] * lastv : lastv;
   }
-  if (!last_block_arrive(&st->counters[0], gridDim.x, &last)) return;
-  if (threadIdx.x == 0) {
-    const double sig_last = st->sigma[m];
-    for (int u = 0; u < kk; ++u) st->sigma[u] = 1.0;
-    st->sigma[kk] = sig_last;
-    st->j = kk;
-    st->cycles += 1;
-    st->status = SA_RUNNING;
-  }
+}
+
+// The state after the rotation: kept vectors are unit (sigma 1), the last Lanczos vector moves to
+// row kept, and the cycle continues from j = kept.
+__global__ void lz_thick_finish_kernel(void* work) {
+  sa_state* st = state_of(work);
+  if (st->status != SA_RESTART || threadIdx.x != 0) return;
+  const int m = st->j;
+  const int kk = st->kept;
+  const double sig_last = st->sigma[m];
+  for (int u = 0; u < kk; ++u) st->sigma[u] = 1.0;
+  st->sigma[kk] = sig_last;
+  st->j = kk;
+  st->cycles += 1;
+  st->status = SA_RUNNING;
 }
 
 template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void lz_ritz_kernel(void* work, int64_t n, const T* __restrict__ V,
-                                                           const T* __restrict__ y_last, T* __restrict__ y,
-                                                           int32_t only_if) {
+                                                           int64_t ldv, const T* __restrict__ y_last,
+                                                           T* __restrict__ y, int32_t only_if) {
   __shared__ double coef[SA_MAX_NCV];
   sa_state* st = state_of(work);
   if (only_if >= 0 && st->status != only_if) return;
@@ -656,20 +790,16 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_ritz_kernel(void* work, int64_t n
     int i = 0;
     for (; i + 3 < m; i += 4) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v4[q] += coef[i + q] * (double)V[(int64_t)(i + q) * n + r];
+      for (int q = 0; q < 4; ++q) v4[q] += coef[i + q] * (double)V[(int64_t)(i + q) * ldv + r];
     }
-    for (; i < m; ++i) v4[0] += coef[i] * (double)V[(int64_t)i * n + r];
+    for (; i < m; ++i) v4[0] += coef[i] * (double)V[(int64_t)i * ldv + r];
     const double v = (v4[0] + v4[1]) + (v4[2] + v4[3]);
     const T vt = (T)v;
     y[r] = vt;
     acc[0] += (double)vt * (double)vt;
     if (y_last) acc[1] += (double)vt * (double)y_last[r];
   }
-  double out[2];
-  if (grid_reduce<2, RED_SUM>(acc, work, out) && threadIdx.x == 0) {
-    st->r[4] = sqrt(out[0]);
-    st->r[5] = out[1];
-  }
+  block_partials<2, RED_SUM>(acc, work);
 }
 
 static inline int red_grid(int64_t n) { return grid_for(n, SA_BLOCK * 4, SA_RED_GRID); }
@@ -677,20 +807,27 @@ static inline int red_grid(int64_t n) { return grid_for(n, SA_BLOCK * 4, SA_RED_
 // The Gram-Schmidt update reduces a single column, so it can afford more workgroups; the projections
 // keep (j+1) partials per workgroup, so their grid is capped by the partials area (2048 x 136).
 constexpr int SA_ORTH_GRID = 2048;
-constexpr int SA_DOTS_GRID = 1024;
+constexpr int SA_DOTS_GRID = 512;
 
 template <typename T>
 static int lanczos_start(void* work, int64_t n, const T* s, const T* x, const T* d, double pexp, double cx,
-                         const T* y, double cy, T* V, T* Z, int32_t gate, int32_t only_if, hipStream_t stream) {
-  SA_REQUIRE(work && n > 0 && x && V && Z, "bad arguments");
-  hipLaunchKernelGGL(lz_start_kernel<T>, dim3(red_grid(n)), dim3(SA_BLOCK), 0, stream, work, n, s, x, d, (T)pexp, cx,
-                     y, cy, V, Z, gate, only_if);
-  return check_launch("lanczos_start");
+                         const T* y, double cy, T* V, int64_t ldv, T* Z, int32_t gate, int32_t only_if,
+                         hipStream_t stream) {
+  SA_REQUIRE(work && n > 0 && x && V && Z && ldv >= n, "bad arguments");
+  const int G = red_grid(n);
+  hipLaunchKernelGGL(lz_start_kernel<T>, dim3(G), dim3(SA_BLOCK), 0, stream, work, n, s, x, d, (T)pexp, cx, y, cy, V,
+                     Z, gate, only_if);
+  int rc = check_launch("lanczos_start");
+  if (rc) return rc;
+  hipLaunchKernelGGL(lz_start_finish_kernel, dim3(1), dim3(SA_BLOCK), 0, stream, work, G, gate, only_if);
+  return check_launch("lanczos_start_finish");
 }
 
 template <typename T>
-static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, T* Z, T* W, hipStream_t stream) {
-  SA_REQUIRE(A && work && V && Z && W && A->tile_rows, "bad arguments");
+static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, int64_t ldv, T* Z, T* W, hipStream_t stream) {
+  SA_REQUIRE(A && work && V && Z && W && A->tile_rows && ldv >= A->n, "bad arguments");
+  SA_REQUIRE(aligned16(V) && aligned16(W) && ldv % vec16<T>::L == 0,
+             "V and W must be 16-byte aligned and ldv a multiple of 16 bytes");
   const int64_t n = A->n;
   SpmvParams<T> p = params_from<T>(A);
   p.g = Z;
@@ -701,38 +838,56 @@ static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, T* Z, T* 
   if (rc) return rc;
   if (A->npanels > 0) {
     hipLaunchKernelGGL(lz_combine_kernel<T>, dim3(grid_for(n, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0, stream, work,
-                       n, A->npanels, (const T*)A->partial, s, W);
+                       n, A->npanels, (const T*)A->partial, A->pld, s, W);
     rc = check_launch("lanczos_combine");
     if (rc) return rc;
   }
-  const int G = red_grid(n);
-  hipLaunchKernelGGL(lz_dots_kernel<T>, dim3(grid_for(n, SA_BLOCK * 4, SA_DOTS_GRID)), dim3(SA_BLOCK), 0, stream,
-                     work, n, (const T*)V, (const T*)W);
+  const int Gd = grid_for(n, SA_BLOCK * 4, SA_DOTS_GRID);
+  hipLaunchKernelGGL(lz_dots_kernel<T>, dim3(Gd), dim3(SA_BLOCK), 0, stream, work, n, (const T*)V, ldv, (const T*)W);
   rc = check_launch("lanczos_dots");
   if (rc) return rc;
-  hipLaunchKernelGGL(lz_orth_kernel<T>, dim3(grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID)), dim3(SA_BLOCK), 0, stream, work,
-                     n, s, V, Z, (const T*)W);
-  return check_launch("lanczos_orth");
+  hipLaunchKernelGGL(lz_dots_finish_kernel, dim3(1), dim3(SA_BLOCK), 0, stream, work, Gd);
+  rc = check_launch("lanczos_dots_finish");
+  if (rc) return rc;
+  const int Go = grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID);
+  hipLaunchKernelGGL(lz_orth_kernel<T>, dim3(Go), dim3(SA_BLOCK), 0, stream, work, n, s, V, ldv, Z, (const T*)W);
+  rc = check_launch("lanczos_orth");
+  if (rc) return rc;
+  hipLaunchKernelGGL(lz_orth_finish_kernel, dim3(1), dim3(SA_BLOCK), 0, stream, work, Go);
+  return check_launch("lanczos_orth_finish");
 }
 
 template <typename T>
-static int lanczos_thick_restart(void* work, int64_t n, const T* s, T* V, T* Z, hipStream_t stream) {
-  SA_REQUIRE(work && n > 0 && V && Z, "bad arguments");
+static int lanczos_thick_restart(void* work, int64_t n, const T* s, T* V, int64_t ldv, T* Z, hipStream_t stream) {
+  SA_REQUIRE(work && n > 0 && V && Z && ldv >= n, "bad arguments");
+  SA_REQUIRE(aligned16(V) && aligned16(Z) && aligned16(s) && ldv % vec16<T>::L == 0,
+             "V, Z and s must be 16-byte aligned and ldv a multiple of 16 bytes");
   hipLaunchKernelGGL(lz_thick_prepare_kernel, dim3(1), dim3(SA_BLOCK), 0, stream, work);
   int rc = check_launch("lanczos_thick_prepare");
   if (rc) return rc;
   hipLaunchKernelGGL(lz_thick_vectors_kernel<T>, dim3(grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID)), dim3(SA_BLOCK), 0,
-                     stream, work, n, s, V, Z);
-  return check_launch("lanczos_thick_vectors");
+                     stream, work, n, s, V, ldv, Z);
+  rc = check_launch("lanczos_thick_vectors");
+  if (rc) return rc;
+  hipLaunchKernelGGL(lz_thick_finish_kernel, dim3(1), dim3(64), 0, stream, work);
+  return check_launch("lanczos_thick_finish");
 }
 
 template <typename T>
-static int lanczos_ritz(void* work, int64_t n, const T* V, const T* y_last, T* y, int32_t only_if,
+static int lanczos_ritz(void* work, int64_t n, const T* V, int64_t ldv, const T* y_last, T* y, int32_t only_if,
                         hipStream_t stream) {
-  SA_REQUIRE(work && n > 0 && V && y, "bad arguments");
-  hipLaunchKernelGGL(lz_ritz_kernel<T>, dim3(red_grid(n)), dim3(SA_BLOCK), 0, stream, work, n, V, y_last, y,
-                     only_if);
-  return check_launch("lanczos_ritz");
+  SA_REQUIRE(work && n > 0 && V && y && ldv >= n, "bad arguments");
+  const int G = red_grid(n);
+  hipLaunchKernelGGL(lz_ritz_kernel<T>, dim3(G), dim3(SA_BLOCK), 0, stream, work, n, V, ldv, y_last, y, only_if);
+  int rc = check_launch("lanczos_ritz");
+  if (rc) return rc;
+  FinishSpec f{};
+  f.cols = 2;
+  f.only_if = only_if;
+  f.take_sqrt[0] = 1;
+  f.dst[0] = (int32_t)offsetof(sa_state, r[4]);
+  f.dst[1] = (int32_t)offsetof(sa_state, r[5]);
+  return launch_finish(work, G, f, stream);
 }
 
 }  // namespace sa
@@ -750,43 +905,45 @@ int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, doubl
 }
 
 int sa_lanczos_start_f32(void* work, int64_t n, const float* s, const float* x, const float* d, double pexp,
-                         double cx, const float* y, double cy, float* V, float* Z, int32_t gate, int32_t only_if,
-                         hipStream_t stream) {
-  return lanczos_start<float>(work, n, s, x, d, pexp, cx, y, cy, V, Z, gate, only_if, stream);
+                         double cx, const float* y, double cy, float* V, int64_t ldv, float* Z, int32_t gate,
+                         int32_t only_if, hipStream_t stream) {
+  return lanczos_start<float>(work, n, s, x, d, pexp, cx, y, cy, V, ldv, Z, gate, only_if, stream);
 }
 
 int sa_lanczos_start_f64(void* work, int64_t n, const double* s, const double* x, const double* d, double pexp,
-                         double cx, const double* y, double cy, double* V, double* Z, int32_t gate, int32_t only_if,
-                         hipStream_t stream) {
-  return lanczos_start<double>(work, n, s, x, d, pexp, cx, y, cy, V, Z, gate, only_if, stream);
+                         double cx, const double* y, double cy, double* V, int64_t ldv, double* Z, int32_t gate,
+                         int32_t only_if, hipStream_t stream) {
+  return lanczos_start<double>(work, n, s, x, d, pexp, cx, y, cy, V, ldv, Z, gate, only_if, stream);
 }
 
-int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, float* Z, float* W,
+int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, int64_t ldv, float* Z, float* W,
                         hipStream_t stream) {
-  return lanczos_step<float>(A, work, s, V, Z, W, stream);
+  return lanczos_step<float>(A, work, s, V, ldv, Z, W, stream);
 }
 
-int sa_lanczos_step_f64(const sa_csr* A, void* work, const double* s, double* V, double* Z, double* W,
+int sa_lanczos_step_f64(const sa_csr* A, void* work, const double* s, double* V, int64_t ldv, double* Z, double* W,
                         hipStream_t stream) {
-  return lanczos_step<double>(A, work, s, V, Z, W, stream);
+  return lanczos_step<double>(A, work, s, V, ldv, Z, W, stream);
 }
 
-int sa_lanczos_thick_restart_f32(void* work, int64_t n, const float* s, float* V, float* Z, hipStream_t stream) {
-  return lanczos_thick_restart<float>(work, n, s, V, Z, stream);
+int sa_lanczos_thick_restart_f32(void* work, int64_t n, const float* s, float* V, int64_t ldv, float* Z,
+                                 hipStream_t stream) {
+  return lanczos_thick_restart<float>(work, n, s, V, ldv, Z, stream);
 }
 
-int sa_lanczos_thick_restart_f64(void* work, int64_t n, const double* s, double* V, double* Z, hipStream_t stream) {
-  return lanczos_thick_restart<double>(work, n, s, V, Z, stream);
+int sa_lanczos_thick_restart_f64(void* work, int64_t n, const double* s, double* V, int64_t ldv, double* Z,
+                                 hipStream_t stream) {
+  return lanczos_thick_restart<double>(work, n, s, V, ldv, Z, stream);
 }
 
-int sa_lanczos_ritz_f32(void* work, int64_t n, const float* V, const float* y_last, float* y, int32_t only_if,
-                        hipStream_t stream) {
-  return lanczos_ritz<float>(work, n, V, y_last, y, only_if, stream);
+int sa_lanczos_ritz_f32(void* work, int64_t n, const float* V, int64_t ldv, const float* y_last, float* y,
+                        int32_t only_if, hipStream_t stream) {
+  return lanczos_ritz<float>(work, n, V, ldv, y_last, y, only_if, stream);
 }
 
-int sa_lanczos_ritz_f64(void* work, int64_t n, const double* V, const double* y_last, double* y, int32_t only_if,
-                        hipStream_t stream) {
-  return lanczos_ritz<double>(work, n, V, y_last, y, only_if, stream);
+int sa_lanczos_ritz_f64(void* work, int64_t n, const double* V, int64_t ldv, const double* y_last, double* y,
+                        int32_t only_if, hipStream_t stream) {
+  return lanczos_ritz<double>(work, n, V, ldv, y_last, y, only_if, stream);
 }
 
 }  // extern "C"

@@ -9,6 +9,8 @@
 // The flip order is a sequence of colour classes (independent sets). Within a class no two nodes
 // are adjacent, so flipping all improving nodes of the class at once equals flipping them one at a
 // time in any order: the parallel pass is a sequential single-flip descent, bit for bit.
+#include <cstddef>
+
 #include "sa_reduce.hpp"
 
 namespace sa {
@@ -209,8 +211,7 @@ __global__ __launch_bounds__(SA_BLOCK) void colour_round_kernel(int64_t n, const
     }
     __hip_atomic_store(&colour[i], col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  uint64_t out;
-  if (grid_reduce_u64(left, work, &out) && threadIdx.x == 0) state_of(work)->i64[1] = (int64_t)out;
+  block_partials_u64(left, work);
 }
 
 }  // namespace sa
@@ -285,9 +286,17 @@ int sa_ls_extract_f64(int64_t n, int32_t groups, int32_t c, const int8_t* spins,
 
 int sa_colour_round(const sa_csr* A, uint64_t seed, int32_t* colour, void* work, hipStream_t stream) {
   SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && colour && work, "bad arguments");
-  hipLaunchKernelGGL(colour_round_kernel, dim3(grid_for(A->n, SA_BLOCK * 4, SA_RED_GRID)), dim3(SA_BLOCK), 0, stream,
-                     A->n, A->indptr, A->indices, seed, colour, work);
-  return check_launch("colour_round");
+  const int G = grid_for(A->n, SA_BLOCK * 4, SA_RED_GRID);
+  hipLaunchKernelGGL(colour_round_kernel, dim3(G), dim3(SA_BLOCK), 0, stream, A->n, A->indptr, A->indices, seed,
+                     colour, work);
+  int rc = check_launch("colour_round");
+  if (rc) return rc;
+  FinishSpec f{};
+  f.cols = 1;
+  f.only_if = -1;
+  f.op[0] = FIN_SUM_U64;
+  f.dst[0] = (int32_t)offsetof(sa_state, i64[1]);
+  return launch_finish(work, G, f, stream);
 }
 
 }  // extern "C"

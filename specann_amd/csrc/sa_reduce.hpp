@@ -29,14 +29,14 @@ __device__ inline double* restart_of(void* work) { return partials_of(work) + kP
 
 enum RedOp { RED_SUM = 0, RED_MAX = 1 };
 
-// Reduce C per-thread values over the whole grid. Every thread of every block must call this.
-// Results land in out[0..C) (device memory, written by the last block only). Returns true in the
-// last block (after the results are written), so callers may run a serial epilogue there.
+// Grid-wide reductions run in two launches. Phase 1 (any kernel): each workgroup reduces its C
+// values and thread 0 stores them to part[b * C + c] with plain stores. Phase 2 (a one-workgroup
+// finisher launched next on the same stream) reads the partials, which the kernel boundary has made
+// visible, and sums them in workgroup order. No workgroup ever waits on another, and no device-scope
+// fence is issued per workgroup (on gfx950 such a fence writes back the XCD's L2).
 template <int C, int OP>
-__device__ bool grid_reduce(double (&v)[C], void* work, double* out) {
+__device__ void block_partials(double (&v)[C], void* work) {
   __shared__ double red[SA_WAVES_PER_BLOCK];
-  __shared__ bool last;
-  sa_state* st = state_of(work);
   double* part = partials_of(work);
 #pragma unroll
   for (int c = 0; c < C; ++c) {
@@ -53,47 +53,42 @@ __device__ bool grid_reduce(double (&v)[C], void* work, double* out) {
     }
     if (threadIdx.x == 0) part[(int64_t)blockIdx.x * C + c] = x;
   }
-  if (!last_block_arrive(&st->counters[0], gridDim.x, &last)) return false;
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    double x = (OP == RED_SUM) ? 0.0 : -INFINITY;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-      double p = load_coherent(&part[(int64_t)b * C + c]);
-      x = (OP == RED_SUM) ? x + p : fmax(x, p);
-    }
-    if (OP == RED_SUM) {
-      x = block_sum(x, red);
-    } else {
-      x = wave_max(x);
-      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      if (lane == 0) red[wave] = x;
-      __syncthreads();
-      if (threadIdx.x == 0) x = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[c] = x;
-  }
-  __threadfence();
-  __syncthreads();
-  return true;
 }
 
-// Same for wrapping uint64 sums (signatures) and exact int64 counts.
-__device__ inline bool grid_reduce_u64(uint64_t v, void* work, uint64_t* out) {
+__device__ inline void block_partials_u64(uint64_t v, void* work) {
   __shared__ uint64_t red[SA_WAVES_PER_BLOCK];
-  __shared__ bool last;
-  sa_state* st = state_of(work);
   uint64_t* part = reinterpret_cast<uint64_t*>(partials_of(work));
-  uint64_t x = block_sum(v, red);
+  const uint64_t x = block_sum(v, red);
   if (threadIdx.x == 0) part[blockIdx.x] = x;
-  if (!last_block_arrive(&st->counters[0], gridDim.x, &last)) return false;
-  uint64_t acc = 0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) acc += load_coherent(&part[b]);
-  acc = block_sum(acc, red);
-  if (threadIdx.x == 0) *out = acc;
-  __threadfence();
-  __syncthreads();
-  return true;
 }
+
+// Phase 2 helper for one workgroup: sum of part[b * stride + off] over b < G (each thread in
+// workgroup order, then a fixed tree); the result is returned to every thread.
+__device__ inline double sum_partials(const double* part, int G, int stride, int off) {
+  __shared__ double red[SA_WAVES_PER_BLOCK];
+  __shared__ double bc;
+  double x = 0.0;
+  for (int b = threadIdx.x; b < G; b += blockDim.x) x += part[(int64_t)b * stride + off];
+  x = block_sum(x, red);
+  if (threadIdx.x == 0) bc = x;
+  __syncthreads();
+  x = bc;
+  __syncthreads();
+  return x;
+}
+
+// Generic phase 2: up to four columns, each summed (FP64 or wrapping uint64) or maximised, optionally
+// square-rooted, and stored at a byte offset of sa_state. Skipped unless only_if < 0 or the state's
+// status equals only_if (the phase-1 kernel is gated the same way).
+enum FinishOp { FIN_SUM = 0, FIN_MAX = 1, FIN_SUM_U64 = 2 };
+struct FinishSpec {
+  int32_t cols;
+  int32_t only_if;
+  int32_t op[4];
+  int32_t take_sqrt[4];
+  int32_t dst[4];
+};
+
+int launch_finish(void* work, int G, const FinishSpec& spec, hipStream_t stream);
 
 }  // namespace sa

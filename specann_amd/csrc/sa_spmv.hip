@@ -158,7 +158,7 @@ __global__ __launch_bounds__(SA_BLOCK) void spmv_panel_kernel(SpmvParams<T> p) {
   if (t >= nt) return;
   const int32_t r0 = p.ptile_rows[toff + t], r1 = p.ptile_rows[toff + t + 1];
   if (r0 >= r1) return;
-  PanelStore<T> st{p.partial + (int64_t)panel * p.n};
+  PanelStore<T> st{p.partial + (int64_t)panel * p.pld};
   tile_body<T, TILE>(p.pindptr + (int64_t)panel * (p.n + 1), p.pindices, p.pvals, p.g, r0, r1, p.lanes_fixed,
                      prod, red, st);
 }
@@ -166,11 +166,12 @@ __global__ __launch_bounds__(SA_BLOCK) void spmv_panel_kernel(SpmvParams<T> p) {
 // y_r = a x_r + b s_r sum_p partial[p][r] (panel order fixed).
 template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void combine_kernel(int64_t n, int32_t npanels, const T* __restrict__ partial,
-                                                           const T* __restrict__ s, const T* __restrict__ x,
-                                                           double a, double b, T* __restrict__ y) {
+                                                           int64_t pld, const T* __restrict__ s,
+                                                           const T* __restrict__ x, double a, double b,
+                                                           T* __restrict__ y) {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
-    for (int p = 0; p < npanels; ++p) acc += (double)partial[(int64_t)p * n + r];
+    for (int p = 0; p < npanels; ++p) acc += (double)partial[(int64_t)p * pld + r];
     double v = b * (s ? (double)s[r] : 1.0) * acc;
     if (x) v += a * (double)x[r];
     y[r] = (T)v;
@@ -282,6 +283,7 @@ SpmvParams<T> params_from(const sa_csr* A) {
   p.ptile_off = A->ptile_off;
   p.pmax_tiles = A->pmax_tiles;
   p.partial = (T*)A->partial;
+  p.pld = A->pld > 0 ? A->pld : A->n;
   return p;
 }
 
@@ -352,7 +354,7 @@ static int spmv_apply(const sa_csr* A, const T* s, const T* g, double a, const T
   int rc = launch_spmv<T>(p, A->tile_nnz, stream);
   if (rc || A->npanels == 0) return rc;
   hipLaunchKernelGGL(combine_kernel<T>, dim3(grid_for(A->n, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0, stream, A->n,
-                     A->npanels, (const T*)A->partial, s, p.x, a, b, y);
+                     A->npanels, (const T*)A->partial, p.pld, s, p.x, a, b, y);
   return check_launch("spmv_combine");
 }
 

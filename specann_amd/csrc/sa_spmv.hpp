@@ -31,6 +31,7 @@ struct SpmvParams {
   const int64_t* ptile_off;
   int64_t pmax_tiles;
   T* partial;
+  int64_t pld;
 };
 
 template <typename T>

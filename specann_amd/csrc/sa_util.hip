@@ -1,6 +1,7 @@
 // Set-up and vector kernels of libspecann: value preparation, degrees, normalisation, SpMV tiling,
 // per-alpha scaling, norms, the counter-based normal generator, rounding with signature, and the
 // error/version entry points.
+#include <cstddef>
 #include <cstring>
 #include <mutex>
 
@@ -20,6 +21,58 @@ int check_launch(const char* what) {
   }
   return SA_OK;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Phase 2 of the two-phase grid reductions (see sa_reduce.hpp).
+__global__ __launch_bounds__(SA_BLOCK) void finish_kernel(void* work, int G, FinishSpec spec) {
+  __shared__ double red[SA_WAVES_PER_BLOCK];
+  __shared__ uint64_t redu[SA_WAVES_PER_BLOCK];
+  sa_state* st = state_of(work);
+  if (spec.only_if >= 0 && st->status != spec.only_if) return;
+  const double* part = partials_of(work);
+  const uint64_t* partu = reinterpret_cast<const uint64_t*>(part);
+  const int C = spec.cols;
+  for (int c = 0; c < C; ++c) {
+    char* dst = reinterpret_cast<char*>(st) + spec.dst[c];
+    if (spec.op[c] == FIN_SUM_U64) {
+      uint64_t x = 0;
+      for (int b = threadIdx.x; b < G; b += blockDim.x) x += partu[(int64_t)b * C + c];
+      x = block_sum(x, redu);
+      if (threadIdx.x == 0) *reinterpret_cast<uint64_t*>(dst) = x;
+    } else if (spec.op[c] == FIN_MAX) {
+      double x = -INFINITY;
+      for (int b = threadIdx.x; b < G; b += blockDim.x) x = fmax(x, part[(int64_t)b * C + c]);
+      x = wave_max(x);
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      if (lane == 0) red[wave] = x;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        x = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        *reinterpret_cast<double*>(dst) = spec.take_sqrt[c] ? sqrt(x) : x;
+      }
+      __syncthreads();
+    } else {
+      double x = 0.0;
+      for (int b = threadIdx.x; b < G; b += blockDim.x) x += part[(int64_t)b * C + c];
+      x = block_sum(x, red);
+      if (threadIdx.x == 0) *reinterpret_cast<double*>(dst) = spec.take_sqrt[c] ? sqrt(x) : x;
+    }
+  }
+}
+
+int launch_finish(void* work, int G, const FinishSpec& spec, hipStream_t stream) {
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(SA_BLOCK), 0, stream, work, G, spec);
+  return check_launch("finish");
+}
+
+static FinishSpec finish_spec(int cols, int only_if = -1) {
+  FinishSpec f{};
+  f.cols = cols;
+  f.only_if = only_if;
+  return f;
+}
+
+#define SA_STATE_OFF(field) ((int32_t)offsetof(sa_state, field))
 
 // ---------------------------------------------------------------------------------------------
 template <typename T>
@@ -49,7 +102,7 @@ __global__ __launch_bounds__(SA_BLOCK) void abs_degree_kernel(int64_t n, const i
     deg[r] = dv;
     mx[0] = fmax(mx[0], (double)dv);
   }
-  grid_reduce<1, RED_MAX>(mx, work, &state_of(work)->r[0]);
+  block_partials<1, RED_MAX>(mx, work);
 }
 
 template <typename T>
@@ -91,49 +144,28 @@ __global__ __launch_bounds__(SA_BLOCK) void alpha_scaling_kernel(int64_t n, cons
     acc[0] += (double)dv;
     acc[1] = fmax(acc[1], (double)pow(dr, one_minus_alpha));
   }
-  // One handshake for both results: column 0 is summed, column 1 is maximised.
+  // Column 0 is summed, column 1 is maximised (phase 2 on the host side of the launch).
+  double sm[1] = {acc[0]}, mx[1] = {acc[1]};
   __shared__ double red[SA_WAVES_PER_BLOCK];
-  __shared__ bool last;
-  sa_state* st = state_of(work);
   double* part = partials_of(work);
-  double bs = block_sum(acc[0], red);
-  double bm = wave_max(acc[1]);
+  const double bs = block_sum(sm[0], red);
+  double bm = wave_max(mx[0]);
   {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) red[wave] = bm;
     __syncthreads();
     if (threadIdx.x == 0) bm = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    __syncthreads();
   }
   if (threadIdx.x == 0) {
     part[2 * (int64_t)blockIdx.x] = bs;
     part[2 * (int64_t)blockIdx.x + 1] = bm;
-  }
-  if (!last_block_arrive(&st->counters[0], gridDim.x, &last)) return;
-  double xs = 0.0, xm = -INFINITY;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-    xs += load_coherent(&part[2 * (int64_t)b]);
-    xm = fmax(xm, load_coherent(&part[2 * (int64_t)b + 1]));
-  }
-  xs = block_sum(xs, red);
-  xm = wave_max(xm);
-  {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) red[wave] = xm;
-    __syncthreads();
-    if (threadIdx.x == 0) xm = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    st->r[1] = xs;
-    st->r[2] = xm;
   }
 }
 
 template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void norm_dot_kernel(int64_t n, const T* __restrict__ x,
                                                             const T* __restrict__ y, double a, double b,
-                                                            int32_t slot, void* work) {
+                                                            void* work) {
   double acc[2] = {0.0, 0.0};
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     const double xv = (double)x[r];
@@ -142,12 +174,7 @@ __global__ __launch_bounds__(SA_BLOCK) void norm_dot_kernel(int64_t n, const T* 
     acc[0] += u * u;
     acc[1] += xv * yv;
   }
-  sa_state* st = state_of(work);
-  double out[2];
-  if (grid_reduce<2, RED_SUM>(acc, work, out) && threadIdx.x == 0) {
-    st->r[slot] = sqrt(out[0]);
-    st->r[slot + 1] = out[1];
-  }
+  block_partials<2, RED_SUM>(acc, work);
 }
 
 template <typename T>
@@ -197,7 +224,7 @@ __global__ __launch_bounds__(SA_BLOCK) void round_kernel(int64_t n, const T* __r
       hsum += splitmix64(word ^ splitmix64((uint64_t)w));
     }
   }
-  grid_reduce_u64(hsum, work, &state_of(work)->hash);
+  block_partials_u64(hsum, work);
 }
 
 __global__ __launch_bounds__(SA_BLOCK) void bits_compare_kernel(int64_t nwords, const uint64_t* __restrict__ a,
@@ -205,9 +232,24 @@ __global__ __launch_bounds__(SA_BLOCK) void bits_compare_kernel(int64_t nwords, 
   uint64_t diff = 0;
   for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < nwords; w += (int64_t)gridDim.x * blockDim.x)
     diff += (a[w] != b[w]);
-  uint64_t out;
-  sa_state* st = state_of(work);
-  if (grid_reduce_u64(diff, work, &out) && threadIdx.x == 0) st->i64[0] = (int64_t)out;
+  block_partials_u64(diff, work);
+}
+
+static int finish_alpha_scaling(void* work, int G, hipStream_t stream) {
+  FinishSpec f = finish_spec(2);
+  f.op[0] = FIN_SUM;
+  f.op[1] = FIN_MAX;
+  f.dst[0] = SA_STATE_OFF(r[1]);
+  f.dst[1] = SA_STATE_OFF(r[2]);
+  return launch_finish(work, G, f, stream);
+}
+
+static int finish_norm_dot(void* work, int G, int slot, hipStream_t stream) {
+  FinishSpec f = finish_spec(2);
+  f.take_sqrt[0] = 1;
+  f.dst[0] = SA_STATE_OFF(r[0]) + 8 * slot;
+  f.dst[1] = SA_STATE_OFF(r[0]) + 8 * (slot + 1);
+  return launch_finish(work, G, f, stream);
 }
 
 }  // namespace sa
@@ -258,16 +300,28 @@ int sa_prepare_values_f64(int64_t nnz, const double* raw, double* vals, int32_t*
 
 int sa_abs_degree_f32(const sa_csr* A, float* deg, void* work, hipStream_t stream) {
   SA_REQUIRE(A && A->n > 0 && A->indptr && A->vals && deg && work, "bad arguments");
-  hipLaunchKernelGGL(abs_degree_kernel<float>, dim3(SA_RGRID(A->n)), dim3(SA_BLOCK), 0, stream, A->n, A->indptr,
+  const int G = SA_RGRID(A->n);
+  hipLaunchKernelGGL(abs_degree_kernel<float>, dim3(G), dim3(SA_BLOCK), 0, stream, A->n, A->indptr,
                      (const float*)A->vals, deg, work);
-  return check_launch("abs_degree_f32");
+  int rc = check_launch("abs_degree_f32");
+  if (rc) return rc;
+  FinishSpec f = finish_spec(1);
+  f.op[0] = FIN_MAX;
+  f.dst[0] = SA_STATE_OFF(r[0]);
+  return launch_finish(work, G, f, stream);
 }
 
 int sa_abs_degree_f64(const sa_csr* A, double* deg, void* work, hipStream_t stream) {
   SA_REQUIRE(A && A->n > 0 && A->indptr && A->vals && deg && work, "bad arguments");
-  hipLaunchKernelGGL(abs_degree_kernel<double>, dim3(SA_RGRID(A->n)), dim3(SA_BLOCK), 0, stream, A->n, A->indptr,
+  const int G = SA_RGRID(A->n);
+  hipLaunchKernelGGL(abs_degree_kernel<double>, dim3(G), dim3(SA_BLOCK), 0, stream, A->n, A->indptr,
                      (const double*)A->vals, deg, work);
-  return check_launch("abs_degree_f64");
+  int rc = check_launch("abs_degree_f64");
+  if (rc) return rc;
+  FinishSpec f = finish_spec(1);
+  f.op[0] = FIN_MAX;
+  f.dst[0] = SA_STATE_OFF(r[0]);
+  return launch_finish(work, G, f, stream);
 }
 
 int sa_normalize_f32(int64_t nnz, float* vals, int64_t n, const float* deg, float* d, double c, hipStream_t stream) {
@@ -298,32 +352,43 @@ int sa_csr_tiles(int64_t n, const int32_t* indptr, int32_t tile_nnz, int32_t* ti
 int sa_alpha_scaling_f32(int64_t n, const float* d, double alpha, double clamp, float* s, void* work,
                          hipStream_t stream) {
   SA_REQUIRE(n > 0 && d && s && work, "bad arguments");
-  hipLaunchKernelGGL(alpha_scaling_kernel<float>, dim3(SA_RGRID(n)), dim3(SA_BLOCK), 0, stream, n, d, (float)alpha,
+  const int G = SA_RGRID(n);
+  hipLaunchKernelGGL(alpha_scaling_kernel<float>, dim3(G), dim3(SA_BLOCK), 0, stream, n, d, (float)alpha,
                      (float)(1.0 - alpha), (float)clamp, s, work);
-  return check_launch("alpha_scaling_f32");
+  int rc = check_launch("alpha_scaling_f32");
+  if (rc) return rc;
+  return finish_alpha_scaling(work, G, stream);
 }
 
 int sa_alpha_scaling_f64(int64_t n, const double* d, double alpha, double clamp, double* s, void* work,
                          hipStream_t stream) {
   SA_REQUIRE(n > 0 && d && s && work, "bad arguments");
-  hipLaunchKernelGGL(alpha_scaling_kernel<double>, dim3(SA_RGRID(n)), dim3(SA_BLOCK), 0, stream, n, d, alpha,
-                     1.0 - alpha, clamp, s, work);
-  return check_launch("alpha_scaling_f64");
+  const int G = SA_RGRID(n);
+  hipLaunchKernelGGL(alpha_scaling_kernel<double>, dim3(G), dim3(SA_BLOCK), 0, stream, n, d, alpha, 1.0 - alpha,
+                     clamp, s, work);
+  int rc = check_launch("alpha_scaling_f64");
+  if (rc) return rc;
+  return finish_alpha_scaling(work, G, stream);
 }
 
 int sa_norm_dot_f32(void* work, int64_t n, const float* x, const float* y, double a, double b, int32_t slot,
                     hipStream_t stream) {
   SA_REQUIRE(work && n > 0 && x && slot >= 0 && slot <= 6, "bad arguments");
-  hipLaunchKernelGGL(norm_dot_kernel<float>, dim3(SA_RGRID(n)), dim3(SA_BLOCK), 0, stream, n, x, y, a, b, slot, work);
-  return check_launch("norm_dot_f32");
+  const int G = SA_RGRID(n);
+  hipLaunchKernelGGL(norm_dot_kernel<float>, dim3(G), dim3(SA_BLOCK), 0, stream, n, x, y, a, b, work);
+  int rc = check_launch("norm_dot_f32");
+  if (rc) return rc;
+  return finish_norm_dot(work, G, slot, stream);
 }
 
 int sa_norm_dot_f64(void* work, int64_t n, const double* x, const double* y, double a, double b, int32_t slot,
                     hipStream_t stream) {
   SA_REQUIRE(work && n > 0 && x && slot >= 0 && slot <= 6, "bad arguments");
-  hipLaunchKernelGGL(norm_dot_kernel<double>, dim3(SA_RGRID(n)), dim3(SA_BLOCK), 0, stream, n, x, y, a, b, slot,
-                     work);
-  return check_launch("norm_dot_f64");
+  const int G = SA_RGRID(n);
+  hipLaunchKernelGGL(norm_dot_kernel<double>, dim3(G), dim3(SA_BLOCK), 0, stream, n, x, y, a, b, work);
+  int rc = check_launch("norm_dot_f64");
+  if (rc) return rc;
+  return finish_norm_dot(work, G, slot, stream);
 }
 
 int sa_scale_copy_f32(int64_t n, const float* x, double scale, float* out, hipStream_t stream) {
@@ -355,24 +420,40 @@ int sa_round_f32(void* work, int64_t n, const float* y, double scale, float* y_l
                  hipStream_t stream) {
   SA_REQUIRE(work && n > 0 && y && bits, "bad arguments");
   const int64_t nwords = (n + 63) / 64;
-  hipLaunchKernelGGL(round_kernel<float>, dim3(grid_for(nwords, SA_WAVES_PER_BLOCK * 4, SA_RED_GRID)),
-                     dim3(SA_BLOCK), 0, stream, n, y, scale, y_last, bits, work);
-  return check_launch("round_f32");
+  const int G = grid_for(nwords, SA_WAVES_PER_BLOCK * 4, SA_RED_GRID);
+  hipLaunchKernelGGL(round_kernel<float>, dim3(G), dim3(SA_BLOCK), 0, stream, n, y, scale, y_last, bits, work);
+  int rc = check_launch("round_f32");
+  if (rc) return rc;
+  FinishSpec f = finish_spec(1);
+  f.op[0] = FIN_SUM_U64;
+  f.dst[0] = SA_STATE_OFF(hash);
+  return launch_finish(work, G, f, stream);
 }
 
 int sa_round_f64(void* work, int64_t n, const double* y, double scale, double* y_last, uint64_t* bits,
                  hipStream_t stream) {
   SA_REQUIRE(work && n > 0 && y && bits, "bad arguments");
   const int64_t nwords = (n + 63) / 64;
-  hipLaunchKernelGGL(round_kernel<double>, dim3(grid_for(nwords, SA_WAVES_PER_BLOCK * 4, SA_RED_GRID)),
-                     dim3(SA_BLOCK), 0, stream, n, y, scale, y_last, bits, work);
-  return check_launch("round_f64");
+  const int G = grid_for(nwords, SA_WAVES_PER_BLOCK * 4, SA_RED_GRID);
+  hipLaunchKernelGGL(round_kernel<double>, dim3(G), dim3(SA_BLOCK), 0, stream, n, y, scale, y_last, bits, work);
+  int rc = check_launch("round_f64");
+  if (rc) return rc;
+  FinishSpec f = finish_spec(1);
+  f.op[0] = FIN_SUM_U64;
+  f.dst[0] = SA_STATE_OFF(hash);
+  return launch_finish(work, G, f, stream);
 }
 
 int sa_bits_compare(void* work, int64_t nwords, const uint64_t* a, const uint64_t* b, hipStream_t stream) {
   SA_REQUIRE(work && nwords > 0 && a && b, "bad arguments");
-  hipLaunchKernelGGL(bits_compare_kernel, dim3(SA_RGRID(nwords)), dim3(SA_BLOCK), 0, stream, nwords, a, b, work);
-  return check_launch("bits_compare");
+  const int G = SA_RGRID(nwords);
+  hipLaunchKernelGGL(bits_compare_kernel, dim3(G), dim3(SA_BLOCK), 0, stream, nwords, a, b, work);
+  int rc = check_launch("bits_compare");
+  if (rc) return rc;
+  FinishSpec f = finish_spec(1);
+  f.op[0] = FIN_SUM_U64;
+  f.dst[0] = SA_STATE_OFF(i64[0]);
+  return launch_finish(work, G, f, stream);
 }
 
 }  // extern "C"

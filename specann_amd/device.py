@@ -187,7 +187,9 @@ class DeviceProblem:
             call("sa_panel_tiles", n, ctypes.c_void_p(self.pindptr.data_ptr() + 4 * p * (n + 1)), self.tile_nnz,
                  ctypes.c_void_p(self.ptile_rows.data_ptr() + 4 * int(off[p])), ntiles[p], s)
         self.ptile_off = torch.from_numpy(off).to(dev)
-        self.partial = torch.empty((npanels, n), dtype=self.dtype, device=dev)
+        # Panel rows padded like the Lanczos basis (row starts off a common HBM channel offset).
+        self.pld = _lib.padded_ld(n, self.dtype.itemsize)
+        self.partial = torch.empty((npanels, self.pld), dtype=self.dtype, device=dev)
         self.npanels = npanels
         c = self.csr
         c.npanels = npanels
@@ -198,6 +200,7 @@ class DeviceProblem:
         c.ptile_off = self.ptile_off.data_ptr()
         c.pmax_tiles = int(max(ntiles))
         c.partial = self.partial.data_ptr()
+        c.pld = self.pld
 
     def time_spmv(self, reps: int = 5) -> float:
         """Average seconds of one fused SpMV application (the Lanczos step's launch, plus the panel

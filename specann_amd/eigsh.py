@@ -59,7 +59,11 @@ class LanczosSolver:
         # vector alone, the reference eigsh's k = 1 thick restart.
         self.keep = max(1, min(_lib.SA_MAX_KEEP, self.ncv // 2 if keep is None else int(keep), self.ncv - 1))
         dev, dt = prob.device, prob.dtype
-        self.V = torch.empty((self.ncv + 1, self.n), dtype=dt, device=dev)
+        # Basis rows padded so that consecutive rows do not start at the same HBM channel offset
+        # (n = 2^k would put every row start in one channel group); V is the n-wide view.
+        self.ldv = _lib.padded_ld(self.n, 4 if dt == torch.float32 else 8)
+        self.Vbuf = torch.empty((self.ncv + 1, self.ldv), dtype=dt, device=dev)
+        self.V = self.Vbuf[:, : self.n]
         self.Z = torch.empty(self.n, dtype=dt, device=dev)
         self.W = torch.empty(self.n, dtype=dt, device=dev)
         self.Y = torch.empty(self.n, dtype=dt, device=dev)
@@ -75,18 +79,18 @@ class LanczosSolver:
               cx: float = 1.0, y: torch.Tensor | None = None, cy: float = 0.0, gate: int = 0,
               only_if: int = -1) -> None:
         call(f"sa_lanczos_start_{self.sfx}", self.work.ptr, self.n, ptr(s), ptr(x), ptr(d), float(pexp), float(cx),
-             ptr(y), float(cy), ptr(self.V), ptr(self.Z), int(gate), int(only_if), stream())
+             ptr(y), float(cy), ptr(self.V), self.ldv, ptr(self.Z), int(gate), int(only_if), stream())
 
     def _step(self, s: torch.Tensor | None) -> None:
-        call(f"sa_lanczos_step_{self.sfx}", self.prob.csr_ref, self.work.ptr, ptr(s), ptr(self.V), ptr(self.Z),
-             ptr(self.W), stream())
+        call(f"sa_lanczos_step_{self.sfx}", self.prob.csr_ref, self.work.ptr, ptr(s), ptr(self.V), self.ldv,
+             ptr(self.Z), ptr(self.W), stream())
 
     def thick_restart(self, s: torch.Tensor | None) -> None:
-        call(f"sa_lanczos_thick_restart_{self.sfx}", self.work.ptr, self.n, ptr(s), ptr(self.V), ptr(self.Z),
-             stream())
+        call(f"sa_lanczos_thick_restart_{self.sfx}", self.work.ptr, self.n, ptr(s), ptr(self.V), self.ldv,
+             ptr(self.Z), stream())
 
     def ritz(self, y_last: torch.Tensor | None = None, only_if: int = -1) -> None:
-        call(f"sa_lanczos_ritz_{self.sfx}", self.work.ptr, self.n, ptr(self.V), ptr(y_last), ptr(self.Y),
+        call(f"sa_lanczos_ritz_{self.sfx}", self.work.ptr, self.n, ptr(self.V), self.ldv, ptr(y_last), ptr(self.Y),
              int(only_if), stream())
 
     # -- the solve ------------------------------------------------------------------------------
