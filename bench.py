@@ -76,6 +76,39 @@ def spmv_roofline(prob, reps=30):
     return sec, alg
 
 
+def measured_traffic(workload, dtype, layout):
+    """HBM bytes per SpMV application measured by the FETCH_SIZE / WRITE_SIZE counter passes committed
+    under profiles/ (scaled by a copy kernel), if they were taken on this workload and layout."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"spmv_traffic_{workload}_{dtype}.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("layout") != {"npanels": layout["npanels"], "tile_nnz": layout["tile_nnz"]}:
+        return None
+    return float(rec["traffic"])
+
+
+def copy_calibration(n=1 << 26, reps=10):
+    """A plain copy of n floats (sa_scale_copy_f32 with scale 1: 4n bytes read, 4n written), the known-size
+    kernel the FETCH_SIZE / WRITE_SIZE counters are scaled by."""
+    import torch
+    from specann_amd._lib import call
+    from specann_amd.device import ptr, stream
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    call("sa_scale_copy_f32", n, ptr(x), 1.0, ptr(y), stream())
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        call("sa_scale_copy_f32", n, ptr(x), 1.0, ptr(y), stream())
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps, 8 * n
+
+
 def cpu_operator_sample(J, seconds=12.0):
     """Seconds per application of the reference's CPU operator S_rho = rho I - N (scipy, FP32)."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
@@ -103,7 +136,10 @@ def main(argv=None) -> int:
     ap.add_argument("--alphas", type=int, default=128)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--spmv-only", action="store_true", help="only the SpMV timing loop (for profiling)")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the SpMV timing loop and a copy kernel of known size (for counter profiling)")
+    ap.add_argument("--layout", default=None, metavar="NPANELS,TILE",
+                    help="fixed SpMV layout instead of the autotuner's choice, e.g. 8,1024")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -119,15 +155,24 @@ def main(argv=None) -> int:
     J = make_graph(w)
     log(f"graph {args.workload}: n={J.shape[0]} nnz={J.nnz} generated in {time.time() - t0:.1f}s")
     prob32 = DeviceProblem(J, torch.float32)
-    tune32 = prob32.autotune()
+    if args.layout:
+        npanels, tile = (int(v) for v in args.layout.split(","))
+        prob32.set_tiling(tile)
+        if npanels:
+            prob32.build_panels(npanels)
+        tune32 = {"choice": {"npanels": npanels, "tile_nnz": tile, "seconds": None}, "tried": []}
+    else:
+        tune32 = prob32.autotune()
     torch.cuda.synchronize()
-    log(f"uploaded FP32 problem; autotuned SpMV: {tune32['choice']} of {tune32['tried']}")
+    log(f"uploaded FP32 problem; SpMV layout: {tune32['choice']} of {tune32['tried']}")
 
     spmv_sec, spmv_bytes = spmv_roofline(prob32)
     achieved = spmv_bytes / spmv_sec / 1e9
     log(f"spmv f32: {spmv_sec * 1e6:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
     if args.spmv_only:
-        print(json.dumps({"spmv_us": spmv_sec * 1e6, "spmv_gbs": achieved}))
+        copy_sec, copy_bytes = copy_calibration()
+        print(json.dumps({"spmv_us": spmv_sec * 1e6, "spmv_gbs": achieved, "spmv_alg_bytes": spmv_bytes,
+                          "copy_us": copy_sec * 1e6, "copy_bytes": copy_bytes}))
         return 0
 
     def solver(precision):
@@ -192,7 +237,8 @@ def main(argv=None) -> int:
         "best_energy_fp32": e32, "best_energy_fp64": e64, "energy_gap": gap,
         "operator_applications": matvecs, "converged_fraction": conv,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, "f32", tune32["choice"]),
                      "kernel": ("spmv_panel_kernel<float> + combine (XCD-panel fused scaled SpMV)"
                                 if tune32["choice"]["npanels"] else "spmv_tile_kernel<float> (fused scaled CSR SpMV)"),
                      "bytes_per_launch": spmv_bytes, "us_per_launch": spmv_sec * 1e6},

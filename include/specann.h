@@ -263,6 +263,65 @@ int sa_ls_extract_f64(int64_t n, int32_t groups, int32_t c, const int8_t* spins,
  * still uncoloured after the round. */
 int sa_colour_round(const sa_csr* A, uint64_t seed, int32_t* colour, void* work, hipStream_t stream);
 
+/* ---- Eigenvector tangent and its adjoint: deflated MINRES -----------------------------------
+ * No counterpart in the reference (a new predictor beside utils/predictors.py:6-56). For the unit
+ * leading eigenvector y of N(alpha) = s (.) (J/c) (.) s, s = d^(-alpha/2), with eigenvalue lambda:
+ *   dN/dalpha = -1/2 (L N + N L), L = diag(ln d), and with u = 1/2 L y, P = I - y y^T,
+ *   dy/dalpha = P u + 2 lambda z,   z = (N - lambda I)^+ P u,
+ * and the adjoint of a cotangent g is g . dy/dalpha = g . P u + 2 lambda (N - lambda I)^+ P g . u.
+ * Both solves are MINRES (Paige & Saunders 1975) on P (N - shift I) P with a right-hand side P r,
+ * FP32 or FP64 vectors with FP64 scalars, every scalar on the device. The state lives at the start
+ * of its own work buffer (sa_work_bytes bytes); buffers are caller-owned, n entries of T each. */
+typedef struct sa_minres_state {
+  int32_t status;      /* SA_RUNNING, SA_CONVERGED (phibar <= tol * beta1), SA_MAXITER, SA_BROKEN */
+  int32_t itn;         /* iterations done */
+  int32_t maxit;
+  int32_t pending;     /* 1 while the last iteration's update of x is still to be applied */
+  double shift;        /* lambda of P (N - lambda I) P */
+  double tol;
+  double beta1;        /* ||P r|| */
+  double beta;         /* ||r2|| (current raw Lanczos vector) */
+  double oldb;         /* ||r1|| */
+  double cr1;          /* coefficient of r1 in the recurrence (0 in the first iteration) */
+  double alfa;
+  double g;            /* y . (N - shift I) r2: the deflated component */
+  double dbar, epsln, cs, sn, phibar;
+  double pe_oldeps, pe_delta, pe_denom, pe_phi; /* coefficients of the pending x update */
+  double c0;           /* y . r of the right-hand side, then y . x at the end */
+  double rnorm;        /* phibar after the last iteration: ||P r - A x|| */
+} sa_minres_state;
+
+typedef struct sa_minres_bufs {
+  void* r[2];          /* the two latest raw Lanczos vectors of the recurrence */
+  void* w[3];          /* MINRES search directions */
+  void* x;             /* the solution */
+  void* z;             /* s (.) r2, the SpMV's gathered vector */
+  void* q;             /* N r2 */
+} sa_minres_bufs;
+
+size_t sa_minres_state_bytes(void);
+
+/* u = 1/2 ln(max(d, clamp)) (.) y: the tangent's right-hand side. */
+int sa_tangent_rhs_f32(int64_t n, const float* y, const float* d, double clamp, float* u, hipStream_t stream);
+int sa_tangent_rhs_f64(int64_t n, const double* y, const double* d, double clamp, double* u, hipStream_t stream);
+
+/* Start a solve of P (N - shift I) P x = P r (y unit): b = P r, x = 0, beta1 = ||b||. */
+int sa_minres_begin_f32(void* work, int64_t n, const float* s, const float* y, const float* r, double shift,
+                        double tol, int32_t maxit, const sa_minres_bufs* B, hipStream_t stream);
+int sa_minres_begin_f64(void* work, int64_t n, const double* s, const double* y, const double* r, double shift,
+                        double tol, int32_t maxit, const sa_minres_bufs* B, hipStream_t stream);
+
+/* Queue `count` iterations (each: the fused SpMV, one projection pass, one update pass and two
+ * one-workgroup finishers); every launch is skipped once the solve has stopped. */
+int sa_minres_iterate_f32(const sa_csr* A, void* work, const float* s, const float* y, int32_t count,
+                          const sa_minres_bufs* B, hipStream_t stream);
+int sa_minres_iterate_f64(const sa_csr* A, void* work, const double* s, const double* y, int32_t count,
+                          const sa_minres_bufs* B, hipStream_t stream);
+
+/* Apply the pending update of x and project x onto y's complement (x -= (y . x) y). */
+int sa_minres_end_f32(void* work, int64_t n, const float* y, const sa_minres_bufs* B, hipStream_t stream);
+int sa_minres_end_f64(void* work, int64_t n, const double* y, const sa_minres_bufs* B, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -186,6 +186,74 @@ def tangent(Jn, d: np.ndarray, alpha: float, clamp: float = 1e-12):
     return lam, y, yp
 
 
+def minres_deflated(matvec, y: np.ndarray, r: np.ndarray, shift: float, tol: float, maxit: int):
+    """MINRES (Paige & Saunders 1975; the recurrence of scipy.sparse.linalg.minres without a
+    preconditioner) on P (N - shift I) P x = P r, P = I - y y^T with y unit, in FP64, in the order
+    the build's sa_minres kernels use: the deflation is applied to each product,
+    A v = (N - shift I) v - y (y . (N - shift I) v). Returns (x, iterations, phibar, beta1); x is
+    projected onto y's complement at the end. Parity for the build's eigenvector tangent and its
+    adjoint (no counterpart in the reference)."""
+    y = np.asarray(y, dtype=np.float64)
+    r = np.asarray(r, dtype=np.float64)
+    b = r - (y @ r) * y
+    beta1 = float(np.linalg.norm(b))
+    x = np.zeros_like(b)
+    if beta1 == 0.0 or maxit <= 0:
+        return x, 0, beta1, beta1
+    r1 = np.zeros_like(b)
+    r2 = b.copy()
+    w = np.zeros_like(b)
+    w1 = np.zeros_like(b)
+    w2 = np.zeros_like(b)
+    beta, oldb = beta1, 0.0
+    dbar = epsln = 0.0
+    cs, sn = -1.0, 0.0
+    phibar = beta1
+    itn = 0
+    while itn < maxit:
+        q = np.asarray(matvec(r2), dtype=np.float64) - shift * r2
+        g = float(y @ q)
+        av = (q - g * y) / beta
+        v = r2 / beta
+        cr1 = beta / oldb if itn >= 1 else 0.0
+        yv = av - cr1 * r1
+        alfa = float(v @ yv)
+        rn = yv - (alfa / beta) * r2
+        beta_new = float(np.linalg.norm(rn))
+        oldeps = epsln
+        delta = cs * dbar + sn * alfa
+        gbar = sn * dbar - cs * alfa
+        epsln = sn * beta_new
+        dbar = -cs * beta_new
+        gamma = max(float(np.hypot(gbar, beta_new)), np.finfo(np.float64).eps)
+        cs, sn = gbar / gamma, beta_new / gamma
+        phi = cs * phibar
+        phibar = sn * phibar
+        w1, w2 = w2, w
+        w = (v - oldeps * w1 - delta * w2) / gamma
+        x = x + phi * w
+        r1, r2 = r2, rn
+        oldb, beta = beta, beta_new
+        itn += 1
+        if phibar <= tol * beta1 or beta_new == 0.0:
+            break
+    x = x - (y @ x) * y
+    return x, itn, float(phibar), beta1
+
+
+def tangent_minres(Jn, d: np.ndarray, alpha: float, y: np.ndarray, lam: float, steps: int, tol: float = 0.0,
+                   clamp: float = 1e-12):
+    """The tangent dy/dalpha = P u + 2 lambda z with z from `steps` iterations of minres_deflated
+    (u = 1/2 ln max(d, clamp) (.) y), FP64."""
+    s, _, _ = alpha_scaling(d.astype(np.float64), alpha, clamp)
+    Jd = Jn.astype(np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    u = 0.5 * np.log(np.maximum(d.astype(np.float64), clamp)) * y
+    z, itn, phibar, beta1 = minres_deflated(lambda v: s * (Jd @ (s * v)), y, u, lam, tol, steps)
+    pu = u - (y @ u) * y
+    return pu + 2.0 * lam * z, z, itn
+
+
 # ---------------------------------------------------------------------------------------------
 # Rounding, signatures, exact energies
 # ---------------------------------------------------------------------------------------------

@@ -110,6 +110,44 @@ class SaState(ctypes.Structure):
 # Bytes of the leading part of SaState the host reads after each batch of launches.
 STATE_HEAD_BYTES = SaState.a.offset
 
+
+class SaMinresState(ctypes.Structure):
+    _fields_ = [
+        ("status", c_int32),
+        ("itn", c_int32),
+        ("maxit", c_int32),
+        ("pending", c_int32),
+        ("shift", c_double),
+        ("tol", c_double),
+        ("beta1", c_double),
+        ("beta", c_double),
+        ("oldb", c_double),
+        ("cr1", c_double),
+        ("alfa", c_double),
+        ("g", c_double),
+        ("dbar", c_double),
+        ("epsln", c_double),
+        ("cs", c_double),
+        ("sn", c_double),
+        ("phibar", c_double),
+        ("pe_oldeps", c_double),
+        ("pe_delta", c_double),
+        ("pe_denom", c_double),
+        ("pe_phi", c_double),
+        ("c0", c_double),
+        ("rnorm", c_double),
+    ]
+
+
+class SaMinresBufs(ctypes.Structure):
+    _fields_ = [
+        ("r", c_void_p * 2),
+        ("w", c_void_p * 3),
+        ("x", c_void_p),
+        ("z", c_void_p),
+        ("q", c_void_p),
+    ]
+
 _P = c_void_p
 _SIGNATURES = {
     "sa_version": (c_int32, []),
@@ -162,6 +200,17 @@ _SIGNATURES = {
     "sa_ls_extract_f32": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P]),
     "sa_ls_extract_f64": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P]),
     "sa_colour_round": (c_int32, [POINTER(SaCsr), c_uint64, _P, _P, _P]),
+    "sa_minres_state_bytes": (c_size_t, []),
+    "sa_tangent_rhs_f32": (c_int32, [c_int64, _P, _P, c_double, _P, _P]),
+    "sa_tangent_rhs_f64": (c_int32, [c_int64, _P, _P, c_double, _P, _P]),
+    "sa_minres_begin_f32": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, c_int32,
+                                      POINTER(SaMinresBufs), _P]),
+    "sa_minres_begin_f64": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, c_int32,
+                                      POINTER(SaMinresBufs), _P]),
+    "sa_minres_iterate_f32": (c_int32, [POINTER(SaCsr), _P, _P, _P, c_int32, POINTER(SaMinresBufs), _P]),
+    "sa_minres_iterate_f64": (c_int32, [POINTER(SaCsr), _P, _P, _P, c_int32, POINTER(SaMinresBufs), _P]),
+    "sa_minres_end_f32": (c_int32, [_P, c_int64, _P, POINTER(SaMinresBufs), _P]),
+    "sa_minres_end_f64": (c_int32, [_P, c_int64, _P, POINTER(SaMinresBufs), _P]),
 }
 
 _lib = None
@@ -185,6 +234,8 @@ def load(path: Path | str | None = None) -> ctypes.CDLL:
     if lib.sa_state_bytes() != ctypes.sizeof(SaState):
         raise SpecAnnLibraryError(
             f"sa_state layout mismatch: library {lib.sa_state_bytes()} bytes, binding {ctypes.sizeof(SaState)}")
+    if lib.sa_minres_state_bytes() != ctypes.sizeof(SaMinresState):
+        raise SpecAnnLibraryError("sa_minres_state layout mismatch between the library and the binding")
     if path is None:
         _lib = lib
     return lib

@@ -833,6 +833,7 @@ static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, int64_t l
   p.g = Z;
   p.s = s;
   p.lz = reinterpret_cast<const sa_state*>(work);
+  p.gate = &p.lz->status;
   p.y = W;
   int rc = launch_spmv<T>(p, A->tile_nnz, stream);
   if (rc) return rc;

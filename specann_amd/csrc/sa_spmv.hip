@@ -131,10 +131,8 @@ __global__ __launch_bounds__(SA_BLOCK) void spmv_tile_kernel(SpmvParams<T> p) {
   __shared__ double prod[TILE];
   __shared__ double red[SA_WAVES_PER_BLOCK];
   double bb = p.b;
-  if (p.lz) {
-    if (p.lz->status != SA_RUNNING) return;
-    bb *= p.lz->sigma[p.lz->j];
-  }
+  if (p.gate && *p.gate != SA_RUNNING) return;
+  if (p.lz) bb *= p.lz->sigma[p.lz->j];
   const int64_t t = blockIdx.x;
   const int32_t r0 = p.tile_rows[t], r1 = p.tile_rows[t + 1];
   if (r0 >= r1) return;
@@ -147,7 +145,7 @@ template <typename T, int TILE>
 __global__ __launch_bounds__(SA_BLOCK) void spmv_panel_kernel(SpmvParams<T> p) {
   __shared__ double prod[TILE];
   __shared__ double red[SA_WAVES_PER_BLOCK];
-  if (p.lz && p.lz->status != SA_RUNNING) return;
+  if (p.gate && *p.gate != SA_RUNNING) return;
   const int64_t w = blockIdx.x;
   const int64_t q = w >> 3;
   const int panel = (int)((q / p.pmax_tiles) * 8 + (w & 7));
@@ -168,7 +166,8 @@ template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void combine_kernel(int64_t n, int32_t npanels, const T* __restrict__ partial,
                                                            int64_t pld, const T* __restrict__ s,
                                                            const T* __restrict__ x, double a, double b,
-                                                           T* __restrict__ y) {
+                                                           T* __restrict__ y, const int32_t* gate) {
+  if (gate && *gate != SA_RUNNING) return;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
     for (int p = 0; p < npanels; ++p) acc += (double)partial[(int64_t)p * pld + r];
@@ -274,6 +273,7 @@ SpmvParams<T> params_from(const sa_csr* A) {
   p.a = 0.0;
   p.b = 1.0;
   p.lz = nullptr;
+  p.gate = nullptr;
   p.y = nullptr;
   p.npanels = A->npanels;
   p.pindptr = A->pindptr;
@@ -340,10 +340,11 @@ template int launch_spmv<double>(const SpmvParams<double>&, int32_t, hipStream_t
 template int launch_prescale<float>(int64_t, const float*, const float*, float*, hipStream_t);
 template int launch_prescale<double>(int64_t, const double*, const double*, double*, hipStream_t);
 
-// y = a x + b s (.) (J g): the tile kernel, or the panel kernel plus the combine.
+// y = a x + b s (.) (J g): the tile kernel, or the panel kernel plus the combine; skipped unless
+// gate is NULL or *gate == SA_RUNNING.
 template <typename T>
-static int spmv_apply(const sa_csr* A, const T* s, const T* g, double a, const T* x, double b, T* y,
-                      hipStream_t stream) {
+int spmv_apply(const sa_csr* A, const T* s, const T* g, double a, const T* x, double b, T* y,
+               const int32_t* gate, hipStream_t stream) {
   SpmvParams<T> p = params_from<T>(A);
   p.g = g;
   p.s = s;
@@ -351,12 +352,18 @@ static int spmv_apply(const sa_csr* A, const T* s, const T* g, double a, const T
   p.a = a;
   p.b = b;
   p.y = y;
+  p.gate = gate;
   int rc = launch_spmv<T>(p, A->tile_nnz, stream);
   if (rc || A->npanels == 0) return rc;
   hipLaunchKernelGGL(combine_kernel<T>, dim3(grid_for(A->n, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0, stream, A->n,
-                     A->npanels, (const T*)A->partial, p.pld, s, p.x, a, b, y);
+                     A->npanels, (const T*)A->partial, p.pld, s, p.x, a, b, y, gate);
   return check_launch("spmv_combine");
 }
+
+template int spmv_apply<float>(const sa_csr*, const float*, const float*, double, const float*, double, float*,
+                               const int32_t*, hipStream_t);
+template int spmv_apply<double>(const sa_csr*, const double*, const double*, double, const double*, double, double*,
+                                const int32_t*, hipStream_t);
 
 template <typename T>
 static int spmv_scaled(const sa_csr* A, const T* s, const T* x, double a, double b, T* z, T* y, hipStream_t stream) {
@@ -365,7 +372,7 @@ static int spmv_scaled(const sa_csr* A, const T* s, const T* x, double a, double
   SA_REQUIRE(A->nnz >= 0 && A->nnz < (1ll << 31), "nnz must fit int32 indices");
   int rc = launch_prescale<T>(A->n, s, x, z, stream);
   if (rc) return rc;
-  return spmv_apply<T>(A, s, z, a, x, b, y, stream);
+  return spmv_apply<T>(A, s, z, a, x, b, y, nullptr, stream);
 }
 
 template <typename T>
@@ -373,7 +380,7 @@ static int spmv_gather(const sa_csr* A, const T* s, const T* g, double a, const 
                        hipStream_t stream) {
   SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && A->vals && A->tile_rows && g && y, "bad arguments");
   SA_REQUIRE(a == 0.0 || x, "x is required when a != 0");
-  return spmv_apply<T>(A, s, g, a, x, b, y, stream);
+  return spmv_apply<T>(A, s, g, a, x, b, y, nullptr, stream);
 }
 
 template <typename T>

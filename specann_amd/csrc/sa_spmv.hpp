@@ -21,6 +21,9 @@ struct SpmvParams {
   // NULL, or the Lanczos state: the launch then runs only while status == SA_RUNNING and b is
   // multiplied by sigma[j], the scale of the raw basis vector being applied.
   const sa_state* lz;
+  // NULL, or a status word: the launch then runs only while *gate == SA_RUNNING (set to &lz->status
+  // for the Lanczos step; the MINRES solve points it at its own status).
+  const int32_t* gate;
   T* y;
   // XCD panel layout (npanels > 0): partial row sums per panel go to partial[p * n + r].
   int32_t npanels;
@@ -39,6 +42,10 @@ SpmvParams<T> params_from(const sa_csr* A);
 
 template <typename T>
 int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream);
+
+template <typename T>
+int spmv_apply(const sa_csr* A, const T* s, const T* g, double a, const T* x, double b, T* y, const int32_t* gate,
+               hipStream_t stream);
 
 template <typename T>
 int launch_prescale(int64_t n, const T* s, const T* x, T* z, hipStream_t stream);

@@ -33,7 +33,7 @@ from .eigsh import LanczosSolver
 from .failures import NumericalFailure, is_numerical, is_out_of_memory
 from .results import store_alphas, store_general
 
-PRED_MODES = ("diagonal", "secant", "prev")
+PRED_MODES = ("diagonal", "secant", "prev", "tangent")
 LS_POLICIES = ("default", "fixed_guard", "adaptive_guard", "independent_set", "sequential_oracle")
 SCHEDULES = ("sqrt", "linear")
 
@@ -78,6 +78,7 @@ class SweepOutcome:
     sweep_time: float = 0.0
     post_sweep_time: float = 0.0
     matvecs: int = 0
+    tangent_matvecs: int = 0          # operator applications of the tangent predictor's MINRES
     cand_bits: object = None          # device int64 [k][nwords]: the distinct rounded candidates
     raw_energies: list = field(default_factory=list)
     refined_energies: list = field(default_factory=list)
@@ -97,7 +98,7 @@ class SpectralAnnealing:
                  schedule: str = "sqrt", num_alphas: int = 128, schedule_interval=(0.0, 1.0),
                  eigsh_ncv: int = None, fp32_spmv: bool = None, ls_policy: str = "default",
                  ls_batch: bool = True, verbose: bool = True, tile_nnz: int = None, lanes_per_row: int = 0,
-                 autotune: bool = True, **kwargs):
+                 autotune: bool = True, tangent_steps: int = 8, tangent_tol: float = 1e-2, **kwargs):
         self.instance_name = instance_name
         self.dataset = dataset
         self.random_seed = random_seed
@@ -145,6 +146,12 @@ class SpectralAnnealing:
         self.tile_nnz = tile_nnz
         self.lanes_per_row = lanes_per_row
         self.autotune = bool(autotune) and tile_nnz is None
+        # pred_mode "tangent" (no reference counterpart): the diagonal predictor plus the eigenvector
+        # tangent's correction, from `tangent_steps` deflated MINRES iterations (specann_amd.tangent)
+        self.tangent_steps = int(tangent_steps)
+        self.tangent_tol = float(tangent_tol)
+        if self.tangent_steps < 0:
+            raise ValueError("tangent_steps must be >= 0")
         self.verbose = verbose
         if verbose:
             print(f"Initialized {self.solver_name} solver. Results will be saved under '{self.solver_name}'.")
@@ -236,6 +243,12 @@ class SpectralAnnealing:
         self._alphas_skipped = self._alphas_retried = self._alpha_errors = 0
         self._last_alpha_error = None
         self._matvecs = 0
+        self._tangent_matvecs = 0
+        self._tangent = None
+        if self.pred_mode == "tangent" and not self.disable_warm_start:
+            from .tangent import TangentPredictor
+            self._tangent = TangentPredictor(prob, self.tangent_steps, self.tangent_tol)
+        self._tangent_alpha = None
         raw_known: dict = {}
 
         for alpha_idx, alpha in enumerate(alpha_list):
@@ -259,6 +272,13 @@ class SpectralAnnealing:
                 if not self.disable_warm_start:
                     y_prev, y_last = y_last, free
                     lambda1_prev, alpha_prev = eo.lambda1, alpha
+                tangent_iters = None
+                if self._tangent is not None and alpha_idx + 1 < len(alpha_list) and self.tangent_steps > 0:
+                    # tangent at this alpha (s = s(alpha), y_last unit) for the next prediction
+                    tres = self._tangent.compute(s, y_last, eo.lambda1)
+                    tangent_iters = tres.iterations
+                    self._tangent_matvecs += tres.iterations
+                    self._tangent_alpha = alpha
                 cand = None
                 for c in by_hash.get(sig, []):
                     call("sa_bits_compare", prob.work.ptr, nwords, ptr(cand_bits[slot]), ptr(cand_bits[c]), stream())
@@ -285,6 +305,7 @@ class SpectralAnnealing:
                     'candidate': cand if self.use_local_search else None,
                     'timestamp': datetime.now().strftime('%Y-%m-%d %H:%M:%S'),
                     'residual_estimate': eo.res_estimate, 'residual': eo.residual, 'lanczos_restarts': eo.cycles,
+                    'tangent_iterations': tangent_iters,
                 })
                 cand_of_row.append(cand)
             except Exception as e:
@@ -352,6 +373,7 @@ class SpectralAnnealing:
         out.sweep_time = sweep_end - start_time
         out.post_sweep_time = time.time() - sweep_end
         out.matvecs = self._matvecs
+        out.tangent_matvecs = self._tangent_matvecs
         return out
 
     def _refine(self, prob: DeviceProblem, cand_bits: torch.Tensor, k: int) -> LS.LocalSearchState:
@@ -398,8 +420,13 @@ class SpectralAnnealing:
                 res = lz.run(s, self.eigsh_maxiter)
             else:
                 lz.begin(self.eigsh_tol, self.eigsh_maxiter, bound, pred_tol=self.pred_tol)
-                if self.pred_mode == "diagonal":
-                    lz.start(s, y_last, d=prob.d, pexp=0.5 * (alpha - alpha_prev), gate=1)
+                da = alpha - alpha_prev
+                if self.pred_mode == "tangent" and self._tangent is not None and self._tangent_alpha == alpha_prev:
+                    # D^(da/2) (y + 2 lambda da z): the diagonal predictor plus the tangent correction
+                    lz.start(s, y_last, d=prob.d, pexp=0.5 * da, y=self._tangent.z,
+                             cy=2.0 * self._tangent.lam * da, gate=1)
+                elif self.pred_mode in ("diagonal", "tangent"):
+                    lz.start(s, y_last, d=prob.d, pexp=0.5 * da, gate=1)
                 else:
                     self._secant_start(prob, lz, s, y_last, y_prev)
                 res = lz.run(s, self.eigsh_maxiter, first_check=1)

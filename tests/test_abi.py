@@ -32,6 +32,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_library():
     lib = _lib.load()
     assert lib.sa_state_bytes() == ctypes.sizeof(_lib.SaState)
+    assert lib.sa_minres_state_bytes() == ctypes.sizeof(_lib.SaMinresState)
+    assert ctypes.sizeof(_lib.SaMinresState) <= ctypes.sizeof(_lib.SaState)   # shares the work-buffer layout
     assert lib.sa_work_bytes(1000) >= ctypes.sizeof(_lib.SaState)
     assert lib.sa_version() >= 100
 

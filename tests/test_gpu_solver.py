@@ -109,6 +109,7 @@ def test_fp64_form_tight_residuals():
 
 
 @pytest.mark.parametrize("opts", [dict(pred_mode="secant"), dict(pred_mode="prev"), dict(disable_warm_start=True),
+                                  dict(pred_mode="tangent"), dict(pred_mode="tangent", tangent_steps=3),
                                   dict(use_local_search=False), dict(schedule="linear", num_alphas=9),
                                   dict(ls_batch=False, num_alphas=6), dict(eigsh_ncv=8),
                                   dict(schedule_interval=(0, 1.25), num_alphas=7)])

@@ -137,6 +137,51 @@ def test_tangent_matches_finite_differences():
     assert np.linalg.norm(fd - yp) <= 1e-5 * max(1.0, np.linalg.norm(yp))
 
 
+def _small_problem(n=120, seed=3):
+    J = graphs.erdos_renyi(n, 12 / (n - 1), seed=seed, kind="pm1")
+    Jn, d, c = O.normalize(J, np.float64)
+    return Jn, d
+
+
+def test_minres_deflated_matches_scipy_minres_iterates():
+    """The restated recurrence equals scipy's MINRES on the explicitly deflated operator, iterate by
+    iterate (same algorithm, FP64), and reaches the pseudo-inverse solution."""
+    from scipy.sparse.linalg import LinearOperator, minres
+    Jn, d = _small_problem()
+    alpha = 0.4
+    s, _, _ = O.alpha_scaling(d, alpha)
+    N = O.dense_n(Jn, s)
+    w, V = np.linalg.eigh(N)
+    lam, y = w[-1], V[:, -1]
+    rng = np.random.default_rng(0)
+    r = rng.standard_normal(N.shape[0])
+    P = np.eye(N.shape[0]) - np.outer(y, y)
+    A = P @ (N - lam * np.eye(N.shape[0])) @ P
+    op = LinearOperator(A.shape, matvec=lambda v: A @ v, dtype=np.float64)
+    b = P @ r
+    for k in (1, 2, 5, 9):
+        x, itn, _, _ = O.minres_deflated(lambda v: N @ v, y, r, lam, 0.0, k)
+        xs, _ = minres(op, b, maxiter=k, rtol=0.0)
+        xs = xs - (y @ xs) * y
+        assert itn == k
+        assert np.linalg.norm(x - xs) <= 1e-9 * np.linalg.norm(xs), k
+    x, itn, phibar, beta1 = O.minres_deflated(lambda v: N @ v, y, r, lam, 1e-13, 400)
+    ref = np.linalg.pinv(N - lam * np.eye(N.shape[0]), rcond=1e-12) @ b
+    assert np.linalg.norm(x - ref) <= 1e-7 * np.linalg.norm(ref)
+    assert phibar <= 1e-13 * beta1 * 10
+
+
+def test_tangent_minres_equals_dense_tangent():
+    Jn, d = _small_problem(150, 5)
+    alpha = 0.6
+    lam, y, yp = O.tangent(Jn, d, alpha)
+    t, z, itn = O.tangent_minres(Jn, d, alpha, y, lam, steps=500, tol=1e-13)
+    assert np.linalg.norm(t - yp) <= 1e-7 * np.linalg.norm(yp)
+    # a planted sign error in the correction term is visible
+    bad = t - 4.0 * lam * z
+    assert np.linalg.norm(bad - yp) > 1e-3 * np.linalg.norm(yp)
+
+
 def test_round_and_bits():
     x = np.array([0.0, -0.0, -1e-30, 2.0, -3.0])
     np.testing.assert_array_equal(O.round_spins(x), [1, 1, -1, 1, -1])
