@@ -6,14 +6,17 @@ Lanczos per alpha, rounding, deduplication) plus the batched local search of the
 candidates and the exact energies. The graph is generated and uploaded (J/c, degrees) before the
 timed region, so every timed step starts with its inputs resident in HBM.
 
-Reported (one JSON line): value = seconds per FP32 sweep (diagonal predictor, the SpecAnn run);
-the FP64 form (SpecAnn-64) is timed once beside it, with the best-energy gap between the two; the
+Reported (one JSON line): value = seconds per FP32 sweep with the tangent predictor (the diagonal
+predictor plus the eigenvector tangent's correction from two deflated MINRES steps seeded with the
+backward difference); the FP32 sweep with the reference's default diagonal predictor (the Lanczos
+applications the tangent saves) and the plain FP64 diagonal-predictor form (SpecAnn-64) are timed
+once beside it, with the best-energy gap between the FP32 tangent sweep and the FP64 one; the
 roofline object gives the fused scaled SpMV's algorithmic bandwidth (8 nnz + 4 (n+1) + 12 n bytes
 per launch in FP32) against the 8 TB/s HBM peak, timed with HIP events on the launch stream; the
 cpu_baseline object times the reference's CPU operator (scipy CSR, S_rho in FP32) on a bounded
 sample and scales it to the sweep's operator applications.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--workload er22|ba20|er1k]
+    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--workload er22|ba20|er1k] [--pred tangent|diagonal|secant]
 """
 from __future__ import annotations
 
@@ -134,6 +137,13 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="er22", choices=sorted(WORKLOADS))
     ap.add_argument("--alphas", type=int, default=128)
+    ap.add_argument("--pred", default="tangent", choices=["diagonal", "tangent", "secant", "prev"],
+                    help="predictor of the timed FP32 sweep (the sweeps beside it use the diagonal one)")
+    ap.add_argument("--tangent-steps", type=int, default=2)
+    ap.add_argument("--skip-fp32-diagonal", action="store_true",
+                    help="skip the FP32 diagonal-predictor sweep (the Lanczos applications the predictor saves)")
+    ap.add_argument("--ncv", type=int, default=None, help="Lanczos basis size (default: the reference's min(33, n-1))")
+    ap.add_argument("--rows-out", default=None, help="write the FP32 sweep's per-alpha rows (JSON) to this file")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--spmv-only", action="store_true",
@@ -175,30 +185,46 @@ def main(argv=None) -> int:
                           "copy_us": copy_sec * 1e6, "copy_bytes": copy_bytes}))
         return 0
 
-    def solver(precision):
+    def solver(precision, pred="diagonal"):
         return SpectralAnnealing("bench", "synthetic", random_seed=42, precision=precision, verbose=False,
-                                 num_alphas=args.alphas)
+                                 num_alphas=args.alphas, pred_mode=pred, tangent_steps=args.tangent_steps,
+                                 eigsh_ncv=args.ncv)
 
     for i in range(args.warmup):
-        out = solver(32).run_on_device(prob32)
+        out = solver(32, args.pred).run_on_device(prob32)
         log(f"warmup {i}: {out.sweep_time + out.post_sweep_time:.2f}s")
     torch.cuda.synchronize()
     times, outs = [], []
     for i in range(args.steps):
         torch.cuda.synchronize()
         t = time.perf_counter()
-        out = solver(32).run_on_device(prob32)
+        out = solver(32, args.pred).run_on_device(prob32)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t)
         outs.append(out)
         log(f"step {i}: {times[-1]:.3f}s (sweep {out.sweep_time:.3f}s, local search {out.post_sweep_time:.3f}s, "
-            f"{out.matvecs} operator applications, {out.num_candidates} candidates, best {out.best_energy})")
+            f"{out.matvecs} Lanczos + {out.tangent_matvecs} tangent operator applications, {out.num_candidates} "
+            f"candidates, best {out.best_energy})")
+    if args.rows_out:
+        keys = ("alpha_index", "alpha", "eigsh_iterations", "pred_residual", "pred_accepted", "lanczos_restarts",
+                "lambda1", "residual_estimate", "tangent_iterations", "candidate", "eigsh_retry")
+        with open(args.rows_out, "w") as f:
+            json.dump([{k: r.get(k) for k in keys} for r in outs[-1].rows], f)
     sec32 = float(np.mean(times))
     e32 = outs[-1].best_energy
-    matvecs = outs[-1].matvecs
+    matvecs = outs[-1].matvecs + outs[-1].tangent_matvecs
     conv = float(np.mean([bool(r['eigsh_converged']) for r in outs[-1].rows]))
 
-    sec64 = e64 = gap = None
+    sec32d = e32d = apps32d = None
+    if args.pred != "diagonal" and not args.skip_fp32_diagonal:
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        outd = solver(32, "diagonal").run_on_device(prob32)
+        torch.cuda.synchronize()
+        sec32d, e32d, apps32d = time.perf_counter() - t, outd.best_energy, outd.matvecs
+        log(f"fp32 diagonal sweep: {sec32d:.3f}s, best {e32d}, {apps32d} operator applications")
+
+    sec64 = e64 = gap = apps64 = None
     if not args.skip_fp64:
         del prob32
         torch.cuda.empty_cache()
@@ -211,6 +237,7 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         sec64 = time.perf_counter() - t
         e64 = out64.best_energy
+        apps64 = out64.matvecs
         gap = abs(e32 - e64) / abs(e64) if e64 else None
         log(f"fp64 sweep: {sec64:.3f}s, best {e64}, {out64.matvecs} operator applications")
         del prob64
@@ -230,12 +257,17 @@ def main(argv=None) -> int:
         "metric": METRIC, "value": sec32, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": sec32 * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": w["text"], "alphas": args.alphas, "predictor": "diagonal",
+        "config": {"workload": w["text"], "alphas": args.alphas, "predictor": args.pred,
                    "local_search": "batched deterministic", "precision": "FP32 vectors, FP64 Lanczos scalars",
                    "spmv_layout": tune32["choice"]},
         "fp64_sweep_s": sec64, "fp32_vs_fp64_speedup": (sec64 / sec32) if sec64 else None,
+        "fp64_operator_applications": apps64,
+        "fp32_diagonal_sweep_s": sec32d, "fp32_diagonal_operator_applications": apps32d,
+        "best_energy_fp32_diagonal": e32d,
+        "lanczos_applications_saved_vs_diagonal": (apps32d - matvecs) if apps32d is not None else None,
         "best_energy_fp32": e32, "best_energy_fp64": e64, "energy_gap": gap,
-        "operator_applications": matvecs, "converged_fraction": conv,
+        "operator_applications": matvecs, "lanczos_applications": outs[-1].matvecs,
+        "tangent_applications": outs[-1].tangent_matvecs, "converged_fraction": conv,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, "f32", tune32["choice"]),

@@ -214,6 +214,13 @@ int sa_norm_dot_f64(void* work, int64_t n, const double* x, const double* y, dou
 /* out_i = scale * x_i (in T). */
 int sa_scale_copy_f32(int64_t n, const float* x, double scale, float* out, hipStream_t stream);
 int sa_scale_copy_f64(int64_t n, const double* x, double scale, double* out, hipStream_t stream);
+
+/* out = a x + b y + c z (FP64 arithmetic; y or z may be NULL when its coefficient is 0). The vector
+ * bookkeeping of the predictors (cf. utils/predictors.py:29-56). */
+int sa_combine3_f32(int64_t n, double a, const float* x, double b, const float* y, double c, const float* z,
+                    float* out, hipStream_t stream);
+int sa_combine3_f64(int64_t n, double a, const double* x, double b, const double* y, double c, const double* z,
+                    double* out, hipStream_t stream);
 /* Counter-based standard normal vector (Box-Muller on SplitMix64 streams (seed, stream_id, i)). */
 int sa_fill_normal_f32(int64_t n, uint64_t seed, uint64_t stream_id, float* out, hipStream_t stream);
 int sa_fill_normal_f64(int64_t n, uint64_t seed, uint64_t stream_id, double* out, hipStream_t stream);

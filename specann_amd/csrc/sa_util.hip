@@ -184,6 +184,18 @@ __global__ __launch_bounds__(SA_BLOCK) void scale_copy_kernel(int64_t n, const T
     out[r] = (T)(scale * (double)x[r]);
 }
 
+template <typename T>
+__global__ __launch_bounds__(SA_BLOCK) void combine3_kernel(int64_t n, double a, const T* __restrict__ x, double b,
+                                                            const T* __restrict__ y, double c,
+                                                            const T* __restrict__ z, T* __restrict__ out) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double v = a * (double)x[r];
+    if (y) v += b * (double)y[r];
+    if (z) v += c * (double)z[r];
+    out[r] = (T)v;
+  }
+}
+
 // Box-Muller on two 53-bit uniforms of the stream (seed, stream_id, i); u1 is in (0, 1].
 __host__ __device__ inline double normal_at(uint64_t seed, uint64_t stream_id, uint64_t i) {
   const uint64_t h1 = mix3(seed, 0x5A17C0DE00000000ull ^ stream_id, i);
@@ -401,6 +413,22 @@ int sa_scale_copy_f64(int64_t n, const double* x, double scale, double* out, hip
   SA_REQUIRE(n > 0 && x && out, "bad arguments");
   hipLaunchKernelGGL(scale_copy_kernel<double>, dim3(SA_GRID(n)), dim3(SA_BLOCK), 0, stream, n, x, scale, out);
   return check_launch("scale_copy_f64");
+}
+
+int sa_combine3_f32(int64_t n, double a, const float* x, double b, const float* y, double c, const float* z,
+                    float* out, hipStream_t stream) {
+  SA_REQUIRE(n > 0 && x && out && (y || b == 0.0) && (z || c == 0.0), "bad arguments");
+  hipLaunchKernelGGL(combine3_kernel<float>, dim3(SA_GRID(n)), dim3(SA_BLOCK), 0, stream, n, a, x, y ? b : 0.0, y,
+                     z ? c : 0.0, z, out);
+  return check_launch("combine3_f32");
+}
+
+int sa_combine3_f64(int64_t n, double a, const double* x, double b, const double* y, double c, const double* z,
+                    double* out, hipStream_t stream) {
+  SA_REQUIRE(n > 0 && x && out && (y || b == 0.0) && (z || c == 0.0), "bad arguments");
+  hipLaunchKernelGGL(combine3_kernel<double>, dim3(SA_GRID(n)), dim3(SA_BLOCK), 0, stream, n, a, x, y ? b : 0.0, y,
+                     z ? c : 0.0, z, out);
+  return check_launch("combine3_f64");
 }
 
 int sa_fill_normal_f32(int64_t n, uint64_t seed, uint64_t stream_id, float* out, hipStream_t stream) {

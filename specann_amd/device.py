@@ -293,6 +293,13 @@ class DeviceProblem:
              stream())
         return out
 
+    def combine3(self, a: float, x: torch.Tensor, b: float, y: torch.Tensor | None, c: float,
+                 z: torch.Tensor | None, out: torch.Tensor) -> torch.Tensor:
+        """out = a x + b y + c z."""
+        call(f"sa_combine3_{self.sfx}", self.n, float(a), ptr(x), float(b), ptr(y), float(c), ptr(z), ptr(out),
+             stream())
+        return out
+
     def scale_copy(self, x: torch.Tensor, scale: float, out: torch.Tensor) -> torch.Tensor:
         call(f"sa_scale_copy_{self.sfx}", self.n, ptr(x), float(scale), ptr(out), stream())
         return out

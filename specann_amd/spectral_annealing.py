@@ -98,7 +98,7 @@ class SpectralAnnealing:
                  schedule: str = "sqrt", num_alphas: int = 128, schedule_interval=(0.0, 1.0),
                  eigsh_ncv: int = None, fp32_spmv: bool = None, ls_policy: str = "default",
                  ls_batch: bool = True, verbose: bool = True, tile_nnz: int = None, lanes_per_row: int = 0,
-                 autotune: bool = True, tangent_steps: int = 8, tangent_tol: float = 1e-2, **kwargs):
+                 autotune: bool = True, tangent_steps: int = 2, tangent_tol: float = 1e-2, **kwargs):
         self.instance_name = instance_name
         self.dataset = dataset
         self.random_seed = random_seed
@@ -249,6 +249,7 @@ class SpectralAnnealing:
             from .tangent import TangentPredictor
             self._tangent = TangentPredictor(prob, self.tangent_steps, self.tangent_tol)
         self._tangent_alpha = None
+        self._last_ok_idx = None
         raw_known: dict = {}
 
         for alpha_idx, alpha in enumerate(alpha_list):
@@ -269,16 +270,23 @@ class SpectralAnnealing:
                 call(f"sa_round_{prob.sfx}", prob.work.ptr, n, ptr(lz.Y), sign / nrm, ptr(free),
                      ptr(cand_bits[slot]), stream())
                 sig = int(prob.work.head().hash)
+                alpha_before = alpha_prev
                 if not self.disable_warm_start:
                     y_prev, y_last = y_last, free
                     lambda1_prev, alpha_prev = eo.lambda1, alpha
                 tangent_iters = None
                 if self._tangent is not None and alpha_idx + 1 < len(alpha_list) and self.tangent_steps > 0:
-                    # tangent at this alpha (s = s(alpha), y_last unit) for the next prediction
-                    tres = self._tangent.compute(s, y_last, eo.lambda1)
-                    tangent_iters = tres.iterations
-                    self._tangent_matvecs += tres.iterations
+                    # tangent at this alpha (s = s(alpha), y_last unit) for the next prediction, its MINRES
+                    # started from the backward difference when the previous alpha is the one before
+                    hist = None
+                    if y_prev is not None and alpha_before is not None and alpha_idx >= 1 and \
+                            self._last_ok_idx == alpha_idx - 1:
+                        hist = (y_prev, alpha - alpha_before)
+                    self._tangent.compute(s, y_last, eo.lambda1, hist)
+                    tangent_iters = self._tangent.matvecs
+                    self._tangent_matvecs += tangent_iters
                     self._tangent_alpha = alpha
+                self._last_ok_idx = alpha_idx
                 cand = None
                 for c in by_hash.get(sig, []):
                     call("sa_bits_compare", prob.work.ptr, nwords, ptr(cand_bits[slot]), ptr(cand_bits[c]), stream())
@@ -420,15 +428,7 @@ class SpectralAnnealing:
                 res = lz.run(s, self.eigsh_maxiter)
             else:
                 lz.begin(self.eigsh_tol, self.eigsh_maxiter, bound, pred_tol=self.pred_tol)
-                da = alpha - alpha_prev
-                if self.pred_mode == "tangent" and self._tangent is not None and self._tangent_alpha == alpha_prev:
-                    # D^(da/2) (y + 2 lambda da z): the diagonal predictor plus the tangent correction
-                    lz.start(s, y_last, d=prob.d, pexp=0.5 * da, y=self._tangent.z,
-                             cy=2.0 * self._tangent.lam * da, gate=1)
-                elif self.pred_mode in ("diagonal", "tangent"):
-                    lz.start(s, y_last, d=prob.d, pexp=0.5 * da, gate=1)
-                else:
-                    self._secant_start(prob, lz, s, y_last, y_prev)
+                self._predict_start(prob, lz, s, alpha, alpha_prev, y_last, y_prev)
                 res = lz.run(s, self.eigsh_maxiter, first_check=1)
                 if res.status == SA_BROKEN and not (res.start_norm >= 1e-12):
                     # zero predicted vector: random start (utils/predictors.py:22-25)
@@ -466,6 +466,17 @@ class SpectralAnnealing:
                       f"{self.eigsh_tol:g}; energy bound not certified")
         return EigenOutcome(res.theta, rho - res.theta, rho, g, degree_scale, res.steps, retry, converged, res.res,
                             residual, pred_residual, pred_accepted, res.cycles)
+
+    def _predict_start(self, prob, lz, s, alpha, alpha_prev, y_last, y_prev):
+        """Queue the predicted start vector of a warm-started alpha (gated: step 0 tests it)."""
+        da = alpha - alpha_prev
+        if self.pred_mode == "tangent" and self._tangent is not None and self._tangent_alpha == alpha_prev:
+            # D^(da/2) (y + 2 lambda da z): the diagonal predictor plus the tangent correction
+            lz.start(s, y_last, d=prob.d, pexp=0.5 * da, y=self._tangent.z, cy=2.0 * self._tangent.lam * da, gate=1)
+        elif self.pred_mode in ("diagonal", "tangent"):
+            lz.start(s, y_last, d=prob.d, pexp=0.5 * da, gate=1)
+        else:
+            self._secant_start(prob, lz, s, y_last, y_prev)
 
     def _secant_start(self, prob, lz, s, y_last, y_prev):
         """Secant predictor (utils/predictors.py:29-56) as the start vector cx y_last + cy y_prev."""

@@ -101,19 +101,45 @@ class TangentPredictor:
         self.minres = DeflatedMinres(prob)
         self.u = torch.empty(prob.n, dtype=prob.dtype, device=prob.device)
         self.z = torch.empty_like(self.u)
+        self.z0 = torch.empty_like(self.u)
+        self.w = torch.empty_like(self.u)
+        self.scratch = torch.empty_like(self.u)
         self.lam = None
         self.last: MinresResult | None = None
+        self.matvecs = 0
 
     def rhs(self, y: torch.Tensor) -> torch.Tensor:
         call(f"sa_tangent_rhs_{self.prob.sfx}", self.prob.n, ptr(y), ptr(self.prob.d), CLAMP, ptr(self.u), stream())
         return self.u
 
-    def compute(self, s: torch.Tensor | None, y: torch.Tensor, lam: float) -> MinresResult:
-        """Tangent correction at (alpha, y, lambda): self.z = (N - lambda I)^+ P u (y unit, s = s(alpha))."""
+    def compute(self, s: torch.Tensor | None, y: torch.Tensor, lam: float,
+                history: tuple[torch.Tensor, float] | None = None) -> MinresResult:
+        """Tangent correction at (alpha, y, lambda): self.z = (N - lambda I)^+ P u (y unit, s = s(alpha)).
+
+        With ``history = (y_prev, dalpha)`` (the previous alpha's unit eigenvector, sign-aligned with y,
+        and alpha - alpha_prev), MINRES starts from the secant estimate of z,
+        z0 = ((y - y_prev) / dalpha - P u) / (2 lambda): the backward difference carries the components
+        along the eigenvectors next to lambda, which a few Krylov steps cannot resolve, and MINRES
+        corrects the rest (one extra operator application for the initial residual)."""
+        prob = self.prob
         self.rhs(y)
-        res = self.minres.solve(s, y, self.u, lam, self.tol, self.steps)
-        self.z.copy_(self.minres.x)
         self.lam = float(lam)
+        if history is None or not (history[1] > 0) or lam == 0.0:
+            res = self.minres.solve(s, y, self.u, lam, self.tol, self.steps)
+            self.z.copy_(self.minres.x)
+            self.matvecs = res.iterations
+            self.last = res
+            return res
+        y_prev, dprev = history
+        _, yu = prob.norm_dot(y, self.u)
+        inv = 1.0 / (2.0 * lam)
+        prob.combine3((1.0 / dprev + yu) * inv, y, -inv / dprev, y_prev, -inv, self.u, self.z0)
+        # residual of the derivative equation at z0: r = u - (N - lambda I) z0 (deflated inside MINRES)
+        prob.matvec(s, self.z0, -lam, 1.0, self.w, self.scratch)
+        prob.combine3(1.0, self.u, -1.0, self.w, 0.0, None, self.w)
+        res = self.minres.solve(s, y, self.w, lam, self.tol, self.steps)
+        prob.combine3(1.0, self.z0, 1.0, self.minres.x, 0.0, None, self.z)
+        self.matvecs = res.iterations + 1
         self.last = res
         return res
 

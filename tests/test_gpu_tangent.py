@@ -103,7 +103,7 @@ def test_autograd_adjoint_matches_finite_differences(gpu, tdtype, rtol):
     J = _graph(600, 12, 4)
     print(f"adjoint dtype={tdtype}", flush=True)
     prob = DeviceProblem(J, tdtype)
-    eig = EigenContext(prob, adjoint_steps=300, adjoint_tol=1e-12 if tdtype == torch.float64 else 1e-6)
+    eig = EigenContext(prob, adjoint_steps=600, adjoint_tol=1e-13 if tdtype == torch.float64 else 1e-6)
     cvec = torch.from_numpy(O.normal_vector(prob.n, 3, 1)).to(gpu)
     alpha0 = 0.55
     y0, _, _ = eig.solve(alpha0)
@@ -118,8 +118,10 @@ def test_autograd_adjoint_matches_finite_differences(gpu, tdtype, rtol):
         yy, ll, _ = eig.solve(al, sign_ref=ref)
         return float((cvec * yy.double()).sum()) + 0.3 * ll
 
+    # fourth-order central differences (truncation O(h^4))
     h = 1e-3 if tdtype == torch.float64 else 1e-2
-    fd = (L(alpha0 + h) - L(alpha0 - h)) / (2 * h)
+    fd = (-L(alpha0 + 2 * h) + 8 * L(alpha0 + h) - 8 * L(alpha0 - h) + L(alpha0 - 2 * h)) / (12 * h)
+    print(f"adjoint gradient {g:.12g} finite differences {fd:.12g}", flush=True)
     assert g == pytest.approx(fd, rel=rtol, abs=rtol * 1e-2)
 
 
