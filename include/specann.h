@@ -73,6 +73,22 @@ typedef struct sa_csr {
   int64_t pld;
 } sa_csr;
 
+/* A batch of independent instances packed block-diagonally into one CSR (no counterpart in the
+ * reference, which solves one instance per call). Instance b owns the packed rows
+ * [row_off[b], row_off[b+1]) (multiples of 64 apart; padding rows have no entries) and the
+ * workgroups [blk_off[b], blk_off[b+1]) of the batched vector kernels, rows_per_block rows each; its
+ * solve state lives in the work buffer works + b * stride (sa_work_bytes bytes each). */
+typedef struct sa_batch {
+  int32_t B;
+  int32_t rows_per_block;   /* a multiple of 64 */
+  const int64_t* row_off;   /* B + 1, device */
+  const int64_t* row_end;   /* B, device: end of instance b's real rows (then padding up to row_off[b+1]) */
+  const int32_t* blk_off;   /* B + 1, device: blk_off[b+1] - blk_off[b] = ceil(real rows / rows_per_block) */
+  void* works;
+  int64_t stride;
+  int64_t nblocks;          /* blk_off[B] */
+} sa_batch;
+
 /* Device-side state of one eigensolve; lives at the start of the work buffer. */
 typedef struct sa_state {
   int32_t status;      /* SA_RUNNING, ... */
@@ -123,6 +139,11 @@ int sa_abs_degree_f32(const sa_csr* A, float* deg, void* work, hipStream_t strea
 int sa_abs_degree_f64(const sa_csr* A, double* deg, void* work, hipStream_t stream);
 /* vals /= c in place, d = deg / c (models/spectral_annealing.py:282-288). */
 int sa_normalize_f32(int64_t nnz, float* vals, int64_t n, const float* deg, float* d, double c, hipStream_t stream);
+/* Block-diagonal batch: row r and its entries divided by c_row[r] (its instance's factor). */
+int sa_normalize_rows_f32(const sa_csr* A, float* vals, const float* deg, float* d, const float* c_row,
+                          hipStream_t stream);
+int sa_normalize_rows_f64(const sa_csr* A, double* vals, const double* deg, double* d, const double* c_row,
+                          hipStream_t stream);
 int sa_normalize_f64(int64_t nnz, double* vals, int64_t n, const double* deg, double* d, double c,
                      hipStream_t stream);
 /* tile_rows[t] = first row r with indptr[r] >= t * tile_nnz (t = 0..ntiles; tile_rows[ntiles] = n). */
@@ -328,6 +349,50 @@ int sa_minres_iterate_f64(const sa_csr* A, void* work, const double* s, const do
 /* Apply the pending update of x and project x onto y's complement (x -= (y . x) y). */
 int sa_minres_end_f32(void* work, int64_t n, const float* y, const sa_minres_bufs* B, hipStream_t stream);
 int sa_minres_end_f64(void* work, int64_t n, const double* y, const sa_minres_bufs* B, hipStream_t stream);
+
+/* ---- Batched sweep: many instances packed block-diagonally (sa_batch) ------------------------
+ * No counterpart in the reference (it solves one instance per call); these replace, for a batch,
+ * the calls above that cite it. Each instance runs its own eigensolve in its own work buffer (its
+ * own j, status, Ritz solve and restarts); every launch covers all instances and is skipped per
+ * workgroup by instances not in the status it needs. The batched step uses the plain CSR layout. */
+int sa_batch_alpha_scaling_f32(const sa_batch* Bt, const float* d, double alpha, double clamp, float* s,
+                               hipStream_t stream);
+int sa_batch_alpha_scaling_f64(const sa_batch* Bt, const double* d, double alpha, double clamp, double* s,
+                               hipStream_t stream);
+/* bounds: device array of B breakdown bounds (rho_b + g_b). */
+int sa_batch_lanczos_begin(const sa_batch* Bt, int32_t ncv, int32_t maxiter, double tol, const double* bounds,
+                           double pred_tol, int32_t keep, hipStream_t stream);
+int sa_batch_lanczos_start_f32(const sa_batch* Bt, const float* s, const float* x, const float* d, double pexp,
+                               double cx, const float* y, double cy, float* V, int64_t ldv, float* Z, int32_t gate,
+                               int32_t only_if, hipStream_t stream);
+int sa_batch_lanczos_start_f64(const sa_batch* Bt, const double* s, const double* x, const double* d, double pexp,
+                               double cx, const double* y, double cy, double* V, int64_t ldv, double* Z,
+                               int32_t gate, int32_t only_if, hipStream_t stream);
+int sa_batch_lanczos_step_f32(const sa_csr* A, const sa_batch* Bt, const float* s, float* V, int64_t ldv, float* Z,
+                              float* W, hipStream_t stream);
+int sa_batch_lanczos_step_f64(const sa_csr* A, const sa_batch* Bt, const double* s, double* V, int64_t ldv,
+                              double* Z, double* W, hipStream_t stream);
+int sa_batch_lanczos_thick_restart_f32(const sa_batch* Bt, const float* s, float* V, int64_t ldv, float* Z,
+                                       hipStream_t stream);
+int sa_batch_lanczos_thick_restart_f64(const sa_batch* Bt, const double* s, double* V, int64_t ldv, double* Z,
+                                       hipStream_t stream);
+int sa_batch_lanczos_ritz_f32(const sa_batch* Bt, const float* V, int64_t ldv, const float* y_last, float* y,
+                              int32_t only_if, hipStream_t stream);
+int sa_batch_lanczos_ritz_f64(const sa_batch* Bt, const double* V, int64_t ldv, const double* y_last, double* y,
+                              int32_t only_if, hipStream_t stream);
+/* Per instance: y_last = (sign / r[4]) y (sign = -1 if align and r[5] < 0), its bits, state.hash. */
+int sa_batch_round_f32(const sa_batch* Bt, const float* y, float* y_last, uint64_t* bits, int32_t align,
+                       hipStream_t stream);
+int sa_batch_round_f64(const sa_batch* Bt, const double* y, double* y_last, uint64_t* bits, int32_t align,
+                       hipStream_t stream);
+/* e2[b * k + c] += sum over instance b's rows (callers zero e2 first). */
+int sa_batch_energy_bits_i64(const sa_csr* A, const sa_batch* Bt, int32_t k, const uint64_t* cand_bits,
+                             int64_t nwords, int64_t* e2, hipStream_t stream);
+int sa_batch_ls_energy(const sa_batch* Bt, int32_t k, int32_t groups, const int8_t* spins, const int32_t* fields,
+                       int64_t* e2, hipStream_t stream);
+/* out[c][w] = in[src[c * B + b(w)]][w], b(w) the instance owning word w (word_off: B + 1). */
+int sa_bits_gather(int64_t nwords, int32_t k, int32_t B, const int64_t* word_off, const int32_t* src,
+                   const uint64_t* in, uint64_t* out, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -111,6 +111,19 @@ class SaState(ctypes.Structure):
 STATE_HEAD_BYTES = SaState.a.offset
 
 
+class SaBatch(ctypes.Structure):
+    _fields_ = [
+        ("B", c_int32),
+        ("rows_per_block", c_int32),
+        ("row_off", c_void_p),
+        ("row_end", c_void_p),
+        ("blk_off", c_void_p),
+        ("works", c_void_p),
+        ("stride", c_int64),
+        ("nblocks", c_int64),
+    ]
+
+
 class SaMinresState(ctypes.Structure):
     _fields_ = [
         ("status", c_int32),
@@ -203,6 +216,26 @@ _SIGNATURES = {
     "sa_combine3_f32": (c_int32, [c_int64, c_double, _P, c_double, _P, c_double, _P, _P, _P]),
     "sa_combine3_f64": (c_int32, [c_int64, c_double, _P, c_double, _P, c_double, _P, _P, _P]),
     "sa_minres_state_bytes": (c_size_t, []),
+    "sa_normalize_rows_f32": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P, _P]),
+    "sa_normalize_rows_f64": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P, _P]),
+    "sa_batch_alpha_scaling_f32": (c_int32, [POINTER(SaBatch), _P, c_double, c_double, _P, _P]),
+    "sa_batch_alpha_scaling_f64": (c_int32, [POINTER(SaBatch), _P, c_double, c_double, _P, _P]),
+    "sa_batch_lanczos_begin": (c_int32, [POINTER(SaBatch), c_int32, c_int32, c_double, _P, c_double, c_int32, _P]),
+    "sa_batch_lanczos_start_f32": (c_int32, [POINTER(SaBatch), _P, _P, _P, c_double, c_double, _P, c_double, _P,
+                                             c_int64, _P, c_int32, c_int32, _P]),
+    "sa_batch_lanczos_start_f64": (c_int32, [POINTER(SaBatch), _P, _P, _P, c_double, c_double, _P, c_double, _P,
+                                             c_int64, _P, c_int32, c_int32, _P]),
+    "sa_batch_lanczos_step_f32": (c_int32, [POINTER(SaCsr), POINTER(SaBatch), _P, _P, c_int64, _P, _P, _P]),
+    "sa_batch_lanczos_step_f64": (c_int32, [POINTER(SaCsr), POINTER(SaBatch), _P, _P, c_int64, _P, _P, _P]),
+    "sa_batch_lanczos_thick_restart_f32": (c_int32, [POINTER(SaBatch), _P, _P, c_int64, _P, _P]),
+    "sa_batch_lanczos_thick_restart_f64": (c_int32, [POINTER(SaBatch), _P, _P, c_int64, _P, _P]),
+    "sa_batch_lanczos_ritz_f32": (c_int32, [POINTER(SaBatch), _P, c_int64, _P, _P, c_int32, _P]),
+    "sa_batch_lanczos_ritz_f64": (c_int32, [POINTER(SaBatch), _P, c_int64, _P, _P, c_int32, _P]),
+    "sa_batch_round_f32": (c_int32, [POINTER(SaBatch), _P, _P, _P, c_int32, _P]),
+    "sa_batch_round_f64": (c_int32, [POINTER(SaBatch), _P, _P, _P, c_int32, _P]),
+    "sa_batch_energy_bits_i64": (c_int32, [POINTER(SaCsr), POINTER(SaBatch), c_int32, _P, c_int64, _P, _P]),
+    "sa_batch_ls_energy": (c_int32, [POINTER(SaBatch), c_int32, c_int32, _P, _P, _P, _P]),
+    "sa_bits_gather": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P, _P, _P]),
     "sa_tangent_rhs_f32": (c_int32, [c_int64, _P, _P, c_double, _P, _P]),
     "sa_tangent_rhs_f64": (c_int32, [c_int64, _P, _P, c_double, _P, _P]),
     "sa_minres_begin_f32": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, c_int32,

@@ -1,0 +1,377 @@
+"""Batched SpecAnn sweep: many instances packed block-diagonally and swept together on one GPU.
+
+No counterpart in the reference, which solves one instance per call (utils/core.py loops over
+instances). Instance b occupies the packed rows [row_off[b], row_off[b] + n_b), each instance
+starting on a multiple of 64 rows (empty padding rows fill the gaps), and is normalised by its own
+max absolute degree c_b. Every Lanczos step is one set of launches for all instances: the fused
+SpMV over the packed CSR, then projections, update, Ritz solves and restarts per instance (each in
+its own work buffer with its own j, status and residual test; see sa_batch_* in specann.h).
+
+Per alpha and instance the control flow is the one-instance sweep's (specann_amd.spectral_annealing,
+after models/spectral_annealing.py:311-421): Gershgorin bound and rho from the instance's last
+lambda_1, the diagonal (or secant) predictor gated by its step-0 residual, a fall-back start from
+y_last when the gate rejects it, sign alignment, rounding and first-appearance deduplication. Two
+differences: an instance whose Lanczos breaks down skips the alpha (the one-instance sweep retries it
+from a random vector once), and the local search runs on the packed graph with column c holding every
+instance's c-th distinct candidate (an instance with fewer repeats its last one), so its pass
+count is the batch's; per-instance energies are exact int64 sums over the instance's rows.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _lib
+from . import local_search as LS
+from ._lib import (SA_BROKEN, SA_CONVERGED, SA_MAXITER, SA_REJECTED, SA_RUNNING, SaBatch, SaState, call)
+from .device import DeviceProblem, ptr, require_gpu, stream
+from .eigsh import default_ncv
+from .spectral_annealing import compute_shift_rho
+
+TERMINAL = (SA_CONVERGED, SA_MAXITER, SA_BROKEN)
+ALIGN = 64
+
+
+def pack(Js: list[sp.spmatrix]) -> tuple[sp.csr_matrix, np.ndarray, np.ndarray]:
+    """Block-diagonal CSR of the instances, each starting on a multiple of 64 rows; returns
+    (J_packed, row_off[B+1], sizes[B])."""
+    sizes = np.array([J.shape[0] for J in Js], dtype=np.int64)
+    padded = (sizes + ALIGN - 1) // ALIGN * ALIGN
+    row_off = np.zeros(len(Js) + 1, dtype=np.int64)
+    row_off[1:] = np.cumsum(padded)
+    blocks = []
+    for J, p in zip(Js, padded):
+        J = sp.csr_matrix(J)
+        if p > J.shape[0]:
+            J = sp.block_diag([J, sp.csr_matrix((int(p - J.shape[0]), int(p - J.shape[0])))], format="csr")
+        blocks.append(J)
+    Jp = sp.block_diag(blocks, format="csr")
+    Jp.sort_indices()
+    return Jp, row_off, sizes
+
+
+class BatchProblem:
+    """The packed couplings on the GPU plus the batch descriptor (row and workgroup tables and one
+    work buffer per instance)."""
+
+    def __init__(self, Js: list[sp.spmatrix], dtype: torch.dtype = torch.float32, rows_per_block: int = 2048):
+        self.device = require_gpu()
+        self.B = len(Js)
+        if self.B < 1:
+            raise ValueError("empty batch")
+        Jp, row_off, sizes = pack(Js)
+        self.sizes, self.row_off_h = sizes, row_off
+        self.c_h = np.array([float(abs(sp.csr_matrix(J)).sum(axis=1).max()) for J in Js])
+        if not np.all(self.c_h > 0):
+            raise ValueError("an instance has no nonzero coupling")
+        row_c = np.ones(Jp.shape[0])
+        for b in range(self.B):
+            row_c[row_off[b]:row_off[b] + sizes[b]] = self.c_h[b]
+        self.prob = DeviceProblem(Jp, dtype, row_c=row_c)
+        if self.prob.jint is None:
+            raise NotImplementedError("the batched sweep needs integer couplings (exact energies)")
+        self.n = self.prob.n
+        self.dtype = dtype
+        self.rpb = int(rows_per_block)
+        if self.rpb % 64 or self.rpb < 64:
+            raise ValueError("rows_per_block must be a positive multiple of 64")
+        blk = (sizes + self.rpb - 1) // self.rpb
+        if blk.max() > 2048:
+            # the per-instance reduction partials hold (SA_MAX_NCV + 8) x 2048 values
+            raise ValueError(f"an instance needs {blk.max()} workgroups of {self.rpb} rows (at most 2048): "
+                             "raise rows_per_block")
+        blk_off = np.zeros(self.B + 1, dtype=np.int64)
+        blk_off[1:] = np.cumsum(blk)
+        self.blk_off_h = blk_off
+        self.word_off_h = row_off // 64
+        dev = self.device
+        self.row_off = torch.from_numpy(row_off).to(dev)
+        self.row_end = torch.from_numpy(row_off[:-1] + sizes).to(dev)
+        self.blk_off = torch.from_numpy(blk_off.astype(np.int32)).to(dev)
+        self.word_off = torch.from_numpy(self.word_off_h.astype(np.int64)).to(dev)
+        self.stride = int(_lib.load().sa_work_bytes(0))
+        self.works = torch.zeros(self.B * self.stride, dtype=torch.uint8, device=dev)
+        self.desc = SaBatch(B=self.B, rows_per_block=self.rpb, row_off=self.row_off.data_ptr(),
+                            row_end=self.row_end.data_ptr(), blk_off=self.blk_off.data_ptr(),
+                            works=self.works.data_ptr(), stride=self.stride, nblocks=int(blk_off[-1]))
+        self._head = torch.empty((self.B, _lib.STATE_HEAD_BYTES), dtype=torch.uint8, pin_memory=True)
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.desc)
+
+    def heads(self) -> list[SaState]:
+        """The scalar heads of every instance's state (one copy, one synchronisation)."""
+        hb = _lib.STATE_HEAD_BYTES
+        view = self.works.view(self.B, self.stride)[:, :hb]
+        self._head.copy_(view, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        out = []
+        base = self._head.data_ptr()
+        for b in range(self.B):
+            st = SaState()
+            ctypes.memmove(ctypes.addressof(st), base + b * hb, hb)
+            out.append(st)
+        return out
+
+    def rows_of(self, b: int) -> slice:
+        return slice(int(self.row_off_h[b]), int(self.row_off_h[b] + self.sizes[b]))
+
+    def words_of(self, b: int) -> slice:
+        return slice(int(self.word_off_h[b]), int(self.word_off_h[b] + (self.sizes[b] + 63) // 64))
+
+
+@dataclass
+class InstanceResult:
+    best_energy: float
+    best_spins: np.ndarray | None
+    best_alpha: float | None
+    rows: list = field(default_factory=list)
+    num_candidates: int = 0
+    raw_energies: list = field(default_factory=list)
+    refined_energies: list = field(default_factory=list)
+
+
+@dataclass
+class BatchOutcome:
+    results: list
+    sweep_time: float
+    post_sweep_time: float
+    steps: int                 # batched Lanczos steps (launch sets)
+    matvecs: int               # operator applications summed over instances
+    local_passes: int
+    packed_candidates: object = None   # device int64 [K][nwords]: column c = every instance's c-th candidate
+    ls_state: object = None
+
+
+class BatchSweep:
+    """The alpha sweep of SpectralAnnealing (diagonal, secant or prev predictor) over a batch."""
+
+    def __init__(self, random_seed: int = 0, num_alphas: int = 128, schedule: str = "sqrt",
+                 schedule_interval=(0.0, 1.0), pred_mode: str = "diagonal", pred_tol: float = 1e-3,
+                 shift_eps: float = 1e-3, eigsh_tol: float = 1e-6, eigsh_maxiter: int = 1000,
+                 eigsh_ncv: int | None = None, max_local_iterations: int = 1000, use_local_search: bool = True,
+                 chunk: int = 8):
+        if pred_mode not in ("diagonal", "secant", "prev"):
+            raise ValueError("the batched sweep supports pred_mode diagonal, secant or prev")
+        self.seed = 0 if random_seed is None else int(random_seed)
+        t0, t1 = (float(v) for v in schedule_interval)
+        t = np.linspace(t0, t1, int(num_alphas))
+        self.alphas = np.sqrt(t) if schedule == "sqrt" else t
+        self.pred_mode, self.pred_tol, self.shift_eps = pred_mode, float(pred_tol), float(shift_eps)
+        self.tol, self.maxiter = float(eigsh_tol), int(eigsh_maxiter)
+        self.ncv_arg = eigsh_ncv
+        self.max_local_iterations = int(max_local_iterations)
+        self.use_local_search = bool(use_local_search)
+        self.chunk = int(chunk)
+
+    def run(self, bp: BatchProblem) -> BatchOutcome:
+        t_start = time.time()
+        prob, B, n, sfx = bp.prob, bp.B, bp.n, bp.prob.sfx
+        dev, dt = bp.device, bp.dtype
+        ncv = default_ncv(int(bp.sizes.min()), self.ncv_arg)
+        keep = max(1, min(_lib.SA_MAX_KEEP, ncv // 2, ncv - 1))
+        ldv = _lib.padded_ld(n, 4 if dt == torch.float32 else 8)
+        Vbuf = torch.zeros((ncv + 1, ldv), dtype=dt, device=dev)
+        V = Vbuf[:, :n]
+        Z, W, Y = (torch.zeros(n, dtype=dt, device=dev) for _ in range(3))
+        s = torch.zeros(n, dtype=dt, device=dev)
+        x_rand = torch.zeros(n, dtype=dt, device=dev)
+        ybuf = [torch.zeros(n, dtype=dt, device=dev) for _ in range(3)]
+        nwords = (n + 63) // 64
+        A = len(self.alphas)
+        cand_bits = torch.zeros((A, nwords), dtype=torch.int64, device=dev)
+        bounds = torch.empty(B, dtype=torch.float64, device=dev)
+        R = bp.ref
+
+        def step_all():
+            call(f"sa_batch_lanczos_step_{sfx}", prob.csr_ref, R, ptr(s), ptr(V), ldv, ptr(Z), ptr(W), stream())
+            call(f"sa_batch_lanczos_thick_restart_{sfx}", R, ptr(s), ptr(V), ldv, ptr(Z), stream())
+
+        def start(x, **kw):
+            call(f"sa_batch_lanczos_start_{sfx}", R, ptr(s), ptr(x), ptr(kw.get("d")), float(kw.get("pexp", 0.0)),
+                 float(kw.get("cx", 1.0)), ptr(kw.get("y")), float(kw.get("cy", 0.0)), ptr(V), ldv, ptr(Z),
+                 int(kw.get("gate", 0)), int(kw.get("only_if", -1)), stream())
+
+        def begin(pred_tol):
+            call("sa_batch_lanczos_begin", R, ncv, self.maxiter, self.tol, ptr(bounds), float(pred_tol), keep,
+                 stream())
+
+        def run_until_done(first_check: int):
+            """Queue steps until every instance stopped; restart REJECTED ones from y_last. Returns heads
+            and each instance's first_beta (the predictor residual, read at the first check)."""
+            check = first_check
+            first = None
+            steps = 0
+            limit = self.maxiter + 8 * ncv + 64
+            while True:
+                for _ in range(check):
+                    step_all()
+                    steps += 1
+                hs = bp.heads()
+                if first is None:
+                    first = np.array([h.first_beta for h in hs])
+                stats = np.array([h.status for h in hs])
+                if np.any(stats == SA_REJECTED):
+                    begin_rej = stats == SA_REJECTED
+                    start(y_last, gate=0, only_if=SA_REJECTED)
+                    self._rejected |= begin_rej
+                    hs = bp.heads()
+                    stats = np.array([h.status for h in hs])
+                if np.all(np.isin(stats, TERMINAL)):
+                    return hs, first, steps
+                if steps > limit:
+                    raise RuntimeError(f"batched Lanczos did not stop after {steps} steps")
+                check = self.chunk
+
+        results = [InstanceResult(float("inf"), None, None) for _ in range(B)]
+        lam_prev = np.full(B, np.nan)
+        y_last = y_prev = None
+        alpha_prev = None
+        total_steps = total_matvecs = 0
+        by_hash = [dict() for _ in range(B)]
+        cand_alpha = [[] for _ in range(B)]       # per instance: alpha slot of each distinct candidate
+        cand_of_row = [[] for _ in range(B)]
+        for ai, alpha in enumerate(self.alphas):
+            call(f"sa_batch_alpha_scaling_{sfx}", R, ptr(prob.d), float(alpha), 1e-12, ptr(s), stream())
+            hs = bp.heads()
+            dsum = np.array([h.r[1] for h in hs])
+            g = np.array([h.r[2] for h in hs])
+            rho = np.array([compute_shift_rho(None if np.isnan(lam_prev[b]) else lam_prev[b], g[b], self.shift_eps)
+                            for b in range(B)])
+            bounds.copy_(torch.from_numpy(rho + g))
+            self._rejected = np.zeros(B, dtype=bool)
+            cold = ai == 0 or y_last is None
+            if cold:
+                begin(np.inf)
+                prob.fill_normal(self.seed, ai * 4, x_rand)
+                start(x_rand)
+                hs, first, steps = run_until_done(self.chunk)
+                pred_res = np.full(B, np.nan)
+            else:
+                if self.pred_mode == "prev":
+                    begin(np.inf)
+                    start(y_last)
+                    hs, first, steps = run_until_done(self.chunk)
+                    pred_res = np.full(B, np.nan)
+                else:
+                    begin(self.pred_tol)
+                    if self.pred_mode == "diagonal" or y_prev is None:
+                        start(y_last, d=prob.d, pexp=0.5 * (alpha - alpha_prev), gate=1)
+                    else:
+                        start(y_last, cx=2.0, y=y_prev, cy=-1.0, gate=1)
+                    hs, first, steps = run_until_done(1)
+                    pred_res = first
+            total_steps += steps
+            stats = np.array([h.status for h in hs])
+            theta = np.array([h.theta for h in hs])
+            isteps = np.array([h.steps for h in hs])
+            total_matvecs += int(isteps.sum())
+            ok = (stats != SA_BROKEN) & np.isfinite(theta) & (theta <= g * (1.0 + max(0.5 * self.shift_eps, 1e-5)))
+            call(f"sa_batch_lanczos_ritz_{sfx}", R, ptr(V), ldv, ptr(y_last), ptr(Y), -1, stream())
+            free = next(b for b in ybuf if b is not y_last and b is not y_prev)
+            call(f"sa_batch_round_{sfx}", R, ptr(Y), ptr(free), ptr(cand_bits[ai]), int(y_last is not None), stream())
+            hs2 = bp.heads()
+            if y_last is not None and not ok.all():
+                for b in np.nonzero(~ok)[0]:          # a failed instance keeps its previous vector
+                    free[bp.rows_of(b)].copy_(y_last[bp.rows_of(b)])
+            bits_h = cand_bits[ai].cpu().numpy()
+            for b in range(B):
+                row = {'alpha': float(alpha), 'alpha_index': ai, 'eigsh_iterations': int(isteps[b]),
+                       'eigsh_converged': bool(stats[b] == SA_CONVERGED and ok[b]), 'lambda1': float(theta[b]),
+                       'rho': float(rho[b]), 'gershgorin': float(g[b]),
+                       'energy_bound': float(-0.5 * theta[b] * dsum[b] * bp.c_h[b]) if ok[b] else float('nan'),
+                       'pred_residual': float(pred_res[b]), 'pred_accepted':
+                           (None if np.isnan(pred_res[b]) else bool(not pred_res[b] > self.pred_tol)),
+                       'residual_estimate': float(hs[b].res), 'lanczos_restarts': int(hs[b].cycles),
+                       'failed': not bool(ok[b]), 'candidate': None}
+                if ok[b]:
+                    key = bits_h[bp.words_of(b)].tobytes()
+                    c = by_hash[b].get(key)
+                    if c is None:
+                        c = len(cand_alpha[b])
+                        by_hash[b][key] = c
+                        cand_alpha[b].append(ai)
+                    row['candidate'] = c
+                results[b].rows.append(row)
+                cand_of_row[b].append(row['candidate'])
+                if ok[b]:
+                    lam_prev[b] = theta[b]
+            y_prev, y_last = y_last, free
+            alpha_prev = alpha
+        torch.cuda.synchronize()
+        sweep_time = time.time() - t_start
+
+        # ---- post-sweep: packed candidates, exact energies, local search, selection ------------
+        t_post = time.time()
+        K = max(len(c) for c in cand_alpha)
+        passes = 0
+        if K == 0:
+            return BatchOutcome(results, sweep_time, 0.0, total_steps, total_matvecs, 0)
+        src = np.zeros((K, B), dtype=np.int32)
+        for b in range(B):
+            ca = cand_alpha[b] or [0]
+            for c in range(K):
+                src[c, b] = ca[min(c, len(ca) - 1)]
+        src_d = torch.from_numpy(src).to(dev)
+        packed = torch.zeros((K, nwords), dtype=torch.int64, device=dev)
+        call("sa_bits_gather", nwords, K, B, ptr(bp.word_off), ptr(src_d), ptr(cand_bits), ptr(packed), stream())
+        e2 = torch.zeros((B, K), dtype=torch.int64, device=dev)
+        call("sa_batch_energy_bits_i64", prob.csr_ref, R, K, ptr(packed), nwords, ptr(e2), stream())
+        raw = -(e2.cpu().numpy() // 2)
+        refined = raw.copy()
+        state = None
+        if self.use_local_search:
+            state = LS.flip_refinement(prob, packed, K, max_iterations=self.max_local_iterations, seed=self.seed)
+            passes = state.passes
+            e2.zero_()
+            call("sa_batch_ls_energy", R, K, state.groups, ptr(state.spins), ptr(state.fields), ptr(e2), stream())
+            refined = -(e2.cpu().numpy() // 2)
+        spins_all = {}
+        for b in range(B):
+            res = results[b]
+            kb = len(cand_alpha[b])
+            res.num_candidates = kb
+            res.raw_energies = [int(v) for v in raw[b, :kb]]
+            res.refined_energies = [int(v) for v in refined[b, :kb]]
+            best_e, best_src, best_alpha = float("inf"), None, None
+            for r, c in zip(res.rows, cand_of_row[b]):
+                if c is None:
+                    continue
+                e = float(raw[b, c])
+                r['raw_energy'] = e
+                r['energy'] = e
+                if e < best_e:
+                    best_e, best_src, best_alpha = e, ('original', c), r['alpha']
+            if self.use_local_search:
+                for r, c in zip(res.rows, cand_of_row[b]):
+                    if c is None:
+                        continue
+                    if refined[b, c] > raw[b, c]:
+                        src_e = (('original', c), float(raw[b, c]))
+                    else:
+                        src_e = (('refined', c), float(refined[b, c]))
+                    r['energy'] = src_e[1]
+                    r['local_search_iterations'] = passes
+                    if src_e[1] < best_e:
+                        best_e, best_src, best_alpha = src_e[1], src_e[0], r['alpha']
+            res.best_energy, res.best_alpha = best_e, best_alpha
+            if best_src is not None:
+                kind, c = best_src
+                if kind == 'refined':
+                    if c not in spins_all:
+                        spins_all[c] = LS.extract(prob, state, c).cpu().numpy()
+                    res.best_spins = spins_all[c][bp.rows_of(b)].astype(np.int8)
+                else:
+                    words = cand_bits[cand_alpha[b][c]].cpu().numpy()[bp.words_of(b)]
+                    bitsv = np.unpackbits(words.view(np.uint8), bitorder="little")[: bp.sizes[b]]
+                    res.best_spins = np.where(bitsv > 0, 1, -1).astype(np.int8)
+        torch.cuda.synchronize()
+        return BatchOutcome(results, sweep_time, time.time() - t_post, total_steps, total_matvecs, passes, packed,
+                            state)

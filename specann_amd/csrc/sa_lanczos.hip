@@ -23,10 +23,12 @@
 
 namespace sa {
 
-__global__ void lz_begin_kernel(void* work, int32_t ncv, int32_t maxiter, double tol, double bound,
+template <class F>
+__global__ void lz_begin_kernel(F fin, int32_t ncv, int32_t maxiter, double tol, double bound, const double* bounds,
                                 double pred_tol, int32_t keep) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  sa_state* st = state_of(work);
+  if (threadIdx.x != 0) return;
+  sa_state* st = state_of(fin.w());
+  if (bounds) bound = bounds[fin.idx()];
   st->status = SA_RUNNING;
   st->j = 0;
   st->steps = 0;
@@ -44,19 +46,17 @@ __global__ void lz_begin_kernel(void* work, int32_t ncv, int32_t maxiter, double
   st->pred_tol = pred_tol;
 }
 
-template <typename T>
-__global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(void* work, int64_t n, const T* __restrict__ s,
-                                                            const T* __restrict__ x, const T* __restrict__ d,
-                                                            T pexp, double cx, const T* __restrict__ y, double cy,
-                                                            T* __restrict__ V, T* __restrict__ Z, int32_t gate,
-                                                            int32_t only_if) {
-  const sa_state* st = state_of(work);
+template <typename T, class M>
+__global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(M map, const T* __restrict__ s, const T* __restrict__ x,
+                                                            const T* __restrict__ d, T pexp, double cx,
+                                                            const T* __restrict__ y, double cy, T* __restrict__ V,
+                                                            T* __restrict__ Z, int32_t gate, int32_t only_if) {
+  const Slice sl = map.slice();
+  const sa_state* st = state_of(sl.work);
   if (only_if >= 0 && st->status != only_if) return;
   double acc[1] = {0.0};
   const T clampv = (T)1e-12;
-  int64_t lo, hi;
-  block_rows(n, lo, hi);
-  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+  for (int64_t r = sl.lo + threadIdx.x; r < sl.hi; r += blockDim.x) {
     T v;
     if (y) {
       v = (T)(cx * (double)x[r] + cy * (double)y[r]);
@@ -72,14 +72,15 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(void* work, int64_t 
     Z[r] = s ? s[r] * v : v;
     acc[0] += (double)v * (double)v;
   }
-  block_partials<1, RED_SUM>(acc, work);
+  block_partials<1, RED_SUM>(acc, sl.work, sl.blk);
 }
 
 // ||x0||, sigma_0 and the fresh state of a started solve (phase 2 of the norm of lz_start).
-__global__ __launch_bounds__(SA_BLOCK) void lz_start_finish_kernel(void* work, int G, int32_t gate, int32_t only_if) {
-  sa_state* st = state_of(work);
+template <class F>
+__global__ __launch_bounds__(SA_BLOCK) void lz_start_finish_kernel(F fin, int32_t gate, int32_t only_if) {
+  sa_state* st = state_of(fin.w());
   if (only_if >= 0 && st->status != only_if) return;
-  const double tot = sum_partials(partials_of(work), G, 1, 0);
+  const double tot = sum_partials(partials_of(fin.w()), fin.g(), 1, 0);
   if (threadIdx.x == 0) {
     const double nrm = sqrt(tot);
     st->r[3] = nrm;
@@ -116,21 +117,21 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_combine_kernel(void* work, int64_
 // h_m = sigma_m v_m . W for every basis vector of the cycle. 16-byte loads, chunks of 8 vectors per
 // pass over the workgroup's rows; per-workgroup partials are stored column-major (part[m * G + b]) and
 // the last workgroup sums them in workgroup order, a 32-lane group per column.
-template <typename T>
-__global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(void* work, int64_t n, const T* __restrict__ V,
-                                                           int64_t ldv, const T* __restrict__ W) {
+template <typename T, class M>
+__global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(M map, const T* __restrict__ V, int64_t ldv,
+                                                           const T* __restrict__ W) {
   using VT = typename vec16<T>::type;
   constexpr int L = vec16<T>::L;
   __shared__ double red[2][SA_WAVES_PER_BLOCK][SA_RED_COLS];
-  const sa_state* st = state_of(work);
+  const Slice sl = map.slice();
+  const sa_state* st = state_of(sl.work);
   if (st->status != SA_RUNNING) return;
   const int j = st->j;
   const int nch = j / SA_RED_COLS + 1;
-  double* part = partials_of(work);
-  const int G = (int)gridDim.x;
+  double* part = partials_of(sl.work);
+  const int G = sl.G;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int64_t lo, hi;
-  block_rows(n, lo, hi);
+  const int64_t lo = sl.lo, hi = sl.hi;
   const int64_t nv = hi > lo ? (hi - lo) / L : 0;
   for (int ch = 0; ch < nch; ++ch) {
     const int m0 = ch * SA_RED_COLS;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(void* work, int64_t n
     __syncthreads();
     if ((int)threadIdx.x < mc) {
       const int c = threadIdx.x;
-      part[(int64_t)(m0 + c) * G + blockIdx.x] =
+      part[(int64_t)(m0 + c) * G + sl.blk] =
           (red[ch & 1][0][c] + red[ch & 1][1][c]) + (red[ch & 1][2][c] + red[ch & 1][3][c]);
     }
   }
@@ -173,11 +174,14 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(void* work, int64_t n
 
 // Phase 2 of the projections: column m summed over the G workgroups by a 32-lane group, eight loads
 // in flight per lane, in workgroup order per lane and a fixed shuffle tree across lanes.
-__global__ __launch_bounds__(SA_BLOCK) void lz_dots_finish_kernel(void* work, int G) {
-  sa_state* st = state_of(work);
+template <class F>
+__global__ __launch_bounds__(SA_BLOCK) void lz_dots_finish_kernel(F fin) {
+  sa_state* st = state_of(fin.w());
   if (st->status != SA_RUNNING) return;
   const int j = st->j;
-  const double* part = partials_of(work);
+  const int G = fin.g();
+  const double ws = F::kScaleW ? st->sigma[j] : 1.0;  // batched: W was not scaled by sigma_j
+  const double* part = partials_of(fin.w());
   const int lane32 = threadIdx.x & 31, grp = threadIdx.x >> 5;
   for (int mb = 0; mb <= j; mb += SA_BLOCK / 32) {
     const int m = mb + grp;
@@ -196,7 +200,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_finish_kernel(void* work, in
     }
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) x += __shfl_xor(x, o, 32);
-    if (lane32 == 0 && m <= j) st->h[m] = st->sigma[m] * x;
+    if (lane32 == 0 && m <= j) st->h[m] = st->sigma[m] * ws * x;
   }
   __syncthreads();
   if (threadIdx.x == 0) st->a[j] = st->h[j];
@@ -447,24 +451,23 @@ __device__ double largest_eigenvalue(const Proj& T, const double* bounds) {
   return 0.5 * (lo + hi);
 }
 
-template <typename T>
-__global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n, const T* __restrict__ s,
-                                                           T* __restrict__ V, int64_t ldv, T* __restrict__ Z,
-                                                           const T* __restrict__ W) {
+template <typename T, class M>
+__global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(M map, const T* __restrict__ s, T* __restrict__ V,
+                                                           int64_t ldv, T* __restrict__ Z, const T* __restrict__ W) {
   __shared__ double coef[SA_MAX_NCV + 1];
   __shared__ double red[SA_WAVES_PER_BLOCK];
-  const sa_state* st = state_of(work);
+  const Slice sl = map.slice();
+  const sa_state* st = state_of(sl.work);
   if (st->status != SA_RUNNING) return;
   const int j = st->j;
+  const double ws = M::kScaleW ? st->sigma[j] : 1.0;  // batched: W was not scaled by sigma_j
   for (int m = threadIdx.x; m <= j; m += blockDim.x) coef[m] = st->h[m] * st->sigma[m];
   __syncthreads();
   T* __restrict__ vn = V + (int64_t)(j + 1) * ldv;
   double acc = 0.0;
-  int64_t lo, hi;
-  block_rows(n, lo, hi);
-  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+  for (int64_t r = sl.lo + threadIdx.x; r < sl.hi; r += blockDim.x) {
     double wv[8];
-    wv[0] = (double)W[r];
+    wv[0] = ws * (double)W[r];
 #pragma unroll
     for (int u = 1; u < 8; ++u) wv[u] = 0.0;
     int m = 0;
@@ -482,18 +485,19 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(void* work, int64_t n
     Z[r] = s ? s[r] * wt : wt;
     acc += (double)wt * (double)wt;
   }
-  double* part = partials_of(work);
+  double* part = partials_of(sl.work);
   const double bs = block_sum(acc, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+  if (threadIdx.x == 0) part[sl.blk] = bs;
 }
 
 // Phase 2 of the Gram-Schmidt step: beta_j from the G partial squared norms, then the FP64 Ritz solve
 // of the projected matrix and the stopping test, in one workgroup.
-__global__ __launch_bounds__(SA_BLOCK) void lz_orth_finish_kernel(void* work, int G) {
-  sa_state* st = state_of(work);
+template <class F>
+__global__ __launch_bounds__(SA_BLOCK) void lz_orth_finish_kernel(F fin) {
+  sa_state* st = state_of(fin.w());
   if (st->status != SA_RUNNING) return;
   const int j = st->j;
-  const double beta = sqrt(sum_partials(partials_of(work), G, 1, 0));
+  const double beta = sqrt(sum_partials(partials_of(fin.w()), fin.g(), 1, 0));
 
   __shared__ double la[SA_MAX_NCV], lb[SA_MAX_NCV], lb2[SA_MAX_NCV], lar[SA_MAX_NCV], lar2[SA_MAX_NCV];
   __shared__ double lvec[SA_MAX_NCV], lwork[6 * SA_MAX_NCV], lbounds[4];
@@ -563,7 +567,9 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_finish_kernel(void* work, in
 // bordered by arrow_u = beta_{m-1} s_u[m-1]. Scratch lives in LDS while m <= kPrepLds.
 constexpr int kPrepLds = 48;
 
-__global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(void* work) {
+template <class F>
+__global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(F fin) {
+  void* work = fin.w();
   __shared__ double la[SA_MAX_NCV], lb[SA_MAX_NCV], lb2[SA_MAX_NCV], lar[SA_MAX_NCV], lar2[SA_MAX_NCV];
   __shared__ double lbounds[4], ltheta[SA_MAX_KEEP];
   __shared__ double lscr[SA_MAX_KEEP * 8 * kPrepLds];
@@ -667,10 +673,12 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(void* work) 
 // before writing any, so rows 0..kept-1 can take the Ritz vectors and row kept the last Lanczos vector.
 // A thread owns 16 bytes of rows (4 floats or 2 doubles) and keeps four basis loads in flight; the
 // coefficients are stored basis-row-major in LDS so one broadcast read serves a whole row group.
-template <typename T>
-__global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(void* work, int64_t n, const T* __restrict__ s,
+template <typename T, class M>
+__global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(M map, const T* __restrict__ s,
                                                                     T* __restrict__ V, int64_t ldv,
                                                                     T* __restrict__ Z) {
+  const Slice sl = map.slice();
+  void* work = sl.work;
   using VT = typename vec16<T>::type;
   constexpr int L = vec16<T>::L;
   __shared__ double coef[SA_MAX_NCV * SA_MAX_KEEP];   // coef[p * SA_MAX_KEEP + u]
@@ -684,8 +692,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(void* work, 
     coef[p * SA_MAX_KEEP + u] = S[(int64_t)u * SA_MAX_NCV + p] * st->sigma[p];
   }
   __syncthreads();
-  int64_t lo, hi;
-  block_rows(n, lo, hi);
+  const int64_t lo = sl.lo, hi = sl.hi;
   const int64_t nv = hi > lo ? (hi - lo) / L : 0;
   for (int64_t g = threadIdx.x; g < nv; g += blockDim.x) {
     const int64_t r = lo + g * L;
@@ -759,8 +766,9 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(void* work, 
 
 // The state after the rotation: kept vectors are unit (sigma 1), the last Lanczos vector moves to
 // row kept, and the cycle continues from j = kept.
-__global__ void lz_thick_finish_kernel(void* work) {
-  sa_state* st = state_of(work);
+template <class F>
+__global__ void lz_thick_finish_kernel(F fin) {
+  sa_state* st = state_of(fin.w());
   if (st->status != SA_RESTART || threadIdx.x != 0) return;
   const int m = st->j;
   const int kk = st->kept;
@@ -772,20 +780,19 @@ __global__ void lz_thick_finish_kernel(void* work) {
   st->status = SA_RUNNING;
 }
 
-template <typename T>
-__global__ __launch_bounds__(SA_BLOCK) void lz_ritz_kernel(void* work, int64_t n, const T* __restrict__ V,
-                                                           int64_t ldv, const T* __restrict__ y_last,
-                                                           T* __restrict__ y, int32_t only_if) {
+template <typename T, class M>
+__global__ __launch_bounds__(SA_BLOCK) void lz_ritz_kernel(M map, const T* __restrict__ V, int64_t ldv,
+                                                           const T* __restrict__ y_last, T* __restrict__ y,
+                                                           int32_t only_if) {
   __shared__ double coef[SA_MAX_NCV];
-  sa_state* st = state_of(work);
+  const Slice sl = map.slice();
+  const sa_state* st = state_of(sl.work);
   if (only_if >= 0 && st->status != only_if) return;
   const int m = st->j;
   for (int i = threadIdx.x; i < m; i += blockDim.x) coef[i] = st->ritz[i] * st->sigma[i];
   __syncthreads();
   double acc[2] = {0.0, 0.0};
-  int64_t lo, hi;
-  block_rows(n, lo, hi);
-  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+  for (int64_t r = sl.lo + threadIdx.x; r < sl.hi; r += blockDim.x) {
     double v4[4] = {0.0, 0.0, 0.0, 0.0};
     int i = 0;
     for (; i + 3 < m; i += 4) {
@@ -799,7 +806,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_ritz_kernel(void* work, int64_t n
     acc[0] += (double)vt * (double)vt;
     if (y_last) acc[1] += (double)vt * (double)y_last[r];
   }
-  block_partials<2, RED_SUM>(acc, work);
+  block_partials<2, RED_SUM>(acc, sl.work, sl.blk);
 }
 
 static inline int red_grid(int64_t n) { return grid_for(n, SA_BLOCK * 4, SA_RED_GRID); }
@@ -815,11 +822,12 @@ static int lanczos_start(void* work, int64_t n, const T* s, const T* x, const T*
                          hipStream_t stream) {
   SA_REQUIRE(work && n > 0 && x && V && Z && ldv >= n, "bad arguments");
   const int G = red_grid(n);
-  hipLaunchKernelGGL(lz_start_kernel<T>, dim3(G), dim3(SA_BLOCK), 0, stream, work, n, s, x, d, (T)pexp, cx, y, cy, V,
-                     Z, gate, only_if);
+  hipLaunchKernelGGL((lz_start_kernel<T, OneMap>), dim3(G), dim3(SA_BLOCK), 0, stream, OneMap{work, n}, s, x, d,
+                     (T)pexp, cx, y, cy, V, Z, gate, only_if);
   int rc = check_launch("lanczos_start");
   if (rc) return rc;
-  hipLaunchKernelGGL(lz_start_finish_kernel, dim3(1), dim3(SA_BLOCK), 0, stream, work, G, gate, only_if);
+  hipLaunchKernelGGL(lz_start_finish_kernel<OneFin>, dim3(1), dim3(SA_BLOCK), 0, stream, OneFin{work, G}, gate,
+                     only_if);
   return check_launch("lanczos_start_finish");
 }
 
@@ -844,17 +852,19 @@ static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, int64_t l
     if (rc) return rc;
   }
   const int Gd = grid_for(n, SA_BLOCK * 4, SA_DOTS_GRID);
-  hipLaunchKernelGGL(lz_dots_kernel<T>, dim3(Gd), dim3(SA_BLOCK), 0, stream, work, n, (const T*)V, ldv, (const T*)W);
+  hipLaunchKernelGGL((lz_dots_kernel<T, OneMap>), dim3(Gd), dim3(SA_BLOCK), 0, stream, OneMap{work, n},
+                     (const T*)V, ldv, (const T*)W);
   rc = check_launch("lanczos_dots");
   if (rc) return rc;
-  hipLaunchKernelGGL(lz_dots_finish_kernel, dim3(1), dim3(SA_BLOCK), 0, stream, work, Gd);
+  hipLaunchKernelGGL(lz_dots_finish_kernel<OneFin>, dim3(1), dim3(SA_BLOCK), 0, stream, OneFin{work, Gd});
   rc = check_launch("lanczos_dots_finish");
   if (rc) return rc;
   const int Go = grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID);
-  hipLaunchKernelGGL(lz_orth_kernel<T>, dim3(Go), dim3(SA_BLOCK), 0, stream, work, n, s, V, ldv, Z, (const T*)W);
+  hipLaunchKernelGGL((lz_orth_kernel<T, OneMap>), dim3(Go), dim3(SA_BLOCK), 0, stream, OneMap{work, n}, s, V, ldv,
+                     Z, (const T*)W);
   rc = check_launch("lanczos_orth");
   if (rc) return rc;
-  hipLaunchKernelGGL(lz_orth_finish_kernel, dim3(1), dim3(SA_BLOCK), 0, stream, work, Go);
+  hipLaunchKernelGGL(lz_orth_finish_kernel<OneFin>, dim3(1), dim3(SA_BLOCK), 0, stream, OneFin{work, Go});
   return check_launch("lanczos_orth_finish");
 }
 
@@ -863,14 +873,14 @@ static int lanczos_thick_restart(void* work, int64_t n, const T* s, T* V, int64_
   SA_REQUIRE(work && n > 0 && V && Z && ldv >= n, "bad arguments");
   SA_REQUIRE(aligned16(V) && aligned16(Z) && aligned16(s) && ldv % vec16<T>::L == 0,
              "V, Z and s must be 16-byte aligned and ldv a multiple of 16 bytes");
-  hipLaunchKernelGGL(lz_thick_prepare_kernel, dim3(1), dim3(SA_BLOCK), 0, stream, work);
+  hipLaunchKernelGGL(lz_thick_prepare_kernel<OneFin>, dim3(1), dim3(SA_BLOCK), 0, stream, OneFin{work, 1});
   int rc = check_launch("lanczos_thick_prepare");
   if (rc) return rc;
-  hipLaunchKernelGGL(lz_thick_vectors_kernel<T>, dim3(grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID)), dim3(SA_BLOCK), 0,
-                     stream, work, n, s, V, ldv, Z);
+  hipLaunchKernelGGL((lz_thick_vectors_kernel<T, OneMap>), dim3(grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID)),
+                     dim3(SA_BLOCK), 0, stream, OneMap{work, n}, s, V, ldv, Z);
   rc = check_launch("lanczos_thick_vectors");
   if (rc) return rc;
-  hipLaunchKernelGGL(lz_thick_finish_kernel, dim3(1), dim3(64), 0, stream, work);
+  hipLaunchKernelGGL(lz_thick_finish_kernel<OneFin>, dim3(1), dim3(64), 0, stream, OneFin{work, 1});
   return check_launch("lanczos_thick_finish");
 }
 
@@ -879,7 +889,8 @@ static int lanczos_ritz(void* work, int64_t n, const T* V, int64_t ldv, const T*
                         hipStream_t stream) {
   SA_REQUIRE(work && n > 0 && V && y && ldv >= n, "bad arguments");
   const int G = red_grid(n);
-  hipLaunchKernelGGL(lz_ritz_kernel<T>, dim3(G), dim3(SA_BLOCK), 0, stream, work, n, V, ldv, y_last, y, only_if);
+  hipLaunchKernelGGL((lz_ritz_kernel<T, OneMap>), dim3(G), dim3(SA_BLOCK), 0, stream, OneMap{work, n}, V, ldv,
+                     y_last, y, only_if);
   int rc = check_launch("lanczos_ritz");
   if (rc) return rc;
   FinishSpec f{};
@@ -889,6 +900,105 @@ static int lanczos_ritz(void* work, int64_t n, const T* V, int64_t ldv, const T*
   f.dst[0] = (int32_t)offsetof(sa_state, r[4]);
   f.dst[1] = (int32_t)offsetof(sa_state, r[5]);
   return launch_finish(work, G, f, stream);
+}
+
+// ---- batched forms ------------------------------------------------------------------------------
+bool batch_ok(const sa_batch* Bt) {
+  return Bt && Bt->B >= 1 && Bt->row_off && Bt->row_end && Bt->blk_off && Bt->works && Bt->nblocks >= Bt->B &&
+         Bt->nblocks < (1ll << 31) && Bt->rows_per_block >= 64 && Bt->rows_per_block % 64 == 0 &&
+         Bt->stride >= (int64_t)(state_bytes_padded() + (kPartials + kRestart) * sizeof(double));
+}
+
+BatchMap bmap(const sa_batch* Bt) {
+  return BatchMap{reinterpret_cast<char*>(Bt->works), Bt->stride, Bt->row_off, Bt->row_end, Bt->blk_off, Bt->B,
+                  Bt->rows_per_block};
+}
+
+BatchFin bfin(const sa_batch* Bt) {
+  return BatchFin{reinterpret_cast<char*>(Bt->works), Bt->stride, Bt->blk_off};
+}
+
+static int batch_begin(const sa_batch* Bt, int32_t ncv, int32_t maxiter, double tol, const double* bounds,
+                       double pred_tol, int32_t keep, hipStream_t stream) {
+  SA_REQUIRE(batch_ok(Bt) && bounds, "bad arguments");
+  SA_REQUIRE(ncv >= 2 && ncv <= SA_MAX_NCV && maxiter >= 1, "bad arguments (2 <= ncv <= 128)");
+  SA_REQUIRE(keep >= 1 && keep <= SA_MAX_KEEP && keep < ncv, "keep must be in [1, min(16, ncv - 1)]");
+  hipLaunchKernelGGL(lz_begin_kernel<BatchFin>, dim3(Bt->B), dim3(64), 0, stream, bfin(Bt), ncv, maxiter, tol, 0.0,
+                     bounds, pred_tol, keep);
+  return check_launch("batch_lanczos_begin");
+}
+
+template <typename T>
+static int batch_start(const sa_batch* Bt, const T* s, const T* x, const T* d, double pexp, double cx, const T* y,
+                       double cy, T* V, int64_t ldv, T* Z, int32_t gate, int32_t only_if, hipStream_t stream) {
+  SA_REQUIRE(batch_ok(Bt) && x && V && Z, "bad arguments");
+  hipLaunchKernelGGL((lz_start_kernel<T, BatchMap>), dim3((unsigned)Bt->nblocks), dim3(SA_BLOCK), 0, stream, bmap(Bt),
+                     s, x, d, (T)pexp, cx, y, cy, V, Z, gate, only_if);
+  int rc = check_launch("batch_lanczos_start");
+  if (rc) return rc;
+  hipLaunchKernelGGL(lz_start_finish_kernel<BatchFin>, dim3(Bt->B), dim3(SA_BLOCK), 0, stream, bfin(Bt), gate,
+                     only_if);
+  return check_launch("batch_lanczos_start_finish");
+}
+
+// One step for every running instance: the plain fused SpMV over the packed CSR (sigma_j is applied
+// by the projections and the update, per instance), projections, update, Ritz solves.
+template <typename T>
+static int batch_step(const sa_csr* A, const sa_batch* Bt, const T* s, T* V, int64_t ldv, T* Z, T* W,
+                      hipStream_t stream) {
+  SA_REQUIRE(A && batch_ok(Bt) && V && Z && W && A->tile_rows && ldv >= A->n, "bad arguments");
+  SA_REQUIRE(A->npanels == 0, "the batched step uses the plain CSR layout");
+  SA_REQUIRE(aligned16(V) && aligned16(W) && ldv % vec16<T>::L == 0,
+             "V and W must be 16-byte aligned and ldv a multiple of 16 bytes");
+  int rc = spmv_apply<T>(A, s, Z, 0.0, nullptr, 1.0, W, nullptr, stream);
+  if (rc) return rc;
+  const dim3 grid((unsigned)Bt->nblocks), fgrid(Bt->B);
+  hipLaunchKernelGGL((lz_dots_kernel<T, BatchMap>), grid, dim3(SA_BLOCK), 0, stream, bmap(Bt), (const T*)V, ldv,
+                     (const T*)W);
+  rc = check_launch("batch_lanczos_dots");
+  if (rc) return rc;
+  hipLaunchKernelGGL(lz_dots_finish_kernel<BatchFin>, fgrid, dim3(SA_BLOCK), 0, stream, bfin(Bt));
+  rc = check_launch("batch_lanczos_dots_finish");
+  if (rc) return rc;
+  hipLaunchKernelGGL((lz_orth_kernel<T, BatchMap>), grid, dim3(SA_BLOCK), 0, stream, bmap(Bt), s, V, ldv, Z,
+                     (const T*)W);
+  rc = check_launch("batch_lanczos_orth");
+  if (rc) return rc;
+  hipLaunchKernelGGL(lz_orth_finish_kernel<BatchFin>, fgrid, dim3(SA_BLOCK), 0, stream, bfin(Bt));
+  return check_launch("batch_lanczos_orth_finish");
+}
+
+template <typename T>
+static int batch_thick_restart(const sa_batch* Bt, const T* s, T* V, int64_t ldv, T* Z, hipStream_t stream) {
+  SA_REQUIRE(batch_ok(Bt) && V && Z, "bad arguments");
+  SA_REQUIRE(aligned16(V) && aligned16(Z) && aligned16(s) && ldv % vec16<T>::L == 0,
+             "V, Z and s must be 16-byte aligned and ldv a multiple of 16 bytes");
+  hipLaunchKernelGGL(lz_thick_prepare_kernel<BatchFin>, dim3(Bt->B), dim3(SA_BLOCK), 0, stream, bfin(Bt));
+  int rc = check_launch("batch_thick_prepare");
+  if (rc) return rc;
+  hipLaunchKernelGGL((lz_thick_vectors_kernel<T, BatchMap>), dim3((unsigned)Bt->nblocks), dim3(SA_BLOCK), 0, stream,
+                     bmap(Bt), s, V, ldv, Z);
+  rc = check_launch("batch_thick_vectors");
+  if (rc) return rc;
+  hipLaunchKernelGGL(lz_thick_finish_kernel<BatchFin>, dim3(Bt->B), dim3(64), 0, stream, bfin(Bt));
+  return check_launch("batch_thick_finish");
+}
+
+template <typename T>
+static int batch_ritz(const sa_batch* Bt, const T* V, int64_t ldv, const T* y_last, T* y, int32_t only_if,
+                      hipStream_t stream) {
+  SA_REQUIRE(batch_ok(Bt) && V && y, "bad arguments");
+  hipLaunchKernelGGL((lz_ritz_kernel<T, BatchMap>), dim3((unsigned)Bt->nblocks), dim3(SA_BLOCK), 0, stream, bmap(Bt),
+                     V, ldv, y_last, y, only_if);
+  int rc = check_launch("batch_lanczos_ritz");
+  if (rc) return rc;
+  FinishSpec f{};
+  f.cols = 2;
+  f.only_if = only_if;
+  f.take_sqrt[0] = 1;
+  f.dst[0] = (int32_t)offsetof(sa_state, r[4]);
+  f.dst[1] = (int32_t)offsetof(sa_state, r[5]);
+  return launch_finish_batch(Bt, f, stream);
 }
 
 }  // namespace sa
@@ -901,7 +1011,8 @@ int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, doubl
                      int32_t keep, hipStream_t stream) {
   SA_REQUIRE(work && ncv >= 2 && ncv <= SA_MAX_NCV && maxiter >= 1, "bad arguments (2 <= ncv <= 128)");
   SA_REQUIRE(keep >= 1 && keep <= SA_MAX_KEEP && keep < ncv, "keep must be in [1, min(16, ncv - 1)]");
-  hipLaunchKernelGGL(lz_begin_kernel, dim3(1), dim3(64), 0, stream, work, ncv, maxiter, tol, bound, pred_tol, keep);
+  hipLaunchKernelGGL(lz_begin_kernel<OneFin>, dim3(1), dim3(64), 0, stream, OneFin{work, 1}, ncv, maxiter, tol, bound,
+                     (const double*)nullptr, pred_tol, keep);
   return check_launch("lanczos_begin");
 }
 
@@ -945,6 +1056,53 @@ int sa_lanczos_ritz_f32(void* work, int64_t n, const float* V, int64_t ldv, cons
 int sa_lanczos_ritz_f64(void* work, int64_t n, const double* V, int64_t ldv, const double* y_last, double* y,
                         int32_t only_if, hipStream_t stream) {
   return lanczos_ritz<double>(work, n, V, ldv, y_last, y, only_if, stream);
+}
+
+int sa_batch_lanczos_begin(const sa_batch* Bt, int32_t ncv, int32_t maxiter, double tol, const double* bounds,
+                           double pred_tol, int32_t keep, hipStream_t stream) {
+  return batch_begin(Bt, ncv, maxiter, tol, bounds, pred_tol, keep, stream);
+}
+
+int sa_batch_lanczos_start_f32(const sa_batch* Bt, const float* s, const float* x, const float* d, double pexp,
+                               double cx, const float* y, double cy, float* V, int64_t ldv, float* Z, int32_t gate,
+                               int32_t only_if, hipStream_t stream) {
+  return batch_start<float>(Bt, s, x, d, pexp, cx, y, cy, V, ldv, Z, gate, only_if, stream);
+}
+
+int sa_batch_lanczos_start_f64(const sa_batch* Bt, const double* s, const double* x, const double* d, double pexp,
+                               double cx, const double* y, double cy, double* V, int64_t ldv, double* Z,
+                               int32_t gate, int32_t only_if, hipStream_t stream) {
+  return batch_start<double>(Bt, s, x, d, pexp, cx, y, cy, V, ldv, Z, gate, only_if, stream);
+}
+
+int sa_batch_lanczos_step_f32(const sa_csr* A, const sa_batch* Bt, const float* s, float* V, int64_t ldv, float* Z,
+                              float* W, hipStream_t stream) {
+  return batch_step<float>(A, Bt, s, V, ldv, Z, W, stream);
+}
+
+int sa_batch_lanczos_step_f64(const sa_csr* A, const sa_batch* Bt, const double* s, double* V, int64_t ldv,
+                              double* Z, double* W, hipStream_t stream) {
+  return batch_step<double>(A, Bt, s, V, ldv, Z, W, stream);
+}
+
+int sa_batch_lanczos_thick_restart_f32(const sa_batch* Bt, const float* s, float* V, int64_t ldv, float* Z,
+                                       hipStream_t stream) {
+  return batch_thick_restart<float>(Bt, s, V, ldv, Z, stream);
+}
+
+int sa_batch_lanczos_thick_restart_f64(const sa_batch* Bt, const double* s, double* V, int64_t ldv, double* Z,
+                                       hipStream_t stream) {
+  return batch_thick_restart<double>(Bt, s, V, ldv, Z, stream);
+}
+
+int sa_batch_lanczos_ritz_f32(const sa_batch* Bt, const float* V, int64_t ldv, const float* y_last, float* y,
+                              int32_t only_if, hipStream_t stream) {
+  return batch_ritz<float>(Bt, V, ldv, y_last, y, only_if, stream);
+}
+
+int sa_batch_lanczos_ritz_f64(const sa_batch* Bt, const double* V, int64_t ldv, const double* y_last, double* y,
+                              int32_t only_if, hipStream_t stream) {
+  return batch_ritz<double>(Bt, V, ldv, y_last, y, only_if, stream);
 }
 
 }  // extern "C"

@@ -6,7 +6,7 @@
 // reproducible for a fixed (n, grid).
 #pragma once
 
-#include "sa_common.hpp"
+#include "sa_map.hpp"
 
 namespace sa {
 
@@ -35,7 +35,7 @@ enum RedOp { RED_SUM = 0, RED_MAX = 1 };
 // visible, and sums them in workgroup order. No workgroup ever waits on another, and no device-scope
 // fence is issued per workgroup (on gfx950 such a fence writes back the XCD's L2).
 template <int C, int OP>
-__device__ void block_partials(double (&v)[C], void* work) {
+__device__ void block_partials(double (&v)[C], void* work, int slot) {
   __shared__ double red[SA_WAVES_PER_BLOCK];
   double* part = partials_of(work);
 #pragma unroll
@@ -51,16 +51,23 @@ __device__ void block_partials(double (&v)[C], void* work) {
       if (threadIdx.x == 0) x = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
       __syncthreads();
     }
-    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * C + c] = x;
+    if (threadIdx.x == 0) part[(int64_t)slot * C + c] = x;
   }
 }
 
-__device__ inline void block_partials_u64(uint64_t v, void* work) {
+template <int C, int OP>
+__device__ void block_partials(double (&v)[C], void* work) {
+  block_partials<C, OP>(v, work, (int)blockIdx.x);
+}
+
+__device__ inline void block_partials_u64(uint64_t v, void* work, int slot) {
   __shared__ uint64_t red[SA_WAVES_PER_BLOCK];
   uint64_t* part = reinterpret_cast<uint64_t*>(partials_of(work));
   const uint64_t x = block_sum(v, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = x;
+  if (threadIdx.x == 0) part[slot] = x;
 }
+
+__device__ inline void block_partials_u64(uint64_t v, void* work) { block_partials_u64(v, work, (int)blockIdx.x); }
 
 // Phase 2 helper for one workgroup: sum of part[b * stride + off] over b < G (each thread in
 // workgroup order, then a fixed tree); the result is returned to every thread.
@@ -90,5 +97,11 @@ struct FinishSpec {
 };
 
 int launch_finish(void* work, int G, const FinishSpec& spec, hipStream_t stream);
+int launch_finish_batch(const sa_batch* Bt, const FinishSpec& spec, hipStream_t stream);
+
+// The batch descriptor's validity and its device-side maps (sa_lanczos.hip).
+bool batch_ok(const sa_batch* Bt);
+BatchMap bmap(const sa_batch* Bt);
+BatchFin bfin(const sa_batch* Bt);
 
 }  // namespace sa

@@ -24,9 +24,12 @@ int check_launch(const char* what) {
 
 // ---------------------------------------------------------------------------------------------
 // Phase 2 of the two-phase grid reductions (see sa_reduce.hpp).
-__global__ __launch_bounds__(SA_BLOCK) void finish_kernel(void* work, int G, FinishSpec spec) {
+template <class F>
+__global__ __launch_bounds__(SA_BLOCK) void finish_kernel(F fin, FinishSpec spec) {
   __shared__ double red[SA_WAVES_PER_BLOCK];
   __shared__ uint64_t redu[SA_WAVES_PER_BLOCK];
+  void* work = fin.w();
+  const int G = fin.g();
   sa_state* st = state_of(work);
   if (spec.only_if >= 0 && st->status != spec.only_if) return;
   const double* part = partials_of(work);
@@ -61,8 +64,14 @@ __global__ __launch_bounds__(SA_BLOCK) void finish_kernel(void* work, int G, Fin
 }
 
 int launch_finish(void* work, int G, const FinishSpec& spec, hipStream_t stream) {
-  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(SA_BLOCK), 0, stream, work, G, spec);
+  hipLaunchKernelGGL(finish_kernel<OneFin>, dim3(1), dim3(SA_BLOCK), 0, stream, OneFin{work, G}, spec);
   return check_launch("finish");
+}
+
+int launch_finish_batch(const sa_batch* Bt, const FinishSpec& spec, hipStream_t stream) {
+  hipLaunchKernelGGL(finish_kernel<BatchFin>, dim3(Bt->B), dim3(SA_BLOCK), 0, stream,
+                     BatchFin{reinterpret_cast<char*>(Bt->works), Bt->stride, Bt->blk_off}, spec);
+  return check_launch("finish_batch");
 }
 
 static FinishSpec finish_spec(int cols, int only_if = -1) {
@@ -111,6 +120,19 @@ __global__ __launch_bounds__(SA_BLOCK) void normalize_kernel(int64_t nnz, T* __r
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += stride) vals[e] = vals[e] / c;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += stride) d[r] = deg[r] / c;
+}
+
+// Per-row normalisation of a block-diagonal batch: row r and its entries are divided by c_row[r]
+// (its instance's factor), the same operations as normalize_kernel per instance.
+template <typename T>
+__global__ __launch_bounds__(SA_BLOCK) void normalize_rows_kernel(int64_t n, const int32_t* __restrict__ indptr,
+                                                                  T* __restrict__ vals, const T* __restrict__ deg,
+                                                                  T* __restrict__ d, const T* __restrict__ c_row) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const T c = c_row[r];
+    for (int32_t e = indptr[r]; e < indptr[r + 1]; ++e) vals[e] = vals[e] / c;
+    d[r] = deg[r] / c;
+  }
 }
 
 __global__ __launch_bounds__(SA_BLOCK) void tiles_kernel(int64_t n, const int32_t* __restrict__ indptr,
@@ -349,6 +371,22 @@ int sa_normalize_f64(int64_t nnz, double* vals, int64_t n, const double* deg, do
   hipLaunchKernelGGL(normalize_kernel<double>, dim3(SA_GRID(nnz > n ? nnz : n)), dim3(SA_BLOCK), 0, stream, nnz,
                      vals, n, deg, d, c);
   return check_launch("normalize_f64");
+}
+
+int sa_normalize_rows_f32(const sa_csr* A, float* vals, const float* deg, float* d, const float* c_row,
+                          hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->indptr && vals && deg && d && c_row, "bad arguments");
+  hipLaunchKernelGGL(normalize_rows_kernel<float>, dim3(SA_GRID(A->n)), dim3(SA_BLOCK), 0, stream, A->n, A->indptr,
+                     vals, deg, d, c_row);
+  return check_launch("normalize_rows_f32");
+}
+
+int sa_normalize_rows_f64(const sa_csr* A, double* vals, const double* deg, double* d, const double* c_row,
+                          hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->indptr && vals && deg && d && c_row, "bad arguments");
+  hipLaunchKernelGGL(normalize_rows_kernel<double>, dim3(SA_GRID(A->n)), dim3(SA_BLOCK), 0, stream, A->n, A->indptr,
+                     vals, deg, d, c_row);
+  return check_launch("normalize_rows_f64");
 }
 
 int sa_csr_tiles(int64_t n, const int32_t* indptr, int32_t tile_nnz, int32_t* tile_rows, int64_t ntiles,

@@ -83,7 +83,9 @@ class DeviceProblem:
     """
 
     def __init__(self, J: sp.spmatrix, dtype: torch.dtype = torch.float32, tile_nnz: int | None = None,
-                 lanes_per_row: int = 0, device: torch.device | None = None):
+                 lanes_per_row: int = 0, device: torch.device | None = None, row_c: np.ndarray | None = None):
+        """row_c (host, one factor per row): normalise row r and its entries by row_c[r] instead of one
+        global max degree (a block-diagonal batch, each instance by its own max degree); c is then 1."""
         device = device or require_gpu()
         J = sp.csr_matrix(J)
         if not J.has_sorted_indices:
@@ -129,7 +131,13 @@ class DeviceProblem:
         if not (self.c > 0.0 and math.isfinite(self.c)):
             raise ValueError("J has no nonzero coupling")
         self.d = torch.empty(n, dtype=dtype, device=device)
-        call(f"sa_normalize_{sfx}", nnz, ptr(self.vals), n, ptr(self.deg), ptr(self.d), self.c, s)
+        if row_c is not None:
+            rc = torch.from_numpy(np.ascontiguousarray(row_c, dtype=np.float64)).to(device=device, dtype=dtype)
+            call(f"sa_normalize_rows_{sfx}", ctypes.byref(self.csr), ptr(self.vals), ptr(self.deg), ptr(self.d),
+                 ptr(rc), s)
+            self.c = 1.0
+        else:
+            call(f"sa_normalize_{sfx}", nnz, ptr(self.vals), n, ptr(self.deg), ptr(self.d), self.c, s)
         self.sfx = sfx
         self.row_counts = np.diff(J.indptr).astype(np.int64)   # entries per row = per column (symmetric J)
         self.npanels = 0

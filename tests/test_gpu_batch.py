@@ -1,0 +1,77 @@
+"""The batched (block-diagonal) sweep against one-instance sweeps and the CPU oracle (GPU)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import specann_oracle as O
+from specann_amd import graphs
+from specann_amd.batch import BatchProblem, BatchSweep, pack
+from specann_amd.spectral_annealing import SpectralAnnealing
+from specann_amd.device import DeviceProblem
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances():
+    return [graphs.erdos_renyi(300, 12 / 299, seed=1, kind="neg"),
+            graphs.barabasi_albert(517, 4, seed=2, kind="neg"),
+            graphs.erdos_renyi(1000, 20 / 999, seed=3, kind="pm1"),
+            graphs.barabasi_albert(130, 3, seed=4, kind="pm1"),
+            graphs.erdos_renyi(2200, 10 / 2199, seed=5, kind="neg")]
+
+
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("pred", ["diagonal", "secant"])
+def test_batched_sweep_matches_single_instance_sweeps(gpu, tdtype, pred):
+    Js = _instances()
+    print(f"batch dtype={tdtype} pred={pred} sizes={[J.shape[0] for J in Js]}", flush=True)
+    bp = BatchProblem(Js, tdtype, rows_per_block=256)
+    tol = 1e-6 if tdtype == torch.float32 else 1e-10
+    out = BatchSweep(random_seed=7, num_alphas=10, pred_mode=pred, eigsh_tol=tol).run(bp)
+    prec = 32 if tdtype == torch.float32 else 64
+    for b, J in enumerate(Js):
+        res = out.results[b]
+        assert len(res.rows) == 10
+        Jn, d, c = O.normalize(J, np.float64)
+        for r in res.rows:
+            assert r["eigsh_converged"], (b, r)
+            s, _, g = O.alpha_scaling(d, r["alpha"])
+            lam, _ = O.leading_eigpair(Jn, s)
+            assert abs(r["lambda1"] - lam) <= (2e-5 if prec == 32 else 1e-8) * max(1.0, abs(lam)), (b, r["alpha"])
+            assert r["gershgorin"] == pytest.approx(g, rel=1e-5)
+        # the same instance swept alone gives the same eigenvalues
+        single = SpectralAnnealing("i", "d", random_seed=7, precision=prec, num_alphas=10, pred_mode=pred,
+                                   eigsh_tol=tol, verbose=False).run_on_device(DeviceProblem(J, tdtype))
+        for r1, r2 in zip(res.rows, single.rows):
+            assert r1["lambda1"] == pytest.approx(r2["lambda1"], rel=2e-5 if prec == 32 else 1e-8)
+        # energies are exact and the best spins realise the best energy
+        Jint = O.integer_couplings(J)
+        assert res.best_spins is not None and res.best_spins.shape == (J.shape[0],)
+        assert res.best_energy == O.energy_int64(Jint, res.best_spins.astype(np.int64))
+        assert res.best_energy <= min(res.raw_energies)
+        assert min(res.refined_energies) >= res.best_energy
+
+
+def test_batched_local_search_equals_oracle_on_packed_graph(gpu):
+    Js = _instances()[:3]
+    bp = BatchProblem(Js, torch.float32, rows_per_block=256)
+    out = BatchSweep(random_seed=3, num_alphas=8).run(bp)
+    Jp, row_off, sizes = pack(Js)
+    Jint = O.integer_couplings(Jp)
+    K = out.packed_candidates.shape[0]
+    words = out.packed_candidates.cpu().numpy()
+    cand = np.stack([O.round_spins(np.unpackbits(w.view(np.uint8), bitorder="little")[: Jp.shape[0]] - 0.5)
+                     for w in words])                                  # K x n
+    ref = O.local_search(Jint, cand, 3, max_iterations=1000)
+    state = out.ls_state
+    got = state.spins[:, :K].cpu().numpy().T                             # K x n
+    np.testing.assert_array_equal(got, ref.spins)
+    assert state.passes == ref.passes
+    spins = got.T.astype(np.int64)
+    cand = cand.T
+    for b, J in enumerate(Js):
+        rows = slice(int(row_off[b]), int(row_off[b] + sizes[b]))
+        Jb = O.integer_couplings(J)
+        for c in range(out.results[b].num_candidates):
+            assert out.results[b].refined_energies[c] == O.energy_int64(Jb, spins[rows, c])
+            assert out.results[b].raw_energies[c] == O.energy_int64(Jb, cand[rows, c].astype(np.int64))
