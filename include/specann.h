@@ -87,6 +87,9 @@ typedef struct sa_batch {
   void* works;
   int64_t stride;
   int64_t nblocks;          /* blk_off[B] */
+  /* NULL, or the instance of every SpMV tile of the packed CSR, whose tiles then must not cross
+   * instances: the batched step skips the tiles of instances that are not running. */
+  const int32_t* tile_inst;
 } sa_batch;
 
 /* Device-side state of one eigensolve; lives at the start of the work buffer. */

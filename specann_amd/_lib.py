@@ -121,6 +121,7 @@ class SaBatch(ctypes.Structure):
         ("works", c_void_p),
         ("stride", c_int64),
         ("nblocks", c_int64),
+        ("tile_inst", c_void_p),
     ]
 
 

@@ -97,9 +97,29 @@ class BatchProblem:
         self.word_off = torch.from_numpy(self.word_off_h.astype(np.int64)).to(dev)
         self.stride = int(_lib.load().sa_work_bytes(0))
         self.works = torch.zeros(self.B * self.stride, dtype=torch.uint8, device=dev)
+        # SpMV tiles cut per instance (no tile crosses two), so a step skips stopped instances' tiles
+        tile = self.prob.tile_nnz
+        indptr = Jp.indptr.astype(np.int64)
+        starts, inst = [], []
+        for b in range(self.B):
+            r0, r1 = int(row_off[b]), int(row_off[b + 1])
+            e0, e1 = indptr[r0], indptr[r1]
+            targets = e0 + np.arange(0, max(int(e1 - e0), 1), tile, dtype=np.int64)
+            rows = r0 + np.searchsorted(indptr[r0:r1 + 1], targets, side="left")
+            rows[0] = r0
+            starts.append(np.minimum(rows, r1))
+            inst.append(np.full(rows.size, b, dtype=np.int32))
+        tile_rows = np.concatenate(starts + [np.array([self.n], dtype=np.int64)]).astype(np.int32)
+        self.tile_rows = torch.from_numpy(tile_rows).to(dev)
+        self.tile_inst = torch.from_numpy(np.concatenate(inst)).to(dev)
+        self.prob.tile_rows = self.tile_rows
+        self.prob.ntiles = int(tile_rows.size - 1)
+        self.prob.csr.tile_rows = self.tile_rows.data_ptr()
+        self.prob.csr.ntiles = self.prob.ntiles
         self.desc = SaBatch(B=self.B, rows_per_block=self.rpb, row_off=self.row_off.data_ptr(),
                             row_end=self.row_end.data_ptr(), blk_off=self.blk_off.data_ptr(),
-                            works=self.works.data_ptr(), stride=self.stride, nblocks=int(blk_off[-1]))
+                            works=self.works.data_ptr(), stride=self.stride, nblocks=int(blk_off[-1]),
+                            tile_inst=self.tile_inst.data_ptr())
         self._head = torch.empty((self.B, _lib.STATE_HEAD_BYTES), dtype=torch.uint8, pin_memory=True)
 
     @property

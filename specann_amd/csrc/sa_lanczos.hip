@@ -18,6 +18,8 @@
 //      tridiagonal LU) is computed only in the step that stops the solve.
 // Every launch reads the status in the work buffer first and does nothing unless the solve is
 // running, so the host may queue several steps (and the restart pair) without a round trip.
+#include <cstddef>
+
 #include "sa_reduce.hpp"
 #include "sa_spmv.hpp"
 
@@ -950,7 +952,16 @@ static int batch_step(const sa_csr* A, const sa_batch* Bt, const T* s, T* V, int
   SA_REQUIRE(A->npanels == 0, "the batched step uses the plain CSR layout");
   SA_REQUIRE(aligned16(V) && aligned16(W) && ldv % vec16<T>::L == 0,
              "V and W must be 16-byte aligned and ldv a multiple of 16 bytes");
-  int rc = spmv_apply<T>(A, s, Z, 0.0, nullptr, 1.0, W, nullptr, stream);
+  SpmvParams<T> p = params_from<T>(A);
+  p.g = Z;
+  p.s = s;
+  p.y = W;
+  if (Bt->tile_inst) {  // skip the tiles of instances that are not running
+    p.tile_inst = Bt->tile_inst;
+    p.gate_base = reinterpret_cast<const char*>(Bt->works) + offsetof(sa_state, status);
+    p.gate_stride = Bt->stride;
+  }
+  int rc = launch_spmv<T>(p, A->tile_nnz, stream);
   if (rc) return rc;
   const dim3 grid((unsigned)Bt->nblocks), fgrid(Bt->B);
   hipLaunchKernelGGL((lz_dots_kernel<T, BatchMap>), grid, dim3(SA_BLOCK), 0, stream, bmap(Bt), (const T*)V, ldv,

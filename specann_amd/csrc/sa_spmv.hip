@@ -134,6 +134,9 @@ __global__ __launch_bounds__(SA_BLOCK) void spmv_tile_kernel(SpmvParams<T> p) {
   if (p.gate && *p.gate != SA_RUNNING) return;
   if (p.lz) bb *= p.lz->sigma[p.lz->j];
   const int64_t t = blockIdx.x;
+  if (p.tile_inst &&
+      *reinterpret_cast<const int32_t*>(p.gate_base + (int64_t)p.tile_inst[t] * p.gate_stride) != SA_RUNNING)
+    return;
   const int32_t r0 = p.tile_rows[t], r1 = p.tile_rows[t + 1];
   if (r0 >= r1) return;
   PlainStore<T> st{p.s, p.x, p.a, bb, p.y};
@@ -274,6 +277,9 @@ SpmvParams<T> params_from(const sa_csr* A) {
   p.b = 1.0;
   p.lz = nullptr;
   p.gate = nullptr;
+  p.tile_inst = nullptr;
+  p.gate_base = nullptr;
+  p.gate_stride = 0;
   p.y = nullptr;
   p.npanels = A->npanels;
   p.pindptr = A->pindptr;

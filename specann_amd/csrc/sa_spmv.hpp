@@ -24,6 +24,12 @@ struct SpmvParams {
   // NULL, or a status word: the launch then runs only while *gate == SA_RUNNING (set to &lz->status
   // for the Lanczos step; the MINRES solve points it at its own status).
   const int32_t* gate;
+  // NULL, or the instance of every tile of a block-diagonal batch (tiles do not cross instances):
+  // a tile then runs only while its instance's status, the int32 at gate_base + inst * gate_stride,
+  // is SA_RUNNING.
+  const int32_t* tile_inst;
+  const char* gate_base;
+  int64_t gate_stride;
   T* y;
   // XCD panel layout (npanels > 0): partial row sums per panel go to partial[p * n + r].
   int32_t npanels;
