@@ -41,6 +41,9 @@ class LanczosResult:
     start_norm: float
 
 
+GRAPH_MAX_N = 1 << 18
+
+
 def default_ncv(n: int, ncv: int | None) -> int:
     """CuPy's default for k = 1 is min(max(2k, k + 32), n - 1) = min(33, n - 1)."""
     v = min(33, n - 1) if ncv is None else int(ncv)
@@ -50,7 +53,8 @@ def default_ncv(n: int, ncv: int | None) -> int:
 class LanczosSolver:
     """Buffers and launch sequence of the warm-started Lanczos for one DeviceProblem."""
 
-    def __init__(self, prob: DeviceProblem, ncv: int | None = None, chunk: int = 8, keep: int | None = None):
+    def __init__(self, prob: DeviceProblem, ncv: int | None = None, chunk: int = 8, keep: int | None = None,
+                 graphs: bool | None = None):
         self.prob = prob
         self.n = prob.n
         self.ncv = default_ncv(prob.n, ncv)
@@ -69,6 +73,12 @@ class LanczosSolver:
         self.Y = torch.empty(self.n, dtype=dt, device=dev)
         self.sfx = prob.sfx
         self.work = prob.work
+        # Small problems are launch-bound: a chunk of steps is then captured once as a HIP graph and
+        # replayed (every launch reads the solve's state on the device, so one graph serves every
+        # solve on these buffers).
+        self.use_graphs = (self.n <= GRAPH_MAX_N) if graphs is None else bool(graphs)
+        self._graph = None
+        self._graph_key = None
 
     # -- launch helpers ------------------------------------------------------------------------
     def begin(self, tol: float, maxiter: int, bound: float, pred_tol: float = float("inf")) -> None:
@@ -93,33 +103,42 @@ class LanczosSolver:
         call(f"sa_lanczos_ritz_{self.sfx}", self.work.ptr, self.n, ptr(self.V), self.ldv, ptr(y_last), ptr(self.Y),
              int(only_if), stream())
 
+    def _group(self, s: torch.Tensor | None) -> None:
+        """One step and the thick-restart launches after it (no-ops unless that step filled the basis)."""
+        self._step(s)
+        self.thick_restart(s)
+
+    def _queue(self, s: torch.Tensor | None, count: int) -> None:
+        if self.use_graphs and count == self.chunk:
+            key = (None if s is None else s.data_ptr(), self.prob.csr.tile_rows, self.prob.csr.npanels,
+                   self.prob.csr.tile_nnz)
+            if self._graph is None or self._graph_key != key:
+                g = torch.cuda.CUDAGraph()
+                torch.cuda.synchronize()
+                with torch.cuda.graph(g):
+                    for _ in range(count):
+                        self._group(s)
+                self._graph, self._graph_key = g, key
+            self._graph.replay()
+            return
+        for _ in range(count):
+            self._group(s)
+
     # -- the solve ------------------------------------------------------------------------------
     def run(self, s: torch.Tensor | None, maxiter: int, first_check: int | None = None) -> LanczosResult:
         """Iterate from the started vector until converged, stopped at maxiter, rejected or broken."""
-        j = 0
         launched = 0
         check = first_check or self.chunk
         limit = int(maxiter) + 4 * self.chunk + self.ncv + 8
         first_beta = None
         while True:
-            for _ in range(check):
-                self._step(s)
-                launched += 1
-                j += 1
-                if j >= self.ncv:
-                    # basis full: thick restart on the device (a no-op unless the solve is at SA_RESTART)
-                    self.thick_restart(s)
-                    j = self.keep
+            self._queue(s, check)
+            launched += check
             st = self.work.head()
             if first_beta is None:
                 first_beta = st.first_beta
             if st.status in TERMINAL:
                 return LanczosResult(st.status, st.theta, st.res, st.steps, st.cycles, first_beta, st.r[3])
-            if st.status == SA_RESTART:
-                # the restart came earlier than the host expected (a kept vector was dropped)
-                self.thick_restart(s)
-                st = self.work.head()
-            j = st.j
             if launched > limit:
                 raise EigensolverBreakdown(f"Lanczos did not stop after {launched} steps (status {st.status})")
             check = self.chunk

@@ -98,7 +98,8 @@ class SpectralAnnealing:
                  schedule: str = "sqrt", num_alphas: int = 128, schedule_interval=(0.0, 1.0),
                  eigsh_ncv: int = None, fp32_spmv: bool = None, ls_policy: str = "default",
                  ls_batch: bool = True, verbose: bool = True, tile_nnz: int = None, lanes_per_row: int = 0,
-                 autotune: bool = True, tangent_steps: int = 2, tangent_tol: float = 1e-2, **kwargs):
+                 autotune: bool = True, tangent_steps: int = 2, tangent_tol: float = 1e-2,
+                 lanczos_graphs: bool = None, **kwargs):
         self.instance_name = instance_name
         self.dataset = dataset
         self.random_seed = random_seed
@@ -148,6 +149,7 @@ class SpectralAnnealing:
         self.autotune = bool(autotune) and tile_nnz is None
         # pred_mode "tangent" (no reference counterpart): the diagonal predictor plus the eigenvector
         # tangent's correction, from `tangent_steps` deflated MINRES iterations (specann_amd.tangent)
+        self.lanczos_graphs = lanczos_graphs      # None: HIP-graph replay of Lanczos chunks for n <= 2^18
         self.tangent_steps = int(tangent_steps)
         self.tangent_tol = float(tangent_tol)
         if self.tangent_steps < 0:
@@ -230,7 +232,7 @@ class SpectralAnnealing:
         self._normalization_factor = prob.c
         alpha_list = self._get_alpha_values(self.mode)
         nwords = (n + 63) // 64
-        lz = LanczosSolver(prob, self.eigsh_ncv)
+        lz = LanczosSolver(prob, self.eigsh_ncv, graphs=self.lanczos_graphs)
         s = torch.empty(n, dtype=tdt, device=dev)
         ybuf = [torch.empty(n, dtype=tdt, device=dev) for _ in range(3)]
         x_rand = torch.empty(n, dtype=tdt, device=dev)
