@@ -71,6 +71,11 @@ typedef struct sa_csr {
   int64_t pmax_tiles;       /* max over panels of ntiles_p */
   void* partial;            /* npanels rows of n entries of T, leading dimension pld */
   int64_t pld;
+  /* psigned = 1: every coupling has the magnitude pmag (J in {-1,+1} or J = -1 ...): pindices then
+   * hold column | (sign bit << 31) and pvals is not read (sa_panel_sign_fold). */
+  int32_t psigned;
+  int32_t pad2;
+  double pmag;
 } sa_csr;
 
 /* A batch of independent instances packed block-diagonally into one CSR (no counterpart in the
@@ -163,6 +168,12 @@ int sa_panel_scatter_f32(const sa_csr* A, int32_t npanels, const int32_t* pbound
                          float* pvals, hipStream_t stream);
 int sa_panel_scatter_f64(const sa_csr* A, int32_t npanels, const int32_t* pbound, int32_t* pindptr, int32_t* pindices,
                          double* pvals, hipStream_t stream);
+/* Fold the signs of the panel values into bit 31 of pindices when every |pvals[e]| equals mag;
+ * *mismatch (device, zeroed by the caller) is set to 1 if one does not (pindices are then unusable). */
+int sa_panel_sign_fold_f32(int64_t nnz, int32_t* pindices, const float* pvals, double mag, int32_t* mismatch,
+                           hipStream_t stream);
+int sa_panel_sign_fold_f64(int64_t nnz, int32_t* pindices, const double* pvals, double mag, int32_t* mismatch,
+                           hipStream_t stream);
 int sa_panel_tiles(int64_t n, const int32_t* pindptr_p, int32_t tile_nnz, int32_t* tile_rows, int64_t ntiles,
                    hipStream_t stream);
 

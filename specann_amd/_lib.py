@@ -73,6 +73,9 @@ class SaCsr(ctypes.Structure):
         ("pmax_tiles", c_int64),
         ("partial", c_void_p),
         ("pld", c_int64),
+        ("psigned", c_int32),
+        ("pad2", c_int32),
+        ("pmag", c_double),
     ]
 
 
@@ -217,6 +220,8 @@ _SIGNATURES = {
     "sa_combine3_f32": (c_int32, [c_int64, c_double, _P, c_double, _P, c_double, _P, _P, _P]),
     "sa_combine3_f64": (c_int32, [c_int64, c_double, _P, c_double, _P, c_double, _P, _P, _P]),
     "sa_minres_state_bytes": (c_size_t, []),
+    "sa_panel_sign_fold_f32": (c_int32, [c_int64, _P, _P, c_double, _P, _P]),
+    "sa_panel_sign_fold_f64": (c_int32, [c_int64, _P, _P, c_double, _P, _P]),
     "sa_normalize_rows_f32": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P, _P]),
     "sa_normalize_rows_f64": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P, _P]),
     "sa_batch_alpha_scaling_f32": (c_int32, [POINTER(SaBatch), _P, c_double, c_double, _P, _P]),

@@ -40,9 +40,24 @@ struct PanelStore {
   __device__ void operator()(int32_t r, double acc) const { part[r] = (T)acc; }
 };
 
+// Entry decoding: plain (index, value) pairs, or sign-packed indices (bit 31 = sign, magnitude mag)
+// when every coupling has one magnitude, which halves the stream of a panel.
+template <typename T, bool kSigned>
+__device__ inline void load_entry(const int32_t* __restrict__ indices, const T* __restrict__ vals, T mag, int32_t e,
+                                  int32_t& c, T& v) {
+  const int32_t raw = stream_load(&indices[e]);
+  if constexpr (kSigned) {
+    c = raw & 0x7FFFFFFF;
+    v = raw < 0 ? -mag : mag;
+  } else {
+    c = raw;
+    v = stream_load(&vals[e]);
+  }
+}
+
 // Products of entries [e0, e0 + cnt) into prod[0, cnt), eight gathers in flight per thread.
-template <typename T>
-__device__ inline void stage_products(const int32_t* __restrict__ indices, const T* __restrict__ vals,
+template <typename T, bool kSigned>
+__device__ inline void stage_products(const int32_t* __restrict__ indices, const T* __restrict__ vals, T mag,
                                       const T* __restrict__ g, int32_t e0, int32_t cnt, double* prod) {
   for (int32_t base = 0; base < cnt; base += SA_BLOCK * kUnroll) {
     int32_t c[kUnroll];
@@ -50,8 +65,9 @@ __device__ inline void stage_products(const int32_t* __restrict__ indices, const
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int32_t e = base + u * SA_BLOCK + (int32_t)threadIdx.x;
-      c[u] = e < cnt ? stream_load(&indices[e0 + e]) : 0;
-      v[u] = e < cnt ? stream_load(&vals[e0 + e]) : (T)0;
+      c[u] = 0;
+      v[u] = (T)0;
+      if (e < cnt) load_entry<T, kSigned>(indices, vals, mag, e0 + e, c[u], v[u]);
     }
     T gv[kUnroll];
 #pragma unroll
@@ -68,9 +84,9 @@ __device__ inline void stage_products(const int32_t* __restrict__ indices, const
 }
 
 // One tile: rows [r0, r1) of the CSR (indptr, indices, vals); store(r, row sum) for each row.
-template <typename T, int TILE, class Store>
+template <typename T, int TILE, bool kSigned, class Store>
 __device__ inline void tile_body(const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
-                                 const T* __restrict__ vals, const T* __restrict__ g, int32_t r0, int32_t r1,
+                                 const T* __restrict__ vals, T mag, const T* __restrict__ g, int32_t r0, int32_t r1,
                                  int lanes_fixed, double* prod, double* red, const Store& store) {
   const int32_t e0 = indptr[r0];
   const int32_t e_last = indptr[r1 - 1];
@@ -79,7 +95,7 @@ __device__ inline void tile_body(const int32_t* __restrict__ indptr, const int32
   const int32_t rs = long_last ? r1 - 1 : r1;
   const int32_t es = long_last ? e_last : e1;
 
-  stage_products<T>(indices, vals, g, e0, es - e0, prod);
+  stage_products<T, kSigned>(indices, vals, mag, g, e0, es - e0, prod);
   __syncthreads();
 
   const int32_t nrows = rs - r0;
@@ -112,8 +128,9 @@ __device__ inline void tile_body(const int32_t* __restrict__ indptr, const int32
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         const int32_t e = base + u * SA_BLOCK + (int32_t)threadIdx.x;
-        c[u] = e < e1 ? stream_load(&indices[e]) : 0;
-        v[u] = e < e1 ? stream_load(&vals[e]) : (T)0;
+        c[u] = 0;
+        v[u] = (T)0;
+        if (e < e1) load_entry<T, kSigned>(indices, vals, mag, e, c[u], v[u]);
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
@@ -140,11 +157,11 @@ __global__ __launch_bounds__(SA_BLOCK) void spmv_tile_kernel(SpmvParams<T> p) {
   const int32_t r0 = p.tile_rows[t], r1 = p.tile_rows[t + 1];
   if (r0 >= r1) return;
   PlainStore<T> st{p.s, p.x, p.a, bb, p.y};
-  tile_body<T, TILE>(p.indptr, p.indices, p.vals, p.g, r0, r1, p.lanes_fixed, prod, red, st);
+  tile_body<T, TILE, false>(p.indptr, p.indices, p.vals, (T)0, p.g, r0, r1, p.lanes_fixed, prod, red, st);
 }
 
 // Workgroup w -> XCD w % 8; panel (w / (8 maxT)) * 8 + w % 8, tile (w / 8) % maxT.
-template <typename T, int TILE>
+template <typename T, int TILE, bool kSigned>
 __global__ __launch_bounds__(SA_BLOCK) void spmv_panel_kernel(SpmvParams<T> p) {
   __shared__ double prod[TILE];
   __shared__ double red[SA_WAVES_PER_BLOCK];
@@ -160,8 +177,8 @@ __global__ __launch_bounds__(SA_BLOCK) void spmv_panel_kernel(SpmvParams<T> p) {
   const int32_t r0 = p.ptile_rows[toff + t], r1 = p.ptile_rows[toff + t + 1];
   if (r0 >= r1) return;
   PanelStore<T> st{p.partial + (int64_t)panel * p.pld};
-  tile_body<T, TILE>(p.pindptr + (int64_t)panel * (p.n + 1), p.pindices, p.pvals, p.g, r0, r1, p.lanes_fixed,
-                     prod, red, st);
+  tile_body<T, TILE, kSigned>(p.pindptr + (int64_t)panel * (p.n + 1), p.pindices, p.pvals, p.pmag, p.g, r0, r1,
+                              p.lanes_fixed, prod, red, st);
 }
 
 // y_r = a x_r + b s_r sum_p partial[p][r] (panel order fixed).
@@ -240,6 +257,20 @@ __global__ __launch_bounds__(SA_BLOCK) void panel_scatter_kernel(int64_t n, cons
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(SA_BLOCK) void panel_sign_fold_kernel(int64_t nnz, int32_t* __restrict__ pindices,
+                                                                   const T* __restrict__ pvals, T mag,
+                                                                   int32_t* __restrict__ mismatch) {
+  bool bad = false;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+    const T v = pvals[e];
+    const bool neg = v < (T)0;
+    bad |= (neg ? -v : v) != mag;
+    pindices[e] = pindices[e] | (neg ? (int32_t)0x80000000 : 0);
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(mismatch, 1);
+}
+
 __global__ __launch_bounds__(SA_BLOCK) void panel_tiles_kernel(int64_t n, const int32_t* __restrict__ indptr,
                                                                int32_t tile_nnz, int32_t* __restrict__ tile_rows,
                                                                int64_t ntiles) {
@@ -290,6 +321,8 @@ SpmvParams<T> params_from(const sa_csr* A) {
   p.pmax_tiles = A->pmax_tiles;
   p.partial = (T*)A->partial;
   p.pld = A->pld > 0 ? A->pld : A->n;
+  p.psigned = A->psigned;
+  p.pmag = (T)A->pmag;
   return p;
 }
 
@@ -297,8 +330,8 @@ template <typename T>
 int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream) {
   const dim3 block(SA_BLOCK);
   if (p.npanels > 0) {
-    if (p.npanels % 8 != 0 || p.npanels > 64 || p.pmax_tiles < 1 || !p.pindptr || !p.pindices || !p.pvals ||
-        !p.ptile_rows || !p.ptile_off || !p.partial) {
+    if (p.npanels % 8 != 0 || p.npanels > 64 || p.pmax_tiles < 1 || !p.pindptr || !p.pindices ||
+        (!p.pvals && !p.psigned) || !p.ptile_rows || !p.ptile_off || !p.partial) {
       set_error("spmv: incomplete XCD panel layout");
       return SA_E_ARG;
     }
@@ -308,13 +341,19 @@ int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream) {
       return SA_E_ARG;
     }
     const dim3 grid((unsigned)nblocks);
+#define SA_PANEL_CASE(TN)                                                                          \
+  case TN:                                                                                         \
+    if (p.psigned) hipLaunchKernelGGL((spmv_panel_kernel<T, TN, true>), grid, block, 0, stream, p);  \
+    else hipLaunchKernelGGL((spmv_panel_kernel<T, TN, false>), grid, block, 0, stream, p);           \
+    break;
     switch (tile_nnz) {
-      case 512: hipLaunchKernelGGL((spmv_panel_kernel<T, 512>), grid, block, 0, stream, p); break;
-      case 1024: hipLaunchKernelGGL((spmv_panel_kernel<T, 1024>), grid, block, 0, stream, p); break;
-      case 2048: hipLaunchKernelGGL((spmv_panel_kernel<T, 2048>), grid, block, 0, stream, p); break;
-      case 4096: hipLaunchKernelGGL((spmv_panel_kernel<T, 4096>), grid, block, 0, stream, p); break;
+      SA_PANEL_CASE(512)
+      SA_PANEL_CASE(1024)
+      SA_PANEL_CASE(2048)
+      SA_PANEL_CASE(4096)
       default: set_error("spmv: tile_nnz must be 512, 1024, 2048 or 4096"); return SA_E_ARG;
     }
+#undef SA_PANEL_CASE
     return check_launch("spmv_panel");
   }
   if (p.ntiles < 1 || p.ntiles >= (1ll << 31)) {
@@ -440,6 +479,24 @@ int sa_panel_scatter_f32(const sa_csr* A, int32_t npanels, const int32_t* pbound
 int sa_panel_scatter_f64(const sa_csr* A, int32_t npanels, const int32_t* pbound, int32_t* pindptr, int32_t* pindices,
                          double* pvals, hipStream_t stream) {
   return sa::panel_scatter<double>(A, npanels, pbound, pindptr, pindices, pvals, stream);
+}
+
+int sa_panel_sign_fold_f32(int64_t nnz, int32_t* pindices, const float* pvals, double mag, int32_t* mismatch,
+                           hipStream_t stream) {
+  SA_REQUIRE(nnz >= 0 && (nnz == 0 || (pindices && pvals)) && mismatch && mag > 0.0, "bad arguments");
+  if (nnz == 0) return SA_OK;
+  hipLaunchKernelGGL(sa::panel_sign_fold_kernel<float>, dim3(sa::grid_for(nnz, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0,
+                     stream, nnz, pindices, pvals, (float)mag, mismatch);
+  return sa::check_launch("panel_sign_fold_f32");
+}
+
+int sa_panel_sign_fold_f64(int64_t nnz, int32_t* pindices, const double* pvals, double mag, int32_t* mismatch,
+                           hipStream_t stream) {
+  SA_REQUIRE(nnz >= 0 && (nnz == 0 || (pindices && pvals)) && mismatch && mag > 0.0, "bad arguments");
+  if (nnz == 0) return SA_OK;
+  hipLaunchKernelGGL(sa::panel_sign_fold_kernel<double>, dim3(sa::grid_for(nnz, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK),
+                     0, stream, nnz, pindices, pvals, mag, mismatch);
+  return sa::check_launch("panel_sign_fold_f64");
 }
 
 int sa_panel_tiles(int64_t n, const int32_t* pindptr_p, int32_t tile_nnz, int32_t* tile_rows, int64_t ntiles,

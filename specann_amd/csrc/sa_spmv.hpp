@@ -41,6 +41,8 @@ struct SpmvParams {
   int64_t pmax_tiles;
   T* partial;
   int64_t pld;
+  int32_t psigned;   // panel entries carry their sign in bit 31 of the index; magnitude pmag
+  T pmag;
 };
 
 template <typename T>

@@ -140,6 +140,11 @@ class DeviceProblem:
             call(f"sa_normalize_{sfx}", nnz, ptr(self.vals), n, ptr(self.deg), ptr(self.d), self.c, s)
         self.sfx = sfx
         self.row_counts = np.diff(J.indptr).astype(np.int64)   # entries per row = per column (symmetric J)
+        # one magnitude for every normalised coupling (J in {-1, +1}, J = -1, ...): the stored |J/c|
+        absj = np.abs(J.data)
+        self.uniform_magnitude = None
+        if nnz and row_c is None and absj.min() == absj.max():
+            self.uniform_magnitude = float(torch.abs(self.vals[:1]).item())
         self.npanels = 0
         self.tuning = None
 
@@ -161,8 +166,10 @@ class DeviceProblem:
                 return P
         return 0
 
-    def build_panels(self, npanels: int) -> None:
-        """Regroup the couplings by column panel (sa_panel_count / scan / sa_panel_scatter)."""
+    def build_panels(self, npanels: int, signed: bool | None = None) -> None:
+        """Regroup the couplings by column panel (sa_panel_count / scan / sa_panel_scatter). When every
+        coupling has the same magnitude (signed None: detect it), the panel entries carry their sign in
+        the index and the panel values are dropped (sa_panel_sign_fold)."""
         if npanels == 0:
             self.npanels = 0
             self.csr.npanels = 0
@@ -186,6 +193,19 @@ class DeviceProblem:
         self.pvals = torch.empty(self.nnz, dtype=self.dtype, device=dev)
         call(f"sa_panel_scatter_{self.sfx}", self.csr_ref, npanels, ptr(self.pbound), ptr(self.pindptr),
              ptr(self.pindices), ptr(self.pvals), s)
+        psigned, pmag = 0, 0.0
+        if signed is None:
+            signed = self.uniform_magnitude is not None
+        if signed and self.uniform_magnitude is not None and self.nnz > 0:
+            mismatch = torch.zeros(1, dtype=torch.int32, device=dev)
+            pmag = self.uniform_magnitude
+            call(f"sa_panel_sign_fold_{self.sfx}", self.nnz, ptr(self.pindices), ptr(self.pvals), pmag,
+                 ptr(mismatch), s)
+            if int(mismatch.item()) != 0:
+                raise RuntimeError("panel values do not share one magnitude")
+            psigned = 1
+            del self.pvals
+            self.pvals = None
         totals = totals.cpu().numpy()
         ntiles = [max(1, math.ceil(int(t) / self.tile_nnz)) for t in totals]
         off = np.zeros(npanels + 1, dtype=np.int64)
@@ -203,7 +223,9 @@ class DeviceProblem:
         c.npanels = npanels
         c.pindptr = self.pindptr.data_ptr()
         c.pindices = self.pindices.data_ptr()
-        c.pvals = self.pvals.data_ptr()
+        c.pvals = self.pvals.data_ptr() if self.pvals is not None else None
+        c.psigned = psigned
+        c.pmag = pmag
         c.ptile_rows = self.ptile_rows.data_ptr()
         c.ptile_off = self.ptile_off.data_ptr()
         c.pmax_tiles = int(max(ntiles))

@@ -36,6 +36,25 @@ def _dt(torch_dtype):
     return np.float32 if torch_dtype == torch.float32 else np.float64
 
 
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+def test_sign_packed_panels_equal_plain_panels(gpu, tdtype):
+    """Couplings of one magnitude: the sign-packed panel entries give bit-identical SpMV results."""
+    J = graphs.erdos_renyi(5000, 30 / 4999, seed=11, kind="pm1")
+    outs = []
+    for signed in (False, True):
+        prob = DeviceProblem(J, tdtype, tile_nnz=1024)
+        assert prob.uniform_magnitude is not None
+        prob.build_panels(8, signed=signed)
+        assert prob.csr.psigned == int(signed)
+        s = torch.empty(prob.n, dtype=tdtype, device=gpu)
+        prob.alpha_scaling(0.6, 1e-12, s)
+        x = torch.from_numpy(O.normal_vector(prob.n, 2, 0)).to(gpu, tdtype)
+        y = torch.empty_like(x)
+        prob.matvec(s, x, 0.25, -1.0, y, torch.empty_like(x))
+        outs.append(y.cpu().numpy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("case", sorted(CASES))
 @pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("tile,panels", [(512, 0), (2048, 0), (1024, 8), (2048, 16)])
