@@ -38,6 +38,8 @@ WORKLOADS = {
                  text="signed Barabasi-Albert n=2^20, m=20, J uniform in {-1,+1} (seed 42)"),
     "er1k": dict(kind="er", n=1024, deg=40, coupling="neg", seed=42,
                  text="Erdos-Renyi n=1024, p=40/1023, J=-1 (seed 42)"),
+    "c5": dict(kind="ba", n=1 << 25, m=10, coupling="neg", seed=42,
+               text="Barabasi-Albert n=2^25, m=10, J=-1 (seed 42)"),
     "c3": dict(kind="batch", count=256, lo=10, hi=16, deg=40, m=20, coupling="neg", seed=42,
                text="256 instances packed block-diagonally, n = floor(2^U(10,16)) (seed 42), alternating "
                     "Erdos-Renyi (p = 40/(n-1)) and Barabasi-Albert (m = 20), J = -1"),

@@ -256,6 +256,11 @@ class DeviceProblem:
         P = self.auto_panels() if self.n >= (1 << 17) else 0
         if P:
             configs += [(P, t) for t in (1024, 2048)]
+        if self.n >= (1 << 20):
+            # large vectors: more (narrower) panels trade partial-sum traffic for L2-resident gathers,
+            # and skewed degree distributions make the entry-balanced panels' widths uneven, so the
+            # rule above is only a starting point: time the other panel counts too
+            configs += [(p, 1024) for p in (8, 16, 32, 64) if p != P]
         best, results = None, []
         for panels, tile in configs:
             self.set_tiling(tile, self.csr.lanes_per_row)
