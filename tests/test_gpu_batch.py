@@ -75,3 +75,26 @@ def test_batched_local_search_equals_oracle_on_packed_graph(gpu):
         for c in range(out.results[b].num_candidates):
             assert out.results[b].refined_energies[c] == O.energy_int64(Jb, spins[rows, c])
             assert out.results[b].raw_energies[c] == O.energy_int64(Jb, cand[rows, c].astype(np.int64))
+
+
+def test_batch_edge_cases(gpu):
+    """One instance; instances smaller than a 64-row word and than the Lanczos basis."""
+    tiny = graphs.erdos_renyi(12, 0.5, seed=9, kind="neg")
+    star = graphs.barabasi_albert(40, 1, seed=3, kind="pm1")
+    for Js in ([graphs.erdos_renyi(700, 10 / 699, seed=6, kind="pm1")], [tiny, star, tiny]):
+        bp = BatchProblem(Js, torch.float64, rows_per_block=64)
+        out = BatchSweep(random_seed=1, num_alphas=6, eigsh_tol=1e-9).run(bp)
+        for b, J in enumerate(Js):
+            res = out.results[b]
+            Jn, d, c = O.normalize(J, np.float64)
+            for r in res.rows:
+                assert r["eigsh_converged"], (b, r)
+                s, _, _ = O.alpha_scaling(d, r["alpha"])
+                lam, _ = O.leading_eigpair(Jn, s)
+                assert abs(r["lambda1"] - lam) <= 1e-7 * max(1.0, abs(lam))
+            Jint = O.integer_couplings(J)
+            assert res.best_energy == O.energy_int64(Jint, res.best_spins.astype(np.int64))
+            if J.shape[0] <= 16:   # brute force the ground state: the sweep's best is never below it
+                best = min(O.energy_int64(Jint, np.array([1 if (m >> i) & 1 else -1 for i in range(J.shape[0])]))
+                           for m in range(1 << J.shape[0]))
+                assert res.best_energy >= best
