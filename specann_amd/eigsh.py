@@ -121,8 +121,13 @@ class LanczosSolver:
                 self._graph, self._graph_key = g, key
             self._graph.replay()
             return
+        # eager launches: the host tracks j and queues the restart pair only when the basis fills
         for _ in range(count):
-            self._group(s)
+            self._step(s)
+            self._j += 1
+            if self._j >= self.ncv:
+                self.thick_restart(s)
+                self._j = self.keep
 
     # -- the solve ------------------------------------------------------------------------------
     def run(self, s: torch.Tensor | None, maxiter: int, first_check: int | None = None) -> LanczosResult:
@@ -131,6 +136,7 @@ class LanczosSolver:
         check = first_check or self.chunk
         limit = int(maxiter) + 4 * self.chunk + self.ncv + 8
         first_beta = None
+        self._j = 0
         while True:
             self._queue(s, check)
             launched += check
@@ -139,6 +145,11 @@ class LanczosSolver:
                 first_beta = st.first_beta
             if st.status in TERMINAL:
                 return LanczosResult(st.status, st.theta, st.res, st.steps, st.cycles, first_beta, st.r[3])
+            if st.status == SA_RESTART:
+                # the restart came earlier than the host expected (a kept vector was dropped)
+                self.thick_restart(s)
+                st = self.work.head()
+            self._j = st.j
             if launched > limit:
                 raise EigensolverBreakdown(f"Lanczos did not stop after {launched} steps (status {st.status})")
             check = self.chunk

@@ -211,3 +211,58 @@ def test_cli_on_bundled_instances(tmp_path, monkeypatch):
                      "--instance", "ba_25_1_1.txt", "--seed", "1"]) == 0
     assert main(["--solver", "SpecAnn-guard-adaptive", "--dataset", "barabasi_albert_3ins_m10_tiny",
                  "--datasets-path", inst, "--instance", "ba_25_1_1.txt", "--seed", "1"]) == 2
+
+
+def test_breakdown_detection_from_the_host(gpu):
+    """The Ritz-matrix guard (an oversized recurrence coefficient) and a non-finite start vector stop
+    the solve with SA_BROKEN; every kernel still completes normally."""
+    from specann_amd._lib import SA_BROKEN
+    from specann_amd.eigsh import LanczosSolver
+    J = graphs.erdos_renyi(500, 10 / 499, seed=2, kind="pm1")
+    prob = DeviceProblem(J, torch.float32)
+    s = torch.empty(prob.n, dtype=torch.float32, device=gpu)
+    prob.alpha_scaling(0.5, 1e-12, s)
+    lz = LanczosSolver(prob, 20)
+    x = torch.from_numpy(O.normal_vector(prob.n, 1, 0)).to(gpu, torch.float32)
+    lz.begin(1e-6, 200, bound=1e-9)                 # |alpha_0| > 10 bound: the guard trips at step 0
+    lz.start(s, x)
+    assert lz.run(s, 200).status == SA_BROKEN
+    bad = x.clone()
+    bad[7] = float("nan")
+    lz.begin(1e-6, 200, bound=10.0)
+    lz.start(s, bad)                                 # non-finite norm: broken before the first step
+    assert lz.run(s, 200).status == SA_BROKEN
+    lz.begin(1e-6, 200, bound=10.0)                  # and the solver recovers on the next begin
+    lz.start(s, x)
+    assert lz.run(s, 200).status != SA_BROKEN
+
+
+@pytest.mark.parametrize("fail_calls", [1, 2])
+def test_breakdown_retry_and_skip(gpu, monkeypatch, fail_calls):
+    """A breakdown at one alpha is retried once from a random vector (the row says eigsh_retry);
+    when the retry breaks down too, the alpha is skipped and counted (models/spectral_annealing.py:616-647,
+    :323-329)."""
+    from specann_amd import eigsh as E
+    J = graphs.erdos_renyi(600, 12 / 599, seed=5, kind="pm1")
+    orig = E.LanczosSolver.begin
+    calls = {"n": 0}
+
+    def begin(self, tol, maxiter, bound, pred_tol=float("inf")):
+        calls["n"] += 1
+        # the third alpha's begin calls: trip the breakdown guard with a tiny bound
+        if 3 <= calls["n"] < 3 + fail_calls:
+            bound = 1e-12
+        return orig(self, tol, maxiter, bound, pred_tol)
+
+    monkeypatch.setattr(E.LanczosSolver, "begin", begin)
+    sa = SpectralAnnealing("t", "synthetic", random_seed=42, verbose=False, num_alphas=8, pred_mode="prev")
+    prob = DeviceProblem(J, torch.float32)
+    out = sa.run_on_device(prob)
+    if fail_calls == 1:
+        assert len(out.rows) == 8
+        assert out.rows[2]["eigsh_retry"] and not any(r["eigsh_retry"] for i, r in enumerate(out.rows) if i != 2)
+        assert sa._alphas_skipped == 0
+    else:
+        assert len(out.rows) == 7 and sa._alphas_skipped == 1
+        assert [r["alpha_index"] for r in out.rows] == [0, 1, 3, 4, 5, 6, 7]
+    check_rows_against_dense(J, out, np.float32, 5e-5)
