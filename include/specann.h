@@ -76,6 +76,23 @@ typedef struct sa_csr {
   int32_t psigned;
   int32_t pad2;
   double pmag;
+  /* Optional LDS-blocked layout (bmode = 1; needs npanels = 8, pbound, partial, pld, and psigned /
+   * pmag as above). Panel p's columns are cut into blocks of bcols, rows into bnr blocks of brows;
+   * tile t = (p * bnr + rb) * bnc + cb holds the entries of row block rb in column block cb of
+   * panel p, in 64-entry chunks [bchunk[t], bchunk[t+1]). Workgroup w = rb * 8 + p (XCD w % 8 = p)
+   * stages each column block of the gathered vector in LDS and sums its rows in LDS. An entry is
+   * sign << 31 | cont << 30 | local column << 15 | local row (cont: the next entry has the same row;
+   * a row's entries in one tile never straddle a chunk); padding entries have local row = brows. */
+  int32_t bmode;
+  int32_t brows;            /* rows per row block, a multiple of 64, < 32768 */
+  int32_t bcols;            /* columns per column block, a multiple of 64, <= 32768 */
+  int32_t bnc;              /* column blocks per panel (the widest panel's count) */
+  int32_t bnr;              /* row blocks */
+  int32_t pad3;
+  const int32_t* pbound;    /* npanels + 1 column boundaries, multiples of 64 except pbound[npanels] = n */
+  const int32_t* bchunk;    /* 8 * bnr * bnc + 1 tile starts, in chunks of 64 entries */
+  const uint32_t* bent;     /* 64 * bchunk[8 * bnr * bnc] entries */
+  const void* bvals;        /* entry values (T; 0 for padding), or NULL when psigned */
 } sa_csr;
 
 /* A batch of independent instances packed block-diagonally into one CSR (no counterpart in the
@@ -134,6 +151,8 @@ int sa_last_error(char* buf, size_t len);
 size_t sa_work_bytes(int64_t n);
 /* sizeof(sa_state), for bindings that mirror the struct. */
 size_t sa_state_bytes(void);
+/* sizeof(sa_csr), for bindings that mirror the struct. */
+size_t sa_csr_bytes(void);
 
 /* ---- Set-up: values, degrees, normalisation (models/spectral_annealing.py:270-288) ---------- */
 /* vals_out = (T) raw (the host's float64 data), jint = (int32) raw; *nonint_dev (device int32)
@@ -176,6 +195,25 @@ int sa_panel_sign_fold_f64(int64_t nnz, int32_t* pindices, const double* pvals, 
                            hipStream_t stream);
 int sa_panel_tiles(int64_t n, const int32_t* pindptr_p, int32_t tile_nnz, int32_t* tile_rows, int64_t ntiles,
                    hipStream_t stream);
+
+/* LDS-blocked layout from the plain CSR (sorted columns); no counterpart in the reference (whose SpMV
+ * is cuSPARSE's). ntiles = 8 * bnr * bnc; pbound as for the panels, multiples of 64. A run is a row's
+ * entries in one tile; it takes a slot of 1, 2, 4, ..., 64 entries (size class 0..6), and a tile
+ * lays out its slots from the largest class down, so no slot straddles a 64-entry chunk.
+ * sa_block_plan counts the runs of each tile and class into runs[16 t + k] (runs: 16 * ntiles int32,
+ * zeroed by the caller; entries 16 t + 8 .. 16 t + 14 are the fill's cursors) and scans the tiles'
+ * chunk counts into bchunk (ntiles + 1); *flag (zeroed by the caller) is set to 1 if a row has more
+ * than 64 entries in one tile, which the layout cannot hold. sa_block_fill then writes the
+ * nent = 64 * bchunk[ntiles] entries and, unless mag > 0 (sign-packed: every |J/c| equals mag, else
+ * *flag is set to 2), their values; *flag bit 8 means the plan did not come from this matrix. */
+int sa_block_plan(const sa_csr* A, const int32_t* pbound, int32_t brows, int32_t bcols, int32_t bnr, int32_t bnc,
+                  int32_t* runs, int32_t* bchunk, int32_t* flag, hipStream_t stream);
+int sa_block_fill_f32(const sa_csr* A, const int32_t* pbound, int32_t brows, int32_t bcols, int32_t bnr, int32_t bnc,
+                      int32_t* runs, const int32_t* bchunk, int64_t nent, uint32_t* bent, float* bvals, double mag,
+                      int32_t* flag, hipStream_t stream);
+int sa_block_fill_f64(const sa_csr* A, const int32_t* pbound, int32_t brows, int32_t bcols, int32_t bnr, int32_t bnc,
+                      int32_t* runs, const int32_t* bchunk, int64_t nent, uint32_t* bent, double* bvals, double mag,
+                      int32_t* flag, hipStream_t stream);
 
 /* ---- Per-alpha scaling (models/spectral_annealing.py:316, :34) ------------------------------ */
 /* s_i = 1 / sqrt(max(d_i^alpha, clamp)); state.r[1] = sum_i d_i^alpha (degree_scale, :326);

@@ -76,6 +76,16 @@ class SaCsr(ctypes.Structure):
         ("psigned", c_int32),
         ("pad2", c_int32),
         ("pmag", c_double),
+        ("bmode", c_int32),
+        ("brows", c_int32),
+        ("bcols", c_int32),
+        ("bnc", c_int32),
+        ("bnr", c_int32),
+        ("pad3", c_int32),
+        ("pbound", c_void_p),
+        ("bchunk", c_void_p),
+        ("bent", c_void_p),
+        ("bvals", c_void_p),
     ]
 
 
@@ -182,6 +192,12 @@ _SIGNATURES = {
     "sa_panel_scatter_f32": (c_int32, [POINTER(SaCsr), c_int32, _P, _P, _P, _P, _P]),
     "sa_panel_scatter_f64": (c_int32, [POINTER(SaCsr), c_int32, _P, _P, _P, _P, _P]),
     "sa_panel_tiles": (c_int32, [c_int64, _P, c_int32, _P, c_int64, _P]),
+    "sa_csr_bytes": (c_size_t, []),
+    "sa_block_plan": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, c_int32, c_int32, _P, _P, _P, _P]),
+    "sa_block_fill_f32": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, c_int32, c_int32, _P, _P, c_int64, _P, _P,
+                                    c_double, _P, _P]),
+    "sa_block_fill_f64": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, c_int32, c_int32, _P, _P, c_int64, _P, _P,
+                                    c_double, _P, _P]),
     "sa_alpha_scaling_f32": (c_int32, [c_int64, _P, c_double, c_double, _P, _P, _P]),
     "sa_alpha_scaling_f64": (c_int32, [c_int64, _P, c_double, c_double, _P, _P, _P]),
     "sa_spmv_scaled_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, c_double, _P, _P, _P]),
@@ -275,6 +291,9 @@ def load(path: Path | str | None = None) -> ctypes.CDLL:
     if lib.sa_state_bytes() != ctypes.sizeof(SaState):
         raise SpecAnnLibraryError(
             f"sa_state layout mismatch: library {lib.sa_state_bytes()} bytes, binding {ctypes.sizeof(SaState)}")
+    if lib.sa_csr_bytes() != ctypes.sizeof(SaCsr):
+        raise SpecAnnLibraryError(
+            f"sa_csr layout mismatch: library {lib.sa_csr_bytes()} bytes, binding {ctypes.sizeof(SaCsr)}")
     if lib.sa_minres_state_bytes() != ctypes.sizeof(SaMinresState):
         raise SpecAnnLibraryError("sa_minres_state layout mismatch between the library and the binding")
     if path is None:

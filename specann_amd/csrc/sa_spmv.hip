@@ -323,12 +323,26 @@ SpmvParams<T> params_from(const sa_csr* A) {
   p.pld = A->pld > 0 ? A->pld : A->n;
   p.psigned = A->psigned;
   p.pmag = (T)A->pmag;
+  p.bmode = A->bmode;
+  p.brows = A->brows;
+  p.bcols = A->bcols;
+  p.bnc = A->bnc;
+  p.bnr = A->bnr;
+  p.pbound = A->pbound;
+  p.bchunk = A->bchunk;
+  p.bent = A->bent;
+  p.bvals = A->bvals;
   return p;
 }
 
 template <typename T>
 int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream) {
   const dim3 block(SA_BLOCK);
+  if (p.bmode == 1) return launch_block_spmv<T>(p, stream);
+  if (p.bmode != 0) {
+    set_error("spmv: unknown layout mode");
+    return SA_E_ARG;
+  }
   if (p.npanels > 0) {
     if (p.npanels % 8 != 0 || p.npanels > 64 || p.pmax_tiles < 1 || !p.pindptr || !p.pindices ||
         (!p.pvals && !p.psigned) || !p.ptile_rows || !p.ptile_off || !p.partial) {

@@ -43,6 +43,12 @@ struct SpmvParams {
   int64_t pld;
   int32_t psigned;   // panel entries carry their sign in bit 31 of the index; magnitude pmag
   T pmag;
+  // LDS-blocked layout (bmode = 1): tiles of brows x bcols per panel, entries in 64-entry chunks.
+  int32_t bmode, brows, bcols, bnc, bnr;
+  const int32_t* pbound;
+  const int32_t* bchunk;
+  const uint32_t* bent;
+  const void* bvals;
 };
 
 template <typename T>
@@ -50,6 +56,9 @@ SpmvParams<T> params_from(const sa_csr* A);
 
 template <typename T>
 int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream);
+
+template <typename T>
+int launch_block_spmv(const SpmvParams<T>& p, hipStream_t stream);
 
 template <typename T>
 int spmv_apply(const sa_csr* A, const T* s, const T* g, double a, const T* x, double b, T* y, const int32_t* gate,

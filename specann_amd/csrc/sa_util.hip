@@ -306,6 +306,8 @@ int sa_last_error(char* buf, size_t len) {
 
 size_t sa_state_bytes(void) { return sizeof(sa_state); }
 
+size_t sa_csr_bytes(void) { return sizeof(sa_csr); }
+
 size_t sa_work_bytes(int64_t n) {
   (void)n;
   return state_bytes_padded() + (size_t)(kPartials + kRestart) * sizeof(double);

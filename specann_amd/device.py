@@ -17,6 +17,8 @@ from . import _lib
 from ._lib import SaCsr, SaState, SpecAnnLibraryError, call
 
 TILE_CHOICES = (512, 1024, 2048, 4096)
+LDS_BYTES = 163840          # LDS of one gfx950 CU, all of it taken by one blocked-SpMV workgroup
+CUS_PER_XCD = 32
 
 
 def require_gpu() -> torch.device:
@@ -170,6 +172,7 @@ class DeviceProblem:
         """Regroup the couplings by column panel (sa_panel_count / scan / sa_panel_scatter). When every
         coupling has the same magnitude (signed None: detect it), the panel entries carry their sign in
         the index and the panel values are dropped (sa_panel_sign_fold)."""
+        self._clear_blocks()
         if npanels == 0:
             self.npanels = 0
             self.csr.npanels = 0
@@ -232,6 +235,79 @@ class DeviceProblem:
         c.partial = self.partial.data_ptr()
         c.pld = self.pld
 
+    # -- LDS-blocked layout ----------------------------------------------------------------------
+    def block_geometry(self) -> tuple[int, int, int]:
+        """(brows, bcols, bnr) of the LDS-blocked layout: the fewest row blocks, in whole rounds of one
+        workgroup per CU of an XCD, whose row sums fit LDS with room for a column block of at least
+        2048 entries; the column block takes the rest of LDS."""
+        es = self.dtype.itemsize
+        rb_max = min(32704, ((LDS_BYTES - 2048 * es) // es - 1) // 64 * 64)
+        bnr = CUS_PER_XCD
+        while True:
+            rb = -(-(-(-self.n // bnr)) // 64) * 64
+            if rb <= rb_max:
+                break
+            bnr += CUS_PER_XCD
+        lds_rows = -(-((rb + 1) * es) // 16) * 16
+        cb = min((LDS_BYTES - lds_rows) // es // 64 * 64, 32768, 8 * 1024 * (16 // es))
+        return rb, cb, bnr
+
+    def build_blocks(self, signed: bool | None = None, brows: int | None = None, bcols: int | None = None) -> bool:
+        """Regroup the couplings into the LDS-blocked layout (sa_block_plan / sa_block_fill) on the eight
+        entry-balanced panels; False (layout unchanged) if a row has more than 64 entries in one tile.
+        brows / bcols override the geometry of block_geometry (multiples of 64 that fit LDS)."""
+        n, dev, s = self.n, self.device, stream()
+        rb, cb, bnr = self.block_geometry()
+        if brows is not None or bcols is not None:
+            rb = int(brows or rb)
+            cb = int(bcols or cb)
+            bnr = -(-n // rb)
+        bounds = self.panel_bounds(8)
+        bounds[1:-1] = (bounds[1:-1] // 64) * 64
+        bounds = np.maximum.accumulate(bounds)
+        bnc = max(1, int(-(-int(np.max(np.diff(bounds))) // cb)))
+        ntiles = 8 * bnr * bnc
+        pbound = torch.from_numpy(bounds.astype(np.int32)).to(dev)
+        runs = torch.zeros(16 * ntiles, dtype=torch.int32, device=dev)
+        bchunk = torch.empty(ntiles + 1, dtype=torch.int32, device=dev)
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        call("sa_block_plan", self.csr_ref, ptr(pbound), rb, cb, bnr, bnc, ptr(runs), ptr(bchunk), ptr(flag), s)
+        if int(flag.item()) != 0:
+            return False
+        nent = 64 * int(bchunk[-1].item())
+        if signed is None:
+            signed = self.uniform_magnitude is not None
+        signed = bool(signed and self.uniform_magnitude is not None)
+        mag = self.uniform_magnitude if signed else 0.0
+        bent = torch.empty(max(nent, 64), dtype=torch.int32, device=dev)
+        bvals = None if signed else torch.empty(max(nent, 64), dtype=self.dtype, device=dev)
+        call(f"sa_block_fill_{self.sfx}", self.csr_ref, ptr(pbound), rb, cb, bnr, bnc, ptr(runs), ptr(bchunk), nent,
+             ptr(bent), ptr(bvals), mag, ptr(flag), s)
+        if int(flag.item()) != 0:
+            raise RuntimeError(f"LDS-blocked layout fill failed (flag {int(flag.item())})")
+        del runs
+        self.build_panels(0)
+        self.pbound, self.bchunk, self.bent, self.bvals = pbound, bchunk, bent, bvals
+        self.pld = _lib.padded_ld(n, self.dtype.itemsize)
+        self.partial = torch.empty((8, self.pld), dtype=self.dtype, device=dev)
+        self.npanels = 8
+        self.block = {"brows": rb, "bcols": cb, "bnr": bnr, "bnc": bnc, "entries": nent}
+        c = self.csr
+        c.npanels, c.partial, c.pld = 8, self.partial.data_ptr(), self.pld
+        c.psigned, c.pmag = (1, mag) if signed else (0, 0.0)
+        c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 1, rb, cb, bnr, bnc
+        c.pbound, c.bchunk, c.bent = pbound.data_ptr(), bchunk.data_ptr(), bent.data_ptr()
+        c.bvals = bvals.data_ptr() if bvals is not None else None
+        return True
+
+    def _clear_blocks(self) -> None:
+        c = self.csr
+        c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 0, 0, 0, 0, 0
+        c.pbound = c.bchunk = c.bent = c.bvals = None
+        for name in ("bchunk", "bent", "bvals", "block"):
+            if hasattr(self, name):
+                delattr(self, name)
+
     def time_spmv(self, reps: int = 5) -> float:
         """Average seconds of one fused SpMV application (the Lanczos step's launch, plus the panel
         combine when panels are on), measured with HIP events on the current stream."""
@@ -249,9 +325,9 @@ class DeviceProblem:
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / 1e3 / reps
 
-    def autotune(self, reps: int = 4, verbose: bool = False) -> dict:
-        """Pick the SpMV layout (plain tiles or XCD panels) and tile size by timing them on this
-        matrix; small problems keep the plain layout."""
+    def autotune(self, reps: int = 4, verbose: bool = False, blocks: bool = True) -> dict:
+        """Pick the SpMV layout (plain tiles, XCD panels or LDS blocks) and tile size by timing them on
+        this matrix; small problems keep the plain layout."""
         configs = [(0, t) for t in (1024, 2048, 4096)]
         P = self.auto_panels() if self.n >= (1 << 17) else 0
         if P:
@@ -261,18 +337,33 @@ class DeviceProblem:
             # and skewed degree distributions make the entry-balanced panels' widths uneven, so the
             # rule above is only a starting point: time the other panel counts too
             configs += [(p, 1024) for p in (8, 16, 32, 64) if p != P]
+        if self.n >= (1 << 16) and blocks:
+            # the LDS-blocked layout (its tile size does not matter; the plain tiles stay for other uses)
+            configs.append((-1, self.tile_nnz))
         best, results = None, []
         for panels, tile in configs:
             self.set_tiling(tile, self.csr.lanes_per_row)
-            self.build_panels(panels) if panels else self.build_panels(0)
+            if panels < 0:
+                if not self.build_blocks():
+                    continue
+            else:
+                self.build_panels(panels) if panels else self.build_panels(0)
             sec = self.time_spmv(reps)
-            results.append({"npanels": panels, "tile_nnz": tile, "seconds": sec})
+            layout = "blocked" if panels < 0 else "panels" if panels else "csr"
+            results.append({"layout": layout, "npanels": 8 if panels < 0 else panels, "tile_nnz": tile,
+                            "seconds": sec})
+            if panels < 0:
+                results[-1].update({k: v for k, v in self.block.items() if k != "entries"})
             if verbose:
-                print(f"[autotune] npanels={panels} tile_nnz={tile}: {sec * 1e6:.1f} us", flush=True)
+                print(f"[autotune] {layout} npanels={results[-1]['npanels']} tile_nnz={tile}: {sec * 1e6:.1f} us",
+                      flush=True)
             if best is None or sec < best["seconds"]:
                 best = results[-1]
         self.set_tiling(best["tile_nnz"], self.csr.lanes_per_row)
-        if best["npanels"] != self.npanels or best["npanels"]:
+        if best["layout"] == "blocked":
+            if not self.csr.bmode:
+                self.build_blocks()
+        elif best["npanels"] != self.npanels or best["npanels"] or self.csr.bmode:
             self.build_panels(best["npanels"])
         self._free_unused_panels()
         self.tuning = {"choice": best, "tried": results}

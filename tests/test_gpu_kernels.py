@@ -90,6 +90,64 @@ def test_scaled_spmv_matches_oracle(gpu, case, tdtype, tile, panels):
     assert np.all(err <= (6 + panels) * eps * mag + 1e-300), f"max excess {np.max(err / (mag + 1e-300))}"
 
 
+def _max_run(J, bounds, brows, bcols):
+    """Longest run (a row's entries in one tile) of the LDS-blocked layout, on the host."""
+    J = sp.csr_matrix(J)
+    rows = np.repeat(np.arange(J.shape[0]), np.diff(J.indptr))
+    p = np.minimum(np.searchsorted(bounds, J.indices, side="right") - 1, 7)
+    t = ((p * (J.shape[0] // brows + 1) + rows // brows) * (J.shape[0] // bcols + 2)
+         + (J.indices - bounds[p]) // bcols)
+    key = rows.astype(np.int64) * (t.max() + 1) + t
+    _, counts = np.unique(key, return_counts=True)
+    return int(counts.max())
+
+
+@pytest.mark.parametrize("case", sorted(CASES) + ["er_100k"])
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("geom", [None, (1024, 2048), (8192, 8192)])
+@pytest.mark.parametrize("signed", [True, False])
+def test_blocked_spmv_matches_oracle(gpu, case, tdtype, geom, signed):
+    """The LDS-blocked layout: same operator as the oracle within the FP summation bound, bitwise
+    reproducible, and refused exactly when a row has more than 64 entries in one tile."""
+    if case == "er_100k":
+        J = graphs.erdos_renyi(100000, 40 / 99999, seed=12, kind="pm1")
+    else:
+        J = CASES[case]()
+    prob = DeviceProblem(J, tdtype)
+    brows, bcols = geom if geom else (None, None)
+    if tdtype == torch.float64 and bcols == 8192:
+        bcols = 4096
+    print(f"blocked spmv case={case} dtype={tdtype} geom={brows, bcols} signed={signed} n={prob.n}", flush=True)
+    ok = prob.build_blocks(signed=signed, brows=brows, bcols=bcols)
+    if not ok:
+        rb, cb, _ = prob.block_geometry()
+        bounds = prob.panel_bounds(8)
+        bounds[1:-1] = (bounds[1:-1] // 64) * 64
+        assert _max_run(J, np.maximum.accumulate(bounds), brows or rb, bcols or cb) > 64
+        return
+    assert prob.csr.bmode == 1 and prob.csr.psigned == int(signed and prob.uniform_magnitude is not None)
+    dt = _dt(tdtype)
+    Jn, d, c = O.normalize(J, dt)
+    s = torch.empty(prob.n, dtype=tdtype, device=gpu)
+    prob.alpha_scaling(0.41, 1e-12, s)
+    x = O.normal_vector(prob.n, 7, 0).astype(dt)
+    xt = torch.from_numpy(x).to(gpu)
+    y = torch.empty_like(xt)
+    z = torch.empty_like(xt)
+    rho = 0.9
+    prob.matvec(s, xt, rho, -1.0, y, z)
+    y1 = y.cpu().numpy()
+    prob.matvec(s, xt, rho, -1.0, y, z)
+    np.testing.assert_array_equal(y.cpu().numpy(), y1)
+    s_dev = s.cpu().numpy()
+    ref = O.s_matvec(Jn, s_dev, x, rho)
+    mag = rho * np.abs(x) + s_dev.astype(np.float64) * (abs(Jn).astype(np.float64) @ np.abs(s_dev * x))
+    # row sums in T within a panel: at most (entries of the row) roundings, then the panel combine
+    m = int(np.max(np.diff(sp.csr_matrix(J).indptr)))
+    err = np.abs(y1.astype(np.float64) - ref)
+    assert np.all(err <= (10 + m) * np.finfo(dt).eps * mag + 1e-300), f"max excess {np.max(err / (mag + 1e-300))}"
+
+
 @pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
 def test_normal_and_rounding_bits(gpu, tdtype):
     n = 10007
@@ -163,13 +221,18 @@ def test_local_search_bit_exact(gpu, case, k):
 
 @pytest.mark.parametrize("case", ["er_2k_pm1", "ba_20k_neg"])
 @pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
-@pytest.mark.parametrize("panels", [0, 8])
+@pytest.mark.parametrize("panels", [0, 8, -1])
 def test_lanczos_leading_eigenpair(gpu, case, tdtype, panels):
+    """panels -1: the LDS-blocked layout (where it holds the matrix)."""
     from specann_amd.eigsh import LanczosSolver
     from specann_amd._lib import SA_CONVERGED
     J = CASES[case]()
     prob = DeviceProblem(J, tdtype)
-    prob.build_panels(panels)
+    if panels < 0:
+        if not prob.build_blocks(brows=1024, bcols=1024):
+            pytest.skip("a row has more than 64 entries in one tile of this graph")
+    else:
+        prob.build_panels(panels)
     dt = _dt(tdtype)
     Jn, d, c = O.normalize(J, dt)
     alpha = 0.5
