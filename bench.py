@@ -167,6 +167,15 @@ def spmv_roofline(prob, reps=30):
     return sec, alg
 
 
+def spmv_kernel_name(choice, tname):
+    """The SpMV kernel(s) one application of the chosen layout launches."""
+    if choice.get("layout") == "blocked":
+        return f"spmv_block_kernel<{tname}> + combine (LDS-blocked fused scaled SpMV)"
+    if choice.get("npanels"):
+        return f"spmv_panel_kernel<{tname}> + combine (XCD-panel fused scaled SpMV)"
+    return f"spmv_tile_kernel<{tname}> (fused scaled CSR SpMV)"
+
+
 def measured_traffic(workload, dtype, layout):
     """HBM bytes per SpMV application measured by the FETCH_SIZE / WRITE_SIZE counter passes committed
     under profiles/ (scaled by a copy kernel), if they were taken on this workload and layout."""
@@ -176,7 +185,10 @@ def measured_traffic(workload, dtype, layout):
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    if rec.get("layout") != {"npanels": layout["npanels"], "tile_nnz": layout["tile_nnz"]}:
+    want = {"npanels": layout["npanels"], "tile_nnz": layout["tile_nnz"]}
+    if layout.get("layout") == "blocked":
+        want = {"layout": "blocked", **{k: layout[k] for k in ("brows", "bcols", "bnr", "bnc")}}
+    if rec.get("layout") != want:
         return None
     return float(rec["traffic"])
 
@@ -239,7 +251,8 @@ def main(argv=None) -> int:
     ap.add_argument("--c3-sample", type=int, default=16,
                     help="batch workload: instances also swept one at a time, for comparison")
     ap.add_argument("--layout", default=None, metavar="NPANELS,TILE",
-                    help="fixed SpMV layout instead of the autotuner's choice, e.g. 8,1024")
+                    help="fixed SpMV layout instead of the autotuner's choice, e.g. 8,1024, or blocked (LDS-blocked "
+                         "layout; blocked:BROWS,BCOLS for a given geometry)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -257,7 +270,13 @@ def main(argv=None) -> int:
     J = make_graph(w)
     log(f"graph {args.workload}: n={J.shape[0]} nnz={J.nnz} generated in {time.time() - t0:.1f}s")
     prob32 = DeviceProblem(J, torch.float32)
-    if args.layout:
+    if args.layout and args.layout.startswith("blocked"):
+        geom = [int(v) for v in args.layout.split(":")[1].split(",")] if ":" in args.layout else [None, None]
+        if not prob32.build_blocks(brows=geom[0], bcols=geom[1]):
+            raise SystemExit("the LDS-blocked layout cannot hold this matrix")
+        tune32 = {"choice": {"layout": "blocked", "npanels": 8, "tile_nnz": prob32.tile_nnz, "seconds": None,
+                             **{k: v for k, v in prob32.block.items() if k != "entries"}}, "tried": []}
+    elif args.layout:
         npanels, tile = (int(v) for v in args.layout.split(","))
         prob32.set_tiling(tile)
         if npanels:
@@ -376,8 +395,7 @@ def main(argv=None) -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, "f32", tune32["choice"]),
-                     "kernel": ("spmv_panel_kernel<float> + combine (XCD-panel fused scaled SpMV)"
-                                if tune32["choice"]["npanels"] else "spmv_tile_kernel<float> (fused scaled CSR SpMV)"),
+                     "kernel": spmv_kernel_name(tune32["choice"], "float"),
                      "bytes_per_launch": spmv_bytes, "us_per_launch": spmv_sec * 1e6},
         "cpu_baseline": cpu,
     }

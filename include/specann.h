@@ -85,8 +85,9 @@ typedef struct sa_csr {
    * a row's entries in one tile never straddle a chunk); padding entries have local row = brows. */
   int32_t bmode;
   int32_t brows;            /* rows per row block, a multiple of 64, < 32768 */
-  int32_t bcols;            /* columns per column block, a multiple of 64, <= 32768 */
-  int32_t bnc;              /* column blocks per panel (the widest panel's count) */
+  int32_t bcols;            /* columns per column block, a multiple of 64, <= 65536 / sizeof(T); the
+                             * row sums (brows + 1, 16-byte aligned) and the block take <= 162816 bytes */
+  int32_t bnc;              /* column blocks per panel (the widest panel's count), at most 248 */
   int32_t bnr;              /* row blocks */
   int32_t pad3;
   const int32_t* pbound;    /* npanels + 1 column boundaries, multiples of 64 except pbound[npanels] = n */

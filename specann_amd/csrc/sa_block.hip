@@ -21,16 +21,9 @@ namespace sa {
 
 constexpr int kBThreads = 1024;                // 16 waves: one workgroup per CU (all of LDS)
 constexpr int kBWaves = kBThreads / 64;
-constexpr int kBLdsBytes = 163840;
+constexpr int kBTable = 256;                   // tile-bound table entries, at the end of LDS
+constexpr int kBMaxTiles = kBTable - 8;        // column blocks per panel
 constexpr int kBPre = 4;                       // chunks of the next tile each wave loads ahead
-constexpr int kBMaxXv = 8;                     // 16-byte loads per thread for one column block
-
-template <typename T>
-__device__ inline int block_xoff(int brows) {
-  // row sums [0, brows] (brows is the padding slot), then the column block, 16-byte aligned
-  const int bytes = (brows + 1) * (int)sizeof(T);
-  return ((bytes + 15) & ~15) / (int)sizeof(T);
-}
 
 template <typename T, bool kSigned>
 __device__ inline T block_entry(uint32_t e, T val, const T* xb) {
@@ -42,123 +35,195 @@ __device__ inline T block_entry(uint32_t e, T val, const T* xb) {
   }
 }
 
-// One chunk: lane values v (already gathered), runs reduced onto their first lane, which adds the
-// run sum into its row.
-template <typename T>
-__device__ inline void block_chunk_add(uint32_t e, T v, T* yacc, int lane) {
-  const uint64_t m = __ballot((e >> 30) & 1u);  // lane l continues into lane l + 1
-  uint64_t c = m;
-  for (int d = 1; c != 0; d <<= 1) {
-    const T o = __shfl_down(v, d, 64);
-    if ((c >> lane) & 1ull) v += o;
-    c &= c >> d;
+// v of lane + 1 (0 on lane 63): a DPP wave_shl:1 move, no LDS round trip.
+__device__ inline float next_lane(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+__device__ inline double next_lane(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, 0x130, 0xf, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x130, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// NC chunks of one tile (on[u]: chunk u exists): lane values v (already gathered) are reduced run by
+// run onto each run's first lane (a segmented reduction over the cont mask: lane l continues into
+// l + 1; runs of two take one step, longer ones log2 of their length), and that lane adds the run
+// sum into its row. A row has one run per tile, so no other lane touches the row while the tile is
+// summed: the row reads of all NC chunks go out before their writes, and a plain read-modify-write
+// suffices (an LDS atomic add is several times slower here).
+template <typename T, int NC>
+__device__ inline void block_chunks_add(const uint32_t* e, T* v, const bool* on, T* yacc, int lane) {
+  uint64_t m[NC];
+#pragma unroll
+  for (int u = 0; u < NC; ++u) m[u] = __ballot((e[u] >> 30) & 1u);
+#pragma unroll
+  for (int u = 0; u < NC; ++u) {
+    if (on[u] && m[u]) {
+      const T o = next_lane(v[u]);
+      if ((m[u] >> lane) & 1ull) v[u] += o;
+      uint64_t c = m[u] & (m[u] >> 1);
+      for (int d = 2; c != 0; d <<= 1) {
+        const T w = __shfl_down(v[u], d, 64);
+        if ((c >> lane) & 1ull) v[u] += w;
+        c &= c >> d;
+      }
+    }
   }
-  const bool head = lane == 0 || !((m >> (lane - 1)) & 1ull);
-  if (head) {
-    const int r = (int)(e & 0x7FFFu);
-    yacc[r] += v;
+  bool head[NC];
+  T y[NC];
+#pragma unroll
+  for (int u = 0; u < NC; ++u) {
+    head[u] = on[u] && (lane == 0 || !((m[u] >> (lane - 1)) & 1ull));
+    y[u] = head[u] ? yacc[e[u] & 0x7FFFu] : (T)0;
+  }
+#pragma unroll
+  for (int u = 0; u < NC; ++u) {
+    if (head[u]) yacc[e[u] & 0x7FFFu] = y[u] + v[u];
   }
 }
 
+// LDS split of one workgroup: row sums for up to kRows rows (and the padding slot), the column block
+// (kCols entries) and the tile-bound table. One workgroup per CU.
+template <typename T>
+struct BlockCfg;
+template <>
+struct BlockCfg<float> {
+  static constexpr int kRows = 26240, kCols = 14336;
+};
+template <>
+struct BlockCfg<double> {
+  static constexpr int kRows = 13312, kCols = 6976;
+};
+constexpr int kBXv = 4;  // 16-byte loads per thread for one column block: kCols <= kBXv * 16 KB
+
+// Workgroup barrier for the walk: this wave's LDS operations complete (lgkmcnt 0) and at most N
+// vector-memory operations still in flight (vmcnt N), then s_barrier. The wavefront-scope fences only
+// keep the compiler from moving memory operations across the barrier.
+template <int N>
+__device__ inline void walk_barrier() {
+  static_assert(N >= 0 && N < 64, "vmcnt out of range");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4));
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The walk over the column blocks is software-pipelined one tile deep: while tile c is summed, the
+// column block and the entries of tile c + 1 are loaded into registers; the block goes to LDS between
+// the two barriers of the next tile. The walk is unrolled by two so that the two register sets of
+// entries never move between registers (a move would make their loads complete early). Every load is
+// issued unconditionally (out-of-range lanes read a valid stand-in address and drop the value), so
+// the loads in flight are counted exactly and each wait covers only the loads it needs.
 template <typename T, bool kSigned>
 __global__ __launch_bounds__(kBThreads) void spmv_block_kernel(SpmvParams<T> p) {
-  __shared__ __attribute__((aligned(16))) char lds[kBLdsBytes];
-  if (p.gate && *p.gate != SA_RUNNING) return;
   typedef typename vec16<T>::type V;
+  constexpr int kRows = BlockCfg<T>::kRows, kCols = BlockCfg<T>::kCols;
   constexpr int L = vec16<T>::L;
+  __shared__ __attribute__((aligned(16))) T yacc[kRows + 4];
+  __shared__ __attribute__((aligned(16))) T xb[kCols];
+  __shared__ int32_t tb[kBTable];
+  if (p.gate && *p.gate != SA_RUNNING) return;
   const int panel = (int)(blockIdx.x & 7);
   const int rb = (int)(blockIdx.x >> 3);
   const int RB = p.brows, CB = p.bcols;
   const int64_t r0 = (int64_t)rb * RB;
   if (r0 >= p.n) return;
   const int rows = (int)((p.n - r0) < RB ? (p.n - r0) : RB);
-  T* yacc = reinterpret_cast<T*>(lds);
-  T* xb = yacc + block_xoff<T>(RB);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   for (int i = tid; i <= RB; i += kBThreads) yacc[i] = (T)0;
 
   const int64_t c0 = p.pbound[panel];
   const int64_t width = (int64_t)p.pbound[panel + 1] - c0;
-  const int nc = (int)min((width + CB - 1) / CB, (int64_t)p.bnc);
+  const int nc = (int)min((width + CB - 1) / CB, (int64_t)min(p.bnc, kBMaxTiles));
   const int64_t tbase = ((int64_t)panel * p.bnr + rb) * p.bnc;
+  const int64_t tlast = (int64_t)8 * p.bnr * p.bnc;  // bchunk has tlast + 1 entries
   const uint32_t* __restrict__ ent = p.bent;
   const T* __restrict__ bv = (const T*)p.bvals;
 
-  // column block c of g into registers (elements beyond the block are not loaded)
-  V xr[kBMaxXv];
+  // The workgroup's tile bounds in LDS (read back as uniform values). The walk reads up to
+  // bound(nc + 2) (steps run to c = nc and read up to bound(c + 2)); entries past the workgroup's own
+  // tiles hold later tiles' bounds, clamped to the last, so every bound is a valid chunk offset.
+  for (int i = tid; i < nc + 8; i += kBThreads) tb[i] = p.bchunk[min(tbase + i, tlast)];
+  __syncthreads();
+  auto bound = [&](int c) -> int64_t { return __builtin_amdgcn_readfirstlane(tb[min(c, nc + 7)]); };
+  auto block_count = [&](int c) -> int {
+    const int64_t rest = width - (int64_t)c * CB;
+    return (int)(rest < CB ? rest : CB);  // <= 0 past the panel
+  };
+
+  V xr[kBXv];
   auto load_block = [&](int c) {
     const int64_t lo = c0 + (int64_t)c * CB;
-    const int cnt = (int)((width - (int64_t)c * CB) < CB ? (width - (int64_t)c * CB) : CB);
+    const int cnt = block_count(c);
 #pragma unroll
-    for (int u = 0; u < kBMaxXv; ++u) {
+    for (int u = 0; u < kBXv; ++u) {
       const int i = (u * kBThreads + tid) * L;
-      if (i + L <= cnt) {
-        xr[u] = *reinterpret_cast<const V*>(p.g + lo + i);
-      } else if (i < cnt) {
-        V t;
-#pragma unroll
-        for (int q = 0; q < L; ++q) t[q] = (i + q < cnt) ? p.g[lo + i + q] : (T)0;
-        xr[u] = t;
-      }
+      xr[u] = *reinterpret_cast<const V*>(i + L <= cnt ? p.g + lo + i : p.g);
     }
   };
   auto store_block = [&](int c) {
-    const int cnt = (int)((width - (int64_t)c * CB) < CB ? (width - (int64_t)c * CB) : CB);
+    const int64_t lo = c0 + (int64_t)c * CB;
+    const int cnt = block_count(c);
 #pragma unroll
-    for (int u = 0; u < kBMaxXv; ++u) {
+    for (int u = 0; u < kBXv; ++u) {
       const int i = (u * kBThreads + tid) * L;
-      if (i < cnt) *reinterpret_cast<V*>(xb + i) = xr[u];
-    }
-  };
-
-  // entries of the first kBPre chunks this wave takes in a tile
-  uint32_t ce[kBPre], ne[kBPre];
-  T cv[kBPre], nv[kBPre];
-  auto fetch = [&](int64_t t, uint32_t* e, T* v) {
-    const int64_t k0 = p.bchunk[t], k1 = p.bchunk[t + 1];
-#pragma unroll
-    for (int u = 0; u < kBPre; ++u) {
-      const int64_t k = k0 + wave + (int64_t)u * kBWaves;
-      e[u] = 0u;
-      v[u] = (T)0;
-      if (k < k1) {
-        e[u] = stream_load(&ent[k * 64 + lane]);
-        if constexpr (!kSigned) v[u] = stream_load(&bv[k * 64 + lane]);
+      if (i + L <= cnt) {
+        *reinterpret_cast<V*>(xb + i) = xr[u];
+      } else if (i < cnt) {  // the ragged end of the last panel
+        for (int q = 0; q < L; ++q) xb[i + q] = (i + q < cnt) ? p.g[lo + i + q] : (T)0;
       }
     }
   };
-
-  if (nc > 0) {
-    load_block(0);
-    fetch(tbase, ce, cv);
-  }
-  for (int c = 0; c < nc; ++c) {
-    const int64_t t = tbase + c;
-    const int64_t k0 = p.bchunk[t], k1 = p.bchunk[t + 1];
-    __syncthreads();  // the previous tile's gathers are done with xb (and yacc is zeroed)
-    store_block(c);
-    __syncthreads();
-    if (c + 1 < nc) {
-      load_block(c + 1);
-      fetch(t + 1, ne, nv);
-    }
+  auto fetch = [&](int64_t k0, int64_t k1, uint32_t* e, T* v) {
 #pragma unroll
     for (int u = 0; u < kBPre; ++u) {
       const int64_t k = k0 + wave + (int64_t)u * kBWaves;
-      if (k < k1) block_chunk_add<T>(ce[u], block_entry<T, kSigned>(ce[u], cv[u], xb), yacc, lane);
+      const int64_t kk = k < k1 ? k : 0;  // chunk 0 always exists; its entries are dropped
+      e[u] = stream_load(&ent[kk * 64 + lane]);
+      if constexpr (!kSigned) v[u] = stream_load(&bv[kk * 64 + lane]);
+      else v[u] = (T)0;
     }
-    for (int64_t k = k0 + wave + (int64_t)kBPre * kBWaves; k < k1; k += kBWaves) {
-      const uint32_t e = stream_load(&ent[k * 64 + lane]);
-      T v = (T)0;
-      if constexpr (!kSigned) v = stream_load(&bv[k * 64 + lane]);
-      block_chunk_add<T>(e, block_entry<T, kSigned>(e, v, xb), yacc, lane);
-    }
+  };
+  // Tile c (summed only if c < nc; a step past the panel keeps the barriers uniform): its entries are
+  // in cur; the block and the entries of tile c + 1 are loaded (into xr and nxt).
+  auto step = [&](int c, const uint32_t* cur, const T* curv, uint32_t* nxt, T* nxtv) {
+    const bool live = c < nc;
+    const int64_t k0 = live ? bound(c) : 0, k1 = live ? bound(c + 1) : 0;
+    const int64_t f0 = bound(c + 1), f1 = bound(c + 2);
+    __syncthreads();  // tile c - 1 is summed: xb is free
+    store_block(c);
+    __syncthreads();
+    load_block(c + 1);
+    fetch(f0, f1, nxt, nxtv);
+    T v[kBPre];
+    bool on[kBPre];
 #pragma unroll
     for (int u = 0; u < kBPre; ++u) {
-      ce[u] = ne[u];
-      cv[u] = nv[u];
+      v[u] = block_entry<T, kSigned>(cur[u], curv[u], xb);
+      on[u] = k0 + wave + (int64_t)u * kBWaves < k1;
     }
+    block_chunks_add<T, kBPre>(cur, v, on, yacc, lane);
+    for (int64_t k = k0 + wave + (int64_t)kBPre * kBWaves; k < k1; k += kBWaves) {
+      uint32_t e[1] = {stream_load(&ent[k * 64 + lane])};
+      T w = (T)0;
+      if constexpr (!kSigned) w = stream_load(&bv[k * 64 + lane]);
+      T x[1] = {block_entry<T, kSigned>(e[0], w, xb)};
+      const bool one[1] = {true};
+      block_chunks_add<T, 1>(e, x, one, yacc, lane);
+    }
+  };
+
+  uint32_t eA[kBPre], eB[kBPre];
+  T vA[kBPre], vB[kBPre];
+  if (nc > 0) {
+    load_block(0);
+    fetch(bound(0), bound(1), eA, vA);
+  }
+  for (int c = 0; c < nc; c += 2) {
+    step(c, eA, vA, eB, vB);
+    step(c + 1, eB, vB, eA, vA);
   }
   __syncthreads();
   const T scale = kSigned ? p.pmag : (T)1;
@@ -172,10 +237,9 @@ int launch_block_spmv(const SpmvParams<T>& p, hipStream_t stream) {
     set_error("spmv: incomplete LDS-blocked layout");
     return SA_E_ARG;
   }
-  const int xoff = (((p.brows + 1) * (int)sizeof(T) + 15) & ~15);
-  if (p.brows < 64 || p.brows % 64 || p.brows >= 32768 || p.bcols < 64 || p.bcols % 64 || p.bcols > 32768 ||
-      p.bcols > kBMaxXv * kBThreads * vec16<T>::L || xoff + p.bcols * (int)sizeof(T) > kBLdsBytes || p.bnr < 1 ||
-      p.bnc < 1 || (int64_t)p.bnr * p.brows < p.n) {
+  if (p.brows < 64 || p.brows % 64 || p.brows > BlockCfg<T>::kRows || p.bcols < 64 || p.bcols % 64 ||
+      p.bcols > BlockCfg<T>::kCols || p.bnr < 1 || p.bnc < 1 || p.bnc > kBMaxTiles ||
+      (int64_t)p.bnr * p.brows < p.n || p.n < 64) {
     set_error("spmv: LDS-blocked layout sizes out of range");
     return SA_E_ARG;
   }

@@ -17,7 +17,11 @@ from . import _lib
 from ._lib import SaCsr, SaState, SpecAnnLibraryError, call
 
 TILE_CHOICES = (512, 1024, 2048, 4096)
-LDS_BYTES = 163840          # LDS of one gfx950 CU, all of it taken by one blocked-SpMV workgroup
+# LDS split of the blocked SpMV's workgroup (one per CU), by element size: the most rows whose sums it
+# holds and the largest column block
+BLOCK_ROWS = {4: 26240, 8: 13312}
+BLOCK_COLS = {4: 14336, 8: 6976}
+BLOCK_MAX_TILES = 248       # column blocks per panel the blocked SpMV can walk
 CUS_PER_XCD = 32
 
 
@@ -238,19 +242,16 @@ class DeviceProblem:
     # -- LDS-blocked layout ----------------------------------------------------------------------
     def block_geometry(self) -> tuple[int, int, int]:
         """(brows, bcols, bnr) of the LDS-blocked layout: the fewest row blocks, in whole rounds of one
-        workgroup per CU of an XCD, whose row sums fit LDS with room for a column block of at least
-        2048 entries; the column block takes the rest of LDS."""
+        workgroup per CU of an XCD, whose row sums fit the workgroup's LDS; the column block is the
+        largest the LDS split holds."""
         es = self.dtype.itemsize
-        rb_max = min(32704, ((LDS_BYTES - 2048 * es) // es - 1) // 64 * 64)
         bnr = CUS_PER_XCD
         while True:
             rb = -(-(-(-self.n // bnr)) // 64) * 64
-            if rb <= rb_max:
+            if rb <= BLOCK_ROWS[es]:
                 break
             bnr += CUS_PER_XCD
-        lds_rows = -(-((rb + 1) * es) // 16) * 16
-        cb = min((LDS_BYTES - lds_rows) // es // 64 * 64, 32768, 8 * 1024 * (16 // es))
-        return rb, cb, bnr
+        return rb, BLOCK_COLS[es], bnr
 
     def build_blocks(self, signed: bool | None = None, brows: int | None = None, bcols: int | None = None) -> bool:
         """Regroup the couplings into the LDS-blocked layout (sa_block_plan / sa_block_fill) on the eight
@@ -262,10 +263,15 @@ class DeviceProblem:
             rb = int(brows or rb)
             cb = int(bcols or cb)
             bnr = -(-n // rb)
+        es = self.dtype.itemsize
+        if not (64 <= rb <= BLOCK_ROWS[es] and rb % 64 == 0 and cb % 64 == 0 and 64 <= cb <= BLOCK_COLS[es]):
+            raise ValueError(f"blocked layout geometry {rb} x {cb} does not fit the workgroup's LDS split")
         bounds = self.panel_bounds(8)
         bounds[1:-1] = (bounds[1:-1] // 64) * 64
         bounds = np.maximum.accumulate(bounds)
         bnc = max(1, int(-(-int(np.max(np.diff(bounds))) // cb)))
+        if bnc > BLOCK_MAX_TILES:
+            return False
         ntiles = 8 * bnr * bnc
         pbound = torch.from_numpy(bounds.astype(np.int32)).to(dev)
         runs = torch.zeros(16 * ntiles, dtype=torch.int32, device=dev)
