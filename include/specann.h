@@ -201,9 +201,11 @@ int sa_panel_tiles(int64_t n, const int32_t* pindptr_p, int32_t tile_nnz, int32_
  * is cuSPARSE's). ntiles = 8 * bnr * bnc; pbound as for the panels, multiples of 64. A run is a row's
  * entries in one tile; it takes a slot of 1, 2, 4, ..., 64 entries (size class 0..6), and a tile
  * lays out its slots from the largest class down, so no slot straddles a 64-entry chunk.
- * sa_block_plan counts the runs of each tile and class into runs[16 t + k] (runs: 16 * ntiles int32,
- * zeroed by the caller; entries 16 t + 8 .. 16 t + 14 are the fill's cursors) and scans the tiles'
- * chunk counts into bchunk (ntiles + 1); *flag (zeroed by the caller) is set to 1 if a row has more
+ * Singletons (runs of one) follow the larger slots from the next multiple of 32, in rounds over their
+ * row banks (local row mod 32), so that a half-wave's row-sum updates hit distinct LDS banks.
+ * sa_block_plan counts the runs of each tile into runs[96 t + ...] (runs: 96 * ntiles int32, zeroed by
+ * the caller: per size class at 0..6, per row bank of the singletons at 16..47; 8..14 and 48..79 are
+ * the fill's cursors) and scans the tiles' chunk counts into bchunk (ntiles + 1); *flag (zeroed by the caller) is set to 1 if a row has more
  * than 64 entries in one tile, which the layout cannot hold. sa_block_fill then writes the
  * nent = 64 * bchunk[ntiles] entries and, unless mag > 0 (sign-packed: every |J/c| equals mag, else
  * *flag is set to 2), their values; *flag bit 8 means the plan did not come from this matrix. */

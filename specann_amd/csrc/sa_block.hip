@@ -287,6 +287,20 @@ __device__ inline void for_each_run(const sa_csr& A, const int32_t* bnd, const B
   }
 }
 
+// Per-tile counters in the caller's runs array, kRunStride int32 per tile: [0, 7) runs per size class,
+// [8, 15) the fill's slot cursors per class, [16, 48) singleton runs per row bank (local row mod 32),
+// [48, 80) the fill's cursors per row bank.
+constexpr int kRunStride = 96;
+constexpr int kRunCur = 8, kBankCnt = 16, kBankCur = 48;
+
+// Slots of the runs of two or more entries (classes 1..6, largest first), rounded up to 32 so that
+// the singletons start on a half-wave boundary.
+__device__ inline int64_t multi_slots(const int32_t* runs, int64_t t) {
+  int64_t s = 0;
+  for (int k = 1; k <= 6; ++k) s += (int64_t)runs[kRunStride * t + k] << k;
+  return (s + 31) & ~(int64_t)31;
+}
+
 __global__ __launch_bounds__(SA_BLOCK) void block_count_kernel(sa_csr A, const int32_t* __restrict__ pbound,
                                                                BlockGeom G, int32_t* __restrict__ runs,
                                                                int32_t* __restrict__ flag) {
@@ -294,17 +308,19 @@ __global__ __launch_bounds__(SA_BLOCK) void block_count_kernel(sa_csr A, const i
   if (threadIdx.x < 9) bnd[threadIdx.x] = pbound[threadIdx.x];
   __syncthreads();
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < G.n; r += (int64_t)gridDim.x * blockDim.x) {
-    for_each_run(A, bnd, G, r, [&](int64_t t, int k, int32_t, int32_t, int, int32_t) {
-      if (k > 6) atomicOr(flag, 1);
-      else atomicAdd(&runs[16 * t + k], 1);
+    for_each_run(A, bnd, G, r, [&](int64_t t, int k, int32_t, int32_t, int lr, int32_t) {
+      if (k > 6) {
+        atomicOr(flag, 1);
+      } else {
+        atomicAdd(&runs[kRunStride * t + k], 1);
+        if (k == 0) atomicAdd(&runs[kRunStride * t + kBankCnt + (lr & 31)], 1);
+      }
     });
   }
 }
 
 __device__ inline int64_t tile_slots(const int32_t* runs, int64_t t) {
-  int64_t s = 0;
-  for (int k = 0; k <= 6; ++k) s += (int64_t)runs[16 * t + k] << k;
-  return s;
+  return multi_slots(runs, t) + runs[kRunStride * t];
 }
 
 // bchunk[t] = sum over earlier tiles of ceil(slots / 64) (one workgroup, in passes of SA_BLOCK tiles).
@@ -361,11 +377,27 @@ __global__ __launch_bounds__(SA_BLOCK) void block_fill_kernel(sa_csr A, const in
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < G.n; r += (int64_t)gridDim.x * blockDim.x) {
     for_each_run(A, bnd, G, r, [&](int64_t t, int k, int32_t e0, int32_t L, int lr, int32_t lo) {
       if (k > 6) return;
-      // slots of class k start after the larger classes' slots
-      int64_t off = 0;
-      for (int q = 6; q > k; --q) off += (int64_t)runs[16 * t + q] << q;
-      const int32_t slot = atomicAdd(&runs[16 * t + 8 + k], 1);
-      const int64_t pos = (int64_t)bchunk[t] * 64 + off + ((int64_t)slot << k);
+      const int32_t* rt = runs + kRunStride * t;
+      int64_t off;
+      if (k > 0) {
+        // slots of class k start after the larger classes' slots
+        off = 0;
+        for (int q = 6; q > k; --q) off += (int64_t)rt[q] << q;
+        off += (int64_t)atomicAdd(&runs[kRunStride * t + kRunCur + k], 1) << k;
+      } else {
+        // singletons: rank r in row bank b goes to position r-th round over the banks, b-th among
+        // the banks that still have an r-th singleton, so each 32 consecutive singletons (one
+        // half-wave: the region starts on a multiple of 32) hit distinct row banks until the banks
+        // with fewer singletons run out
+        const int b = lr & 31;
+        const int32_t rank = atomicAdd(&runs[kRunStride * t + kBankCur + b], 1);
+        off = multi_slots(runs, t);
+        for (int q = 0; q < 32; ++q) {
+          const int32_t c = rt[kBankCnt + q];
+          off += min(c, rank) + ((q < b && c > rank) ? 1 : 0);
+        }
+      }
+      const int64_t pos = (int64_t)bchunk[t] * 64 + off;
       if (pos + L > nent || pos + L > (int64_t)bchunk[t + 1] * 64) {
         atomicOr(flag, 8);  // the plan does not match this matrix
         return;

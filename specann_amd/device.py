@@ -274,7 +274,7 @@ class DeviceProblem:
             return False
         ntiles = 8 * bnr * bnc
         pbound = torch.from_numpy(bounds.astype(np.int32)).to(dev)
-        runs = torch.zeros(16 * ntiles, dtype=torch.int32, device=dev)
+        runs = torch.zeros(96 * ntiles, dtype=torch.int32, device=dev)
         bchunk = torch.empty(ntiles + 1, dtype=torch.int32, device=dev)
         flag = torch.zeros(1, dtype=torch.int32, device=dev)
         call("sa_block_plan", self.csr_ref, ptr(pbound), rb, cb, bnr, bnc, ptr(runs), ptr(bchunk), ptr(flag), s)
