@@ -245,6 +245,8 @@ def main(argv=None) -> int:
     ap.add_argument("--ncv", type=int, default=None, help="Lanczos basis size (default: the reference's min(33, n-1))")
     ap.add_argument("--rows-out", default=None, help="write the FP32 sweep's per-alpha rows (JSON) to this file")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--reorth", default="selective", choices=("selective", "full"),
+                    help="Lanczos reorthogonalisation: selective (default) or full at every step (the reference's)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV timing loop and a copy kernel of known size (for counter profiling)")
@@ -299,7 +301,7 @@ def main(argv=None) -> int:
     def solver(precision, pred="diagonal"):
         return SpectralAnnealing("bench", "synthetic", random_seed=42, precision=precision, verbose=False,
                                  num_alphas=args.alphas, pred_mode=pred, tangent_steps=args.tangent_steps,
-                                 eigsh_ncv=args.ncv)
+                                 eigsh_ncv=args.ncv, reorth=args.reorth)
 
     for i in range(args.warmup):
         out = solver(32, args.pred).run_on_device(prob32)
@@ -392,6 +394,8 @@ def main(argv=None) -> int:
         "best_energy_fp32": e32, "best_energy_fp64": e64, "energy_gap": gap,
         "operator_applications": matvecs, "lanczos_applications": outs[-1].matvecs,
         "tangent_applications": outs[-1].tangent_matvecs, "converged_fraction": conv,
+        "reorth": args.reorth, "orth_full_steps": outs[-1].orth_full_steps,
+        "orth_selective_steps": outs[-1].orth_selective_steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, "f32", tune32["choice"]),

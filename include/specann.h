@@ -126,13 +126,14 @@ typedef struct sa_state {
   int32_t gate;        /* 1 while step 0 runs on a predicted start vector */
   int32_t kept;        /* Ritz vectors kept by the last thick restart (rows 0..kept-1 of the basis) */
   int32_t keep;        /* Ritz vectors a thick restart keeps (1: restart from the Ritz vector alone) */
-  int32_t pad1;
+  int32_t orth_lo;     /* first basis row the current step's Gram-Schmidt update subtracts (0: all) */
   double theta;        /* largest Ritz value of N (the reference's mu = rho - theta) */
   double res;          /* residual estimate |beta_j * s_j| */
   double first_beta;   /* beta_0 of a gated start: the predictor residual */
   double tol;
   double bound;        /* breakdown bound on |alpha|, |beta| (10 x (rho + g) is applied) */
   double pred_tol;
+  double orth_tau;     /* selective reorthogonalisation threshold (0: every step against the whole basis) */
   double r[8];         /* results of scalar reductions */
   uint64_t hash;       /* spin signature of the last rounding */
   int64_t i64[4];
@@ -142,7 +143,8 @@ typedef struct sa_state {
   double h[SA_MAX_NCV + 1];
   double ritz[SA_MAX_NCV];
   double arrow[SA_MAX_NCV]; /* couplings of the kept Ritz vectors to basis row `kept` */
-  unsigned int counters[16];
+  unsigned int counters[16]; /* [0]: steps orthogonalised against the whole basis, [1]: against the
+                              * last two rows only (both since sa_lanczos_begin) */
 } sa_state;
 
 int sa_version(void);
@@ -245,8 +247,12 @@ int sa_spmv_gather_f64(const sa_csr* A, const double* s, const double* g, double
  * Basis V is (ncv + 1) rows of n entries, row-major with leading dimension ldv >= n (pad ldv so that
  * ldv * sizeof(T) is not a multiple of a large power of two: with n = 2^22 unpadded rows start on the
  * same HBM channels and the basis streams camp on them). Z, W are n-vectors of scratch. */
+/* orth_tau: a step's Gram-Schmidt update subtracts only the last two basis rows unless the projection
+ * on an older row exceeds orth_tau * beta_j (then every row); sqrt(eps) / 2 of the vectors' type keeps
+ * the basis semi-orthogonal, which leaves Ritz values as accurate as full reorthogonalisation. 0
+ * subtracts every row at every step (the reference eigsh's full reorthogonalisation). */
 int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, double bound, double pred_tol,
-                     int32_t keep, hipStream_t stream);
+                     int32_t keep, double orth_tau, hipStream_t stream);
 /* Start vector V[0] = x (.) max(d, 1e-12)^pexp (the diagonal predictor, utils/predictors.py:6-26, with
  * pexp = delta_alpha / 2; pexp = 0 copies x) or cx * x + cy * y when y != NULL (the secant predictor,
  * :29-56). Z = s (.) V[0]; sigma_0 = 1 / ||V[0]||; state.r[3] = ||V[0]||. gate = 1 marks the vector as
@@ -416,7 +422,7 @@ int sa_batch_alpha_scaling_f64(const sa_batch* Bt, const double* d, double alpha
                                hipStream_t stream);
 /* bounds: device array of B breakdown bounds (rho_b + g_b). */
 int sa_batch_lanczos_begin(const sa_batch* Bt, int32_t ncv, int32_t maxiter, double tol, const double* bounds,
-                           double pred_tol, int32_t keep, hipStream_t stream);
+                           double pred_tol, int32_t keep, double orth_tau, hipStream_t stream);
 int sa_batch_lanczos_start_f32(const sa_batch* Bt, const float* s, const float* x, const float* d, double pexp,
                                double cx, const float* y, double cy, float* V, int64_t ldv, float* Z, int32_t gate,
                                int32_t only_if, hipStream_t stream);

@@ -100,13 +100,14 @@ class SaState(ctypes.Structure):
         ("gate", c_int32),
         ("kept", c_int32),
         ("keep", c_int32),
-        ("pad1", c_int32),
+        ("orth_lo", c_int32),
         ("theta", c_double),
         ("res", c_double),
         ("first_beta", c_double),
         ("tol", c_double),
         ("bound", c_double),
         ("pred_tol", c_double),
+        ("orth_tau", c_double),
         ("r", c_double * 8),
         ("hash", c_uint64),
         ("i64", c_int64 * 4),
@@ -204,7 +205,7 @@ _SIGNATURES = {
     "sa_spmv_scaled_f64": (c_int32, [POINTER(SaCsr), _P, _P, c_double, c_double, _P, _P, _P]),
     "sa_spmv_gather_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, _P, c_double, _P, _P]),
     "sa_spmv_gather_f64": (c_int32, [POINTER(SaCsr), _P, _P, c_double, _P, c_double, _P, _P]),
-    "sa_lanczos_begin": (c_int32, [_P, c_int32, c_int32, c_double, c_double, c_double, c_int32, _P]),
+    "sa_lanczos_begin": (c_int32, [_P, c_int32, c_int32, c_double, c_double, c_double, c_int32, c_double, _P]),
     "sa_lanczos_thick_restart_f32": (c_int32, [_P, c_int64, _P, _P, c_int64, _P, _P]),
     "sa_lanczos_thick_restart_f64": (c_int32, [_P, c_int64, _P, _P, c_int64, _P, _P]),
     "sa_lanczos_start_f32": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, _P, c_double, _P, c_int64, _P,
@@ -242,7 +243,8 @@ _SIGNATURES = {
     "sa_normalize_rows_f64": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P, _P]),
     "sa_batch_alpha_scaling_f32": (c_int32, [POINTER(SaBatch), _P, c_double, c_double, _P, _P]),
     "sa_batch_alpha_scaling_f64": (c_int32, [POINTER(SaBatch), _P, c_double, c_double, _P, _P]),
-    "sa_batch_lanczos_begin": (c_int32, [POINTER(SaBatch), c_int32, c_int32, c_double, _P, c_double, c_int32, _P]),
+    "sa_batch_lanczos_begin": (c_int32, [POINTER(SaBatch), c_int32, c_int32, c_double, _P, c_double, c_int32, c_double,
+                                         _P]),
     "sa_batch_lanczos_start_f32": (c_int32, [POINTER(SaBatch), _P, _P, _P, c_double, c_double, _P, c_double, _P,
                                              c_int64, _P, c_int32, c_int32, _P]),
     "sa_batch_lanczos_start_f64": (c_int32, [POINTER(SaBatch), _P, _P, _P, c_double, c_double, _P, c_double, _P,

@@ -31,7 +31,7 @@ from . import _lib
 from . import local_search as LS
 from ._lib import (SA_BROKEN, SA_CONVERGED, SA_MAXITER, SA_REJECTED, SA_RUNNING, SaBatch, SaState, call)
 from .device import DeviceProblem, ptr, require_gpu, stream
-from .eigsh import default_ncv
+from .eigsh import default_ncv, default_orth_tau
 from .spectral_annealing import compute_shift_rho
 
 TERMINAL = (SA_CONVERGED, SA_MAXITER, SA_BROKEN)
@@ -221,7 +221,7 @@ class BatchSweep:
 
         def begin(pred_tol):
             call("sa_batch_lanczos_begin", R, ncv, self.maxiter, self.tol, ptr(bounds), float(pred_tol), keep,
-                 stream())
+                 default_orth_tau(bp.dtype), stream())
 
         def run_until_done(first_check: int):
             """Queue steps until every instance stopped; restart REJECTED ones from y_last. Returns heads

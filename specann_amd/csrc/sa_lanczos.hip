@@ -27,7 +27,7 @@ namespace sa {
 
 template <class F>
 __global__ void lz_begin_kernel(F fin, int32_t ncv, int32_t maxiter, double tol, double bound, const double* bounds,
-                                double pred_tol, int32_t keep) {
+                                double pred_tol, int32_t keep, double orth_tau) {
   if (threadIdx.x != 0) return;
   sa_state* st = state_of(fin.w());
   if (bounds) bound = bounds[fin.idx()];
@@ -46,6 +46,10 @@ __global__ void lz_begin_kernel(F fin, int32_t ncv, int32_t maxiter, double tol,
   st->tol = tol;
   st->bound = bound;
   st->pred_tol = pred_tol;
+  st->orth_tau = orth_tau;
+  st->orth_lo = 0;
+  st->counters[0] = 0;
+  st->counters[1] = 0;
 }
 
 template <typename T, class M>
@@ -135,6 +139,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(M map, const T* __res
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t lo = sl.lo, hi = sl.hi;
   const int64_t nv = hi > lo ? (hi - lo) / L : 0;
+  double ww = 0.0;  // ||W||^2, accumulated in the first pass
   for (int ch = 0; ch < nch; ++ch) {
     const int m0 = ch * SA_RED_COLS;
     const int mc = min(SA_RED_COLS, j + 1 - m0);
@@ -144,6 +149,10 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(M map, const T* __res
     for (int64_t g = threadIdx.x; g < nv; g += SA_BLOCK) {
       const int64_t r = lo + g * L;
       const VT w = *reinterpret_cast<const VT*>(&W[r]);
+      if (ch == 0) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) ww += (double)w[l] * (double)w[l];
+      }
       VT v[SA_RED_COLS];
 #pragma unroll
       for (int c = 0; c < SA_RED_COLS; ++c)
@@ -156,6 +165,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(M map, const T* __res
     }
     for (int64_t r = lo + nv * L + threadIdx.x; r < hi; r += SA_BLOCK) {
       const double w = (double)W[r];
+      if (ch == 0) ww += w * w;
 #pragma unroll
       for (int c = 0; c < SA_RED_COLS; ++c)
         if (c < mc) acc[c] += (double)V[(int64_t)(m0 + c) * ldv + r] * w;
@@ -172,6 +182,9 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(M map, const T* __res
           (red[ch & 1][0][c] + red[ch & 1][1][c]) + (red[ch & 1][2][c] + red[ch & 1][3][c]);
     }
   }
+  __shared__ double redw[SA_WAVES_PER_BLOCK];
+  const double wsum = block_sum(ww, redw);
+  if (threadIdx.x == 0) part[(int64_t)(j + 1) * G + sl.blk] = wsum;
 }
 
 // Phase 2 of the projections: column m summed over the G workgroups by a 32-lane group, eight loads
@@ -185,10 +198,11 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_finish_kernel(F fin) {
   const double ws = F::kScaleW ? st->sigma[j] : 1.0;  // batched: W was not scaled by sigma_j
   const double* part = partials_of(fin.w());
   const int lane32 = threadIdx.x & 31, grp = threadIdx.x >> 5;
-  for (int mb = 0; mb <= j; mb += SA_BLOCK / 32) {
+  __shared__ double lww;
+  for (int mb = 0; mb <= j + 1; mb += SA_BLOCK / 32) {
     const int m = mb + grp;
     double x = 0.0;
-    if (m <= j) {
+    if (m <= j + 1) {
       const double* pc = part + (int64_t)m * G;
       int b = lane32;
       for (; b + 32 * 7 < G; b += 32 * 8) {
@@ -203,9 +217,27 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_finish_kernel(F fin) {
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) x += __shfl_xor(x, o, 32);
     if (lane32 == 0 && m <= j) st->h[m] = st->sigma[m] * ws * x;
+    if (lane32 == 0 && m == j + 1) lww = ws * ws * x;
   }
   __syncthreads();
-  if (threadIdx.x == 0) st->a[j] = st->h[j];
+  if (threadIdx.x == 0) {
+    st->a[j] = st->h[j];
+    // Selective reorthogonalisation. The projections h_m of W on rows m < j - 1 vanish in exact
+    // arithmetic past the first step of a cycle (the kept rows couple to row k only); left in, they
+    // become components h_m / beta_j of the next basis vector. The update subtracts every row when one
+    // of them exceeds tau beta_j (beta_j^2 ~ ||W||^2 - sum h_m^2), else only rows j - 1 and j, which
+    // keeps the basis orthogonal to tau (semi-orthogonality, tau ~ sqrt(eps) of the vectors' type).
+    double s2 = 0.0, hmax = 0.0;
+    for (int m = 0; m <= j; ++m) {
+      s2 += st->h[m] * st->h[m];
+      if (m + 1 < j) hmax = fmax(hmax, fabs(st->h[m]));
+    }
+    const double beta_est = sqrt(fmax(lww - s2, 1e-30 * lww));
+    const double tau = st->orth_tau;
+    const bool full = !(hmax <= tau * beta_est);  // NaN: full
+    st->orth_lo = full ? 0 : max(0, j - 1);
+    st->counters[full ? 0 : 1] += 1;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -463,6 +495,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(M map, const T* __res
   if (st->status != SA_RUNNING) return;
   const int j = st->j;
   const double ws = M::kScaleW ? st->sigma[j] : 1.0;  // batched: W was not scaled by sigma_j
+  const int mlo = st->orth_lo;  // rows mlo..j are subtracted (set by the projections' finisher)
   for (int m = threadIdx.x; m <= j; m += blockDim.x) coef[m] = st->h[m] * st->sigma[m];
   __syncthreads();
   T* __restrict__ vn = V + (int64_t)(j + 1) * ldv;
@@ -472,7 +505,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(M map, const T* __res
     wv[0] = ws * (double)W[r];
 #pragma unroll
     for (int u = 1; u < 8; ++u) wv[u] = 0.0;
-    int m = 0;
+    int m = mlo;
     for (; m + 7 <= j; m += 8) {
       T a[8];
 #pragma unroll
@@ -921,12 +954,13 @@ BatchFin bfin(const sa_batch* Bt) {
 }
 
 static int batch_begin(const sa_batch* Bt, int32_t ncv, int32_t maxiter, double tol, const double* bounds,
-                       double pred_tol, int32_t keep, hipStream_t stream) {
+                       double pred_tol, int32_t keep, double orth_tau, hipStream_t stream) {
   SA_REQUIRE(batch_ok(Bt) && bounds, "bad arguments");
   SA_REQUIRE(ncv >= 2 && ncv <= SA_MAX_NCV && maxiter >= 1, "bad arguments (2 <= ncv <= 128)");
   SA_REQUIRE(keep >= 1 && keep <= SA_MAX_KEEP && keep < ncv, "keep must be in [1, min(16, ncv - 1)]");
+  SA_REQUIRE(orth_tau >= 0.0 && orth_tau < 1.0, "orth_tau must be in [0, 1)");
   hipLaunchKernelGGL(lz_begin_kernel<BatchFin>, dim3(Bt->B), dim3(64), 0, stream, bfin(Bt), ncv, maxiter, tol, 0.0,
-                     bounds, pred_tol, keep);
+                     bounds, pred_tol, keep, orth_tau);
   return check_launch("batch_lanczos_begin");
 }
 
@@ -1019,11 +1053,12 @@ using namespace sa;
 extern "C" {
 
 int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, double bound, double pred_tol,
-                     int32_t keep, hipStream_t stream) {
+                     int32_t keep, double orth_tau, hipStream_t stream) {
   SA_REQUIRE(work && ncv >= 2 && ncv <= SA_MAX_NCV && maxiter >= 1, "bad arguments (2 <= ncv <= 128)");
   SA_REQUIRE(keep >= 1 && keep <= SA_MAX_KEEP && keep < ncv, "keep must be in [1, min(16, ncv - 1)]");
+  SA_REQUIRE(orth_tau >= 0.0 && orth_tau < 1.0, "orth_tau must be in [0, 1)");
   hipLaunchKernelGGL(lz_begin_kernel<OneFin>, dim3(1), dim3(64), 0, stream, OneFin{work, 1}, ncv, maxiter, tol, bound,
-                     (const double*)nullptr, pred_tol, keep);
+                     (const double*)nullptr, pred_tol, keep, orth_tau);
   return check_launch("lanczos_begin");
 }
 
@@ -1070,8 +1105,8 @@ int sa_lanczos_ritz_f64(void* work, int64_t n, const double* V, int64_t ldv, con
 }
 
 int sa_batch_lanczos_begin(const sa_batch* Bt, int32_t ncv, int32_t maxiter, double tol, const double* bounds,
-                           double pred_tol, int32_t keep, hipStream_t stream) {
-  return batch_begin(Bt, ncv, maxiter, tol, bounds, pred_tol, keep, stream);
+                           double pred_tol, int32_t keep, double orth_tau, hipStream_t stream) {
+  return batch_begin(Bt, ncv, maxiter, tol, bounds, pred_tol, keep, orth_tau, stream);
 }
 
 int sa_batch_lanczos_start_f32(const sa_batch* Bt, const float* s, const float* x, const float* d, double pexp,

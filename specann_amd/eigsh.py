@@ -1,20 +1,26 @@
 """Warm-started Lanczos eigensolver on the GPU: the counterpart of utils/eigsh.py:10-17.
 
 The reference calls CuPy's thick-restart ``eigsh(S_rho, k=1, which='SA', tol, maxiter, v0, ncv)``
-(models/spectral_annealing.py:671-677). With k = 1 a thick restart keeps only the Ritz vector, so
-it spans the same Krylov space as an explicit restart from it; this solver restarts explicitly
-from the Ritz vector every ``ncv`` steps. It differs from CuPy in three deliberate ways:
+(models/spectral_annealing.py:671-677), which restarts from the Ritz vector every ``ncv`` steps and
+reorthogonalises each new vector against the whole basis. This solver keeps ``keep`` Ritz vectors
+per thick restart (keep = 1 is CuPy's restart) and differs in four deliberate ways:
 
 * the recurrence runs on N_alpha (the reference's S_rho = rho I - N has the same Krylov space and
   T_S = rho I - T_N), which keeps the FP32 basis orthogonal; mu = rho - theta is reported;
 * the stopping test |beta_j s_j| <= tol is applied after every step, not once per restart;
-* ``iterations`` counts operator applications (CuPy's counter is ncv + (ncv - 1) r).
+* ``iterations`` counts operator applications (CuPy's counter is ncv + (ncv - 1) r);
+* reorthogonalisation is selective: the projections on every basis row are measured each step, and
+  the update subtracts all of them only when one on a row older than j - 1 exceeds orth_tau * beta_j
+  (orth_tau = sqrt(eps) / 2: a semi-orthogonal basis, whose Ritz values are as accurate as with full
+  reorthogonalisation); orth_tau = 0 restores the full reorthogonalisation of every step.
 
-Every step is three HIP launches (fused scaled SpMV, projections, Gram-Schmidt + FP64 Ritz solve);
-all launches read the solve's status on the device and do nothing once it stopped, so the host
-queues several steps, and the restart pair, between reads of the status.
+Every step is a fused scaled SpMV, the projections and the Gram-Schmidt update with the FP64 Ritz
+solve; all launches read the solve's status on the device and do nothing once it stopped, so the
+host queues several steps between reads of the status.
 """
 from __future__ import annotations
+
+import math
 
 import ctypes
 from dataclasses import dataclass
@@ -44,6 +50,12 @@ class LanczosResult:
 GRAPH_MAX_N = 1 << 18
 
 
+def default_orth_tau(dtype: torch.dtype) -> float:
+    """sqrt(eps) / 2 of the vectors' type: the semi-orthogonality level the selective
+    reorthogonalisation keeps."""
+    return 0.5 * math.sqrt(torch.finfo(dtype).eps)
+
+
 def default_ncv(n: int, ncv: int | None) -> int:
     """CuPy's default for k = 1 is min(max(2k, k + 32), n - 1) = min(33, n - 1)."""
     v = min(33, n - 1) if ncv is None else int(ncv)
@@ -54,8 +66,11 @@ class LanczosSolver:
     """Buffers and launch sequence of the warm-started Lanczos for one DeviceProblem."""
 
     def __init__(self, prob: DeviceProblem, ncv: int | None = None, chunk: int = 8, keep: int | None = None,
-                 graphs: bool | None = None):
+                 graphs: bool | None = None, orth_tau: float | None = None):
+        """orth_tau: selective reorthogonalisation threshold (None: sqrt(eps) / 2 of the vectors' type;
+        0: reorthogonalise against the whole basis at every step)."""
         self.prob = prob
+        self.orth_tau = default_orth_tau(prob.dtype) if orth_tau is None else float(orth_tau)
         self.n = prob.n
         self.ncv = default_ncv(prob.n, ncv)
         self.chunk = max(1, int(chunk))
@@ -83,7 +98,7 @@ class LanczosSolver:
     # -- launch helpers ------------------------------------------------------------------------
     def begin(self, tol: float, maxiter: int, bound: float, pred_tol: float = float("inf")) -> None:
         call("sa_lanczos_begin", self.work.ptr, self.ncv, int(maxiter), float(tol), float(bound), float(pred_tol),
-             self.keep, stream())
+             self.keep, self.orth_tau, stream())
 
     def start(self, s: torch.Tensor | None, x: torch.Tensor, *, d: torch.Tensor | None = None, pexp: float = 0.0,
               cx: float = 1.0, y: torch.Tensor | None = None, cy: float = 0.0, gate: int = 0,

@@ -79,6 +79,8 @@ class SweepOutcome:
     post_sweep_time: float = 0.0
     matvecs: int = 0
     tangent_matvecs: int = 0          # operator applications of the tangent predictor's MINRES
+    orth_full_steps: int = 0          # Lanczos steps orthogonalised against the whole basis
+    orth_selective_steps: int = 0     # ... against the last two basis rows only
     cand_bits: object = None          # device int64 [k][nwords]: the distinct rounded candidates
     raw_energies: list = field(default_factory=list)
     refined_energies: list = field(default_factory=list)
@@ -99,7 +101,7 @@ class SpectralAnnealing:
                  eigsh_ncv: int = None, fp32_spmv: bool = None, ls_policy: str = "default",
                  ls_batch: bool = True, verbose: bool = True, tile_nnz: int = None, lanes_per_row: int = 0,
                  autotune: bool = True, tangent_steps: int = 2, tangent_tol: float = 1e-2,
-                 lanczos_graphs: bool = None, **kwargs):
+                 lanczos_graphs: bool = None, reorth: str = "selective", **kwargs):
         self.instance_name = instance_name
         self.dataset = dataset
         self.random_seed = random_seed
@@ -150,6 +152,9 @@ class SpectralAnnealing:
         # pred_mode "tangent" (no reference counterpart): the diagonal predictor plus the eigenvector
         # tangent's correction, from `tangent_steps` deflated MINRES iterations (specann_amd.tangent)
         self.lanczos_graphs = lanczos_graphs      # None: HIP-graph replay of Lanczos chunks for n <= 2^18
+        if reorth not in ("selective", "full"):
+            raise ValueError("reorth must be 'selective' or 'full'")
+        self.reorth = reorth                      # full: the reference eigsh's reorthogonalisation every step
         self.tangent_steps = int(tangent_steps)
         self.tangent_tol = float(tangent_tol)
         if self.tangent_steps < 0:
@@ -232,7 +237,8 @@ class SpectralAnnealing:
         self._normalization_factor = prob.c
         alpha_list = self._get_alpha_values(self.mode)
         nwords = (n + 63) // 64
-        lz = LanczosSolver(prob, self.eigsh_ncv, graphs=self.lanczos_graphs)
+        lz = LanczosSolver(prob, self.eigsh_ncv, graphs=self.lanczos_graphs,
+                           orth_tau=0.0 if self.reorth == "full" else None)
         s = torch.empty(n, dtype=tdt, device=dev)
         ybuf = [torch.empty(n, dtype=tdt, device=dev) for _ in range(3)]
         x_rand = torch.empty(n, dtype=tdt, device=dev)
@@ -246,6 +252,7 @@ class SpectralAnnealing:
         self._last_alpha_error = None
         self._matvecs = 0
         self._tangent_matvecs = 0
+        self._orth_steps = [0, 0]                 # Lanczos steps orthogonalised fully / selectively
         self._tangent = None
         if self.pred_mode == "tangent" and not self.disable_warm_start:
             from .tangent import TangentPredictor
@@ -261,6 +268,9 @@ class SpectralAnnealing:
                                       lambda1_prev)
                 self._alphas_retried += int(eo.retry)
                 self._matvecs += eo.steps
+                oc = lz.work.full().counters
+                self._orth_steps[0] += int(oc[0])
+                self._orth_steps[1] += int(oc[1])
                 bound = (-0.5 * eo.lambda1 * eo.degree_scale * prob.c) if eo.converged else float('nan')
                 # final Ritz vector, sign alignment with y_last, normalisation (:653-660)
                 nrm, dot = lz.final_vector(y_last)
@@ -384,6 +394,7 @@ class SpectralAnnealing:
         out.post_sweep_time = time.time() - sweep_end
         out.matvecs = self._matvecs
         out.tangent_matvecs = self._tangent_matvecs
+        out.orth_full_steps, out.orth_selective_steps = self._orth_steps
         return out
 
     def _refine(self, prob: DeviceProblem, cand_bits: torch.Tensor, k: int) -> LS.LocalSearchState:
