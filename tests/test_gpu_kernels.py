@@ -255,6 +255,45 @@ def test_lanczos_leading_eigenpair(gpu, case, tdtype, panels):
     assert true_res <= (1e-5 if tdtype == torch.float32 else 1e-9)
 
 
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+def test_selective_reorthogonalisation_matches_full(gpu, tdtype):
+    """Selective reorthogonalisation (old rows subtracted only when their projection exceeds
+    sqrt(eps)/2 of beta) converges to the same eigenpair as the full one, with the true residual of
+    its Ritz vector within the tolerance, and takes the selective path on most steps."""
+    from specann_amd.eigsh import LanczosSolver, default_orth_tau
+    from specann_amd._lib import SA_CONVERGED
+    J = graphs.erdos_renyi(20000, 30 / 19999, seed=21, kind="pm1")
+    prob = DeviceProblem(J, tdtype)
+    dt = _dt(tdtype)
+    Jn, d, c = O.normalize(J, dt)
+    s = torch.empty(prob.n, dtype=tdtype, device=gpu)
+    prob.alpha_scaling(0.45, 1e-12, s)
+    sv = s.cpu().numpy()
+    lam_ref, _ = O.leading_eigpair(Jn, sv)
+    tol = 1e-6 if tdtype == torch.float32 else 1e-10
+    out = {}
+    for tau in (0.0, None):
+        solver = LanczosSolver(prob, ncv=33, orth_tau=tau)
+        x0 = prob.fill_normal(4, 0, torch.empty(prob.n, dtype=tdtype, device=gpu))
+        solver.begin(tol, 4000, bound=3.0)
+        solver.start(s, x0)
+        res = solver.run(s, 4000)
+        counters = list(prob.work.full().counters[:2])
+        nrm, _ = solver.final_vector(None)
+        y = solver.Y.cpu().numpy().astype(np.float64) / nrm
+        true_res = np.linalg.norm(O.n_matvec(Jn, sv, y) - res.theta * y)
+        out[tau] = (res, counters, true_res)
+        print(f"reorth tau={tau} dtype={tdtype} steps={res.steps} theta={res.theta!r} counters={counters} "
+              f"true residual {true_res:.2e}")
+        assert res.status == SA_CONVERGED
+        assert abs(res.theta - lam_ref) <= 10 * tol
+        assert true_res <= (1e-5 if tdtype == torch.float32 else 1e-9)
+    full, sel = out[0.0], out[None]
+    assert full[1][0] >= full[0].steps - 2 * (full[0].cycles + 1)   # tau = 0: old rows every step
+    assert sel[1][1] > sel[1][0]                                      # mostly the two-row update
+    assert default_orth_tau(tdtype) == pytest.approx(0.5 * np.sqrt(np.finfo(dt).eps))
+
+
 def test_autotune_keeps_results(gpu):
     """Every tiling the autotuner may pick gives the same operator (within FP32 rounding)."""
     J = graphs.erdos_renyi(200000, 20 / 199999, seed=6, kind="pm1")
