@@ -201,16 +201,19 @@ int sa_panel_tiles(int64_t n, const int32_t* pindptr_p, int32_t tile_nnz, int32_
 
 /* LDS-blocked layout from the plain CSR (sorted columns); no counterpart in the reference (whose SpMV
  * is cuSPARSE's). ntiles = 8 * bnr * bnc; pbound as for the panels, multiples of 64. A run is a row's
- * entries in one tile; it takes a slot of 1, 2, 4, ..., 64 entries (size class 0..6), and a tile
- * lays out its slots from the largest class down, so no slot straddles a 64-entry chunk.
- * Singletons (runs of one) follow the larger slots from the next multiple of 32, in rounds over their
- * row banks (local row mod 32), so that a half-wave's row-sum updates hit distinct LDS banks.
- * sa_block_plan counts the runs of each tile into runs[96 t + ...] (runs: 96 * ntiles int32, zeroed by
- * the caller: per size class at 0..6, per row bank of the singletons at 16..47; 8..14 and 48..79 are
- * the fill's cursors) and scans the tiles' chunk counts into bchunk (ntiles + 1); *flag (zeroed by the caller) is set to 1 if a row has more
- * than 64 entries in one tile, which the layout cannot hold. sa_block_fill then writes the
- * nent = 64 * bchunk[ntiles] entries and, unless mag > 0 (sign-packed: every |J/c| equals mag, else
- * *flag is set to 2), their values; *flag bit 8 means the plan did not come from this matrix. */
+ * entries in one tile. A tile holds, in order: the long region (runs of more than 64 entries, cut into
+ * 64-entry pieces stacked in 16 chunk columns by local row mod 16: chunk r * 16 + c is in column c,
+ * and the SpMV kernel gives chunk k of a tile to wave k mod 16, so a row's pieces are summed by one
+ * wave in order); the runs of 2..64 entries, each in a slot of 2, 4, ..., 64 entries, largest first (no
+ * slot straddles a chunk); and, from the next multiple of 32, the singletons (runs of one) in rounds
+ * over their row banks (local row mod 32), so that a half-wave's row-sum updates hit distinct LDS
+ * banks. sa_block_plan counts the runs of each tile into runs[128 t + ...] (runs: 128 * ntiles int32,
+ * zeroed by the caller: per size class at 0..6, per row bank of the singletons at 16..47, long-run
+ * pieces per chunk column at 80..95; 8..14, 48..79 and 96..111 are the fill's cursors) and scans the
+ * tiles' chunk counts into bchunk (ntiles + 1); *flag (zeroed by the caller) is set to 4 if the
+ * entries would pass 2^31. sa_block_fill then writes the nent = 64 * bchunk[ntiles] entries and,
+ * unless mag > 0 (sign-packed: every |J/c| equals mag, else *flag is set to 2), their values; *flag
+ * bit 8 means the plan did not come from this matrix. */
 int sa_block_plan(const sa_csr* A, const int32_t* pbound, int32_t brows, int32_t bcols, int32_t bnr, int32_t bnc,
                   int32_t* runs, int32_t* bchunk, int32_t* flag, hipStream_t stream);
 int sa_block_fill_f32(const sa_csr* A, const int32_t* pbound, int32_t brows, int32_t bcols, int32_t bnr, int32_t bnc,

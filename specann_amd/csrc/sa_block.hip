@@ -46,12 +46,13 @@ __device__ inline double next_lane(double v) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// NC chunks of one tile (on[u]: chunk u exists): lane values v (already gathered) are reduced run by
-// run onto each run's first lane (a segmented reduction over the cont mask: lane l continues into
-// l + 1; runs of two take one step, longer ones log2 of their length), and that lane adds the run
-// sum into its row. A row has one run per tile, so no other lane touches the row while the tile is
-// summed: the row reads of all NC chunks go out before their writes, and a plain read-modify-write
-// suffices (an LDS atomic add is several times slower here).
+// NC chunks of one tile (on[u]: chunk u exists), in the order the wave takes them: lane values v
+// (already gathered) are reduced run by run onto each run's first lane (a segmented reduction over the
+// cont mask: lane l continues into l + 1; runs of two take one step, longer ones log2 of their
+// length), and that lane adds the run sum into its row. A row's entries in one tile sit in one chunk,
+// or, for a run longer than 64, in pieces in chunks that only this wave takes; no other wave touches
+// the row while the tile is summed, so a plain read-modify-write per chunk, in chunk order, suffices
+// (an LDS atomic add is several times slower here).
 template <typename T, int NC>
 __device__ inline void block_chunks_add(const uint32_t* e, T* v, const bool* on, T* yacc, int lane) {
   uint64_t m[NC];
@@ -70,16 +71,13 @@ __device__ inline void block_chunks_add(const uint32_t* e, T* v, const bool* on,
       }
     }
   }
-  bool head[NC];
-  T y[NC];
 #pragma unroll
   for (int u = 0; u < NC; ++u) {
-    head[u] = on[u] && (lane == 0 || !((m[u] >> (lane - 1)) & 1ull));
-    y[u] = head[u] ? yacc[e[u] & 0x7FFFu] : (T)0;
-  }
-#pragma unroll
-  for (int u = 0; u < NC; ++u) {
-    if (head[u]) yacc[e[u] & 0x7FFFu] = y[u] + v[u];
+    const bool head = on[u] && (lane == 0 || !((m[u] >> (lane - 1)) & 1ull));
+    if (head) {
+      T* y = &yacc[e[u] & 0x7FFFu];
+      *y += v[u];
+    }
   }
 }
 
@@ -289,16 +287,26 @@ __device__ inline void for_each_run(const sa_csr& A, const int32_t* bnd, const B
 
 // Per-tile counters in the caller's runs array, kRunStride int32 per tile: [0, 7) runs per size class,
 // [8, 15) the fill's slot cursors per class, [16, 48) singleton runs per row bank (local row mod 32),
-// [48, 80) the fill's cursors per row bank.
-constexpr int kRunStride = 96;
-constexpr int kRunCur = 8, kBankCnt = 16, kBankCur = 48;
+// [48, 80) the fill's cursors per row bank, [80, 96) 64-entry pieces of the long runs (more than 64
+// entries) per chunk column (local row mod 16), [96, 112) the fill's cursors per chunk column.
+constexpr int kRunStride = 128;
+constexpr int kRunCur = 8, kBankCnt = 16, kBankCur = 48, kColCnt = 80, kColCur = 96;
 
-// Slots of the runs of two or more entries (classes 1..6, largest first), rounded up to 32 so that
-// the singletons start on a half-wave boundary.
+// Entries of the long region at the head of a tile: 16 chunk columns (chunk r * 16 + c is in column
+// c; the kernel gives chunk k of a tile to wave k mod 16, so a column is one wave's, in row order) as
+// tall as the tallest column's pieces.
+__device__ inline int64_t long_slots(const int32_t* runs, int64_t t) {
+  int32_t h = 0;
+  for (int c = 0; c < 16; ++c) h = max(h, runs[kRunStride * t + kColCnt + c]);
+  return (int64_t)h * 16 * 64;
+}
+
+// Where the singletons start: after the long region and the slots of the runs of two to 64 entries
+// (classes 1..6, largest first), rounded up to 32 so that they start on a half-wave boundary.
 __device__ inline int64_t multi_slots(const int32_t* runs, int64_t t) {
   int64_t s = 0;
   for (int k = 1; k <= 6; ++k) s += (int64_t)runs[kRunStride * t + k] << k;
-  return (s + 31) & ~(int64_t)31;
+  return long_slots(runs, t) + ((s + 31) & ~(int64_t)31);
 }
 
 __global__ __launch_bounds__(SA_BLOCK) void block_count_kernel(sa_csr A, const int32_t* __restrict__ pbound,
@@ -308,9 +316,9 @@ __global__ __launch_bounds__(SA_BLOCK) void block_count_kernel(sa_csr A, const i
   if (threadIdx.x < 9) bnd[threadIdx.x] = pbound[threadIdx.x];
   __syncthreads();
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < G.n; r += (int64_t)gridDim.x * blockDim.x) {
-    for_each_run(A, bnd, G, r, [&](int64_t t, int k, int32_t, int32_t, int lr, int32_t) {
+    for_each_run(A, bnd, G, r, [&](int64_t t, int k, int32_t, int32_t L, int lr, int32_t) {
       if (k > 6) {
-        atomicOr(flag, 1);
+        atomicAdd(&runs[kRunStride * t + kColCnt + (lr & 15)], (L + 63) >> 6);
       } else {
         atomicAdd(&runs[kRunStride * t + k], 1);
         if (k == 0) atomicAdd(&runs[kRunStride * t + kBankCnt + (lr & 31)], 1);
@@ -376,12 +384,41 @@ __global__ __launch_bounds__(SA_BLOCK) void block_fill_kernel(sa_csr A, const in
   const T* vals = (const T*)A.vals;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < G.n; r += (int64_t)gridDim.x * blockDim.x) {
     for_each_run(A, bnd, G, r, [&](int64_t t, int k, int32_t e0, int32_t L, int lr, int32_t lo) {
-      if (k > 6) return;
       const int32_t* rt = runs + kRunStride * t;
+      const int64_t tbase = (int64_t)bchunk[t] * 64, tend = (int64_t)bchunk[t + 1] * 64;
+      if (k > 6) {
+        // a long run: pieces of 64 entries stacked in chunk column lr mod 16 of the long region, each
+        // piece one run for the kernel (its last entry ends the run)
+        const int c = lr & 15;
+        const int32_t P = (L + 63) >> 6;
+        const int32_t r0 = atomicAdd(&runs[kRunStride * t + kColCur + c], P);
+        for (int32_t q = 0; q < P; ++q) {
+          const int64_t pos = tbase + ((int64_t)(r0 + q) * 16 + c) * 64;
+          if (pos + 64 > nent || pos + 64 > tend || r0 + q >= rt[kColCnt + c]) {
+            atomicOr(flag, 8);  // the plan does not match this matrix
+            return;
+          }
+          for (int i = q * 64; i < L && i < (q + 1) * 64; ++i) {
+            const T v = vals[e0 + i];
+            const uint32_t col = (uint32_t)(A.indices[e0 + i] - lo);
+            const bool cont = i + 1 < L && i + 1 < (q + 1) * 64;
+            uint32_t code = (col << 15) | (uint32_t)lr | (cont ? (1u << 30) : 0u);
+            if (mag > (T)0) {
+              const T a = v < (T)0 ? -v : v;
+              if (a != mag) atomicOr(flag, 2);
+              if (v < (T)0) code |= 0x80000000u;
+            } else {
+              bvals[pos + (i - q * 64)] = v;
+            }
+            bent[pos + (i - q * 64)] = code;
+          }
+        }
+        return;
+      }
       int64_t off;
       if (k > 0) {
-        // slots of class k start after the larger classes' slots
-        off = 0;
+        // slots of class k start after the long region and the larger classes' slots
+        off = long_slots(runs, t);
         for (int q = 6; q > k; --q) off += (int64_t)rt[q] << q;
         off += (int64_t)atomicAdd(&runs[kRunStride * t + kRunCur + k], 1) << k;
       } else {
@@ -397,8 +434,8 @@ __global__ __launch_bounds__(SA_BLOCK) void block_fill_kernel(sa_csr A, const in
           off += min(c, rank) + ((q < b && c > rank) ? 1 : 0);
         }
       }
-      const int64_t pos = (int64_t)bchunk[t] * 64 + off;
-      if (pos + L > nent || pos + L > (int64_t)bchunk[t + 1] * 64) {
+      const int64_t pos = tbase + off;
+      if (pos + L > nent || pos + L > tend) {
         atomicOr(flag, 8);  // the plan does not match this matrix
         return;
       }

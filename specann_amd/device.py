@@ -255,7 +255,8 @@ class DeviceProblem:
 
     def build_blocks(self, signed: bool | None = None, brows: int | None = None, bcols: int | None = None) -> bool:
         """Regroup the couplings into the LDS-blocked layout (sa_block_plan / sa_block_fill) on the eight
-        entry-balanced panels; False (layout unchanged) if a row has more than 64 entries in one tile.
+        entry-balanced panels; False (layout unchanged) if a panel needs more column blocks than the
+        kernel walks.
         brows / bcols override the geometry of block_geometry (multiples of 64 that fit LDS)."""
         n, dev, s = self.n, self.device, stream()
         rb, cb, bnr = self.block_geometry()
@@ -274,7 +275,7 @@ class DeviceProblem:
             return False
         ntiles = 8 * bnr * bnc
         pbound = torch.from_numpy(bounds.astype(np.int32)).to(dev)
-        runs = torch.zeros(96 * ntiles, dtype=torch.int32, device=dev)
+        runs = torch.zeros(128 * ntiles, dtype=torch.int32, device=dev)
         bchunk = torch.empty(ntiles + 1, dtype=torch.int32, device=dev)
         flag = torch.zeros(1, dtype=torch.int32, device=dev)
         call("sa_block_plan", self.csr_ref, ptr(pbound), rb, cb, bnr, bnc, ptr(runs), ptr(bchunk), ptr(flag), s)
