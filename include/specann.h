@@ -263,26 +263,31 @@ int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, doubl
  * and sets SA_REJECTED if it exceeds pred_tol. only_if >= 0 makes the call a no-op unless
  * state.status == only_if (device-side gating, so restarts need no host round trip). */
 int sa_lanczos_start_f32(void* work, int64_t n, const float* s, const float* x, const float* d, double pexp,
-                         double cx, const float* y, double cy, float* V, int64_t ldv, float* Z, int32_t gate,
-                         int32_t only_if, hipStream_t stream);
+                         double cx, const float* y, double cy, float* V, int64_t ldv, void* Vh, float* Z,
+                         int32_t gate, int32_t only_if, hipStream_t stream);
 int sa_lanczos_start_f64(void* work, int64_t n, const double* s, const double* x, const double* d, double pexp,
-                         double cx, const double* y, double cy, double* V, int64_t ldv, double* Z, int32_t gate,
-                         int32_t only_if, hipStream_t stream);
-/* One Lanczos step with full (classical Gram-Schmidt) reorthogonalisation and a Ritz solve of the
- * tridiagonal; no-op unless state.status == SA_RUNNING. */
-int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, int64_t ldv, float* Z, float* W,
-                        hipStream_t stream);
-int sa_lanczos_step_f64(const sa_csr* A, void* work, const double* s, double* V, int64_t ldv, double* Z, double* W,
-                        hipStream_t stream);
+                         double cx, const double* y, double cy, double* V, int64_t ldv, void* Vh, double* Z,
+                         int32_t gate, int32_t only_if, hipStream_t stream);
+/* One Lanczos step: the fused SpMV, the projections on the cycle's basis and the Gram-Schmidt update
+ * (selective reorthogonalisation, see sa_lanczos_begin), and the FP64 Ritz solve of the projected
+ * matrix; no-op unless state.status == SA_RUNNING.
+ * Vh (may be NULL; the same for start, step and thick restart of one solve) is a shadow basis: ncv + 1
+ * rows of leading dimension ldv in _Float16 (_f32) or float (_f64), 8-byte aligned. The calls keep it
+ * equal to V rounded, and the projections on rows older than j - 1 read it instead of V (they only
+ * decide and correct components below sqrt(eps)): about half the projections' traffic. */
+int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, int64_t ldv, void* Vh, float* Z,
+                        float* W, hipStream_t stream);
+int sa_lanczos_step_f64(const sa_csr* A, void* work, const double* s, double* V, int64_t ldv, void* Vh, double* Z,
+                        double* W, hipStream_t stream);
 /* Thick restart (gated on SA_RESTART): keep the `keep` largest Ritz pairs of the full basis. The basis
  * is rotated in place: rows 0..keep-1 become the Ritz vectors, row keep the last Lanczos vector, Z its
  * scaled copy, and the projected matrix becomes diag(theta) bordered by the arrow couplings. With
  * keep = 1 this is the restart from the Ritz vector alone (the k = 1 thick restart of the reference's
  * eigsh). */
-int sa_lanczos_thick_restart_f32(void* work, int64_t n, const float* s, float* V, int64_t ldv, float* Z,
-                                 hipStream_t stream);
-int sa_lanczos_thick_restart_f64(void* work, int64_t n, const double* s, double* V, int64_t ldv, double* Z,
-                                 hipStream_t stream);
+int sa_lanczos_thick_restart_f32(void* work, int64_t n, const float* s, float* V, int64_t ldv, void* Vh,
+                                 float* Z, hipStream_t stream);
+int sa_lanczos_thick_restart_f64(void* work, int64_t n, const double* s, double* V, int64_t ldv, void* Vh,
+                                 double* Z, hipStream_t stream);
 /* Ritz vector y = sum_m ritz_m v_m of the current cycle; state.r[4] = ||y||, state.r[5] = y . y_last
  * (y_last may be NULL). only_if as in sa_lanczos_start. */
 int sa_lanczos_ritz_f32(void* work, int64_t n, const float* V, int64_t ldv, const float* y_last, float* y,

@@ -206,14 +206,14 @@ _SIGNATURES = {
     "sa_spmv_gather_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, _P, c_double, _P, _P]),
     "sa_spmv_gather_f64": (c_int32, [POINTER(SaCsr), _P, _P, c_double, _P, c_double, _P, _P]),
     "sa_lanczos_begin": (c_int32, [_P, c_int32, c_int32, c_double, c_double, c_double, c_int32, c_double, _P]),
-    "sa_lanczos_thick_restart_f32": (c_int32, [_P, c_int64, _P, _P, c_int64, _P, _P]),
-    "sa_lanczos_thick_restart_f64": (c_int32, [_P, c_int64, _P, _P, c_int64, _P, _P]),
+    "sa_lanczos_thick_restart_f32": (c_int32, [_P, c_int64, _P, _P, c_int64, _P, _P, _P]),
+    "sa_lanczos_thick_restart_f64": (c_int32, [_P, c_int64, _P, _P, c_int64, _P, _P, _P]),
     "sa_lanczos_start_f32": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, _P, c_double, _P, c_int64, _P,
-                                       c_int32, c_int32, _P]),
+                                       _P, c_int32, c_int32, _P]),
     "sa_lanczos_start_f64": (c_int32, [_P, c_int64, _P, _P, _P, c_double, c_double, _P, c_double, _P, c_int64, _P,
-                                       c_int32, c_int32, _P]),
-    "sa_lanczos_step_f32": (c_int32, [POINTER(SaCsr), _P, _P, _P, c_int64, _P, _P, _P]),
-    "sa_lanczos_step_f64": (c_int32, [POINTER(SaCsr), _P, _P, _P, c_int64, _P, _P, _P]),
+                                       _P, c_int32, c_int32, _P]),
+    "sa_lanczos_step_f32": (c_int32, [POINTER(SaCsr), _P, _P, _P, c_int64, _P, _P, _P, _P]),
+    "sa_lanczos_step_f64": (c_int32, [POINTER(SaCsr), _P, _P, _P, c_int64, _P, _P, _P, _P]),
     "sa_lanczos_ritz_f32": (c_int32, [_P, c_int64, _P, c_int64, _P, _P, c_int32, _P]),
     "sa_lanczos_ritz_f64": (c_int32, [_P, c_int64, _P, c_int64, _P, _P, c_int32, _P]),
     "sa_norm_dot_f32": (c_int32, [_P, c_int64, _P, _P, c_double, c_double, c_int32, _P]),

@@ -25,6 +25,29 @@
 
 namespace sa {
 
+// The shadow basis: a 16-bit (FP32 form) or 32-bit (FP64 form) copy of every basis row, read only to
+// measure the projections on rows older than j - 1 (which decide and correct components of order
+// sqrt(eps); the copy's rounding, ~1e-7 of ||W|| for FP16, is far below that). Row m holds the unit
+// vector sigma_m v_m times shscale = sqrt(n), so that its entries sit well inside the FP16 range; it is
+// written by the projections of step m (which read v_m in full precision, with sigma_m known) and by
+// the thick restart for the rows it rotates.
+template <typename T>
+struct Shadow;
+template <>
+struct Shadow<float> {
+  typedef _Float16 type;
+  typedef _Float16 vec __attribute__((ext_vector_type(4)));    // the L = 4 entries of a 16-byte load of V
+  typedef _Float16 vec16 __attribute__((ext_vector_type(8)));  // one 16-byte load of the shadow
+};
+template <>
+struct Shadow<double> {
+  typedef float type;
+  typedef float vec __attribute__((ext_vector_type(2)));
+  typedef float vec16 __attribute__((ext_vector_type(4)));
+};
+template <typename T>
+using SH = typename Shadow<T>::type;
+
 template <class F>
 __global__ void lz_begin_kernel(F fin, int32_t ncv, int32_t maxiter, double tol, double bound, const double* bounds,
                                 double pred_tol, int32_t keep, double orth_tau) {
@@ -120,55 +143,127 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_combine_kernel(void* work, int64_
   }
 }
 
-// h_m = sigma_m v_m . W for every basis vector of the cycle. 16-byte loads, chunks of 8 vectors per
-// pass over the workgroup's rows; per-workgroup partials are stored column-major (part[m * G + b]) and
-// the last workgroup sums them in workgroup order, a 32-lane group per column.
+// h_m = sigma_m v_m . W for every basis vector of the cycle. 16-byte loads (from the shadow too, which
+// then covers twice the entries per lane, so a chunk keeps as many bytes in flight as a full-precision
+// one), chunks of 8 rows per pass over the workgroup's rows: first the rows below mh from the shadow, then
+// the rest in full precision (each chunk's loads are unconditional, rows past the chunk clamped to a
+// valid one and dropped, so all eight stay in flight). Per-workgroup partials are stored
+// column-major (part[m * G + b]); column j + 1 takes ||W||^2. The finisher sums them in workgroup
+// order. The pass over row j also writes its shadow row.
 template <typename T, class M>
 __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(M map, const T* __restrict__ V, int64_t ldv,
+                                                           SH<T>* __restrict__ Vh, double shscale,
                                                            const T* __restrict__ W) {
   using VT = typename vec16<T>::type;
+  using SV = typename Shadow<T>::vec;
   constexpr int L = vec16<T>::L;
   __shared__ double red[2][SA_WAVES_PER_BLOCK][SA_RED_COLS];
+  __shared__ double redw[SA_WAVES_PER_BLOCK];
   const Slice sl = map.slice();
   const sa_state* st = state_of(sl.work);
   if (st->status != SA_RUNNING) return;
   const int j = st->j;
-  const int nch = j / SA_RED_COLS + 1;
+  // rows below mh come from the shadow (the first step of a cycle measures the kept rows, whose
+  // projections are the arrow couplings, in full precision)
+  const int mh = (Vh && j > st->kept) ? j - 1 : 0;
+  const T put = (T)(st->sigma[j] * shscale);  // v_j -> its shadow row
   double* part = partials_of(sl.work);
   const int G = sl.G;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t lo = sl.lo, hi = sl.hi;
   const int64_t nv = hi > lo ? (hi - lo) / L : 0;
   double ww = 0.0;  // ||W||^2, accumulated in the first pass
-  for (int ch = 0; ch < nch; ++ch) {
-    const int m0 = ch * SA_RED_COLS;
-    const int mc = min(SA_RED_COLS, j + 1 - m0);
+  int ch = 0;
+  for (int m0 = 0; m0 <= j; m0 += SA_RED_COLS, ++ch) {
+    const bool shadow = m0 < mh;
+    const int mend = shadow ? mh : j + 1;  // chunks do not cross mh
+    const int mc = min(SA_RED_COLS, mend - m0);
     double acc[SA_RED_COLS];
+    T get[SA_RED_COLS];  // shadow row -> raw scale of v_m
 #pragma unroll
-    for (int c = 0; c < SA_RED_COLS; ++c) acc[c] = 0.0;
-    for (int64_t g = threadIdx.x; g < nv; g += SA_BLOCK) {
+    for (int c = 0; c < SA_RED_COLS; ++c) {
+      acc[c] = 0.0;
+      get[c] = (shadow && c < mc) ? (T)(1.0 / (st->sigma[m0 + c] * shscale)) : (T)0;
+    }
+    const bool first = ch == 0;
+    int64_t tail = lo + nv * L;  // first row of the scalar tail
+    if (shadow) {
+      // 16-byte shadow loads: E = 2 L entries per lane per row, so that eight rows keep as many bytes
+      // in flight as the full-precision chunks
+      using S16 = typename Shadow<T>::vec16;
+      constexpr int E = 2 * L;
+      const int64_t ne = hi > lo ? (hi - lo) / E : 0;
+      tail = lo + ne * E;
+      for (int64_t g = threadIdx.x; g < ne; g += SA_BLOCK) {
+        const int64_t r = lo + g * E;
+        const VT w0 = *reinterpret_cast<const VT*>(&W[r]);
+        const VT w1 = *reinterpret_cast<const VT*>(&W[r + L]);
+        if (first) {
+#pragma unroll
+          for (int l = 0; l < L; ++l) ww += (double)w0[l] * (double)w0[l] + (double)w1[l] * (double)w1[l];
+        }
+        S16 h[SA_RED_COLS];
+#pragma unroll
+        for (int c = 0; c < SA_RED_COLS; ++c)
+          h[c] = *reinterpret_cast<const S16*>(&Vh[(int64_t)(m0 + min(c, mc - 1)) * ldv + r]);
+#pragma unroll
+        for (int c = 0; c < SA_RED_COLS; ++c) {
+          if (c < mc) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+              acc[c] += (double)((T)h[c][l] * get[c]) * (double)w0[l];
+              acc[c] += (double)((T)h[c][L + l] * get[c]) * (double)w1[l];
+            }
+          }
+        }
+      }
+    }
+    for (int64_t g = threadIdx.x; !shadow && g < nv; g += SA_BLOCK) {
       const int64_t r = lo + g * L;
       const VT w = *reinterpret_cast<const VT*>(&W[r]);
-      if (ch == 0) {
+      if (first) {
 #pragma unroll
         for (int l = 0; l < L; ++l) ww += (double)w[l] * (double)w[l];
       }
       VT v[SA_RED_COLS];
+      {
 #pragma unroll
-      for (int c = 0; c < SA_RED_COLS; ++c)
-        v[c] = c < mc ? *reinterpret_cast<const VT*>(&V[(int64_t)(m0 + c) * ldv + r]) : VT{};
+        for (int c = 0; c < SA_RED_COLS; ++c)
+          v[c] = *reinterpret_cast<const VT*>(&V[(int64_t)(m0 + min(c, mc - 1)) * ldv + r]);
+        if (Vh && m0 + mc - 1 == j) {
+#pragma unroll
+          for (int c = 0; c < SA_RED_COLS; ++c) {
+            if (c == mc - 1) {
+              SV hj;
+#pragma unroll
+              for (int l = 0; l < L; ++l) hj[l] = (SH<T>)(v[c][l] * put);
+              *reinterpret_cast<SV*>(&Vh[(int64_t)j * ldv + r]) = hj;
+            }
+          }
+        }
+      }
 #pragma unroll
       for (int c = 0; c < SA_RED_COLS; ++c) {
+        if (c < mc) {
 #pragma unroll
-        for (int l = 0; l < L; ++l) acc[c] += (double)v[c][l] * (double)w[l];
+          for (int l = 0; l < L; ++l) acc[c] += (double)v[c][l] * (double)w[l];
+        }
       }
     }
-    for (int64_t r = lo + nv * L + threadIdx.x; r < hi; r += SA_BLOCK) {
+    for (int64_t r = tail + threadIdx.x; r < hi; r += SA_BLOCK) {
       const double w = (double)W[r];
-      if (ch == 0) ww += w * w;
+      if (first) ww += w * w;
 #pragma unroll
-      for (int c = 0; c < SA_RED_COLS; ++c)
-        if (c < mc) acc[c] += (double)V[(int64_t)(m0 + c) * ldv + r] * w;
+      for (int c = 0; c < SA_RED_COLS; ++c) {
+        if (c >= mc) continue;
+        const int64_t at = (int64_t)(m0 + c) * ldv + r;
+        if (shadow) {
+          acc[c] += (double)((T)Vh[at] * get[c]) * w;
+        } else {
+          acc[c] += (double)V[at] * w;
+          if (Vh && m0 + c == j) Vh[at] = (SH<T>)(V[at] * put);
+        }
+      }
     }
 #pragma unroll
     for (int c = 0; c < SA_RED_COLS; ++c) {
@@ -181,8 +276,8 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(M map, const T* __res
       part[(int64_t)(m0 + c) * G + sl.blk] =
           (red[ch & 1][0][c] + red[ch & 1][1][c]) + (red[ch & 1][2][c] + red[ch & 1][3][c]);
     }
+    if (shadow && m0 + mc == mh) m0 = mh - SA_RED_COLS;  // continue with the full-precision rows at mh
   }
-  __shared__ double redw[SA_WAVES_PER_BLOCK];
   const double wsum = block_sum(ww, redw);
   if (threadIdx.x == 0) part[(int64_t)(j + 1) * G + sl.blk] = wsum;
 }
@@ -711,6 +806,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(F fin) {
 template <typename T, class M>
 __global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(M map, const T* __restrict__ s,
                                                                     T* __restrict__ V, int64_t ldv,
+                                                                    SH<T>* __restrict__ Vh, double shscale,
                                                                     T* __restrict__ Z) {
   const Slice sl = map.slice();
   void* work = sl.work;
@@ -722,6 +818,8 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(M map, const
   const int m = st->j;
   const int kk = st->kept;
   const double* S = restart_of(work);
+  // shadow rows of the rotated basis: the kept Ritz vectors are unit, row kept keeps sigma_m
+  const T put_kept = (T)shscale, put_last = (T)(st->sigma[m] * shscale);
   for (int q = threadIdx.x; q < kk * m; q += blockDim.x) {
     const int u = q / m, p = q % m;
     coef[p * SA_MAX_KEEP + u] = S[(int64_t)u * SA_MAX_NCV + p] * st->sigma[p];
@@ -773,9 +871,17 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(M map, const
 #pragma unroll
         for (int l = 0; l < L; ++l) o[l] = (T)acc[u][l];
         *reinterpret_cast<VT*>(&V[(int64_t)u * ldv + r]) = o;
+        if (Vh) {
+#pragma unroll
+          for (int l = 0; l < L; ++l) Vh[(int64_t)u * ldv + r + l] = (SH<T>)(o[l] * put_kept);
+        }
       }
     }
     *reinterpret_cast<VT*>(&V[(int64_t)kk * ldv + r]) = lastv;
+    if (Vh) {
+#pragma unroll
+      for (int l = 0; l < L; ++l) Vh[(int64_t)kk * ldv + r + l] = (SH<T>)(lastv[l] * put_last);
+    }
     VT z = lastv;
     if (s) z = *reinterpret_cast<const VT*>(&s[r]) * lastv;
     *reinterpret_cast<VT*>(&Z[r]) = z;
@@ -793,8 +899,12 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(M map, const
     const T lastv = V[(int64_t)m * ldv + r];
 #pragma unroll
     for (int u = 0; u < SA_MAX_KEEP; ++u)
-      if (u < kk) V[(int64_t)u * ldv + r] = (T)acc[u];
+      if (u < kk) {
+        V[(int64_t)u * ldv + r] = (T)acc[u];
+        if (Vh) Vh[(int64_t)u * ldv + r] = (SH<T>)((T)acc[u] * put_kept);
+      }
     V[(int64_t)kk * ldv + r] = lastv;
+    if (Vh) Vh[(int64_t)kk * ldv + r] = (SH<T>)(lastv * put_last);
     Z[r] = s ? s[r] * lastv : lastv;
   }
 }
@@ -853,7 +963,7 @@ constexpr int SA_DOTS_GRID = 512;
 
 template <typename T>
 static int lanczos_start(void* work, int64_t n, const T* s, const T* x, const T* d, double pexp, double cx,
-                         const T* y, double cy, T* V, int64_t ldv, T* Z, int32_t gate, int32_t only_if,
+                         const T* y, double cy, T* V, int64_t ldv, void* Vh, T* Z, int32_t gate, int32_t only_if,
                          hipStream_t stream) {
   SA_REQUIRE(work && n > 0 && x && V && Z && ldv >= n, "bad arguments");
   const int G = red_grid(n);
@@ -867,10 +977,12 @@ static int lanczos_start(void* work, int64_t n, const T* s, const T* x, const T*
 }
 
 template <typename T>
-static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, int64_t ldv, T* Z, T* W, hipStream_t stream) {
+static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, int64_t ldv, void* Vh, T* Z, T* W,
+                        hipStream_t stream) {
   SA_REQUIRE(A && work && V && Z && W && A->tile_rows && ldv >= A->n, "bad arguments");
   SA_REQUIRE(aligned16(V) && aligned16(W) && ldv % vec16<T>::L == 0,
              "V and W must be 16-byte aligned and ldv a multiple of 16 bytes");
+  SA_REQUIRE(((uintptr_t)Vh & 7) == 0, "the shadow basis must be 8-byte aligned");
   const int64_t n = A->n;
   SpmvParams<T> p = params_from<T>(A);
   p.g = Z;
@@ -888,7 +1000,7 @@ static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, int64_t l
   }
   const int Gd = grid_for(n, SA_BLOCK * 4, SA_DOTS_GRID);
   hipLaunchKernelGGL((lz_dots_kernel<T, OneMap>), dim3(Gd), dim3(SA_BLOCK), 0, stream, OneMap{work, n},
-                     (const T*)V, ldv, (const T*)W);
+                     (const T*)V, ldv, (SH<T>*)Vh, sqrt((double)n), (const T*)W);
   rc = check_launch("lanczos_dots");
   if (rc) return rc;
   hipLaunchKernelGGL(lz_dots_finish_kernel<OneFin>, dim3(1), dim3(SA_BLOCK), 0, stream, OneFin{work, Gd});
@@ -904,7 +1016,8 @@ static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, int64_t l
 }
 
 template <typename T>
-static int lanczos_thick_restart(void* work, int64_t n, const T* s, T* V, int64_t ldv, T* Z, hipStream_t stream) {
+static int lanczos_thick_restart(void* work, int64_t n, const T* s, T* V, int64_t ldv, void* Vh, T* Z,
+                                 hipStream_t stream) {
   SA_REQUIRE(work && n > 0 && V && Z && ldv >= n, "bad arguments");
   SA_REQUIRE(aligned16(V) && aligned16(Z) && aligned16(s) && ldv % vec16<T>::L == 0,
              "V, Z and s must be 16-byte aligned and ldv a multiple of 16 bytes");
@@ -912,7 +1025,7 @@ static int lanczos_thick_restart(void* work, int64_t n, const T* s, T* V, int64_
   int rc = check_launch("lanczos_thick_prepare");
   if (rc) return rc;
   hipLaunchKernelGGL((lz_thick_vectors_kernel<T, OneMap>), dim3(grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID)),
-                     dim3(SA_BLOCK), 0, stream, OneMap{work, n}, s, V, ldv, Z);
+                     dim3(SA_BLOCK), 0, stream, OneMap{work, n}, s, V, ldv, (SH<T>*)Vh, sqrt((double)n), Z);
   rc = check_launch("lanczos_thick_vectors");
   if (rc) return rc;
   hipLaunchKernelGGL(lz_thick_finish_kernel<OneFin>, dim3(1), dim3(64), 0, stream, OneFin{work, 1});
@@ -999,7 +1112,7 @@ static int batch_step(const sa_csr* A, const sa_batch* Bt, const T* s, T* V, int
   if (rc) return rc;
   const dim3 grid((unsigned)Bt->nblocks), fgrid(Bt->B);
   hipLaunchKernelGGL((lz_dots_kernel<T, BatchMap>), grid, dim3(SA_BLOCK), 0, stream, bmap(Bt), (const T*)V, ldv,
-                     (const T*)W);
+                     (SH<T>*)nullptr, 1.0, (const T*)W);
   rc = check_launch("batch_lanczos_dots");
   if (rc) return rc;
   hipLaunchKernelGGL(lz_dots_finish_kernel<BatchFin>, fgrid, dim3(SA_BLOCK), 0, stream, bfin(Bt));
@@ -1022,7 +1135,7 @@ static int batch_thick_restart(const sa_batch* Bt, const T* s, T* V, int64_t ldv
   int rc = check_launch("batch_thick_prepare");
   if (rc) return rc;
   hipLaunchKernelGGL((lz_thick_vectors_kernel<T, BatchMap>), dim3((unsigned)Bt->nblocks), dim3(SA_BLOCK), 0, stream,
-                     bmap(Bt), s, V, ldv, Z);
+                     bmap(Bt), s, V, ldv, (SH<T>*)nullptr, 1.0, Z);
   rc = check_launch("batch_thick_vectors");
   if (rc) return rc;
   hipLaunchKernelGGL(lz_thick_finish_kernel<BatchFin>, dim3(Bt->B), dim3(64), 0, stream, bfin(Bt));
@@ -1063,35 +1176,35 @@ int sa_lanczos_begin(void* work, int32_t ncv, int32_t maxiter, double tol, doubl
 }
 
 int sa_lanczos_start_f32(void* work, int64_t n, const float* s, const float* x, const float* d, double pexp,
-                         double cx, const float* y, double cy, float* V, int64_t ldv, float* Z, int32_t gate,
+                         double cx, const float* y, double cy, float* V, int64_t ldv, void* Vh, float* Z, int32_t gate,
                          int32_t only_if, hipStream_t stream) {
-  return lanczos_start<float>(work, n, s, x, d, pexp, cx, y, cy, V, ldv, Z, gate, only_if, stream);
+  return lanczos_start<float>(work, n, s, x, d, pexp, cx, y, cy, V, ldv, Vh, Z, gate, only_if, stream);
 }
 
 int sa_lanczos_start_f64(void* work, int64_t n, const double* s, const double* x, const double* d, double pexp,
-                         double cx, const double* y, double cy, double* V, int64_t ldv, double* Z, int32_t gate,
+                         double cx, const double* y, double cy, double* V, int64_t ldv, void* Vh, double* Z, int32_t gate,
                          int32_t only_if, hipStream_t stream) {
-  return lanczos_start<double>(work, n, s, x, d, pexp, cx, y, cy, V, ldv, Z, gate, only_if, stream);
+  return lanczos_start<double>(work, n, s, x, d, pexp, cx, y, cy, V, ldv, Vh, Z, gate, only_if, stream);
 }
 
-int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, int64_t ldv, float* Z, float* W,
-                        hipStream_t stream) {
-  return lanczos_step<float>(A, work, s, V, ldv, Z, W, stream);
+int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, int64_t ldv, void* Vh, float* Z,
+                        float* W, hipStream_t stream) {
+  return lanczos_step<float>(A, work, s, V, ldv, Vh, Z, W, stream);
 }
 
-int sa_lanczos_step_f64(const sa_csr* A, void* work, const double* s, double* V, int64_t ldv, double* Z, double* W,
-                        hipStream_t stream) {
-  return lanczos_step<double>(A, work, s, V, ldv, Z, W, stream);
+int sa_lanczos_step_f64(const sa_csr* A, void* work, const double* s, double* V, int64_t ldv, void* Vh, double* Z,
+                        double* W, hipStream_t stream) {
+  return lanczos_step<double>(A, work, s, V, ldv, Vh, Z, W, stream);
 }
 
-int sa_lanczos_thick_restart_f32(void* work, int64_t n, const float* s, float* V, int64_t ldv, float* Z,
-                                 hipStream_t stream) {
-  return lanczos_thick_restart<float>(work, n, s, V, ldv, Z, stream);
+int sa_lanczos_thick_restart_f32(void* work, int64_t n, const float* s, float* V, int64_t ldv, void* Vh,
+                                 float* Z, hipStream_t stream) {
+  return lanczos_thick_restart<float>(work, n, s, V, ldv, Vh, Z, stream);
 }
 
-int sa_lanczos_thick_restart_f64(void* work, int64_t n, const double* s, double* V, int64_t ldv, double* Z,
-                                 hipStream_t stream) {
-  return lanczos_thick_restart<double>(work, n, s, V, ldv, Z, stream);
+int sa_lanczos_thick_restart_f64(void* work, int64_t n, const double* s, double* V, int64_t ldv, void* Vh,
+                                 double* Z, hipStream_t stream) {
+  return lanczos_thick_restart<double>(work, n, s, V, ldv, Vh, Z, stream);
 }
 
 int sa_lanczos_ritz_f32(void* work, int64_t n, const float* V, int64_t ldv, const float* y_last, float* y,

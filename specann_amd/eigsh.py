@@ -66,9 +66,10 @@ class LanczosSolver:
     """Buffers and launch sequence of the warm-started Lanczos for one DeviceProblem."""
 
     def __init__(self, prob: DeviceProblem, ncv: int | None = None, chunk: int = 8, keep: int | None = None,
-                 graphs: bool | None = None, orth_tau: float | None = None):
+                 graphs: bool | None = None, orth_tau: float | None = None, shadow: bool = True):
         """orth_tau: selective reorthogonalisation threshold (None: sqrt(eps) / 2 of the vectors' type;
-        0: reorthogonalise against the whole basis at every step)."""
+        0: reorthogonalise against the whole basis at every step). shadow: keep a half-width copy of the
+        basis for the projections on rows older than j - 1."""
         self.prob = prob
         self.orth_tau = default_orth_tau(prob.dtype) if orth_tau is None else float(orth_tau)
         self.n = prob.n
@@ -83,6 +84,8 @@ class LanczosSolver:
         self.ldv = _lib.padded_ld(self.n, 4 if dt == torch.float32 else 8)
         self.Vbuf = torch.empty((self.ncv + 1, self.ldv), dtype=dt, device=dev)
         self.V = self.Vbuf[:, : self.n]
+        self.Vh = (torch.empty((self.ncv + 1, self.ldv), dtype=torch.float16 if dt == torch.float32 else torch.float32,
+                               device=dev) if shadow else None)
         self.Z = torch.empty(self.n, dtype=dt, device=dev)
         self.W = torch.empty(self.n, dtype=dt, device=dev)
         self.Y = torch.empty(self.n, dtype=dt, device=dev)
@@ -104,15 +107,15 @@ class LanczosSolver:
               cx: float = 1.0, y: torch.Tensor | None = None, cy: float = 0.0, gate: int = 0,
               only_if: int = -1) -> None:
         call(f"sa_lanczos_start_{self.sfx}", self.work.ptr, self.n, ptr(s), ptr(x), ptr(d), float(pexp), float(cx),
-             ptr(y), float(cy), ptr(self.V), self.ldv, ptr(self.Z), int(gate), int(only_if), stream())
+             ptr(y), float(cy), ptr(self.V), self.ldv, ptr(self.Vh), ptr(self.Z), int(gate), int(only_if), stream())
 
     def _step(self, s: torch.Tensor | None) -> None:
         call(f"sa_lanczos_step_{self.sfx}", self.prob.csr_ref, self.work.ptr, ptr(s), ptr(self.V), self.ldv,
-             ptr(self.Z), ptr(self.W), stream())
+             ptr(self.Vh), ptr(self.Z), ptr(self.W), stream())
 
     def thick_restart(self, s: torch.Tensor | None) -> None:
         call(f"sa_lanczos_thick_restart_{self.sfx}", self.work.ptr, self.n, ptr(s), ptr(self.V), self.ldv,
-             ptr(self.Z), stream())
+             ptr(self.Vh), ptr(self.Z), stream())
 
     def ritz(self, y_last: torch.Tensor | None = None, only_if: int = -1) -> None:
         call(f"sa_lanczos_ritz_{self.sfx}", self.work.ptr, self.n, ptr(self.V), self.ldv, ptr(y_last), ptr(self.Y),

@@ -256,7 +256,9 @@ def test_lanczos_leading_eigenpair(gpu, case, tdtype, panels):
 def test_selective_reorthogonalisation_matches_full(gpu, tdtype):
     """Selective reorthogonalisation (old rows subtracted only when their projection exceeds
     sqrt(eps)/2 of beta) converges to the same eigenpair as the full one, with the true residual of
-    its Ritz vector within the tolerance, and takes the selective path on most steps."""
+    its Ritz vector within the tolerance, and takes the selective path on most steps; with and
+    without the half-width shadow basis (projections on rows older than j - 1 read from it), whose
+    rows hold sigma_m sqrt(n) v_m rounded to the shadow type."""
     from specann_amd.eigsh import LanczosSolver, default_orth_tau
     from specann_amd._lib import SA_CONVERGED
     J = graphs.erdos_renyi(20000, 30 / 19999, seed=21, kind="pm1")
@@ -269,23 +271,35 @@ def test_selective_reorthogonalisation_matches_full(gpu, tdtype):
     lam_ref, _ = O.leading_eigpair(Jn, sv)
     tol = 1e-6 if tdtype == torch.float32 else 1e-10
     out = {}
-    for tau in (0.0, None):
-        solver = LanczosSolver(prob, ncv=33, orth_tau=tau)
+    for tau, shadow in ((0.0, False), (None, False), (None, True)):
+        solver = LanczosSolver(prob, ncv=33, orth_tau=tau, shadow=shadow)
         x0 = prob.fill_normal(4, 0, torch.empty(prob.n, dtype=tdtype, device=gpu))
         solver.begin(tol, 4000, bound=3.0)
         solver.start(s, x0)
         res = solver.run(s, 4000)
-        counters = list(prob.work.full().counters[:2])
+        st = prob.work.full()
+        counters = list(st.counters[:2])
+        if shadow:
+            # every row below the current step holds its shadow (written by its step's projections or
+            # by the thick restart that rotated it)
+            assert st.j > st.kept
+            for m in range(st.j):
+                want = solver.V[m].double() * (st.sigma[m] * np.sqrt(prob.n))
+                torch.testing.assert_close(solver.Vh[m, : prob.n].double(), want,
+                                           rtol=1e-3 if tdtype == torch.float32 else 1e-7, atol=1e-4)
         nrm, _ = solver.final_vector(None)
         y = solver.Y.cpu().numpy().astype(np.float64) / nrm
         true_res = np.linalg.norm(O.n_matvec(Jn, sv, y) - res.theta * y)
-        out[tau] = (res, counters, true_res)
-        print(f"reorth tau={tau} dtype={tdtype} steps={res.steps} theta={res.theta!r} counters={counters} "
-              f"true residual {true_res:.2e}")
+        out[tau, shadow] = (res, counters, true_res)
+        print(f"reorth tau={tau} shadow={shadow} dtype={tdtype} steps={res.steps} theta={res.theta!r} "
+              f"counters={counters} true residual {true_res:.2e}")
         assert res.status == SA_CONVERGED
         assert abs(res.theta - lam_ref) <= 10 * tol
         assert true_res <= (1e-5 if tdtype == torch.float32 else 1e-9)
-    full, sel = out[0.0], out[None]
+    full, sel = out[0.0, False], out[None, False]
+    sh = out[None, True]
+    assert abs(sh[0].theta - sel[0].theta) <= 10 * tol
+    assert sh[1][1] > sh[1][0]
     assert full[1][0] >= full[0].steps - 2 * (full[0].cycles + 1)   # tau = 0: old rows every step
     assert sel[1][1] > sel[1][0]                                      # mostly the two-row update
     assert default_orth_tau(tdtype) == pytest.approx(0.5 * np.sqrt(np.finfo(dt).eps))
