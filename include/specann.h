@@ -272,7 +272,8 @@ int sa_lanczos_start_f64(void* work, int64_t n, const double* s, const double* x
  * (selective reorthogonalisation, see sa_lanczos_begin), and the FP64 Ritz solve of the projected
  * matrix; no-op unless state.status == SA_RUNNING.
  * Vh (may be NULL; the same for start, step and thick restart of one solve) is a shadow basis: ncv + 1
- * rows of leading dimension ldv in _Float16 (_f32) or float (_f64), 8-byte aligned. The calls keep it
+ * rows of leading dimension ldv in _Float16 (_f32) or float (_f64), 16-byte aligned, ldv then a
+ * multiple of 8 (_f32) or 4 (_f64) entries. The calls keep it
  * equal to V rounded, and the projections on rows older than j - 1 read it instead of V (they only
  * decide and correct components below sqrt(eps)): about half the projections' traffic. */
 int sa_lanczos_step_f32(const sa_csr* A, void* work, const float* s, float* V, int64_t ldv, void* Vh, float* Z,

@@ -976,22 +976,35 @@ static int lanczos_start(void* work, int64_t n, const T* s, const T* x, const T*
   return check_launch("lanczos_start_finish");
 }
 
+// One Lanczos step in three phases, in this order on one stream:
+//   phase 0: the SpMV (plain CSR layout: also the sigma_j scaling)
+//   phase 1: panel combine, projections and their finisher, the Gram-Schmidt update
+//   phase 2: beta_j, the Ritz solve and the stopping test (one workgroup)
 template <typename T>
-static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, int64_t ldv, void* Vh, T* Z, T* W,
-                        hipStream_t stream) {
+static int lanczos_step_phase(const sa_csr* A, void* work, const T* s, T* V, int64_t ldv, void* Vh, T* Z, T* W,
+                              int phase, hipStream_t stream) {
   SA_REQUIRE(A && work && V && Z && W && A->tile_rows && ldv >= A->n, "bad arguments");
   SA_REQUIRE(aligned16(V) && aligned16(W) && ldv % vec16<T>::L == 0,
              "V and W must be 16-byte aligned and ldv a multiple of 16 bytes");
-  SA_REQUIRE(((uintptr_t)Vh & 7) == 0, "the shadow basis must be 8-byte aligned");
+  SA_REQUIRE(!Vh || (aligned16(Vh) && ldv % (2 * vec16<T>::L) == 0),
+             "the shadow basis must be 16-byte aligned and ldv a multiple of 32 bytes");
+  SA_REQUIRE(phase >= 0 && phase <= 2, "phase must be 0, 1 or 2");
   const int64_t n = A->n;
-  SpmvParams<T> p = params_from<T>(A);
-  p.g = Z;
-  p.s = s;
-  p.lz = reinterpret_cast<const sa_state*>(work);
-  p.gate = &p.lz->status;
-  p.y = W;
-  int rc = launch_spmv<T>(p, A->tile_nnz, stream);
-  if (rc) return rc;
+  const int Go = grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID);
+  if (phase == 0) {
+    SpmvParams<T> p = params_from<T>(A);
+    p.g = Z;
+    p.s = s;
+    p.lz = reinterpret_cast<const sa_state*>(work);
+    p.gate = &p.lz->status;
+    p.y = W;
+    return launch_spmv<T>(p, A->tile_nnz, stream);
+  }
+  if (phase == 2) {
+    hipLaunchKernelGGL(lz_orth_finish_kernel<OneFin>, dim3(1), dim3(SA_BLOCK), 0, stream, OneFin{work, Go});
+    return check_launch("lanczos_orth_finish");
+  }
+  int rc = 0;
   if (A->npanels > 0) {
     hipLaunchKernelGGL(lz_combine_kernel<T>, dim3(grid_for(n, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0, stream, work,
                        n, A->npanels, (const T*)A->partial, A->pld, s, W);
@@ -1006,13 +1019,19 @@ static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, int64_t l
   hipLaunchKernelGGL(lz_dots_finish_kernel<OneFin>, dim3(1), dim3(SA_BLOCK), 0, stream, OneFin{work, Gd});
   rc = check_launch("lanczos_dots_finish");
   if (rc) return rc;
-  const int Go = grid_for(n, SA_BLOCK * 2, SA_ORTH_GRID);
   hipLaunchKernelGGL((lz_orth_kernel<T, OneMap>), dim3(Go), dim3(SA_BLOCK), 0, stream, OneMap{work, n}, s, V, ldv,
                      Z, (const T*)W);
-  rc = check_launch("lanczos_orth");
-  if (rc) return rc;
-  hipLaunchKernelGGL(lz_orth_finish_kernel<OneFin>, dim3(1), dim3(SA_BLOCK), 0, stream, OneFin{work, Go});
-  return check_launch("lanczos_orth_finish");
+  return check_launch("lanczos_orth");
+}
+
+template <typename T>
+static int lanczos_step(const sa_csr* A, void* work, const T* s, T* V, int64_t ldv, void* Vh, T* Z, T* W,
+                        hipStream_t stream) {
+  for (int phase = 0; phase < 3; ++phase) {
+    const int rc = lanczos_step_phase<T>(A, work, s, V, ldv, Vh, Z, W, phase, stream);
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 template <typename T>
