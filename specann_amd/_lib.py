@@ -233,7 +233,8 @@ _SIGNATURES = {
     "sa_ls_energy": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P, _P]),
     "sa_ls_extract_f32": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P]),
     "sa_ls_extract_f64": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P]),
-    "sa_colour_round": (c_int32, [POINTER(SaCsr), c_uint64, _P, _P, _P]),
+    "sa_colour_start": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P]),
+    "sa_colour_rounds": (c_int32, [POINTER(SaCsr), c_uint64, _P, _P, _P, c_int32, c_int32, _P]),
     "sa_combine3_f32": (c_int32, [c_int64, c_double, _P, c_double, _P, c_double, _P, _P, _P]),
     "sa_combine3_f64": (c_int32, [c_int64, c_double, _P, c_double, _P, c_double, _P, _P, _P]),
     "sa_minres_state_bytes": (c_size_t, []),

@@ -82,7 +82,22 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_init_kernel(int64_t n, const int3
   for (int64_t i = (int64_t)blockIdx.x * SA_WAVES_PER_BLOCK + wave; i < n; i += nw) {
     const int32_t e0 = uniform(indptr[i]), e1 = uniform(indptr[i + 1]);
     int32_t h = 0;
-    for (int32_t e = e0; e < e1; ++e) {
+    int32_t e = e0;
+    // eight neighbours' spin words in flight at a time (integer sums: the order does not matter)
+    for (; e + 8 <= e1; e += 8) {
+      int32_t j[8], jv[8];
+      uint64_t w[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        j[u] = uniform(indices[e + u]);
+        jv[u] = uniform(jint[e + u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) w[u] = sb[(int64_t)j[u] * groups + g];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) h += ((w[u] >> lane) & 1ull) ? jv[u] : -jv[u];
+    }
+    for (; e < e1; ++e) {
       const int32_t j = uniform(indices[e]);
       const int32_t jv = uniform(jint[e]);
       const uint64_t w = sb[(int64_t)j * groups + g];
@@ -169,49 +184,101 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_extract_kernel(int64_t n, int32_t
 }
 
 // ---------------------------------------------------------------------------------------------
-// Greedy colouring round (see the header comment of sa_colour_round).
+// Greedy colouring over a worklist (see the header comment of sa_colour_start). Priority order:
+// degree, then hash(seed, i), then i, all descending; the hash is computed only on degree ties.
 // ---------------------------------------------------------------------------------------------
-__device__ inline bool higher(int32_t dj, uint64_t hj, int32_t j, int32_t di, uint64_t hi, int32_t i) {
+__device__ inline bool outranks(const int32_t* __restrict__ indptr, uint64_t seed, int32_t j, int32_t dj, int32_t i,
+                                int32_t di, uint64_t hi) {
   if (dj != di) return dj > di;
+  const uint64_t hj = mix3(seed, 0xC0102ull, (uint64_t)j);
   if (hj != hi) return hj > hi;
   return j > i;
 }
 
-__global__ __launch_bounds__(SA_BLOCK) void colour_round_kernel(int64_t n, const int32_t* __restrict__ indptr,
-                                                                const int32_t* __restrict__ indices, uint64_t seed,
-                                                                int32_t* colour, void* work) {
-  uint64_t left = 0;
+__global__ __launch_bounds__(SA_BLOCK) void colour_start_kernel(int64_t n, const int32_t* __restrict__ indptr,
+                                                                int32_t* __restrict__ colour,
+                                                                int32_t* __restrict__ list0, int32_t* __restrict__ resume,
+                                                                int32_t* __restrict__ cnt) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    if (__hip_atomic_load(&colour[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= 0) continue;
-    const int32_t e0 = indptr[i], e1 = indptr[i + 1];
-    const int32_t di = e1 - e0;
-    const uint64_t hi = mix3(seed, 0xC0102ull, (uint64_t)i);
-    bool ready = true;
-    for (int32_t e = e0; e < e1 && ready; ++e) {
-      const int32_t j = indices[e];
-      if (j == (int32_t)i) continue;
-      const int32_t cj = __hip_atomic_load(&colour[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (cj < 0 && higher(indptr[j + 1] - indptr[j], mix3(seed, 0xC0102ull, (uint64_t)j), j, di, hi, (int32_t)i))
-        ready = false;
-    }
-    if (!ready) {
-      ++left;
-      continue;
-    }
-    int32_t col = -1;
-    for (int32_t base = 0; col < 0; base += 64) {
-      uint64_t used = 0;
-      for (int32_t e = e0; e < e1; ++e) {
-        const int32_t j = indices[e];
-        if (j == (int32_t)i) continue;
-        const int32_t cj = __hip_atomic_load(&colour[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cj >= base && cj < base + 64) used |= 1ull << (cj - base);
-      }
-      if (~used) col = base + __builtin_ctzll(~used);
-    }
-    __hip_atomic_store(&colour[i], col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    colour[i] = -1;
+    list0[i] = (int32_t)i;
+    resume[i] = indptr[i];
   }
-  block_partials_u64(left, work);
+  if (blockIdx.x == 0 && threadIdx.x < 3) cnt[threadIdx.x] = threadIdx.x == 0 ? (int32_t)n : 0;
+}
+
+// Round r: the nodes of list r % 2 (cnt[r % 3] of them). A node whose higher-priority neighbours are
+// all coloured takes the smallest colour none of them has (its lower-priority neighbours wait for it,
+// so this is its colour in the sequential greedy order, whatever this round's timing); the others go
+// to list (r + 1) % 2, counted in cnt[(r + 1) % 3], with one atomic per wave. cnt[(r + 2) % 3] is
+// cleared for round r + 1's successor. A waiting node keeps in resume[i] the neighbour that blocked
+// it: the neighbours before it were coloured or of lower priority, and a coloured node stays coloured,
+// so the next round starts its check there (every adjacency entry is checked about once overall).
+__global__ __launch_bounds__(SA_BLOCK) void colour_list_kernel(const int32_t* __restrict__ indptr,
+                                                               const int32_t* __restrict__ indices, uint64_t seed,
+                                                               int32_t* colour, const int32_t* __restrict__ in,
+                                                               int32_t* __restrict__ out, int32_t* __restrict__ resume,
+                                                               int32_t* cnt, int32_t r) {
+  const int64_t count = cnt[r % 3];
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt[(r + 2) % 3] = 0;
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); t0 < count; t0 += stride) {
+    const int64_t t = t0 + lane;
+    bool pending = false;
+    int32_t i = 0;
+    if (t < count) {
+      i = in[t];
+      const int32_t e0 = indptr[i], e1 = indptr[i + 1];
+      const int32_t di = e1 - e0;
+      const uint64_t hi = mix3(seed, 0xC0102ull, (uint64_t)i);
+      // four neighbours' loads in flight at a time (indices clamped to the list), checked in order
+      for (int32_t e = resume[i]; e < e1 && !pending; e += 4) {
+        int32_t j[4], cj[4], dj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = indices[min(e + u, e1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          cj[u] = __hip_atomic_load(&colour[j[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          dj[u] = indptr[j[u] + 1] - indptr[j[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (!pending && e + u < e1 && j[u] != i && cj[u] < 0 && outranks(indptr, seed, j[u], dj[u], i, di, hi)) {
+            pending = true;
+            resume[i] = e + u;
+          }
+        }
+      }
+      if (!pending) {
+        int32_t col = -1;
+        for (int32_t base = 0; col < 0; base += 64) {
+          uint64_t used = 0;
+          for (int32_t e = e0; e < e1; e += 4) {
+            int32_t j[4], cj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) j[u] = indices[min(e + u, e1 - 1)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              cj[u] = __hip_atomic_load(&colour[j[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              if (e + u < e1 && j[u] != i && cj[u] >= base && cj[u] < base + 64) used |= 1ull << (cj[u] - base);
+            }
+          }
+          if (~used) col = base + __builtin_ctzll(~used);
+        }
+        __hip_atomic_store(&colour[i], col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    const uint64_t m = __ballot(pending);
+    if (m) {
+      int32_t at = 0;
+      if (lane == 0) at = atomicAdd(&cnt[(r + 1) % 3], (int32_t)__popcll(m));
+      at = __shfl(at, 0);
+      if (pending) out[at + (int32_t)__popcll(m & ((1ull << lane) - 1))] = i;
+    }
+  }
 }
 
 }  // namespace sa
@@ -284,19 +351,27 @@ int sa_ls_extract_f64(int64_t n, int32_t groups, int32_t c, const int8_t* spins,
   return check_launch("ls_extract_f64");
 }
 
-int sa_colour_round(const sa_csr* A, uint64_t seed, int32_t* colour, void* work, hipStream_t stream) {
-  SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && colour && work, "bad arguments");
-  const int G = grid_for(A->n, SA_BLOCK * 4, SA_RED_GRID);
-  hipLaunchKernelGGL(colour_round_kernel, dim3(G), dim3(SA_BLOCK), 0, stream, A->n, A->indptr, A->indices, seed,
-                     colour, work);
-  int rc = check_launch("colour_round");
-  if (rc) return rc;
-  FinishSpec f{};
-  f.cols = 1;
-  f.only_if = -1;
-  f.op[0] = FIN_SUM_U64;
-  f.dst[0] = (int32_t)offsetof(sa_state, i64[1]);
-  return launch_finish(work, G, f, stream);
+int sa_colour_start(const sa_csr* A, int32_t* colour, int32_t* lists, int32_t* counts, hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->indptr && colour && lists && counts, "bad arguments");
+  hipLaunchKernelGGL(colour_start_kernel, dim3(grid_for(A->n, SA_BLOCK, 4096)), dim3(SA_BLOCK), 0, stream, A->n,
+                     A->indptr, colour, lists, lists + 2 * A->n, counts);
+  return check_launch("colour_start");
+}
+
+int sa_colour_rounds(const sa_csr* A, uint64_t seed, int32_t* colour, int32_t* lists, int32_t* counts,
+                     int32_t first_round, int32_t rounds, hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && colour && lists && counts, "bad arguments");
+  SA_REQUIRE(first_round >= 0 && rounds >= 0, "bad round numbers");
+  const int64_t n = A->n;
+  const int G = grid_for(n, SA_BLOCK * 4, SA_RED_GRID);
+  for (int32_t q = 0; q < rounds; ++q) {
+    const int32_t r = first_round + q;
+    hipLaunchKernelGGL(colour_list_kernel, dim3(G), dim3(SA_BLOCK), 0, stream, A->indptr, A->indices, seed, colour,
+                       (const int32_t*)(lists + (r & 1) * n), lists + ((r + 1) & 1) * n, lists + 2 * n, counts, r);
+    const int rc = check_launch("colour_rounds");
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 }  // extern "C"

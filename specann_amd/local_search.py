@@ -59,18 +59,25 @@ class Colouring:
         return len(self.offsets) - 1
 
 
+COLOUR_BATCH = 8  # rounds queued between checks of the worklist count
+
+
 def colour_graph(prob: DeviceProblem, seed: int) -> Colouring:
-    """Greedy colouring on the GPU (descending degree, then hash, then index), one round per launch."""
+    """Greedy colouring on the GPU (descending degree, then hash, then index) in rounds over a
+    shrinking worklist of uncoloured nodes; rounds past the last one do nothing."""
     n = prob.n
-    colours = torch.full((n,), -1, dtype=torch.int32, device=prob.device)
+    colours = torch.empty(n, dtype=torch.int32, device=prob.device)
+    lists = torch.empty(3 * n, dtype=torch.int32, device=prob.device)  # two worklists, resume points
+    counts = torch.empty(3, dtype=torch.int32, device=prob.device)
+    call("sa_colour_start", prob.csr_ref, ptr(colours), ptr(lists), ptr(counts), stream())
     rounds = 0
     while True:
-        call("sa_colour_round", prob.csr_ref, int(seed) & _M, ptr(colours), prob.work.ptr, stream())
-        rounds += 1
-        left = int(prob.work.head().i64[1])
-        if left == 0:
+        call("sa_colour_rounds", prob.csr_ref, int(seed) & _M, ptr(colours), ptr(lists), ptr(counts), rounds,
+             COLOUR_BATCH, stream())
+        rounds += COLOUR_BATCH
+        if int(counts[rounds % 3].item()) == 0:
             break
-        if rounds > n + 1:
+        if rounds > n + COLOUR_BATCH:
             raise RuntimeError("colouring did not finish")
     num = int(colours.max().item()) + 1
     class_nodes = torch.argsort(colours, stable=True).to(torch.int32)

@@ -185,10 +185,12 @@ def test_energy_bits_exact(gpu, case):
     np.testing.assert_array_equal(got, ref)
 
 
-@pytest.mark.parametrize("case", ["er_2k_pm1", "ba_20k_neg", "star_9k"])
+@pytest.mark.parametrize("case", ["er_2k_pm1", "ba_20k_neg", "star_9k", "er_100k"])
 def test_colouring_equals_sequential_greedy(gpu, case):
+    """The worklist rounds give the sequential greedy colouring (er_100k: dozens of rounds, several
+    batches of queued rounds, and rounds that find the worklist empty)."""
     from specann_amd.local_search import colour_graph
-    J = CASES[case]()
+    J = graphs.erdos_renyi(100000, 40 / 99999, seed=12, kind="pm1") if case == "er_100k" else CASES[case]()
     prob = DeviceProblem(J, torch.float32)
     col = colour_graph(prob, seed=42)
     ref = O.greedy_colouring(J.indptr, J.indices, 42)
