@@ -133,22 +133,51 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_class_kernel(const int32_t* __res
   const int64_t kp = (int64_t)groups * 64;
   const int64_t nw = (int64_t)gridDim.x * SA_WAVES_PER_BLOCK;
   bool any_flip = false;
-  for (int64_t u = (int64_t)blockIdx.x * SA_WAVES_PER_BLOCK + wave; u < count; u += nw) {
-    const int32_t i = uniform(nodes[u]);
-    if (!all && (mix3(seed, pass, (uint64_t)i) >> 11) >= thresh) continue;
-    const int64_t off = (int64_t)i * kp + c;
-    const int32_t s = spins[off];
-    const int32_t h = fields[off];
-    const bool flip = (c < k) && (s * h < 0);
-    if (!__any(flip)) continue;
-    const int32_t snew = -s;
-    if (flip) spins[off] = (int8_t)snew;
-    any_flip |= flip;
-    const int32_t e0 = uniform(indptr[i]), e1 = uniform(indptr[i + 1]);
-    for (int32_t e = e0; e < e1; ++e) {
-      const int32_t j = uniform(indices[e]);
-      const int32_t jv = uniform(jint[e]);
-      if (flip) atomicAdd(&fields[(int64_t)j * kp + c], 2 * jv * snew);
+  // U class nodes per wave and iteration, their loads issued together. No two nodes of a class are
+  // adjacent and the field updates are integer atomics, so the order of the nodes does not matter.
+  constexpr int U = 4;
+  for (int64_t u0 = (int64_t)blockIdx.x * SA_WAVES_PER_BLOCK + wave; u0 < count; u0 += nw * U) {
+    int32_t node[U], s[U], h[U];
+    bool sel[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int64_t u = u0 + q * nw;
+      node[q] = uniform(nodes[u < count ? u : count - 1]);
+      sel[q] = u < count && (all || (mix3(seed, pass, (uint64_t)node[q]) >> 11) < thresh);
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int64_t off = (int64_t)node[q] * kp + c;
+      s[q] = spins[off];
+      h[q] = fields[off];
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const bool flip = sel[q] && (c < k) && (s[q] * h[q] < 0);
+      if (!__any(flip)) continue;
+      const int32_t i = node[q];
+      const int32_t snew = -s[q];
+      if (flip) spins[(int64_t)i * kp + c] = (int8_t)snew;
+      any_flip |= flip;
+      const int32_t e0 = uniform(indptr[i]), e1 = uniform(indptr[i + 1]);
+      int32_t e = e0;
+      for (; e + 8 <= e1; e += 8) {
+        int32_t j[8], jv[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          j[v] = uniform(indices[e + v]);
+          jv[v] = uniform(jint[e + v]);
+        }
+        if (flip) {
+#pragma unroll
+          for (int v = 0; v < 8; ++v) atomicAdd(&fields[(int64_t)j[v] * kp + c], 2 * jv[v] * snew);
+        }
+      }
+      for (; e < e1; ++e) {
+        const int32_t j = uniform(indices[e]);
+        const int32_t jv = uniform(jint[e]);
+        if (flip) atomicAdd(&fields[(int64_t)j * kp + c], 2 * jv * snew);
+      }
     }
   }
   if (any_flip) flags[c] = 1;
