@@ -16,7 +16,7 @@ per launch in FP32) against the 8 TB/s HBM peak, timed with HIP events on the la
 cpu_baseline object times the reference's CPU operator (scipy CSR, S_rho in FP32) on a bounded
 sample and scales it to the sweep's operator applications.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--workload er22|ba20|er1k] [--pred tangent|diagonal|secant]
+    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--workload er22|ba20|er1k] [--pred tangent|diagonal|secant]
 """
 from __future__ import annotations
 
@@ -234,7 +234,8 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=1,
+                    help="untimed sweeps first (first-use allocations and code loading stay out of the timing)")
     ap.add_argument("--workload", default="er22", choices=sorted(WORKLOADS))
     ap.add_argument("--alphas", type=int, default=128)
     ap.add_argument("--pred", default="tangent", choices=["diagonal", "tangent", "secant", "prev"],
@@ -244,6 +245,8 @@ def main(argv=None) -> int:
                     help="skip the FP32 diagonal-predictor sweep (the Lanczos applications the predictor saves)")
     ap.add_argument("--ncv", type=int, default=None, help="Lanczos basis size (default: the reference's min(33, n-1))")
     ap.add_argument("--rows-out", default=None, help="write the FP32 sweep's per-alpha rows (JSON) to this file")
+    ap.add_argument("--keep", type=int, default=None,
+                    help="Ritz vectors a thick restart keeps (default: min(12, ncv / 2))")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--reorth", default="selective", choices=("selective", "full"),
                     help="Lanczos reorthogonalisation: selective (default) or full at every step (the reference's)")
@@ -301,7 +304,7 @@ def main(argv=None) -> int:
     def solver(precision, pred="diagonal"):
         return SpectralAnnealing("bench", "synthetic", random_seed=42, precision=precision, verbose=False,
                                  num_alphas=args.alphas, pred_mode=pred, tangent_steps=args.tangent_steps,
-                                 eigsh_ncv=args.ncv, reorth=args.reorth)
+                                 eigsh_ncv=args.ncv, reorth=args.reorth, eigsh_keep=args.keep)
 
     for i in range(args.warmup):
         out = solver(32, args.pred).run_on_device(prob32)

@@ -101,7 +101,7 @@ class SpectralAnnealing:
                  eigsh_ncv: int = None, fp32_spmv: bool = None, ls_policy: str = "default",
                  ls_batch: bool = True, verbose: bool = True, tile_nnz: int = None, lanes_per_row: int = 0,
                  autotune: bool = True, tangent_steps: int = 2, tangent_tol: float = 1e-2,
-                 lanczos_graphs: bool = None, reorth: str = "selective", **kwargs):
+                 lanczos_graphs: bool = None, reorth: str = "selective", eigsh_keep: int = None, **kwargs):
         self.instance_name = instance_name
         self.dataset = dataset
         self.random_seed = random_seed
@@ -113,6 +113,7 @@ class SpectralAnnealing:
         self.eigsh_tol = float(eigsh_tol)
         self.eigsh_maxiter = int(eigsh_maxiter)
         self.eigsh_ncv = None if eigsh_ncv is None else int(eigsh_ncv)
+        self.eigsh_keep = None if eigsh_keep is None else int(eigsh_keep)  # thick-restart Ritz vectors kept
         self.disable_warm_start = disable_warm_start
         if pred_mode not in PRED_MODES:
             raise ValueError(f"pred_mode must be one of {PRED_MODES}, got {pred_mode!r}")
@@ -237,7 +238,7 @@ class SpectralAnnealing:
         self._normalization_factor = prob.c
         alpha_list = self._get_alpha_values(self.mode)
         nwords = (n + 63) // 64
-        lz = LanczosSolver(prob, self.eigsh_ncv, graphs=self.lanczos_graphs,
+        lz = LanczosSolver(prob, self.eigsh_ncv, graphs=self.lanczos_graphs, keep=self.eigsh_keep,
                            orth_tau=0.0 if self.reorth == "full" else None)
         s = torch.empty(n, dtype=tdt, device=dev)
         ybuf = [torch.empty(n, dtype=tdt, device=dev) for _ in range(3)]
