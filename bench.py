@@ -247,6 +247,8 @@ def main(argv=None) -> int:
     ap.add_argument("--rows-out", default=None, help="write the FP32 sweep's per-alpha rows (JSON) to this file")
     ap.add_argument("--keep", type=int, default=None,
                     help="Ritz vectors a thick restart keeps (default: min(12, ncv / 2))")
+    ap.add_argument("--lanczos-graphs", type=int, default=None, choices=(0, 1),
+                    help="replay Lanczos step chunks from HIP graphs (default: only for n <= 2^18)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--reorth", default="selective", choices=("selective", "full"),
                     help="Lanczos reorthogonalisation: selective (default) or full at every step (the reference's)")
@@ -304,7 +306,8 @@ def main(argv=None) -> int:
     def solver(precision, pred="diagonal"):
         return SpectralAnnealing("bench", "synthetic", random_seed=42, precision=precision, verbose=False,
                                  num_alphas=args.alphas, pred_mode=pred, tangent_steps=args.tangent_steps,
-                                 eigsh_ncv=args.ncv, reorth=args.reorth, eigsh_keep=args.keep)
+                                 eigsh_ncv=args.ncv, reorth=args.reorth, eigsh_keep=args.keep,
+                                 lanczos_graphs=args.lanczos_graphs)
 
     for i in range(args.warmup):
         out = solver(32, args.pred).run_on_device(prob32)
