@@ -359,8 +359,9 @@ int sa_ls_extract_f64(int64_t n, int32_t groups, int32_t c, const int8_t* spins,
  * uncoloured node all of whose higher-priority neighbours are coloured takes the smallest colour none
  * of them has, so the result equals the sequential greedy colouring in that order.
  * sa_colour_start: colour[i] = -1, the first list holds every node. lists: 3n int32 (two worklists
- * and each node's resume point in its adjacency list); counts: 3 int32. sa_colour_rounds runs rounds first_round .. first_round + rounds - 1; after round
- * r, counts[(r + 1) % 3] nodes are left (0: done, and later rounds do nothing). */
+ * and each node's resume point in its adjacency list); counts: 3 int32.
+ * sa_colour_rounds runs rounds first_round .. first_round + rounds - 1; after round r,
+ * counts[(r + 1) % 3] nodes are left (0: done, and later rounds do nothing). */
 int sa_colour_start(const sa_csr* A, int32_t* colour, int32_t* lists, int32_t* counts, hipStream_t stream);
 int sa_colour_rounds(const sa_csr* A, uint64_t seed, int32_t* colour, int32_t* lists, int32_t* counts,
                      int32_t first_round, int32_t rounds, hipStream_t stream);
