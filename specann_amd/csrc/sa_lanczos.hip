@@ -150,10 +150,14 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_combine_kernel(void* work, int64_
 // valid one and dropped, so all eight stay in flight). Per-workgroup partials are stored
 // column-major (part[m * G + b]); column j + 1 takes ||W||^2. The finisher sums them in workgroup
 // order. The pass over row j also writes its shadow row.
+// With npanels > 0 the workgroup first forms W on its own rows from the SpMV's panel partials (the
+// panel combine: sigma_j s_r times their FP64 sum in panel order, as lz_combine_kernel computes it)
+// and writes it for the update kernel; partial rows and W are then 16-byte aligned.
 template <typename T, class M>
 __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(M map, const T* __restrict__ V, int64_t ldv,
-                                                           SH<T>* __restrict__ Vh, double shscale,
-                                                           const T* __restrict__ W) {
+                                                           SH<T>* __restrict__ Vh, double shscale, T* __restrict__ W,
+                                                           const T* __restrict__ partial, int64_t pld, int npanels,
+                                                           const T* __restrict__ s) {
   using VT = typename vec16<T>::type;
   using SV = typename Shadow<T>::vec;
   constexpr int L = vec16<T>::L;
@@ -172,6 +176,41 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(M map, const T* __res
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t lo = sl.lo, hi = sl.hi;
   const int64_t nv = hi > lo ? (hi - lo) / L : 0;
+  if (npanels > 0) {
+    const double sigma = st->sigma[j];
+    for (int64_t g = threadIdx.x; g < nv; g += SA_BLOCK) {
+      const int64_t r = lo + g * L;
+      double acc[L];
+#pragma unroll
+      for (int l = 0; l < L; ++l) acc[l] = 0.0;
+      for (int p0 = 0; p0 < npanels; p0 += 8) {  // npanels is a multiple of 8
+        VT pv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) pv[q] = *reinterpret_cast<const VT*>(&partial[(int64_t)(p0 + q) * pld + r]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+#pragma unroll
+          for (int l = 0; l < L; ++l) acc[l] += (double)pv[q][l];
+        }
+      }
+      VT w;
+      if (s) {
+        const VT sv = *reinterpret_cast<const VT*>(&s[r]);
+#pragma unroll
+        for (int l = 0; l < L; ++l) w[l] = (T)(sigma * (double)sv[l] * acc[l]);
+      } else {
+#pragma unroll
+        for (int l = 0; l < L; ++l) w[l] = (T)(sigma * 1.0 * acc[l]);
+      }
+      *reinterpret_cast<VT*>(&W[r]) = w;
+    }
+    for (int64_t r = lo + nv * L + threadIdx.x; r < hi; r += SA_BLOCK) {
+      double acc = 0.0;
+      for (int q = 0; q < npanels; ++q) acc += (double)partial[(int64_t)q * pld + r];
+      W[r] = (T)(sigma * (s ? (double)s[r] : 1.0) * acc);
+    }
+    __syncthreads();  // the chunks below read rows other threads of this workgroup wrote
+  }
   double ww = 0.0;  // ||W||^2, accumulated in the first pass
   int ch = 0;
   for (int m0 = 0; m0 <= j; m0 += SA_RED_COLS, ++ch) {
@@ -1005,7 +1044,10 @@ static int lanczos_step_phase(const sa_csr* A, void* work, const T* s, T* V, int
     return check_launch("lanczos_orth_finish");
   }
   int rc = 0;
-  if (A->npanels > 0) {
+  // the panel combine runs inside the projections kernel when the partial rows are 16-byte aligned
+  const bool fuse = A->npanels > 0 && A->npanels % 8 == 0 && aligned16(A->partial) &&
+                    A->pld % vec16<T>::L == 0 && (!s || aligned16(s));
+  if (A->npanels > 0 && !fuse) {
     hipLaunchKernelGGL(lz_combine_kernel<T>, dim3(grid_for(n, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0, stream, work,
                        n, A->npanels, (const T*)A->partial, A->pld, s, W);
     rc = check_launch("lanczos_combine");
@@ -1013,7 +1055,8 @@ static int lanczos_step_phase(const sa_csr* A, void* work, const T* s, T* V, int
   }
   const int Gd = grid_for(n, SA_BLOCK * 4, SA_DOTS_GRID);
   hipLaunchKernelGGL((lz_dots_kernel<T, OneMap>), dim3(Gd), dim3(SA_BLOCK), 0, stream, OneMap{work, n},
-                     (const T*)V, ldv, (SH<T>*)Vh, sqrt((double)n), (const T*)W);
+                     (const T*)V, ldv, (SH<T>*)Vh, sqrt((double)n), W, fuse ? (const T*)A->partial : nullptr,
+                     A->pld, fuse ? (int)A->npanels : 0, s);
   rc = check_launch("lanczos_dots");
   if (rc) return rc;
   hipLaunchKernelGGL(lz_dots_finish_kernel<OneFin>, dim3(1), dim3(SA_BLOCK), 0, stream, OneFin{work, Gd});
@@ -1131,7 +1174,7 @@ static int batch_step(const sa_csr* A, const sa_batch* Bt, const T* s, T* V, int
   if (rc) return rc;
   const dim3 grid((unsigned)Bt->nblocks), fgrid(Bt->B);
   hipLaunchKernelGGL((lz_dots_kernel<T, BatchMap>), grid, dim3(SA_BLOCK), 0, stream, bmap(Bt), (const T*)V, ldv,
-                     (SH<T>*)nullptr, 1.0, (const T*)W);
+                     (SH<T>*)nullptr, 1.0, W, (const T*)nullptr, (int64_t)0, 0, (const T*)nullptr);
   rc = check_launch("batch_lanczos_dots");
   if (rc) return rc;
   hipLaunchKernelGGL(lz_dots_finish_kernel<BatchFin>, fgrid, dim3(SA_BLOCK), 0, stream, bfin(Bt));
