@@ -249,7 +249,9 @@ int sa_spmv_gather_f64(const sa_csr* A, const double* s, const double* g, double
 /* ---- Warm-started Lanczos on N_alpha (utils/eigsh.py:10-17, models/spectral_annealing.py:671-677)
  * Basis V is (ncv + 1) rows of n entries, row-major with leading dimension ldv >= n (pad ldv so that
  * ldv * sizeof(T) is not a multiple of a large power of two: with n = 2^22 unpadded rows start on the
- * same HBM channels and the basis streams camp on them). Z, W are n-vectors of scratch. */
+ * same HBM channels and the basis streams camp on them). Z, W are n-vectors of scratch. The step,
+ * thick restart and start kernels use 16-byte loads: V, Z, W and s 16-byte aligned, ldv * sizeof(T)
+ * a multiple of 16. */
 /* orth_tau: a step's Gram-Schmidt update subtracts only the last two basis rows unless the projection
  * on an older row exceeds orth_tau * beta_j (then every row); sqrt(eps) / 2 of the vectors' type keeps
  * the basis semi-orthogonal, which leaves Ritz values as accurate as full reorthogonalisation. 0

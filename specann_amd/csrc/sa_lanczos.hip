@@ -633,23 +633,67 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(M map, const T* __res
   for (int m = threadIdx.x; m <= j; m += blockDim.x) coef[m] = st->h[m] * st->sigma[m];
   __syncthreads();
   T* __restrict__ vn = V + (int64_t)(j + 1) * ldv;
+  using VT = typename vec16<T>::type;
+  constexpr int L = vec16<T>::L;
   double acc = 0.0;
-  for (int64_t r = sl.lo + threadIdx.x; r < sl.hi; r += blockDim.x) {
-    double wv[8];
-    wv[0] = ws * (double)W[r];
+  const int64_t lo = sl.lo, hi = sl.hi;
+  const int64_t nv = hi > lo ? (hi - lo) / L : 0;
+  // 16-byte loads: L entries per lane, eight rows per group; the rows left after the last full group
+  // (rows j - 1 and j in a selective step) are loaded as one clamped group, all in flight together
+  for (int64_t g = threadIdx.x; g < nv; g += blockDim.x) {
+    const int64_t r = lo + g * L;
+    const VT w0 = *reinterpret_cast<const VT*>(&W[r]);
+    double wv[8][L];
 #pragma unroll
-    for (int u = 1; u < 8; ++u) wv[u] = 0.0;
+    for (int l = 0; l < L; ++l) {
+      wv[0][l] = ws * (double)w0[l];
+#pragma unroll
+      for (int u = 1; u < 8; ++u) wv[u][l] = 0.0;
+    }
     int m = mlo;
     for (; m + 7 <= j; m += 8) {
-      T a[8];
+      VT a[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] = V[(int64_t)(m + u) * ldv + r];
+      for (int u = 0; u < 8; ++u) a[u] = *reinterpret_cast<const VT*>(&V[(int64_t)(m + u) * ldv + r]);
 #pragma unroll
-      for (int u = 0; u < 8; ++u) wv[u] -= coef[m + u] * (double)a[u];
+      for (int u = 0; u < 8; ++u) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) wv[u][l] -= coef[m + u] * (double)a[u][l];
+      }
     }
-    for (; m <= j; ++m) wv[0] -= coef[m] * (double)V[(int64_t)m * ldv + r];
-    const double w = ((wv[0] + wv[1]) + (wv[2] + wv[3])) + ((wv[4] + wv[5]) + (wv[6] + wv[7]));
-    const T wt = (T)w;
+    if (m <= j) {
+      VT a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = *reinterpret_cast<const VT*>(&V[(int64_t)min(m + u, j) * ldv + r]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (m + u <= j) {
+#pragma unroll
+          for (int l = 0; l < L; ++l) wv[u][l] -= coef[m + u] * (double)a[u][l];
+        }
+      }
+    }
+    VT o, z;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const double w = ((wv[0][l] + wv[1][l]) + (wv[2][l] + wv[3][l])) + ((wv[4][l] + wv[5][l]) + (wv[6][l] + wv[7][l]));
+      o[l] = (T)w;
+      acc += (double)o[l] * (double)o[l];
+    }
+    if (s) {
+      const VT sv = *reinterpret_cast<const VT*>(&s[r]);
+#pragma unroll
+      for (int l = 0; l < L; ++l) z[l] = sv[l] * o[l];
+    } else {
+      z = o;
+    }
+    *reinterpret_cast<VT*>(&vn[r]) = o;
+    *reinterpret_cast<VT*>(&Z[r]) = z;
+  }
+  for (int64_t r = lo + nv * L + threadIdx.x; r < hi; r += blockDim.x) {
+    double wv = ws * (double)W[r];
+    for (int m = mlo; m <= j; ++m) wv -= coef[m] * (double)V[(int64_t)m * ldv + r];
+    const T wt = (T)wv;
     vn[r] = wt;
     Z[r] = s ? s[r] * wt : wt;
     acc += (double)wt * (double)wt;
@@ -1023,8 +1067,8 @@ template <typename T>
 static int lanczos_step_phase(const sa_csr* A, void* work, const T* s, T* V, int64_t ldv, void* Vh, T* Z, T* W,
                               int phase, hipStream_t stream) {
   SA_REQUIRE(A && work && V && Z && W && A->tile_rows && ldv >= A->n, "bad arguments");
-  SA_REQUIRE(aligned16(V) && aligned16(W) && ldv % vec16<T>::L == 0,
-             "V and W must be 16-byte aligned and ldv a multiple of 16 bytes");
+  SA_REQUIRE(aligned16(V) && aligned16(W) && aligned16(Z) && (!s || aligned16(s)) && ldv % vec16<T>::L == 0,
+             "V, W, Z and s must be 16-byte aligned and ldv a multiple of 16 bytes");
   SA_REQUIRE(!Vh || (aligned16(Vh) && ldv % (2 * vec16<T>::L) == 0),
              "the shadow basis must be 16-byte aligned and ldv a multiple of 32 bytes");
   SA_REQUIRE(phase >= 0 && phase <= 2, "phase must be 0, 1 or 2");
@@ -1159,8 +1203,8 @@ static int batch_step(const sa_csr* A, const sa_batch* Bt, const T* s, T* V, int
                       hipStream_t stream) {
   SA_REQUIRE(A && batch_ok(Bt) && V && Z && W && A->tile_rows && ldv >= A->n, "bad arguments");
   SA_REQUIRE(A->npanels == 0, "the batched step uses the plain CSR layout");
-  SA_REQUIRE(aligned16(V) && aligned16(W) && ldv % vec16<T>::L == 0,
-             "V and W must be 16-byte aligned and ldv a multiple of 16 bytes");
+  SA_REQUIRE(aligned16(V) && aligned16(W) && aligned16(Z) && (!s || aligned16(s)) && ldv % vec16<T>::L == 0,
+             "V, W, Z and s must be 16-byte aligned and ldv a multiple of 16 bytes");
   SpmvParams<T> p = params_from<T>(A);
   p.g = Z;
   p.s = s;
