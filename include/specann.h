@@ -82,18 +82,22 @@ typedef struct sa_csr {
    * panel p, in 64-entry chunks [bchunk[t], bchunk[t+1]). Workgroup w = rb * 8 + p (XCD w % 8 = p)
    * stages each column block of the gathered vector in LDS and sums its rows in LDS. An entry is
    * sign << 31 | cont << 30 | local column << 15 | local row (cont: the next entry has the same row;
-   * a row's entries in one tile never straddle a chunk); padding entries have local row = brows. */
+   * a row's entries in one tile never straddle a chunk); padding entries have local row = brows.
+   * rbound (optional): bnr + 1 row-block boundaries, multiples of 64 except rbound[bnr] = n, each
+   * block at most brows rows (row blocks of variable height, balanced by entries when a few rows hold
+   * most of them); NULL: row block rb is rows [rb * brows, (rb + 1) * brows). */
   int32_t bmode;
   int32_t brows;            /* rows per row block, a multiple of 64, < 32768 */
   int32_t bcols;            /* columns per column block, a multiple of 64, <= 65536 / sizeof(T); the
                              * row sums (brows + 1, 16-byte aligned) and the block take <= 162816 bytes */
-  int32_t bnc;              /* column blocks per panel (the widest panel's count), at most 248 */
-  int32_t bnr;              /* row blocks */
+  int32_t bnc;              /* column blocks per panel (the widest panel's count), at most 368 */
+  int32_t bnr;              /* row blocks (bnr * brows >= n unless rbound is given) */
   int32_t pad3;
   const int32_t* pbound;    /* npanels + 1 column boundaries, multiples of 64 except pbound[npanels] = n */
   const int32_t* bchunk;    /* 8 * bnr * bnc + 1 tile starts, in chunks of 64 entries */
   const uint32_t* bent;     /* 64 * bchunk[8 * bnr * bnc] entries */
   const void* bvals;        /* entry values (T; 0 for padding), or NULL when psigned */
+  const int32_t* rbound;    /* NULL, or bnr + 1 row-block boundaries (see above) */
 } sa_csr;
 
 /* A batch of independent instances packed block-diagonally into one CSR (no counterpart in the

@@ -86,6 +86,7 @@ class SaCsr(ctypes.Structure):
         ("bchunk", c_void_p),
         ("bent", c_void_p),
         ("bvals", c_void_p),
+        ("rbound", c_void_p),
     ]
 
 

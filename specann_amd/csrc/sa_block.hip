@@ -21,7 +21,7 @@ namespace sa {
 
 constexpr int kBThreads = 1024;                // 16 waves: one workgroup per CU (all of LDS)
 constexpr int kBWaves = kBThreads / 64;
-constexpr int kBTable = 256;                   // tile-bound table entries, at the end of LDS
+constexpr int kBTable = 376;                   // tile-bound table entries, at the end of LDS (fills it)
 constexpr int kBMaxTiles = kBTable - 8;        // column blocks per panel
 constexpr int kBPre = 4;                       // chunks of the next tile each wave loads ahead
 
@@ -125,9 +125,10 @@ __global__ __launch_bounds__(kBThreads) void spmv_block_kernel(SpmvParams<T> p) 
   const int panel = (int)(blockIdx.x & 7);
   const int rb = (int)(blockIdx.x >> 3);
   const int RB = p.brows, CB = p.bcols;
-  const int64_t r0 = (int64_t)rb * RB;
+  const int64_t r0 = p.rbound ? (int64_t)p.rbound[rb] : (int64_t)rb * RB;
   if (r0 >= p.n) return;
-  const int rows = (int)((p.n - r0) < RB ? (p.n - r0) : RB);
+  const int64_t r1 = p.rbound ? (int64_t)p.rbound[rb + 1] : r0 + RB;
+  const int rows = (int)((p.n < r1 ? p.n : r1) - r0);  // at most brows (checked when the layout is built)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   for (int i = tid; i <= RB; i += kBThreads) yacc[i] = (T)0;
@@ -237,7 +238,7 @@ int launch_block_spmv(const SpmvParams<T>& p, hipStream_t stream) {
   }
   if (p.brows < 64 || p.brows % 64 || p.brows > BlockCfg<T>::kRows || p.bcols < 64 || p.bcols % 64 ||
       p.bcols > BlockCfg<T>::kCols || p.bnr < 1 || p.bnc < 1 || p.bnc > kBMaxTiles ||
-      (int64_t)p.bnr * p.brows < p.n || p.n < 64) {
+      (!p.rbound && (int64_t)p.bnr * p.brows < p.n) || p.n < 64) {
     set_error("spmv: LDS-blocked layout sizes out of range");
     return SA_E_ARG;
   }
@@ -261,13 +262,25 @@ __device__ inline int run_class(int L) { return L <= 1 ? 0 : L > 64 ? 7 : 32 - _
 struct BlockGeom {
   int64_t n;
   int32_t brows, bcols, bnr, bnc;
+  const int32_t* rbound;  // NULL or bnr + 1 row-block boundaries
 };
+
+// Row block of row r: r / brows, or the block [rbound[rb], rbound[rb + 1]) that holds r.
+__device__ inline int row_block(const BlockGeom& G, int64_t r) {
+  if (!G.rbound) return (int)(r / G.brows);
+  int lo = 0, hi = G.bnr - 1;  // the last rb with rbound[rb] <= r
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (G.rbound[mid] <= r) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
 
 // Calls f(t, k, e0, L, lr, cb_lo) for each run of row r: entries [e0, e0 + L) in tile t, class k.
 template <class F>
 __device__ inline void for_each_run(const sa_csr& A, const int32_t* bnd, const BlockGeom& G, int64_t r, F f) {
-  const int rb = (int)(r / G.brows);
-  const int lr = (int)(r - (int64_t)rb * G.brows);
+  const int rb = row_block(G, r);
+  const int lr = (int)(r - (G.rbound ? (int64_t)G.rbound[rb] : (int64_t)rb * G.brows));
   const int32_t e_end = A.indptr[r + 1];
   int pnl = 0;
   int32_t e = A.indptr[r];
@@ -317,6 +330,7 @@ __global__ __launch_bounds__(SA_BLOCK) void block_count_kernel(sa_csr A, const i
   __syncthreads();
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < G.n; r += (int64_t)gridDim.x * blockDim.x) {
     for_each_run(A, bnd, G, r, [&](int64_t t, int k, int32_t, int32_t L, int lr, int32_t) {
+      if (lr >= G.brows) atomicOr(flag, 16);  // a row block taller than brows: the plan is refused
       if (k > 6) {
         atomicAdd(&runs[kRunStride * t + kColCnt + (lr & 15)], (L + 63) >> 6);
       } else {
@@ -461,8 +475,10 @@ static int block_geom(const sa_csr* A, const int32_t* pbound, int32_t brows, int
   SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && A->vals && pbound, "bad arguments");
   SA_REQUIRE(brows >= 64 && brows % 64 == 0 && brows < 32768, "brows must be a multiple of 64 below 32768");
   SA_REQUIRE(bcols >= 64 && bcols % 64 == 0 && bcols <= 32768, "bcols must be a multiple of 64 up to 32768");
-  SA_REQUIRE(bnr >= 1 && bnc >= 1 && (int64_t)bnr * brows >= A->n, "bnr row blocks must cover the rows");
+  SA_REQUIRE(bnr >= 1 && bnc >= 1 && (A->rbound || (int64_t)bnr * brows >= A->n),
+             "bnr row blocks must cover the rows");
   SA_REQUIRE((int64_t)8 * bnr * bnc < (1ll << 31), "too many tiles");
+  G.rbound = A->rbound;
   G.n = A->n;
   G.brows = brows;
   G.bcols = bcols;

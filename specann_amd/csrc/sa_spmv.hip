@@ -332,6 +332,7 @@ SpmvParams<T> params_from(const sa_csr* A) {
   p.bchunk = A->bchunk;
   p.bent = A->bent;
   p.bvals = A->bvals;
+  p.rbound = A->rbound;
   return p;
 }
 

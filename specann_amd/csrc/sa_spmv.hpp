@@ -49,6 +49,7 @@ struct SpmvParams {
   const int32_t* bchunk;
   const uint32_t* bent;
   const void* bvals;
+  const int32_t* rbound;  // NULL (row block rb = rows [rb * brows, ...)) or bnr + 1 boundaries
 };
 
 template <typename T>

@@ -21,7 +21,7 @@ TILE_CHOICES = (512, 1024, 2048, 4096)
 # holds and the largest column block
 BLOCK_ROWS = {4: 26240, 8: 13312}
 BLOCK_COLS = {4: 14336, 8: 6976}
-BLOCK_MAX_TILES = 248       # column blocks per panel the blocked SpMV can walk
+BLOCK_MAX_TILES = 368       # column blocks per panel the blocked SpMV can walk
 CUS_PER_XCD = 32
 
 
@@ -253,18 +253,55 @@ class DeviceProblem:
             bnr += CUS_PER_XCD
         return rb, BLOCK_COLS[es], bnr
 
-    def build_blocks(self, signed: bool | None = None, brows: int | None = None, bcols: int | None = None) -> bool:
+    def row_block_imbalance(self, brows: int) -> float:
+        """Largest over mean entry count of the uniform row blocks of brows rows."""
+        ip = self.indptr.cpu().numpy().astype(np.int64)
+        edges = np.minimum(np.arange(0, self.n + brows, brows), self.n)
+        per = np.diff(ip[np.unique(edges)])
+        return float(per.max() / max(per.mean(), 1.0))
+
+    def balanced_row_bounds(self, nblocks: int, cap: int) -> np.ndarray:
+        """Row-block boundaries (multiples of 64, the last = n) holding about nnz / nblocks entries each
+        and at most cap rows: rows with many entries (the hubs of a scale-free graph) get short blocks,
+        so every workgroup of the blocked SpMV sums about as many entries."""
+        n = self.n
+        ip = self.indptr.cpu().numpy().astype(np.int64)
+        gstart = np.arange(0, n, 64)
+        target = max(1.0, ip[-1] / max(1, nblocks))
+        bid = ip[gstart] // target
+        starts = gstart[np.flatnonzero(np.r_[True, bid[1:] != bid[:-1]])]
+        edges = np.r_[starts, n]
+        out = [0]
+        for a, b in zip(edges[:-1].tolist(), edges[1:].tolist()):
+            k = -(-(b - a) // cap)
+            step = -(-(-(-(b - a) // k)) // 64) * 64
+            out += [a + i * step for i in range(1, k)] + [b]
+        return np.asarray(out, dtype=np.int64)
+
+    def build_blocks(self, signed: bool | None = None, brows: int | None = None, bcols: int | None = None,
+                     balanced: bool = False) -> bool:
         """Regroup the couplings into the LDS-blocked layout (sa_block_plan / sa_block_fill) on the eight
         entry-balanced panels; False (layout unchanged) if a panel needs more column blocks than the
         kernel walks.
-        brows / bcols override the geometry of block_geometry (multiples of 64 that fit LDS)."""
+        brows / bcols override the geometry of block_geometry (multiples of 64 that fit LDS).
+        balanced: row blocks of variable height (at most brows rows, by default the most the LDS
+        holds) balanced by entries, for degree distributions whose uniform row blocks are not."""
         n, dev, s = self.n, self.device, stream()
         rb, cb, bnr = self.block_geometry()
+        es = self.dtype.itemsize
         if brows is not None or bcols is not None:
             rb = int(brows or rb)
             cb = int(bcols or cb)
             bnr = -(-n // rb)
-        es = self.dtype.itemsize
+        rbound = None
+        if balanced:
+            cap = int(brows or BLOCK_ROWS[es])
+            bounds_r = self.balanced_row_bounds(bnr, cap)
+            if bounds_r[-1] != n or np.any(np.diff(bounds_r) <= 0) or np.any(np.diff(bounds_r) > cap) or \
+                    np.any(bounds_r[:-1] % 64):
+                raise RuntimeError("balanced row blocks out of shape")
+            rb, bnr = cap, len(bounds_r) - 1
+            rbound = torch.from_numpy(bounds_r.astype(np.int32)).to(dev)
         if not (64 <= rb <= BLOCK_ROWS[es] and rb % 64 == 0 and cb % 64 == 0 and 64 <= cb <= BLOCK_COLS[es]):
             raise ValueError(f"blocked layout geometry {rb} x {cb} does not fit the workgroup's LDS split")
         bounds = self.panel_bounds(8)
@@ -278,8 +315,11 @@ class DeviceProblem:
         runs = torch.zeros(128 * ntiles, dtype=torch.int32, device=dev)
         bchunk = torch.empty(ntiles + 1, dtype=torch.int32, device=dev)
         flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        old_rbound = self.csr.rbound
+        self.csr.rbound = rbound.data_ptr() if rbound is not None else None  # the plan and fill read it
         call("sa_block_plan", self.csr_ref, ptr(pbound), rb, cb, bnr, bnc, ptr(runs), ptr(bchunk), ptr(flag), s)
         if int(flag.item()) != 0:
+            self.csr.rbound = old_rbound
             return False
         nent = 64 * int(bchunk[-1].item())
         if signed is None:
@@ -294,24 +334,25 @@ class DeviceProblem:
             raise RuntimeError(f"LDS-blocked layout fill failed (flag {int(flag.item())})")
         del runs
         self.build_panels(0)
-        self.pbound, self.bchunk, self.bent, self.bvals = pbound, bchunk, bent, bvals
+        self.pbound, self.bchunk, self.bent, self.bvals, self.rbound = pbound, bchunk, bent, bvals, rbound
         self.pld = _lib.padded_ld(n, self.dtype.itemsize)
         self.partial = torch.empty((8, self.pld), dtype=self.dtype, device=dev)
         self.npanels = 8
-        self.block = {"brows": rb, "bcols": cb, "bnr": bnr, "bnc": bnc, "entries": nent}
+        self.block = {"brows": rb, "bcols": cb, "bnr": bnr, "bnc": bnc, "entries": nent, "balanced": rbound is not None}
         c = self.csr
         c.npanels, c.partial, c.pld = 8, self.partial.data_ptr(), self.pld
         c.psigned, c.pmag = (1, mag) if signed else (0, 0.0)
         c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 1, rb, cb, bnr, bnc
         c.pbound, c.bchunk, c.bent = pbound.data_ptr(), bchunk.data_ptr(), bent.data_ptr()
         c.bvals = bvals.data_ptr() if bvals is not None else None
+        c.rbound = rbound.data_ptr() if rbound is not None else None  # cleared by build_panels(0) above
         return True
 
     def _clear_blocks(self) -> None:
         c = self.csr
         c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 0, 0, 0, 0, 0
-        c.pbound = c.bchunk = c.bent = c.bvals = None
-        for name in ("bchunk", "bent", "bvals", "block"):
+        c.pbound = c.bchunk = c.bent = c.bvals = c.rbound = None
+        for name in ("bchunk", "bent", "bvals", "block", "rbound"):
             if hasattr(self, name):
                 delattr(self, name)
 
@@ -345,13 +386,16 @@ class DeviceProblem:
             # rule above is only a starting point: time the other panel counts too
             configs += [(p, 1024) for p in (8, 16, 32, 64) if p != P]
         if self.n >= (1 << 16) and blocks:
-            # the LDS-blocked layout (its tile size does not matter; the plain tiles stay for other uses)
+            # the LDS-blocked layout (its tile size does not matter; the plain tiles stay for other uses),
+            # with row blocks balanced by entries too where the uniform ones are not (hubs)
             configs.append((-1, self.tile_nnz))
+            if self.row_block_imbalance(self.block_geometry()[0]) > 1.25:
+                configs.append((-2, self.tile_nnz))
         best, results = None, []
         for panels, tile in configs:
             self.set_tiling(tile, self.csr.lanes_per_row)
             if panels < 0:
-                if not self.build_blocks():
+                if not self.build_blocks(balanced=panels == -2):
                     continue
             else:
                 self.build_panels(panels) if panels else self.build_panels(0)
@@ -368,8 +412,8 @@ class DeviceProblem:
                 best = results[-1]
         self.set_tiling(best["tile_nnz"], self.csr.lanes_per_row)
         if best["layout"] == "blocked":
-            if not self.csr.bmode:
-                self.build_blocks()
+            if not self.csr.bmode or self.block.get("balanced") != best.get("balanced"):
+                self.build_blocks(balanced=bool(best.get("balanced")))
         elif best["npanels"] != self.npanels or best["npanels"] or self.csr.bmode:
             self.build_panels(best["npanels"])
         self._free_unused_panels()

@@ -115,10 +115,37 @@ def test_blocked_spmv_matches_oracle(gpu, case, tdtype, geom, signed):
         J = graphs.erdos_renyi(100003, 30 / 100002, seed=13, kind="pm1")
     else:
         J = CASES[case]()
+    _check_blocked(gpu, J, case, tdtype, geom, signed, balanced=False)
+
+
+@pytest.mark.parametrize("case", ["ba_20k_neg", "star_9k", "er_100003", "ba_200k"])
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("geom", [None, (1024, 2048)])
+@pytest.mark.parametrize("signed", [True, False])
+def test_blocked_balanced_spmv_matches_oracle(gpu, case, tdtype, geom, signed):
+    """Row blocks of variable height balanced by entries (hubs in short blocks): the same operator
+    within the same bound, bitwise reproducible, every row covered once."""
+    if case == "er_100003":
+        J = graphs.erdos_renyi(100003, 30 / 100002, seed=13, kind="pm1")
+    elif case == "ba_200k":
+        J = graphs.barabasi_albert(200000, 20, seed=5, kind="neg")
+    else:
+        J = CASES[case]()
+    _check_blocked(gpu, J, case, tdtype, geom, signed, balanced=True)
+
+
+def _check_blocked(gpu, J, case, tdtype, geom, signed, balanced):
     prob = DeviceProblem(J, tdtype)
     brows, bcols = geom if geom else (None, None)
-    print(f"blocked spmv case={case} dtype={tdtype} geom={brows, bcols} signed={signed} n={prob.n}", flush=True)
-    assert prob.build_blocks(signed=signed, brows=brows, bcols=bcols)
+    print(f"blocked spmv case={case} dtype={tdtype} geom={brows, bcols} signed={signed} balanced={balanced} "
+          f"n={prob.n}", flush=True)
+    assert prob.build_blocks(signed=signed, brows=brows, bcols=bcols, balanced=balanced)
+    if balanced:
+        rbound = prob.rbound.cpu().numpy()
+        heights = np.diff(rbound)
+        assert rbound[0] == 0 and rbound[-1] == prob.n and np.all(heights > 0)
+        assert np.all(heights <= prob.csr.brows) and prob.csr.bnr == len(heights)
+        print(f"  {len(heights)} row blocks, heights {heights.min()}..{heights.max()}", flush=True)
     rb, cb, _ = prob.block_geometry()
     bounds = prob.panel_bounds(8)
     bounds[1:-1] = (bounds[1:-1] // 64) * 64
