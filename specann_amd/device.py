@@ -260,12 +260,13 @@ class DeviceProblem:
         per = np.diff(ip[np.unique(edges)])
         return float(per.max() / max(per.mean(), 1.0))
 
-    def balanced_row_bounds(self, nblocks: int, cap: int) -> np.ndarray:
+    def balanced_row_bounds(self, nblocks: int, cap: int, ip: np.ndarray | None = None) -> np.ndarray:
         """Row-block boundaries (multiples of 64, the last = n) holding about nnz / nblocks entries each
         and at most cap rows: rows with many entries (the hubs of a scale-free graph) get short blocks,
-        so every workgroup of the blocked SpMV sums about as many entries."""
+        so every workgroup of the blocked SpMV sums about as many entries. ip: indptr on the host."""
         n = self.n
-        ip = self.indptr.cpu().numpy().astype(np.int64)
+        if ip is None:
+            ip = self.indptr.cpu().numpy().astype(np.int64)
         gstart = np.arange(0, n, 64)
         target = max(1.0, ip[-1] / max(1, nblocks))
         bid = ip[gstart] // target
@@ -277,6 +278,43 @@ class DeviceProblem:
             step = -(-(-(-(b - a) // k)) // 64) * 64
             out += [a + i * step for i in range(1, k)] + [b]
         return np.asarray(out, dtype=np.int64)
+
+    def capped_panel_bounds(self, wmax: int) -> np.ndarray | None:
+        """Eight column panels (boundaries multiples of 64, the last = n), none wider than wmax columns,
+        whose largest entry share is as small as a greedy cover allows: a binary search on that share,
+        each panel growing from the left until it holds the share or reaches wmax. The couplings are
+        symmetric, so the entries of columns [0, c) number indptr[c]. None if no cover exists."""
+        n = self.n
+        ip = self.indptr.cpu().numpy().astype(np.int64)
+        grid = np.r_[np.arange(0, n, 64), n]
+        cum = ip[grid]
+
+        def cover(share: float):
+            b, out = 0, [0]
+            for _ in range(8):
+                i = int(np.searchsorted(grid, out[-1]))
+                j = int(np.searchsorted(cum, cum[i] + share, side="right")) - 1  # most columns within share
+                j = max(j, i + 1)
+                end = min(int(grid[min(j, len(grid) - 1)]), out[-1] + wmax // 64 * 64, n)
+                out.append(end)
+                if end >= n:
+                    break
+            return out if out[-1] >= n else None
+
+        lo, hi = ip[-1] / 8.0, float(ip[-1])
+        best = cover(hi)
+        if best is None:
+            return None
+        for _ in range(40):
+            mid = 0.5 * (lo + hi)
+            c = cover(mid)
+            if c is None:
+                lo = mid
+            else:
+                hi, best = mid, c
+        while len(best) < 9:  # fewer than eight panels sufficed: the rest are empty
+            best.append(n)
+        return np.asarray(best, dtype=np.int64)
 
     def build_blocks(self, signed: bool | None = None, brows: int | None = None, bcols: int | None = None,
                      balanced: bool = False) -> bool:
@@ -296,7 +334,16 @@ class DeviceProblem:
         rbound = None
         if balanced:
             cap = int(brows or BLOCK_ROWS[es])
-            bounds_r = self.balanced_row_bounds(bnr, cap)
+            # a block count in whole rounds of one workgroup per CU of an XCD, if a nearby target gives one
+            ip = self.indptr.cpu().numpy().astype(np.int64)
+            bounds_r = self.balanced_row_bounds(bnr, cap, ip)
+            for extra in range(1, 2 * CUS_PER_XCD):
+                if (len(bounds_r) - 1) % CUS_PER_XCD == 0:
+                    break
+                cand = self.balanced_row_bounds(bnr + extra, cap, ip)
+                if (len(cand) - 1) % CUS_PER_XCD == 0:
+                    bounds_r = cand
+                    break
             if bounds_r[-1] != n or np.any(np.diff(bounds_r) <= 0) or np.any(np.diff(bounds_r) > cap) or \
                     np.any(bounds_r[:-1] % 64):
                 raise RuntimeError("balanced row blocks out of shape")
@@ -307,6 +354,13 @@ class DeviceProblem:
         bounds = self.panel_bounds(8)
         bounds[1:-1] = (bounds[1:-1] // 64) * 64
         bounds = np.maximum.accumulate(bounds)
+        wmax = BLOCK_MAX_TILES * cb - 64
+        if np.max(np.diff(bounds)) > wmax and 8 * wmax >= n:
+            # entry-balanced panels too wide for the kernel's walk (a scale-free graph's sparse columns)
+            capped = self.capped_panel_bounds(wmax)
+            if capped is None:
+                return False
+            bounds = capped
         bnc = max(1, int(-(-int(np.max(np.diff(bounds))) // cb)))
         if bnc > BLOCK_MAX_TILES:
             return False

@@ -134,12 +134,25 @@ def test_blocked_balanced_spmv_matches_oracle(gpu, case, tdtype, geom, signed):
     _check_blocked(gpu, J, case, tdtype, geom, signed, balanced=True)
 
 
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("balanced", [False, True])
+def test_blocked_capped_panels_match_oracle(gpu, tdtype, balanced):
+    """Column panels capped at the kernel's walk (368 column blocks): with 64-column blocks the
+    entry-balanced panels of a scale-free graph are too wide, and the capped split must still cover
+    every column once."""
+    J = graphs.barabasi_albert(150000, 10, seed=8, kind="neg")
+    _check_blocked(gpu, J, "ba_150k", tdtype, (1024, 64), True, balanced)
+
+
 def _check_blocked(gpu, J, case, tdtype, geom, signed, balanced):
     prob = DeviceProblem(J, tdtype)
     brows, bcols = geom if geom else (None, None)
     print(f"blocked spmv case={case} dtype={tdtype} geom={brows, bcols} signed={signed} balanced={balanced} "
           f"n={prob.n}", flush=True)
     assert prob.build_blocks(signed=signed, brows=brows, bcols=bcols, balanced=balanced)
+    pb = prob.pbound.cpu().numpy()
+    assert pb[0] == 0 and pb[-1] == prob.n and np.all(np.diff(pb) >= 0)
+    assert np.all(np.diff(pb) <= prob.csr.bnc * prob.csr.bcols) and prob.csr.bnc <= 368
     if balanced:
         rbound = prob.rbound.cpu().numpy()
         heights = np.diff(rbound)
