@@ -1,8 +1,8 @@
 // Deterministic grid-wide reductions over the caller's work buffer.
 //
 // A reduction kernel runs at most SA_RED_GRID workgroups; each workgroup reduces its share to one
-// value per column, stores it in the partials area, and the last workgroup to arrive sums the
-// partials in block order. No floating-point atomics are used, so results are bitwise
+// value per column and stores it in the partials area, and a one-workgroup finisher launched next
+// sums the partials in block order. No floating-point atomics are used, so results are bitwise
 // reproducible for a fixed (n, grid).
 #pragma once
 
@@ -69,13 +69,31 @@ __device__ inline void block_partials_u64(uint64_t v, void* work, int slot) {
 
 __device__ inline void block_partials_u64(uint64_t v, void* work) { block_partials_u64(v, work, (int)blockIdx.x); }
 
+// This thread's share of a phase-2 sum: part[b * stride + off] for b = threadIdx.x, + blockDim.x, ...
+// below G, added in that order. Eight loads are in flight at a time (the partials come from every
+// XCD, so each load is a miss); the order of the additions is the same as one at a time.
+template <typename T>
+__device__ inline T thread_partial_sum(const T* part, int G, int stride, int off) {
+  T x = 0;
+  const int B = blockDim.x;
+  int b = threadIdx.x;
+  for (; b + 7 * B < G; b += 8 * B) {
+    T q[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) q[u] = part[(int64_t)(b + u * B) * stride + off];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x += q[u];
+  }
+  for (; b < G; b += B) x += part[(int64_t)b * stride + off];
+  return x;
+}
+
 // Phase 2 helper for one workgroup: sum of part[b * stride + off] over b < G (each thread in
 // workgroup order, then a fixed tree); the result is returned to every thread.
 __device__ inline double sum_partials(const double* part, int G, int stride, int off) {
   __shared__ double red[SA_WAVES_PER_BLOCK];
   __shared__ double bc;
-  double x = 0.0;
-  for (int b = threadIdx.x; b < G; b += blockDim.x) x += part[(int64_t)b * stride + off];
+  double x = thread_partial_sum(part, G, stride, off);
   x = block_sum(x, red);
   if (threadIdx.x == 0) bc = x;
   __syncthreads();

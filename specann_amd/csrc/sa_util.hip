@@ -38,8 +38,7 @@ __global__ __launch_bounds__(SA_BLOCK) void finish_kernel(F fin, FinishSpec spec
   for (int c = 0; c < C; ++c) {
     char* dst = reinterpret_cast<char*>(st) + spec.dst[c];
     if (spec.op[c] == FIN_SUM_U64) {
-      uint64_t x = 0;
-      for (int b = threadIdx.x; b < G; b += blockDim.x) x += partu[(int64_t)b * C + c];
+      uint64_t x = thread_partial_sum(partu, G, C, c);
       x = block_sum(x, redu);
       if (threadIdx.x == 0) *reinterpret_cast<uint64_t*>(dst) = x;
     } else if (spec.op[c] == FIN_MAX) {
@@ -55,8 +54,7 @@ __global__ __launch_bounds__(SA_BLOCK) void finish_kernel(F fin, FinishSpec spec
       }
       __syncthreads();
     } else {
-      double x = 0.0;
-      for (int b = threadIdx.x; b < G; b += blockDim.x) x += part[(int64_t)b * C + c];
+      double x = thread_partial_sum(part, G, C, c);
       x = block_sum(x, red);
       if (threadIdx.x == 0) *reinterpret_cast<double*>(dst) = spec.take_sqrt[c] ? sqrt(x) : x;
     }
