@@ -295,6 +295,44 @@ def test_lanczos_leading_eigenpair(gpu, case, tdtype, panels):
 
 
 @pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("panels", [8, -1])
+def test_fused_combine_matches_separate_combine(gpu, tdtype, panels):
+    """The projections kernel forms W from the panel partials itself when they are 16-byte aligned;
+    with a partial buffer that is only element-aligned the step runs the separate combine kernel.
+    Both give bitwise the same basis, projections and scales over several steps."""
+    from specann_amd.eigsh import LanczosSolver
+    J = CASES["er_2k_pm1"]()
+    prob = DeviceProblem(J, tdtype)
+    if panels < 0:
+        assert prob.build_blocks(brows=1024, bcols=1024)
+    else:
+        prob.build_panels(panels)
+    s = torch.empty(prob.n, dtype=tdtype, device=gpu)
+    prob.alpha_scaling(0.6, 1e-12, s)
+    outs = []
+    for misaligned in (False, True):
+        saved = prob.csr.partial
+        buf = torch.empty(8 * prob.pld + 4, dtype=tdtype, device=gpu)
+        if misaligned:
+            prob.csr.partial = buf.data_ptr() + buf.element_size()
+        try:
+            solver = LanczosSolver(prob, ncv=12, keep=5, graphs=False)
+            x0 = prob.fill_normal(7, 0, torch.empty(prob.n, dtype=tdtype, device=gpu))
+            solver.begin(1e-14, 50, bound=3.0)
+            solver.start(s, x0)
+            for _ in range(6):  # below ncv: no restart
+                solver._step(s)
+            torch.cuda.synchronize()
+            st = prob.work.full()
+            outs.append((solver.V[:7].clone(), list(st.h[:7]), list(st.sigma[:8]), st.j))
+        finally:
+            prob.csr.partial = saved
+    assert outs[0][3] == outs[1][3] == 6
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1] and outs[0][2] == outs[1][2]
+
+
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
 def test_selective_reorthogonalisation_matches_full(gpu, tdtype):
     """Selective reorthogonalisation (old rows subtracted only when their projection exceeds
     sqrt(eps)/2 of beta) converges to the same eigenpair as the full one, with the true residual of
