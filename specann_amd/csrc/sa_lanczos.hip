@@ -619,6 +619,9 @@ __device__ double largest_eigenvalue(const Proj& T, const double* bounds) {
   return 0.5 * (lo + hi);
 }
 
+// Phase 1 of the Gram-Schmidt step: v_{j+1} = W - sum_{m >= orth_lo} h_m sigma_m v_m, raw (W = N sigma_j v_j;
+// a batch scales W by sigma_j here), in FP64 per entry with the rows in eight partial sums; Z = s v_{j+1}
+// and each workgroup's partial of ||v_{j+1}||^2. orth_lo is j - 1 on a selective step, 0 on a full one.
 template <typename T, class M>
 __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(M map, const T* __restrict__ s, T* __restrict__ V,
                                                            int64_t ldv, T* __restrict__ Z, const T* __restrict__ W) {
