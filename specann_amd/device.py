@@ -385,6 +385,7 @@ class DeviceProblem:
         call(f"sa_block_fill_{self.sfx}", self.csr_ref, ptr(pbound), rb, cb, bnr, bnc, ptr(runs), ptr(bchunk), nent,
              ptr(bent), ptr(bvals), mag, ptr(flag), s)
         if int(flag.item()) != 0:
+            self.csr.rbound = old_rbound  # the previous layout, if any, stays in force
             raise RuntimeError(f"LDS-blocked layout fill failed (flag {int(flag.item())})")
         del runs
         self.build_panels(0)
