@@ -342,13 +342,17 @@ class BatchSweep:
         src_d = torch.from_numpy(src).to(dev)
         packed = torch.zeros((K, nwords), dtype=torch.int64, device=dev)
         call("sa_bits_gather", nwords, K, B, ptr(bp.word_off), ptr(src_d), ptr(cand_bits), ptr(packed), stream())
+        # exact start energies per instance from the local search's initial spins and fields (one pass
+        # over the couplings for 64 candidates at a time; the same int64 sums as the candidate-bit form)
+        init = LS.init_fields(prob, packed, K)
         e2 = torch.zeros((B, K), dtype=torch.int64, device=dev)
-        call("sa_batch_energy_bits_i64", prob.csr_ref, R, K, ptr(packed), nwords, ptr(e2), stream())
+        call("sa_batch_ls_energy", R, K, init[2], ptr(init[0]), ptr(init[1]), ptr(e2), stream())
         raw = -(e2.cpu().numpy() // 2)
         refined = raw.copy()
         state = None
         if self.use_local_search:
-            state = LS.flip_refinement(prob, packed, K, max_iterations=self.max_local_iterations, seed=self.seed)
+            state = LS.flip_refinement(prob, packed, K, max_iterations=self.max_local_iterations, seed=self.seed,
+                                       init=init)
             passes = state.passes
             e2.zero_()
             call("sa_batch_ls_energy", R, K, state.groups, ptr(state.spins), ptr(state.fields), ptr(e2), stream())

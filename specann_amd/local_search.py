@@ -130,11 +130,12 @@ def init_fields(prob: DeviceProblem, cand_bits: torch.Tensor, k: int):
 
 def flip_refinement(prob: DeviceProblem, cand_bits: torch.Tensor, k: int, max_iterations: int = 1000,
                     final_sweep_fraction: float = 1.0, final_sweep_iters: int = 25, num_rounds: int = 25,
-                    subset_fraction: float = 0.9, seed: int = 0, colouring: Colouring | None = None
-                    ) -> LocalSearchState:
-    """Local search of k candidates given as spin bits (flip_refinement, :1039-1112)."""
+                    subset_fraction: float = 0.9, seed: int = 0, colouring: Colouring | None = None,
+                    init: tuple | None = None) -> LocalSearchState:
+    """Local search of k candidates given as spin bits (flip_refinement, :1039-1112). init: the result
+    of init_fields(prob, cand_bits, k) when the caller has already computed it."""
     seed = int(seed) & _M
-    spins, fields, groups, e2 = init_fields(prob, cand_bits, k)
+    spins, fields, groups, e2 = init if init is not None else init_fields(prob, cand_bits, k)
     e2_start = e2[:k].cpu().numpy().copy()
     col = colouring or colour_graph(prob, seed)
     C = col.num_classes

@@ -77,6 +77,36 @@ def test_batched_local_search_equals_oracle_on_packed_graph(gpu):
             assert out.results[b].raw_energies[c] == O.energy_int64(Jb, cand[rows, c].astype(np.int64))
 
 
+def test_batch_energy_entry_points_agree(gpu):
+    """Per-instance exact energies of random candidate bits: the candidate-bit kernel
+    (sa_batch_energy_bits_i64) and the local search's initial fields summed per instance
+    (sa_batch_ls_energy, which the batched sweep uses) agree with each other and the oracle."""
+    from specann_amd import local_search as LS
+    from specann_amd._lib import call
+    from specann_amd.device import ptr, stream
+    Js = _instances()
+    bp = BatchProblem(Js, torch.float32, rows_per_block=256)
+    prob = bp.prob
+    nwords = (prob.n + 63) // 64
+    K = 70
+    words = np.random.default_rng(5).integers(0, 1 << 62, size=(K, nwords), dtype=np.int64)
+    packed = torch.from_numpy(words).to(gpu)
+    e2a = torch.zeros((len(Js), K), dtype=torch.int64, device=gpu)
+    call("sa_batch_energy_bits_i64", prob.csr_ref, bp.ref, K, ptr(packed), nwords, ptr(e2a), stream())
+    init = LS.init_fields(prob, packed, K)
+    e2b = torch.zeros_like(e2a)
+    call("sa_batch_ls_energy", bp.ref, K, init[2], ptr(init[0]), ptr(init[1]), ptr(e2b), stream())
+    assert torch.equal(e2a, e2b)
+    Jp, row_off, sizes = pack(Js)
+    bits = np.unpackbits(words.view(np.uint8), bitorder="little").reshape(K, -1)[:, : Jp.shape[0]]
+    got = -(e2a.cpu().numpy() // 2)
+    for b, J in enumerate(Js):
+        rows = slice(int(row_off[b]), int(row_off[b] + sizes[b]))
+        Jb = O.integer_couplings(J)
+        for c in range(0, K, 7):
+            assert got[b, c] == O.energy_int64(Jb, bits[c, rows].astype(np.int64) * 2 - 1)
+
+
 def test_batch_edge_cases(gpu):
     """One instance; instances smaller than a 64-row word and than the Lanczos basis."""
     tiny = graphs.erdos_renyi(12, 0.5, seed=9, kind="neg")
