@@ -216,9 +216,14 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_extract_kernel(int64_t n, int32_t
 // Greedy colouring over a worklist (see the header comment of sa_colour_start). Priority order:
 // degree, then hash(seed, i), then i, all descending; the hash is computed only on degree ties.
 // ---------------------------------------------------------------------------------------------
-__device__ inline bool outranks(const int32_t* __restrict__ indptr, uint64_t seed, int32_t j, int32_t dj, int32_t i,
-                                int32_t di, uint64_t hi) {
+// hi caches hash(seed, i): computed on the first degree tie only (hi_ready).
+__device__ inline bool outranks(uint64_t seed, int32_t j, int32_t dj, int32_t i, int32_t di, uint64_t& hi,
+                                bool& hi_ready) {
   if (dj != di) return dj > di;
+  if (!hi_ready) {
+    hi = mix3(seed, 0xC0102ull, (uint64_t)i);
+    hi_ready = true;
+  }
   const uint64_t hj = mix3(seed, 0xC0102ull, (uint64_t)j);
   if (hj != hi) return hj > hi;
   return j > i;
@@ -243,6 +248,66 @@ __global__ __launch_bounds__(SA_BLOCK) void colour_start_kernel(int64_t n, const
 // cleared for round r + 1's successor. A waiting node keeps in resume[i] the neighbour that blocked
 // it: the neighbours before it were coloured or of lower priority, and a coloured node stays coloured,
 // so the next round starts its check there (every adjacency entry is checked about once overall).
+// From round 1 on every listed node has such a blocker, which outranks it for good (priorities are
+// fixed): while the blocker is uncoloured the node only re-reads its colour.
+constexpr int32_t kColourHeavy = 128;  // nodes with more neighbours are checked and coloured by a whole wave
+
+// OR of v over the wave's 64 lanes (every lane receives it).
+__device__ inline uint64_t wave_or(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+
+// One round for a node with many neighbours, by the whole wave (the arguments are uniform): the first
+// neighbour from resume[i] on that blocks it (uncoloured and of higher priority) becomes resume[i] and
+// the node waits (true); otherwise it takes the smallest colour none of its neighbours has (false).
+// The same outcome as the one-lane scan, with 64 neighbours per step.
+__device__ inline bool colour_node_wave(const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                                        uint64_t seed, int32_t* colour, int32_t* __restrict__ resume, int32_t i,
+                                        int32_t r, int lane) {
+  const int32_t e0 = indptr[i], e1 = indptr[i + 1];
+  const int32_t di = e1 - e0;
+  int32_t start = resume[i];
+  if (r > 0) {
+    const int32_t jb = indices[start];
+    if (__hip_atomic_load(&colour[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) return true;
+    ++start;
+  }
+  uint64_t hi = 0;
+  bool hi_ready = false;
+  for (int32_t b = start; b < e1; b += 64) {
+    const int32_t e = b + lane;
+    bool blk = false;
+    if (e < e1) {
+      const int32_t j = indices[e];
+      if (j != i) {
+        const int32_t cj = __hip_atomic_load(&colour[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cj < 0) blk = outranks(seed, j, indptr[j + 1] - indptr[j], i, di, hi, hi_ready);
+      }
+    }
+    const uint64_t bm = __ballot(blk);
+    if (bm) {
+      if (lane == 0) resume[i] = b + (int32_t)__builtin_ctzll(bm);
+      return true;
+    }
+  }
+  int32_t col = -1;
+  for (int32_t base = 0; col < 0; base += 64) {
+    uint64_t used = 0;
+    for (int32_t e = e0 + lane; e < e1; e += 64) {
+      const int32_t j = indices[e];
+      if (j == i) continue;
+      const int32_t cj = __hip_atomic_load(&colour[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cj >= base && cj < base + 64) used |= 1ull << (cj - base);
+    }
+    used = wave_or(used);
+    if (~used) col = base + (int32_t)__builtin_ctzll(~used);
+  }
+  if (lane == 0) __hip_atomic_store(&colour[i], col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return false;
+}
+
 __global__ __launch_bounds__(SA_BLOCK) void colour_list_kernel(const int32_t* __restrict__ indptr,
                                                                const int32_t* __restrict__ indices, uint64_t seed,
                                                                int32_t* colour, const int32_t* __restrict__ in,
@@ -254,51 +319,70 @@ __global__ __launch_bounds__(SA_BLOCK) void colour_list_kernel(const int32_t* __
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); t0 < count; t0 += stride) {
     const int64_t t = t0 + lane;
-    bool pending = false;
+    bool pending = false, heavy = false;
     int32_t i = 0;
     if (t < count) {
       i = in[t];
       const int32_t e0 = indptr[i], e1 = indptr[i + 1];
       const int32_t di = e1 - e0;
-      const uint64_t hi = mix3(seed, 0xC0102ull, (uint64_t)i);
-      // four neighbours' loads in flight at a time (indices clamped to the list), checked in order
-      for (int32_t e = resume[i]; e < e1 && !pending; e += 4) {
-        int32_t j[4], cj[4], dj[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) j[u] = indices[min(e + u, e1 - 1)];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          cj[u] = __hip_atomic_load(&colour[j[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          dj[u] = indptr[j[u] + 1] - indptr[j[u]];
+      heavy = di > kColourHeavy;
+      if (!heavy) {
+        int32_t e = resume[i];
+        if (r > 0) {
+          const int32_t jb = indices[e];
+          if (__hip_atomic_load(&colour[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) pending = true;
+          else ++e;
         }
+        uint64_t hi = 0;
+        bool hi_ready = false;
+        // four neighbours' loads in flight at a time (indices clamped to the list), checked in order
+        for (; e < e1 && !pending; e += 4) {
+          int32_t j[4], cj[4], dj[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (!pending && e + u < e1 && j[u] != i && cj[u] < 0 && outranks(indptr, seed, j[u], dj[u], i, di, hi)) {
-            pending = true;
-            resume[i] = e + u;
+          for (int u = 0; u < 4; ++u) j[u] = indices[min(e + u, e1 - 1)];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            cj[u] = __hip_atomic_load(&colour[j[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            dj[u] = indptr[j[u] + 1] - indptr[j[u]];
           }
-        }
-      }
-      if (!pending) {
-        int32_t col = -1;
-        for (int32_t base = 0; col < 0; base += 64) {
-          uint64_t used = 0;
-          for (int32_t e = e0; e < e1; e += 4) {
-            int32_t j[4], cj[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) j[u] = indices[min(e + u, e1 - 1)];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-              cj[u] = __hip_atomic_load(&colour[j[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              if (e + u < e1 && j[u] != i && cj[u] >= base && cj[u] < base + 64) used |= 1ull << (cj[u] - base);
+          for (int u = 0; u < 4; ++u) {
+            if (!pending && e + u < e1 && j[u] != i && cj[u] < 0 && outranks(seed, j[u], dj[u], i, di, hi, hi_ready)) {
+              pending = true;
+              resume[i] = e + u;
             }
           }
-          if (~used) col = base + __builtin_ctzll(~used);
         }
-        __hip_atomic_store(&colour[i], col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!pending) {
+          int32_t col = -1;
+          for (int32_t base = 0; col < 0; base += 64) {
+            uint64_t used = 0;
+            for (int32_t e = e0; e < e1; e += 4) {
+              int32_t j[4], cj[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) j[u] = indices[min(e + u, e1 - 1)];
+#pragma unroll
+              for (int u = 0; u < 4; ++u)
+                cj[u] = __hip_atomic_load(&colour[j[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                if (e + u < e1 && j[u] != i && cj[u] >= base && cj[u] < base + 64) used |= 1ull << (cj[u] - base);
+              }
+            }
+            if (~used) col = base + __builtin_ctzll(~used);
+          }
+          __hip_atomic_store(&colour[i], col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
+    }
+    // the wave's heavy nodes, one at a time, by all 64 lanes
+    uint64_t hm = __ballot(heavy);
+    while (hm) {
+      const int l = __builtin_ctzll(hm);
+      hm &= hm - 1;
+      const int32_t hn = __shfl(i, l);
+      const bool hp = colour_node_wave(indptr, indices, seed, colour, resume, hn, r, lane);
+      if (lane == l) pending = hp;
     }
     const uint64_t m = __ballot(pending);
     if (m) {
