@@ -276,30 +276,41 @@ __device__ inline bool colour_node_wave(const int32_t* __restrict__ indptr, cons
   }
   uint64_t hi = 0;
   bool hi_ready = false;
-  for (int32_t b = start; b < e1; b += 64) {
-    const int32_t e = b + lane;
-    bool blk = false;
-    if (e < e1) {
-      const int32_t j = indices[e];
-      if (j != i) {
-        const int32_t cj = __hip_atomic_load(&colour[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cj < 0) blk = outranks(seed, j, indptr[j + 1] - indptr[j], i, di, hi, hi_ready);
-      }
+  // 256 neighbours per step, four per lane with their loads in flight together (indices clamped to the
+  // list), the four groups of 64 checked in order
+  for (int32_t b = start; b < e1; b += 256) {
+    int32_t j[4], cj[4], dj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) j[u] = indices[min(b + u * 64 + lane, e1 - 1)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cj[u] = __hip_atomic_load(&colour[j[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      dj[u] = indptr[j[u] + 1] - indptr[j[u]];
     }
-    const uint64_t bm = __ballot(blk);
-    if (bm) {
-      if (lane == 0) resume[i] = b + (int32_t)__builtin_ctzll(bm);
-      return true;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int32_t e = b + u * 64 + lane;
+      const bool blk = e < e1 && j[u] != i && cj[u] < 0 && outranks(seed, j[u], dj[u], i, di, hi, hi_ready);
+      const uint64_t bm = __ballot(blk);
+      if (bm) {
+        if (lane == 0) resume[i] = b + u * 64 + (int32_t)__builtin_ctzll(bm);
+        return true;
+      }
     }
   }
   int32_t col = -1;
   for (int32_t base = 0; col < 0; base += 64) {
     uint64_t used = 0;
-    for (int32_t e = e0 + lane; e < e1; e += 64) {
-      const int32_t j = indices[e];
-      if (j == i) continue;
-      const int32_t cj = __hip_atomic_load(&colour[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (cj >= base && cj < base + 64) used |= 1ull << (cj - base);
+    for (int32_t e = e0 + lane; e < e1; e += 256) {
+      int32_t j[4], cj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) j[u] = indices[min(e + u * 64, e1 - 1)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) cj[u] = __hip_atomic_load(&colour[j[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (e + u * 64 < e1 && j[u] != i && cj[u] >= base && cj[u] < base + 64) used |= 1ull << (cj[u] - base);
+      }
     }
     used = wave_or(used);
     if (~used) col = base + (int32_t)__builtin_ctzll(~used);
