@@ -333,6 +333,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_finish_kernel(F fin) {
   const double* part = partials_of(fin.w());
   const int lane32 = threadIdx.x & 31, grp = threadIdx.x >> 5;
   __shared__ double lww;
+  __shared__ double lh[SA_MAX_NCV + 1];  // the projections again, for the decision below (LDS, not L2)
   for (int mb = 0; mb <= j + 1; mb += SA_BLOCK / 32) {
     const int m = mb + grp;
     double x = 0.0;
@@ -350,12 +351,16 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_finish_kernel(F fin) {
     }
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) x += __shfl_xor(x, o, 32);
-    if (lane32 == 0 && m <= j) st->h[m] = st->sigma[m] * ws * x;
+    if (lane32 == 0 && m <= j) {
+      const double h = st->sigma[m] * ws * x;
+      st->h[m] = h;
+      lh[m] = h;
+    }
     if (lane32 == 0 && m == j + 1) lww = ws * ws * x;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    st->a[j] = st->h[j];
+    st->a[j] = lh[j];
     // Selective reorthogonalisation. The projections h_m of W on rows m < j - 1 vanish in exact
     // arithmetic past the first step of a cycle (the kept rows couple to row k only); left in, they
     // become components h_m / beta_j of the next basis vector. The update subtracts every row when one
@@ -363,8 +368,8 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_finish_kernel(F fin) {
     // keeps the basis orthogonal to tau (semi-orthogonality, tau ~ sqrt(eps) of the vectors' type).
     double s2 = 0.0, hmax = 0.0;
     for (int m = 0; m <= j; ++m) {
-      s2 += st->h[m] * st->h[m];
-      if (m + 1 < j) hmax = fmax(hmax, fabs(st->h[m]));
+      s2 += lh[m] * lh[m];
+      if (m + 1 < j) hmax = fmax(hmax, fabs(lh[m]));
     }
     const double beta_est = sqrt(fmax(lww - s2, 1e-30 * lww));
     const double tau = st->orth_tau;
