@@ -89,7 +89,11 @@ class SweepOutcome:
 class SpectralAnnealing:
     """SpecAnn solver (see the reference's docstring for the parameters,
     models/spectral_annealing.py:132-170). Extra keyword-only tuning: ``tile_nnz`` and
-    ``lanes_per_row`` fix the SpMV tiling (default: chosen from the mean row length)."""
+    ``lanes_per_row`` fix the SpMV tiling (default: chosen from the mean row length), ``autotune``
+    times the SpMV layouts, ``pred_mode="tangent"`` (with ``tangent_steps``, ``tangent_tol``) adds the
+    eigenvector-tangent predictor, ``reorth`` ("selective" or "full") sets the Gram-Schmidt policy,
+    ``eigsh_keep`` the Ritz vectors a thick restart keeps (default min(12, ncv / 2)), and
+    ``lanczos_graphs`` the HIP-graph replay of Lanczos step chunks (default for n <= 2^18)."""
 
     def __init__(self, instance_name: str, dataset: str, random_seed: int = None,
                  mode: int = 0, beta: float = None, alpha: float = None,
