@@ -8,13 +8,14 @@ timed region, so every timed step starts with its inputs resident in HBM.
 
 Reported (one JSON line): value = seconds per FP32 sweep with the tangent predictor (the diagonal
 predictor plus the eigenvector tangent's correction from two deflated MINRES steps seeded with the
-backward difference); the FP32 sweep with the reference's default diagonal predictor (the Lanczos
-applications the tangent saves) and the plain FP64 diagonal-predictor form (SpecAnn-64) are timed
-once beside it, with the best-energy gap between the FP32 tangent sweep and the FP64 one; the
-roofline object gives the fused scaled SpMV's algorithmic bandwidth (8 nnz + 4 (n+1) + 12 n bytes
-per launch in FP32) against the 8 TB/s HBM peak, timed with HIP events on the launch stream; the
-cpu_baseline object times the reference's CPU operator (scipy CSR, S_rho in FP32) on a bounded
-sample and scales it to the sweep's operator applications.
+backward difference). Beside it, timed once each after warm-up: the FP32 sweep with the reference's
+default diagonal predictor and the plain FP64 diagonal-predictor form (SpecAnn-64), giving both the
+north star's ratio (FP32 tangent against FP64 diagonal) and the like-for-like one (diagonal in both
+precisions) with their best-energy gaps, and the end-to-end solve() time from the host matrix (the
+reference's `time` column). The roofline object gives the fused scaled SpMV's algorithmic bandwidth
+(8 nnz + 4 (n+1) + 12 n bytes per launch in FP32) against the 8 TB/s HBM peak, timed with HIP events
+on the launch stream; cpu_baseline times the reference's CPU path (scipy ARPACK + numpy local search)
+on one warm-started alpha of the same graph and scales it to the sweep (labelled as extrapolated).
 
     python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--workload er22|ba20|er1k] [--pred tangent|diagonal|secant]
 """
@@ -130,10 +131,15 @@ def run_batch(args, w):
 
 
 def make_graph(w):
+    """The workload's graph: libspecann's parallel host generators for the large graphs, the numpy
+    generators (same distributions, other draws) below 2^16 nodes."""
     from specann_amd import graphs
+    big = w["n"] >= (1 << 16)
     if w["kind"] == "er":
-        return graphs.erdos_renyi(w["n"], w["deg"] / (w["n"] - 1), seed=w["seed"], kind=w["coupling"])
-    return graphs.barabasi_albert(w["n"], w["m"], seed=w["seed"], kind=w["coupling"])
+        f = graphs.erdos_renyi_native if big else graphs.erdos_renyi
+        return f(w["n"], w["deg"] / (w["n"] - 1), seed=w["seed"], kind=w["coupling"])
+    f = graphs.barabasi_albert_native if big else graphs.barabasi_albert
+    return f(w["n"], w["m"], seed=w["seed"], kind=w["coupling"])
 
 
 def log(msg):
@@ -212,22 +218,55 @@ def copy_calibration(n=1 << 26, reps=10):
     return e0.elapsed_time(e1) / 1e3 / reps, 8 * n
 
 
-def cpu_operator_sample(J, seconds=12.0):
-    """Seconds per application of the reference's CPU operator S_rho = rho I - N (scipy, FP32)."""
+def cpu_sample(J, prob32, alphas, candidates, alpha_index=64):
+    """The reference's CPU path (scipy ARPACK eigsh on S_rho with the diagonal predictor, then the numpy
+    local search) timed on one warm-started alpha of the schedule, scaled to the sweep.
+
+    y_last at alpha_index - 1 comes from the GPU (so no cold CPU solve is needed); the CPU then runs the
+    diagonal predictor, its residual gate, ARPACK's eigsh(which='SA', tol 1e-6, ncv 33) and the
+    local search of the rounded candidate. Extrapolation: that alpha's eigensolve seconds x the sweep's
+    alphas + the local search seconds x the sweep's distinct candidates."""
+    import torch
+    import scipy.sparse.linalg as spla
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import specann_oracle as O
+    from specann_amd.eigsh import ScaledOperator, eigsh
+    a = O.alpha_values(alphas)
+    i = min(max(1, alpha_index), len(a) - 1)
+    s_prev = torch.empty(prob32.n, dtype=torch.float32, device=prob32.device)
+    prob32.alpha_scaling(a[i - 1], 1e-12, s_prev)
+    _, v, _ = eigsh(ScaledOperator(prob32, s_prev, None), which="LA", tol=1e-6, maxiter=4000)
+    y_last = v[:, 0].double().cpu().numpy()
+    y_last /= np.linalg.norm(y_last)
     Jn, d, c = O.normalize(J, np.float32)
-    s, _, _ = O.alpha_scaling(d, 0.5)
-    x = O.normal_vector(J.shape[0], 1, 0).astype(np.float32)
-    rho = np.float32(1.001)
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        y = rho * x - s * (Jn @ (s * x))
-        reps += 1
-        if time.perf_counter() - t0 >= seconds and reps >= 2:
-            break
-    del y
-    return (time.perf_counter() - t0) / reps, reps
+    Jint = O.integer_couplings(J)
+    n = J.shape[0]
+    t0 = time.perf_counter()
+    s, _, g = O.alpha_scaling(d, a[i])
+    rho = O.shift_rho(None, g)
+    y0 = O.diagonal_predictor(y_last, d, a[i] - a[i - 1])
+    counter = [0]
+
+    def mv(x):
+        counter[0] += 1
+        x = x.astype(np.float32)
+        return (np.float32(rho) * x - s * (Jn @ (s * x))).astype(np.float32)
+
+    v1 = O.n_matvec(Jn, s, y0)
+    gate = float(np.linalg.norm(v1 - (y0 @ v1) * y0))
+    start = y0 if gate <= 1e-3 else y_last
+    op = spla.LinearOperator((n, n), matvec=mv, dtype=np.float32)
+    try:
+        w, V = spla.eigsh(op, k=1, which="SA", tol=1e-6, maxiter=40, v0=start.astype(np.float32), ncv=33)
+    except spla.ArpackNoConvergence as e:
+        V = e.eigenvectors
+    t_eig = time.perf_counter() - t0
+    spins = O.round_spins(s.astype(np.float64) * V[:, 0])
+    t1 = time.perf_counter()
+    res = O.local_search(Jint, spins[None, :], 42)
+    t_ls = time.perf_counter() - t1
+    return {"eig_s": t_eig, "matvecs": counter[0] + 1, "ls_s": t_ls, "ls_passes": res.total_passes,
+            "alpha": a[i], "value": t_eig * alphas + t_ls * candidates}
 
 
 def main(argv=None) -> int:
@@ -253,6 +292,7 @@ def main(argv=None) -> int:
     ap.add_argument("--reorth", default="selective", choices=("selective", "full"),
                     help="Lanczos reorthogonalisation: selective (default) or full at every step (the reference's)")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-e2e", action="store_true", help="skip the end-to-end solve() timing")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV timing loop and a copy kernel of known size (for counter profiling)")
     ap.add_argument("--c3-sample", type=int, default=16,
@@ -343,13 +383,64 @@ def main(argv=None) -> int:
         sec32d, e32d, apps32d = time.perf_counter() - t, outd.best_energy, outd.matvecs
         log(f"fp32 diagonal sweep: {sec32d:.3f}s, best {e32d}, {apps32d} operator applications")
 
-    sec64 = e64 = gap = apps64 = None
+    cores = len(os.sched_getaffinity(0))
+    cpu = None
+    if not args.skip_cpu and J.shape[0] <= (1 << 14):
+        # small graphs: the whole CPU path of the sweep (scipy ARPACK on S_rho, numpy local search)
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle import specann_oracle as O
+        t = time.perf_counter()
+        cres = O.cpu_sweep(J, precision=32, num_alphas=args.alphas, pred_mode="diagonal", seed=42)
+        csec = time.perf_counter() - t
+        cpu = {"value": csec, "unit": "s", "cores": 1, "kind": "port",
+               "sample": (f"the full {args.alphas}-alpha sweep on the CPU path (scipy ARPACK eigsh on the S_rho "
+                          f"operator, diagonal predictor, numpy local search; {cores} cores visible, "
+                          f"scipy's CSR product and the local search single-threaded), best energy "
+                          f"{cres.best_energy}")}
+        log(f"cpu sweep: {csec:.3f}s, best {cres.best_energy}")
+    elif not args.skip_cpu:
+        k = outs[-1].num_candidates or 1
+        c = cpu_sample(J, prob32, args.alphas, k)
+        cpu = {"value": c["value"], "unit": "s (extrapolated from a measured alpha)", "cores": 1, "kind": "port",
+               "sample": (f"one warm-started alpha ({c['alpha']:.4f}, mid-schedule) of the CPU path on the same "
+                          f"graph, measured: diagonal predictor + gate + scipy ARPACK eigsh(S_rho, which='SA', "
+                          f"tol 1e-6, ncv 33) {c['eig_s']:.1f} s over {c['matvecs']} operator applications, numpy "
+                          f"local search of its candidate {c['ls_s']:.1f} s ({c['ls_passes']} passes); value = "
+                          f"{c['eig_s']:.1f} s x {args.alphas} alphas + {c['ls_s']:.1f} s x {k} distinct "
+                          f"candidates of the GPU sweep (extrapolated). scipy's CSR product and the local search "
+                          f"run single-threaded; {cores} cores visible, OMP_NUM_THREADS="
+                          f"{os.environ.get('OMP_NUM_THREADS')}")}
+        log(f"cpu sample: {c}")
+
+    e2e = None
+    if not args.skip_e2e:
+        # the reference's `time` column: SpectralAnnealing.solve(J) from the host matrix (upload,
+        # normalisation, SpMV layout and autotune, sweep, local search), result files in a scratch dir
+        import tempfile
+        here = os.getcwd()
+        with tempfile.TemporaryDirectory() as tmp:
+            os.chdir(tmp)
+            try:
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                e_e2e, _, _ = solver(32, args.pred).solve(J)
+                torch.cuda.synchronize()
+                e2e = time.perf_counter() - t
+            finally:
+                os.chdir(here)
+        log(f"end-to-end solve(): {e2e:.3f}s, best {e_e2e}")
+
+    sec64 = e64 = apps64 = None
     if not args.skip_fp64:
         del prob32
         torch.cuda.empty_cache()
         prob64 = DeviceProblem(J, torch.float64)
         tune64 = prob64.autotune()
         log(f"FP64 autotuned SpMV: {tune64['choice']}")
+        # untimed FP64 warm-up (code loading, first-use allocations), as the FP32 sweeps had
+        w64 = SpectralAnnealing("bench", "synthetic", random_seed=42, precision=64, verbose=False,
+                                num_alphas=min(16, args.alphas), reorth=args.reorth)
+        w64.run_on_device(prob64)
         torch.cuda.synchronize()
         t = time.perf_counter()
         out64 = solver(64).run_on_device(prob64)
@@ -357,33 +448,11 @@ def main(argv=None) -> int:
         sec64 = time.perf_counter() - t
         e64 = out64.best_energy
         apps64 = out64.matvecs
-        gap = abs(e32 - e64) / abs(e64) if e64 else None
         log(f"fp64 sweep: {sec64:.3f}s, best {e64}, {out64.matvecs} operator applications")
         del prob64
 
-    cpu = None
-    if not args.skip_cpu and J.shape[0] <= (1 << 14):
-        # small graphs: the whole CPU path of the sweep (scipy ARPACK on S_rho, numpy local search)
-        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-        from oracle import specann_oracle as O
-        cores = len(os.sched_getaffinity(0))
-        t = time.perf_counter()
-        cres = O.cpu_sweep(J, precision=32, num_alphas=args.alphas, pred_mode="diagonal", seed=42)
-        csec = time.perf_counter() - t
-        cpu = {"value": csec, "unit": "s", "cores": 1, "kind": "port",
-               "sample": (f"the full {args.alphas}-alpha sweep on the CPU path (scipy ARPACK eigsh on the S_rho "
-                          f"operator, diagonal predictor, numpy local search; {cores} cores visible, "
-                          f"scipy's CSR product single-threaded), best energy {cres.best_energy}")}
-        log(f"cpu sweep: {csec:.3f}s, best {cres.best_energy}")
-    elif not args.skip_cpu:
-        cores = len(os.sched_getaffinity(0))
-        per_op, reps = cpu_operator_sample(J)
-        cpu = {"value": per_op * matvecs, "unit": "s (extrapolated)", "cores": 1, "kind": "port",
-               "sample": (f"{reps} applications of the reference's CPU operator S_rho = rho I - N "
-                          f"(scipy CSR, FP32, single-threaded matvec; {cores} cores visible) on the same graph, "
-                          f"{per_op:.3f} s each, times the {matvecs} operator applications of the GPU FP32 sweep; "
-                          f"the CPU local search is not included")}
-        log(f"cpu operator: {per_op:.3f}s per application ({reps} reps)")
+    def gap(a, b):
+        return abs(a - b) / abs(b) if (a is not None and b) else None
 
     line = {
         "metric": METRIC, "value": sec32, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -392,14 +461,20 @@ def main(argv=None) -> int:
         "config": {"workload": w["text"], "alphas": args.alphas, "predictor": args.pred,
                    "local_search": "batched deterministic", "precision": "FP32 vectors, FP64 Lanczos scalars",
                    "spmv_layout": tune32["choice"]},
-        "fp64_sweep_s": sec64, "fp32_vs_fp64_speedup": (sec64 / sec32) if sec64 else None,
-        "fp64_operator_applications": apps64,
-        "fp32_diagonal_sweep_s": sec32d, "fp32_diagonal_operator_applications": apps32d,
-        "best_energy_fp32_diagonal": e32d,
+        # the north star's comparison: FP32 with this run's predictor against plain FP64 (diagonal)
+        "fp64_diagonal_sweep_s": sec64,
+        f"fp32_{args.pred}_vs_fp64_diagonal_speedup": (sec64 / sec32) if sec64 else None,
+        f"energy_gap_fp32_{args.pred}_vs_fp64_diagonal": gap(e32, e64),
+        # like for like: the same (diagonal) predictor in both precisions, both warmed up
+        "fp32_diagonal_sweep_s": sec32d,
+        "fp32_vs_fp64_speedup_diagonal": (sec64 / sec32d) if (sec64 and sec32d) else None,
+        "energy_gap_diagonal_fp32_vs_fp64": gap(e32d, e64),
+        "fp64_operator_applications": apps64, "fp32_diagonal_operator_applications": apps32d,
+        "best_energy_fp32": e32, "best_energy_fp32_diagonal": e32d, "best_energy_fp64_diagonal": e64,
         "lanczos_applications_saved_vs_diagonal": (apps32d - matvecs) if apps32d is not None else None,
-        "best_energy_fp32": e32, "best_energy_fp64": e64, "energy_gap": gap,
         "operator_applications": matvecs, "lanczos_applications": outs[-1].matvecs,
         "tangent_applications": outs[-1].tangent_matvecs, "converged_fraction": conv,
+        "solve_end_to_end_s": e2e,
         "reorth": args.reorth, "orth_full_steps": outs[-1].orth_full_steps,
         "orth_selective_steps": outs[-1].orth_selective_steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -475,6 +475,25 @@ int sa_batch_ls_energy(const sa_batch* Bt, int32_t k, int32_t groups, const int8
 int sa_bits_gather(int64_t nwords, int32_t k, int32_t B, const int64_t* word_off, const int32_t* src,
                    const uint64_t* in, uint64_t* out, hipStream_t stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Seeded synthetic instances, built on the host (no device memory, no stream): symmetric CSR with
+ * sorted columns, no self-loops, no duplicates, isolated nodes dropped. No counterpart in the
+ * reference (its instances are text files, utils/data_loader.py); they make the large test and
+ * benchmark graphs (Erdos-Renyi n = 2^22, Barabasi-Albert n = 2^25) in seconds. Every draw is a
+ * counter-based hash, so the graph is the same for any thread count. coupling_kind: 0 = every
+ * coupling -1 (MaxCut form), 1 = uniform +-1 per undirected edge. threads <= 0: all hardware threads.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct sa_graph sa_graph;   /* opaque host-side CSR owned by the library until sa_graph_free */
+int sa_graph_erdos_renyi(int64_t n, double p, uint64_t seed, int32_t coupling_kind, int32_t threads,
+                         sa_graph** out);
+int sa_graph_barabasi_albert(int64_t n, int32_t m, uint64_t seed, int32_t coupling_kind, int32_t threads,
+                             sa_graph** out);
+int64_t sa_graph_n(const sa_graph* g);      /* nodes after isolated ones were dropped */
+int64_t sa_graph_nnz(const sa_graph* g);    /* stored couplings (both triangles) */
+/* Copies the CSR into caller arrays of n + 1, nnz and nnz entries. */
+int sa_graph_export(const sa_graph* g, int64_t* indptr, int32_t* indices, double* vals);
+void sa_graph_free(sa_graph* g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -274,6 +274,15 @@ _SIGNATURES = {
     "sa_minres_end_f64": (c_int32, [_P, c_int64, _P, POINTER(SaMinresBufs), _P]),
 }
 
+_SIGNATURES.update({
+    "sa_graph_erdos_renyi": (c_int32, [c_int64, c_double, c_uint64, c_int32, c_int32, POINTER(c_void_p)]),
+    "sa_graph_barabasi_albert": (c_int32, [c_int64, c_int32, c_uint64, c_int32, c_int32, POINTER(c_void_p)]),
+    "sa_graph_nnz": (c_int64, [c_void_p]),
+    "sa_graph_n": (c_int64, [c_void_p]),
+    "sa_graph_export": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sa_graph_free": (None, [c_void_p]),
+})
+
 _lib = None
 
 

@@ -25,7 +25,7 @@ ARCH = os.environ.get("SPECANN_ARCH", "gfx950")
 HIPFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-pthread",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
@@ -55,7 +55,7 @@ def build(verbose: bool = True) -> Path:
     with ThreadPoolExecutor(max_workers=min(8, len(sources))) as pool:
         objs = list(pool.map(_compile, sources))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
+    cmd = [hipcc(), "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")

@@ -1,9 +1,22 @@
-"""Run a solver over the instances of a dataset for several seeds (utils/core.py:1-118).
+"""The run pipeline behind main.py: one solver over a dataset's instances and a range of seeds.
 
-Instances run in increasing size (first number in the name), the run's params are filtered by the
-solver's constructor signature (unknown keys warn and are dropped), an optional warm-up solve of a
-small synthetic instance precedes the timed runs, and every (instance, seed) failure is recorded
-with "out of memory" tagging; the CLI exits with status 2 if any run failed.
+Behaviour kept from the reference's pipeline (utils/core.py:70-118), because scripts and result
+files depend on it:
+
+* instances run in increasing order of the first number in their file name ("G7" before "G12");
+* the solver is built as ``cls(instance_name=, dataset=, run_name=, random_seed=, **params)`` and
+  ``solve(J)`` returns (energy, spins, seconds); params the constructor does not name are dropped
+  with a warning;
+* seeds are random_seed, random_seed + 1, ... (num_seeds of them);
+* an optional warm-up solve of a small synthetic instance comes first, and its result files are
+  removed;
+* a failed load counts as a failure for every seed of that instance; a failed solve is recorded as
+  "out of memory" or its exception type, and the run goes on;
+* the summary carries run_name, dataset, total_runs, successful_runs, failed_runs and failures
+  (instance, seed, message), which main.py turns into exit status 2 when anything failed.
+
+The structure is this module's own: a RunPlan resolved once from the config, a list of Jobs, and a
+RunLog that counts outcomes.
 """
 from __future__ import annotations
 
@@ -11,97 +24,168 @@ import inspect
 import re
 import traceback
 import warnings
+from dataclasses import dataclass, field
 from pathlib import Path
-from typing import Any, Dict
+from typing import Any, Callable
 
 from .data_loader import DataLoader
 from .failures import is_out_of_memory
 from .selector import get_solver_class
 from .solution_io import save_solution, solution_path
 
+# keys the pipeline consumes itself; every other config key is offered to the solver's constructor
 PIPELINE_KEYS = frozenset({'solver', 'dataset', 'instance', 'num_seeds', 'random_seed', 'run_name',
                            'save_spins', 'warmup', 'datasets_path'})
 WARMUP_INSTANCE = '__warmup__.txt'
+_FIRST_NUMBER = re.compile(r"\d+")
 
 
-def _filter_params(solver_class, params: Dict[str, Any]) -> Dict[str, Any]:
-    named = {name for name, p in inspect.signature(solver_class.__init__).parameters.items()
-             if p.kind is not inspect.Parameter.VAR_KEYWORD}
-    unknown = sorted(k for k in params if k not in named)
-    if unknown:
-        warnings.warn(f"{solver_class.__name__} does not take {unknown}; these config keys are ignored.",
+def instance_size_key(name: str) -> int:
+    """Sort key of an instance file: the first number in its name without the extension, 0 if none."""
+    stem = name.rsplit('.', 1)[0]
+    found = _FIRST_NUMBER.search(stem)
+    return int(found.group()) if found else 0
+
+
+def constructor_params(solver_class, offered: dict[str, Any]) -> dict[str, Any]:
+    """The offered params the solver's constructor names explicitly (a **kwargs catch-all does not
+    count); the others trigger one warning listing them."""
+    accepted = set()
+    for name, p in inspect.signature(solver_class.__init__).parameters.items():
+        if p.kind is not inspect.Parameter.VAR_KEYWORD:
+            accepted.add(name)
+    dropped = sorted(set(offered) - accepted)
+    if dropped:
+        warnings.warn(f"config keys {dropped} are not parameters of {solver_class.__name__} and are ignored",
                       UserWarning)
-    return {k: v for k, v in params.items() if k in named}
+    return {k: v for k, v in offered.items() if k in accepted}
 
 
-def _warmup_instance(n: int = 64, seed: int = 0):
+def synthetic_warmup_couplings(n: int = 64, density: float = 0.2, seed: int = 0):
+    """A small symmetric +-1 coupling matrix (no diagonal) for the warm-up solve."""
     import numpy as np
     import scipy.sparse as sp
     rng = np.random.default_rng(seed)
-    A = sp.triu(sp.random(n, n, density=0.2, random_state=rng, data_rvs=lambda k: rng.choice([-1.0, 1.0], k)), 1)
-    return (A + A.T).tocsr()
+    mask = np.triu(rng.random((n, n)) < density, 1)
+    signs = rng.choice(np.array([-1.0, 1.0]), size=(n, n))
+    upper = sp.csr_matrix(np.where(mask, signs, 0.0))
+    return (upper + upper.T).tocsr()
 
 
-def _warm_up(solver_class, params, dataset, run_name, seed, out_dir):
+@dataclass
+class RunPlan:
+    """Everything the pipeline needs, resolved once from the merged config."""
+    solver_class: type
+    params: dict[str, Any]
+    dataset: str
+    run_name: str
+    seeds: list[int]
+    datasets_path: str
+    instances: list[str] | None
+    save_spins: bool
+    warmup: bool
+
+    @classmethod
+    def from_config(cls, config: dict[str, Any]) -> "RunPlan":
+        solver_class = get_solver_class(config['solver'])
+        offered = {k: v for k, v in config.items() if k not in PIPELINE_KEYS}
+        first = int(config.get('random_seed', 42))
+        count = int(config.get('num_seeds', 1))
+        chosen = config.get('instance')
+        if isinstance(chosen, str):
+            chosen = [chosen]
+        return cls(solver_class=solver_class, params=constructor_params(solver_class, offered),
+                   dataset=config['dataset'], run_name=config.get('run_name', config['solver']),
+                   seeds=list(range(first, first + count)), datasets_path=config.get('datasets_path', 'datasets'),
+                   instances=list(chosen) if chosen else None, save_spins=bool(config.get('save_spins')),
+                   warmup=bool(config.get('warmup')))
+
+    @property
+    def out_dir(self) -> Path:
+        return Path("results") / self.dataset / self.run_name
+
+    def make_solver(self, instance_name: str, seed: int):
+        return self.solver_class(instance_name=instance_name, dataset=self.dataset, run_name=self.run_name,
+                                 random_seed=seed, **self.params)
+
+
+@dataclass
+class RunLog:
+    """Outcome counts of the (instance, seed) runs."""
+    plan: RunPlan
+    total: int = 0
+    failures: list = field(default_factory=list)
+
+    def ok(self) -> None:
+        self.total += 1
+
+    def fail(self, instance: str, seed: int, message: str) -> None:
+        self.total += 1
+        self.failures.append((instance, seed, message))
+
+    def summary(self) -> dict[str, Any]:
+        return {'run_name': self.plan.run_name, 'dataset': self.plan.dataset, 'total_runs': self.total,
+                'successful_runs': self.total - len(self.failures), 'failed_runs': len(self.failures),
+                'failures': list(self.failures)}
+
+
+def _files_matching(root: Path, stem: str) -> set:
+    return {p for p in root.rglob(f'*{stem}*') if p.is_file()} if root.exists() else set()
+
+
+def warm_up(plan: RunPlan, solve: Callable | None = None) -> None:
+    """One untimed solve of a synthetic instance, so that one-time device set-up is not charged to the
+    first real instance; whatever result files it writes are deleted again."""
     stem = Path(WARMUP_INSTANCE).stem
-    before = set(out_dir.rglob(f'*{stem}*')) if out_dir.exists() else set()
+    existing = _files_matching(plan.out_dir, stem)
     try:
-        solver_class(instance_name=WARMUP_INSTANCE, dataset=dataset, run_name=run_name, random_seed=seed,
-                     **params).solve(_warmup_instance())
-    except Exception as e:
-        print(f"  WARNING: warm-up failed ({type(e).__name__}: {e}); the first instance's time includes "
-              f"one-time initialization costs")
+        plan.make_solver(WARMUP_INSTANCE, plan.seeds[0]).solve(synthetic_warmup_couplings())
+    except Exception as exc:  # a warm-up failure only costs timing accuracy
+        print(f"  warm-up skipped after {type(exc).__name__}: {exc} "
+              f"(the first instance's time then includes one-time set-up)")
     finally:
-        for path in (sorted(out_dir.rglob(f'*{stem}*')) if out_dir.exists() else []):
-            if path.is_file() and path not in before:
-                path.unlink()
+        for leftover in sorted(_files_matching(plan.out_dir, stem) - existing):
+            leftover.unlink()
 
 
-def _size(name: str) -> int:
-    m = re.search(r"(\d+)", name.rsplit('.', 1)[0])
-    return int(m.group(1)) if m else 0
+def _failure_label(exc: BaseException) -> str:
+    return "out of memory" if is_out_of_memory(exc) else type(exc).__name__
 
 
-def run_solver(config: Dict[str, Any]) -> Dict[str, Any]:
-    loader = DataLoader(config['dataset'], config.get('datasets_path', 'datasets'))
-    instances = list(config['instance']) if config.get('instance') else loader.list_instances()
-    if isinstance(config.get('instance'), str):
-        instances = [config['instance']]
-    instances.sort(key=_size)
-    solver_class = get_solver_class(config['solver'])
-    params = _filter_params(solver_class, {k: v for k, v in config.items() if k not in PIPELINE_KEYS})
-    seeds = [int(config.get('random_seed', 42)) + i for i in range(int(config.get('num_seeds', 1)))]
-    run_name = config.get('run_name', config['solver'])
-    out_dir = Path("results") / config['dataset'] / run_name
-    if config.get('warmup'):
+def _solve_one(plan: RunPlan, log: RunLog, instance: str, seed: int, J, index) -> None:
+    try:
+        _, spins, _ = plan.make_solver(instance, seed).solve(J)
+        if plan.save_spins:
+            save_solution(solution_path(plan.out_dir, instance, seed), spins, J, index, dataset=plan.dataset,
+                          instance_name=instance, run_name=plan.run_name, seed=seed)
+    except Exception as exc:
+        traceback.print_exc()
+        message = f"{_failure_label(exc)}: {exc}"
+        log.fail(instance, seed, message)
+        print(f"  FAILED: {instance} seed={seed}: {message}")
+        return
+    log.ok()
+
+
+def run_solver(config: dict[str, Any]) -> dict[str, Any]:
+    """Run the configured solver on every instance and seed; returns the run summary."""
+    plan = RunPlan.from_config(config)
+    loader = DataLoader(plan.dataset, plan.datasets_path)
+    instances = sorted(plan.instances if plan.instances else loader.list_instances(), key=instance_size_key)
+    log = RunLog(plan)
+    if plan.warmup:
         print("Warm-up solve on a small synthetic instance (not recorded)")
-        _warm_up(solver_class, params, config['dataset'], run_name, seeds[0], out_dir)
-    total, failed = 0, []
-    for instance_name in instances:
-        print(f"Processing instance: {instance_name}")
+        warm_up(plan)
+    for instance in instances:
+        print(f"Processing instance: {instance}")
         try:
-            J, index = loader.load_instance(instance_name, return_index=True)
-        except Exception as e:
+            J, index = loader.load_instance(instance, return_index=True)
+        except Exception as exc:
             traceback.print_exc()
-            failed += [(instance_name, seed, f"load: {type(e).__name__}: {e}") for seed in seeds]
-            total += len(seeds)
+            for seed in plan.seeds:
+                log.fail(instance, seed, f"load: {type(exc).__name__}: {exc}")
             continue
-        for i, seed in enumerate(seeds):
-            print(f"  Seed {i + 1}/{len(seeds)} (seed={seed})")
-            total += 1
-            try:
-                solver = solver_class(instance_name=instance_name, dataset=config['dataset'], run_name=run_name,
-                                      random_seed=seed, **params)
-                energy, spins, _ = solver.solve(J)
-                if config.get('save_spins'):
-                    save_solution(solution_path(out_dir, instance_name, seed), spins, J, index,
-                                  dataset=config['dataset'], instance_name=instance_name, run_name=run_name,
-                                  seed=seed)
-            except Exception as e:
-                traceback.print_exc()
-                what = "out of memory" if is_out_of_memory(e) else type(e).__name__
-                failed.append((instance_name, seed, f"{what}: {e}"))
-                e = None
-                print(f"  FAILED: {instance_name} seed={seed}: {failed[-1][2]}")
-    return {'run_name': run_name, 'dataset': config['dataset'], 'total_runs': total,
-            'successful_runs': total - len(failed), 'failed_runs': len(failed), 'failures': failed}
+        for number, seed in enumerate(plan.seeds, start=1):
+            print(f"  Seed {number}/{len(plan.seeds)} (seed={seed})")
+            _solve_one(plan, log, instance, seed, J, index)
+    return log.summary()

@@ -23,6 +23,13 @@ BLOCK_ROWS = {4: 26240, 8: 13312}
 BLOCK_COLS = {4: 14336, 8: 6976}
 BLOCK_MAX_TILES = 368       # column blocks per panel the blocked SpMV can walk
 CUS_PER_XCD = 32
+INT_FIELD_LIMIT = float(1 << 29)   # largest absolute degree the exact int32-field local search takes
+
+
+def exact_int_fields(all_integer: bool, c: float) -> bool:
+    """Whether the exact local search (int32 fields) may take couplings that are all integers and whose
+    largest absolute degree is c."""
+    return bool(all_integer) and c < INT_FIELD_LIMIT
 
 
 def require_gpu() -> torch.device:
@@ -39,6 +46,36 @@ def stream() -> ctypes.c_void_p:
 
 def ptr(t: torch.Tensor | None) -> ctypes.c_void_p | None:
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+STAGE_BYTES = 64 << 20      # pinned staging buffer size of the chunked host-to-device upload
+
+
+def upload(a: np.ndarray, device: torch.device) -> torch.Tensor:
+    """Copy a host array to the device. Arrays above two staging buffers go through two pinned
+    buffers in turn: the host fills one while the DMA engine drains the other, so the copy runs at
+    pinned-memory speed without pinning (or registering) the caller's array."""
+    a = np.ascontiguousarray(a)
+    src = torch.from_numpy(a)
+    if a.nbytes <= 2 * STAGE_BYTES or device.type != "cuda":
+        return src.to(device)
+    flat = src.view(-1)
+    out = torch.empty(flat.numel(), dtype=flat.dtype, device=device)
+    per = STAGE_BYTES // flat.element_size()
+    stages = [torch.empty(per, dtype=flat.dtype, pin_memory=True) for _ in range(2)]
+    done = [None, None]
+    s = torch.cuda.current_stream(device)
+    for i, lo in enumerate(range(0, flat.numel(), per)):
+        hi = min(lo + per, flat.numel())
+        b = i & 1
+        if done[b] is not None:
+            done[b].synchronize()            # the copy that last read this buffer has finished
+        stages[b][: hi - lo].copy_(flat[lo:hi])
+        out[lo:hi].copy_(stages[b][: hi - lo], non_blocking=True)
+        done[b] = torch.cuda.Event()
+        done[b].record(s)
+    s.synchronize()
+    return out.view(src.shape)
 
 
 def suffix(dtype: torch.dtype) -> str:
@@ -107,9 +144,9 @@ class DeviceProblem:
         self.n, self.nnz, self.dtype, self.device = n, nnz, dtype, device
         sfx = suffix(dtype)
         s = stream()
-        self.indptr = torch.from_numpy(np.ascontiguousarray(J.indptr, dtype=np.int32)).to(device)
-        self.indices = torch.from_numpy(np.ascontiguousarray(J.indices, dtype=np.int32)).to(device)
-        raw = torch.from_numpy(np.ascontiguousarray(J.data, dtype=np.float64)).to(device)
+        self.indptr = upload(np.asarray(J.indptr, dtype=np.int32), device)
+        self.indices = upload(np.asarray(J.indices, dtype=np.int32), device)
+        raw = upload(np.asarray(J.data, dtype=np.float64), device)
         self.vals = torch.empty(nnz, dtype=dtype, device=device)
         self.jint = torch.empty(nnz, dtype=torch.int32, device=device)
         nonint = torch.zeros(1, dtype=torch.int32, device=device)
@@ -130,7 +167,10 @@ class DeviceProblem:
         call(f"sa_abs_degree_{sfx}", ctypes.byref(self.csr), ptr(self.deg), self.work.ptr, s)
         head = self.work.head()
         self.c = float(head.r[0])
-        self.integer = bool(nonint.item() == 0)
+        # The exact local search keeps int32 fields h_i = sum_j J_ij s_j (|h_i| <= the absolute degree,
+        # at most c) and adds 2 J_ij s_j per flip: both fit int32 while c < 2^30. The margin to 2^29
+        # covers the rounding of c in FP32; larger integer couplings take the floating-point path.
+        self.integer = exact_int_fields(nonint.item() == 0, self.c)
         if not self.integer:
             self.jint = None
             self.csr.jint = None

@@ -154,27 +154,34 @@ class LanczosSolver:
 
     # -- the solve ------------------------------------------------------------------------------
     def run(self, s: torch.Tensor | None, maxiter: int, first_check: int | None = None) -> LanczosResult:
-        """Iterate from the started vector until converged, stopped at maxiter, rejected or broken."""
-        launched = 0
+        """Iterate from the started vector until converged, stopped at maxiter, rejected or broken.
+
+        The device stops the solve by itself at maxiter operator applications; the host only queues
+        chunks of steps and reads the state between them. It gives up when a whole chunk (plus a
+        restart) made no progress on the device's step counter, so launches the device skipped (for
+        example after a thick restart that kept fewer vectors than the host assumed) never count
+        against the budget."""
         check = first_check or self.chunk
-        limit = int(maxiter) + 4 * self.chunk + self.ncv + 8
         first_beta = None
         self._j = 0
+        last_steps, idle = -1, 0
         while True:
             self._queue(s, check)
-            launched += check
             st = self.work.head()
             if first_beta is None:
                 first_beta = st.first_beta
             if st.status in TERMINAL:
                 return LanczosResult(st.status, st.theta, st.res, st.steps, st.cycles, first_beta, st.r[3])
             if st.status == SA_RESTART:
-                # the restart came earlier than the host expected (a kept vector was dropped)
+                # the basis filled earlier than the host expected (the restart kept fewer vectors)
                 self.thick_restart(s)
                 st = self.work.head()
             self._j = st.j
-            if launched > limit:
-                raise EigensolverBreakdown(f"Lanczos did not stop after {launched} steps (status {st.status})")
+            idle = idle + 1 if st.steps == last_steps else 0
+            last_steps = st.steps
+            if idle >= 2:
+                raise EigensolverBreakdown(
+                    f"Lanczos made no progress over two chunks (status {st.status}, {st.steps} steps)")
             check = self.chunk
 
     def final_vector(self, y_last: torch.Tensor | None) -> tuple[float, float]:

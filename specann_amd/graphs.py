@@ -14,7 +14,8 @@ from __future__ import annotations
 import numpy as np
 import scipy.sparse as sp
 
-__all__ = ["erdos_renyi", "barabasi_albert", "couplings", "symmetric_csr", "packed_batch"]
+__all__ = ["erdos_renyi", "barabasi_albert", "couplings", "symmetric_csr", "packed_batch", "erdos_renyi_native",
+           "barabasi_albert_native"]
 
 
 def couplings(m: int, kind: str, seed: int) -> np.ndarray:
@@ -157,3 +158,51 @@ def packed_batch(graphs):
     J = sp.block_diag(graphs, format="csr")
     J.sort_indices()
     return J, offsets
+
+
+# ---------------------------------------------------------------------------------------------
+# Native generators (libspecann host code, multi-threaded): the large workloads
+# ---------------------------------------------------------------------------------------------
+_KINDS = {"neg": 0, "pm1": 1}
+
+
+def _native_graph(build, *args, threads: int | None = None) -> sp.csr_matrix:
+    import ctypes
+    import os
+    from . import _lib
+    lib = _lib.load()
+    if threads is None:
+        threads = len(os.sched_getaffinity(0))
+    handle = ctypes.c_void_p()
+    rc = build(*args, int(threads), ctypes.byref(handle))
+    if rc != 0:
+        raise RuntimeError(f"graph generator failed ({rc}): {_lib.last_error()}")
+    try:
+        n = int(lib.sa_graph_n(handle))
+        nnz = int(lib.sa_graph_nnz(handle))
+        indptr = np.empty(n + 1, dtype=np.int64)
+        indices = np.empty(nnz, dtype=np.int32)
+        data = np.empty(nnz, dtype=np.float64)
+        lib.sa_graph_export(handle, indptr.ctypes.data, indices.ctypes.data, data.ctypes.data)
+    finally:
+        lib.sa_graph_free(handle)
+    J = sp.csr_matrix((data, indices, indptr.astype(np.int32) if nnz < 2**31 else indptr), shape=(n, n),
+                      copy=False)
+    J.has_sorted_indices = True
+    return J
+
+
+def erdos_renyi_native(n: int, p: float, seed: int = 0, kind: str = "neg", threads: int | None = None):
+    """G(n, p) from libspecann's parallel generator (sa_graph_erdos_renyi): counter-based draws, so the
+    graph depends on (n, p, seed, kind) only. Isolated nodes are dropped (n may shrink for small p)."""
+    from . import _lib
+    return _native_graph(_lib.load().sa_graph_erdos_renyi, int(n), float(p), int(seed) & (2**64 - 1),
+                         _KINDS[kind], threads=threads)
+
+
+def barabasi_albert_native(n: int, m: int, seed: int = 0, kind: str = "neg", threads: int | None = None):
+    """Barabasi-Albert preferential attachment (m edges per node) from libspecann's parallel generator
+    (sa_graph_barabasi_albert); self-loops dropped, repeated edges merged, isolated nodes dropped."""
+    from . import _lib
+    return _native_graph(_lib.load().sa_graph_barabasi_albert, int(n), int(m), int(seed) & (2**64 - 1),
+                         _KINDS[kind], threads=threads)

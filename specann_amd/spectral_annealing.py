@@ -65,6 +65,8 @@ class EigenOutcome:
     pred_residual: float
     pred_accepted: bool | None
     cycles: int
+    y_norm: float = 1.0       # ||Y|| of the final Ritz vector (unnormalised, in the solver's Y buffer)
+    y_dot: float = 0.0        # Y . y_last
 
 
 @dataclass
@@ -277,10 +279,8 @@ class SpectralAnnealing:
                 self._orth_steps[0] += int(oc[0])
                 self._orth_steps[1] += int(oc[1])
                 bound = (-0.5 * eo.lambda1 * eo.degree_scale * prob.c) if eo.converged else float('nan')
-                # final Ritz vector, sign alignment with y_last, normalisation (:653-660)
-                nrm, dot = lz.final_vector(y_last)
-                if not (nrm > 0 and np.isfinite(nrm)):
-                    raise EigensolverBreakdown("zero or non-finite Ritz vector")
+                # sign alignment with y_last and normalisation of the final Ritz vector (:653-660)
+                nrm, dot = eo.y_norm, eo.y_dot
                 sign = -1.0 if (y_last is not None and dot < 0) else 1.0
                 free = next(b for b in ybuf if b is not y_last and b is not y_prev)
                 slot = n_cand
@@ -461,6 +461,7 @@ class SpectralAnnealing:
                     lz.start(s, y_last)
                     res = lz.run(s, self.eigsh_maxiter)
             self._check(res, g)
+            nrm, dot = self._final_vector(lz, y_last)
         except Exception as e:
             if is_out_of_memory(e):
                 raise
@@ -469,11 +470,11 @@ class SpectralAnnealing:
             retry = True
             res = cold_start(alpha_idx * 4 + 2)
             self._check(res, g)
+            nrm, dot = self._final_vector(lz, y_last)
         converged = res.status == SA_CONVERGED
         residual = None
         if not converged:
             # at maxiter: the true residual ||N y - theta y|| / ||y|| (:640-647)
-            nrm, _ = lz.final_vector(None)
             w = torch.empty_like(lz.Y)
             prob.matvec(s, lz.Y, 0.0, 1.0, w, lz.W)
             r, _ = prob.norm_dot(w, lz.Y, 1.0, -res.theta)
@@ -483,7 +484,16 @@ class SpectralAnnealing:
                 print(f"Warning: eigsh did not converge for alpha={alpha}: residual {residual:.3g} > tol "
                       f"{self.eigsh_tol:g}; energy bound not certified")
         return EigenOutcome(res.theta, rho - res.theta, rho, g, degree_scale, res.steps, retry, converged, res.res,
-                            residual, pred_residual, pred_accepted, res.cycles)
+                            residual, pred_residual, pred_accepted, res.cycles, nrm, dot)
+
+    @staticmethod
+    def _final_vector(lz, y_last) -> tuple[float, float]:
+        """The final Ritz vector into lz.Y; (||Y||, Y . y_last). A zero or non-finite vector is a
+        breakdown, so the caller's cold retry applies to it as to any other (:616-636, :679-691)."""
+        nrm, dot = lz.final_vector(y_last)
+        if not (nrm > 0 and np.isfinite(nrm) and np.isfinite(dot)):
+            raise EigensolverBreakdown("zero or non-finite Ritz vector")
+        return nrm, dot
 
     def _predict_start(self, prob, lz, s, alpha, alpha_prev, y_last, y_prev):
         """Queue the predicted start vector of a warm-started alpha (gated: step 0 tests it)."""

@@ -6,8 +6,12 @@
   force over all 2^n spin vectors, exact integer arithmetic);
 * loader_<instance>.npz: the CSR (indptr, indices, data), node ids and bias-node row that the
   reference's own DataLoader (utils/data_loader.py, host-only numpy/scipy code) produces for the
-  bundled fixture instances; the build's loader is compared with them. The reference is imported
-  from /root/reference only here; the fixtures are data.
+  bundled fixture instances; the build's loader is compared with them;
+* sol_<instance>.sol: the file the reference's own save_solution (utils/solution_io.py:43-76,
+  host-only numpy code) writes for the spin vector sol_spins(n) on a bias-node instance and on a
+  tiny Gset-format instance (weights negated, so the cut value is filled in); the build's writer
+  must produce the same bytes and its reader the same header and vector.
+The reference is imported from /root/reference only here; the fixtures are data.
 """
 from __future__ import annotations
 
@@ -21,6 +25,15 @@ HERE = Path(__file__).resolve().parent
 DATASET = "barabasi_albert_3ins_m10_tiny"
 INSTANCES = ["ba_20_1_0.txt", "ba_25_1_1.txt", "ba_60_1_2.txt"]
 REFERENCE = Path("/root/reference")
+SOL_CASES = [(DATASET, "ba_20_1_0.txt"), ("tiny_gset", "g7.txt")]
+
+
+def sol_spins(n: int) -> np.ndarray:
+    """The spin vector of the .sol fixtures: s_i = +1 iff (5 i + 1) mod 3 == 0; the last spin (the bias
+    row of a bias-node instance) is -1, so the writer's gauge flip is exercised."""
+    s = np.where((5 * np.arange(n) + 1) % 3 == 0, 1, -1).astype(np.int8)
+    s[-1] = -1
+    return s
 
 
 def brute_force_ground(J) -> int:
@@ -55,6 +68,13 @@ def main() -> None:
             grounds[name] = {"n_solve": int(J.shape[0]), "ground_energy": brute_force_ground(J)}
             print(name, grounds[name])
     (HERE / "ground_states.json").write_text(json.dumps(grounds, indent=1) + "\n")
+
+    from utils.solution_io import save_solution as ref_save  # reference host code, numpy only
+    for dataset, name in SOL_CASES:
+        J, index = RefLoader(dataset, datasets_path=str(HERE / "instances")).load_instance(name, return_index=True)
+        out = ref_save(HERE / f"sol_{Path(name).stem}.sol", sol_spins(J.shape[0]), J, index, dataset=dataset,
+                       instance_name=name, run_name="SpecAnn", seed=7)
+        print("wrote", out.name)
 
 
 if __name__ == "__main__":

@@ -14,7 +14,7 @@ ROOT = Path(__file__).resolve().parents[1]
 def header_functions():
     text = (ROOT / "include" / "specann.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|size_t)\s+(sa_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|int64_t|void)\s+(sa_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_and_binding_agree():

@@ -65,6 +65,102 @@ def test_shift_rho_cases():
     assert O.shift_rho(None, -1.0, eps=1e-3) == pytest.approx(1.001)
 
 
+def test_shift_rho_build_equals_closed_forms():
+    """The build's compute_shift_rho (the host scalar of the sweep) on the closed-form cases, and
+    equal to the oracle's over a grid of inputs (models/spectral_annealing.py:56-67)."""
+    from specann_amd.spectral_annealing import compute_shift_rho as build
+    assert build(None, 1.0) == pytest.approx(1.001)
+    assert build(float("inf"), 0.8) == pytest.approx(0.8008)
+    assert build(0.0, 0.7) == pytest.approx(0.7007)                 # lambda1_prev <= 0: g alone
+    assert build(2.0, 1.0) == pytest.approx(2.002)                  # max(lambda1_prev, g)
+    assert build(0.5, 1.0, eps=0.01) == pytest.approx(1.01)
+    assert build(None, -2.0) == pytest.approx(1.001)                # rho <= 0: (1 + eps) max(g, 1)
+    assert build(None, 0.0, eps=0.5) == pytest.approx(1.5)
+    for lp in (None, float("nan"), -1.0, 0.0, 0.3, 1.7):
+        for g in (-1.0, 0.0, 0.2, 1.0, 3.0):
+            for eps in (1e-3, 0.25):
+                assert build(lp, g, eps) == O.shift_rho(lp, g, eps)
+
+
+def test_check_eigenpair_closed_forms():
+    """_check_eigenpair (models/spectral_annealing.py:679-691): the tolerance is max(eps/2, 1e-5)
+    relative to g, and any non-finite value fails."""
+    y = np.ones(4) / 2
+    g = 0.8
+    assert O.check_eigenpair(g, y, g, 1e-3)
+    assert O.check_eigenpair(g * (1 + 4.9e-4), y, g, 1e-3)             # within eps / 2 = 5e-4
+    assert not O.check_eigenpair(g * (1 + 5.1e-4), y, g, 1e-3)
+    assert O.check_eigenpair(g * (1 + 0.9e-5), y, g, 1e-6)             # eps / 2 < 1e-5: 1e-5 applies
+    assert not O.check_eigenpair(g * (1 + 1.1e-5), y, g, 1e-6)
+    assert O.check_eigenpair(-5.0, y, g, 1e-3)                         # no lower limit
+    assert not O.check_eigenpair(float("nan"), y, g, 1e-3)
+    assert not O.check_eigenpair(float("inf"), y, g, 1e-3)
+    bad = y.copy()
+    bad[2] = np.nan
+    assert not O.check_eigenpair(0.5, bad, g, 1e-3)
+    bad[2] = np.inf
+    assert not O.check_eigenpair(0.5, bad, g, 1e-3)
+
+
+def test_build_eigenpair_check_matches_oracle():
+    """The sweep's own check (SpectralAnnealing._check) rejects exactly what the oracle rejects."""
+    from types import SimpleNamespace
+    from specann_amd._lib import SA_BROKEN, SA_CONVERGED, EigensolverBreakdown
+    from specann_amd.spectral_annealing import SpectralAnnealing
+    y = np.ones(3)
+    for eps in (1e-3, 1e-6):
+        sa = SpectralAnnealing("i", "d", shift_eps=eps, verbose=False)
+        for theta in (0.7, 0.8, 0.8 * (1 + 4e-4), 0.8 * (1 + 6e-4), 0.8 * (1 + 2e-5), float("nan"), float("inf")):
+            ok = O.check_eigenpair(theta, y, 0.8, eps)
+            res = SimpleNamespace(status=SA_CONVERGED, theta=theta)
+            if ok:
+                sa._check(res, 0.8)
+            else:
+                with pytest.raises(EigensolverBreakdown):
+                    sa._check(res, 0.8)
+        with pytest.raises(EigensolverBreakdown, match="breakdown"):
+            sa._check(SimpleNamespace(status=SA_BROKEN, theta=0.5), 0.8)
+
+
+def test_energies_float_exact_and_brute_force():
+    """fp32_energies (models/spectral_annealing.py:962-976): -1/2 c s^T (J/c)_fp32 s with FP64
+    accumulation. Pinned by exact rational arithmetic on the FP32 couplings, by the integer energies
+    when J/c is exact in FP32 (c a power of two), and by the ground state of a tiny instance."""
+    from fractions import Fraction
+    rng = np.random.default_rng(4)
+    n = 10
+    A = np.triu(rng.uniform(-2.0, 2.0, (n, n)) * (rng.random((n, n)) < 0.5), 1)
+    J = sp.csr_matrix(A + A.T)
+    Jn, d, c = O.normalize(J, np.float32)
+    S = np.where(rng.random((n, 6)) < 0.5, 1, -1)
+    got = O.energies_float(Jn, S, c)
+    dense = Jn.astype(np.float32).toarray()
+    for k in range(S.shape[1]):
+        exact = Fraction(0)
+        for i in range(n):
+            for j in range(n):
+                if dense[i, j] != 0:
+                    exact += Fraction(float(dense[i, j])) * int(S[i, k]) * int(S[j, k])
+        exact = -Fraction(1, 2) * Fraction(c) * exact
+        assert abs(got[k] - float(exact)) <= 1e-12 * max(1.0, abs(float(exact)))
+    # integer couplings with c = 8 (J/c exact in FP32): equal to the int64 energies
+    B = np.zeros((6, 6))
+    for (i, j, w) in [(0, 1, 3), (0, 2, -2), (1, 2, 1), (2, 3, 2), (3, 4, -1), (4, 5, 1), (0, 5, 2), (1, 4, 1)]:
+        B[i, j] = B[j, i] = w
+    B[3, 5] = B[5, 3] = 2
+    B[0, 4] = B[4, 0] = 1    # node 0 then has the largest absolute degree, 3 + 2 + 2 + 1 = 8
+    Ji = sp.csr_matrix(B)
+    Jn, d, c = O.normalize(Ji, np.float32)
+    assert c == 8.0
+    S = np.array(list(itertools.product([-1, 1], repeat=6))).T
+    e = O.energies_float(Jn, S, c)
+    Jint = O.integer_couplings(Ji)
+    ints = [O.energy_int64(Jint, S[:, k]) for k in range(S.shape[1])]
+    assert [float(v) for v in ints] == list(e)
+    # brute-force ground state agrees with the minimum over all 2^n states
+    assert e.min() == min(ints)
+
+
 def test_matvec_dense():
     J = small_signed(40)
     Jn, d, c = O.normalize(J, np.float64)
