@@ -29,7 +29,7 @@ import torch
 from . import local_search as LS
 from ._lib import SA_BROKEN, SA_CONVERGED, SA_MAXITER, SA_REJECTED, EigensolverBreakdown, call
 from .device import DeviceProblem, ptr, require_gpu, stream
-from .eigsh import LanczosSolver
+from .eigsh import LanczosSolver, use_shadow
 from .failures import NumericalFailure, is_numerical, is_out_of_memory
 from .results import store_alphas, store_general
 
@@ -245,7 +245,8 @@ class SpectralAnnealing:
         alpha_list = self._get_alpha_values(self.mode)
         nwords = (n + 63) // 64
         lz = LanczosSolver(prob, self.eigsh_ncv, graphs=self.lanczos_graphs, keep=self.eigsh_keep,
-                           orth_tau=0.0 if self.reorth == "full" else None)
+                           orth_tau=0.0 if self.reorth == "full" else None,
+                           shadow=use_shadow(prob.dtype, self.eigsh_tol))
         s = torch.empty(n, dtype=tdt, device=dev)
         ybuf = [torch.empty(n, dtype=tdt, device=dev) for _ in range(3)]
         x_rand = torch.empty(n, dtype=tdt, device=dev)

@@ -28,7 +28,7 @@ import torch
 from . import _lib
 from ._lib import SA_BROKEN, SA_CONVERGED, SA_RUNNING, EigensolverBreakdown, SaMinresBufs, SaMinresState, call
 from .device import DeviceProblem, Work, ptr, stream
-from .eigsh import LanczosSolver
+from .eigsh import LanczosSolver, use_shadow
 
 CLAMP = 1e-12
 
@@ -161,12 +161,12 @@ class EigenContext:
     def __init__(self, prob: DeviceProblem, tol: float | None = None, maxiter: int = 5000, ncv: int | None = None,
                  adjoint_steps: int = 200, adjoint_tol: float = 1e-10, seed: int = 0):
         self.prob = prob
-        self.lz = LanczosSolver(prob, ncv)
+        eps = np.finfo(np.float32 if prob.dtype == torch.float32 else np.float64).eps
+        self.tol = float(tol) if tol else float(np.sqrt(eps) * 1e-2 if prob.dtype == torch.float32 else 1e-12)
+        self.lz = LanczosSolver(prob, ncv, shadow=use_shadow(prob.dtype, self.tol))
         self.s = torch.empty(prob.n, dtype=prob.dtype, device=prob.device)
         self.x0 = torch.empty_like(self.s)
         self.tangent = TangentPredictor(prob, adjoint_steps, adjoint_tol)
-        eps = np.finfo(np.float32 if prob.dtype == torch.float32 else np.float64).eps
-        self.tol = float(tol) if tol else float(np.sqrt(eps) * 1e-2 if prob.dtype == torch.float32 else 1e-12)
         self.maxiter = int(maxiter)
         self.seed = int(seed)
         self.alpha = None

@@ -242,8 +242,8 @@ def test_fp64_eigenpairs_relative_residual_1e10(gpu, monkeypatch, reorth):
         w, V = np.linalg.eigh(O.dense_n(Jn, s))
         lam, yd, gap = w[-1], V[:, -1], w[-1] - w[-2]
         assert abs(r["lambda1"] - lam) <= 1e-11 * max(1.0, abs(lam))
-        cosang = min(1.0, abs(y @ yd) / np.linalg.norm(y))
-        angle = np.sqrt(max(0.0, 1 - cosang ** 2))
+        y = y / np.linalg.norm(y)
+        angle = float(np.linalg.norm(y - (y @ yd) * yd))     # sin of the angle, without cancellation
         print(f"alpha {r['alpha']:.3f}: residual {rel:.2e}, gap {gap:.3e}, angle {angle:.2e}")
         assert angle <= 2.0 * rel / gap + 1e-12
 
