@@ -242,6 +242,16 @@ int sa_spmv_scaled_f32(const sa_csr* A, const float* s, const float* x, double a
                        hipStream_t stream);
 int sa_spmv_scaled_f64(const sa_csr* A, const double* s, const double* x, double a, double b, double* z, double* y,
                        hipStream_t stream);
+/* The allocation-free form of sa_spmv_scaled_f32, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. The prescaled vector z = s (.) x lives in the library's own static
+ * device scratch of 32 MiB per device, so n <= 2^23 (larger n: SA_E_ARG with a message); the
+ * panel or block layout of A, if any, brings its own partial-sum buffer as with sa_spmv_scaled_f32.
+ * The scratch's device address is looked up on the first call on each device. Calls on one device
+ * share its scratch: calls that overlap in time on one device (different streams) race. Same
+ * kernels and results as sa_spmv_scaled_f32, bit for bit. */
+int sa_spmv_scaled_static_f32(const sa_csr* A, const float* s, const float* x, double a, double b, float* y,
+                              hipStream_t stream);
 
 /* The bare fused kernel the Lanczos step launches: y = a * x + b * s (.) (J/c) g, with the gather vector
  * g = s (.) x prescaled by the caller (x may be NULL when a == 0). */

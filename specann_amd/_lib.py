@@ -204,6 +204,7 @@ _SIGNATURES = {
     "sa_alpha_scaling_f64": (c_int32, [c_int64, _P, c_double, c_double, _P, _P, _P]),
     "sa_spmv_scaled_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, c_double, _P, _P, _P]),
     "sa_spmv_scaled_f64": (c_int32, [POINTER(SaCsr), _P, _P, c_double, c_double, _P, _P, _P]),
+    "sa_spmv_scaled_static_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, c_double, _P, _P]),
     "sa_spmv_gather_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, _P, c_double, _P, _P]),
     "sa_spmv_gather_f64": (c_int32, [POINTER(SaCsr), _P, _P, c_double, _P, c_double, _P, _P]),
     "sa_lanczos_begin": (c_int32, [_P, c_int32, c_int32, c_double, c_double, c_double, c_int32, c_double, _P]),

@@ -14,6 +14,8 @@
 // per-panel row sums are combined by the consumer in panel order.
 #include "sa_spmv.hpp"
 
+#include <mutex>
+
 #include "sa_reduce.hpp"
 
 namespace sa {
@@ -454,9 +456,63 @@ static int panel_scatter(const sa_csr* A, int32_t npanels, const int32_t* pbound
   return check_launch("panel_scatter");
 }
 
+// ---------------------------------------------------------------------------------------------
+// Allocation-free variant of the fused FP32 SpMV: the prescaled vector s (.) x, which
+// sa_spmv_scaled_f32 takes as the caller's scratch z, lives in a file-scope device array instead. Each device that
+// loads the library holds its own copy until the process ends. Its device address is looked up once
+// per device (hipGetSymbolAddress with that device current) on the first call there; host code only
+// ever passes that looked-up address, never the array's name.
+// ---------------------------------------------------------------------------------------------
+constexpr size_t kStaticScratchBytes = (size_t)32 << 20;   // n <= 2^23 FP32 entries
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[kMaxDevices];
+
+static int static_scratch(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("spmv_scaled_static: no current HIP device (or device index >= 64)");
+    return SA_E_LAUNCH;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("spmv_scaled_static: hipGetSymbolAddress of the static scratch failed: ") +
+                hipGetErrorString(e));
+      return SA_E_LAUNCH;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SA_OK;
+}
+
+template <typename T>
+static int spmv_scaled_static(const sa_csr* A, const T* s, const T* x, double a, double b, T* y,
+                              hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0, "bad arguments");
+  SA_REQUIRE((size_t)A->n * sizeof(T) <= kStaticScratchBytes,
+             "n exceeds the static scratch of the allocation-free SpMV (32 MiB: n <= 2^23); "
+             "use sa_spmv_scaled_f32 with a caller buffer");
+  void* z = nullptr;
+  const int rc = static_scratch(&z);
+  if (rc) return rc;
+  return spmv_scaled<T>(A, s, x, a, b, static_cast<T*>(z), y, stream);
+}
+
 }  // namespace sa
 
 extern "C" {
+
+int sa_spmv_scaled_static_f32(const sa_csr* A, const float* s, const float* x, double a, double b, float* y,
+                              hipStream_t stream) {
+  return sa::spmv_scaled_static<float>(A, s, x, a, b, y, stream);
+}
 
 int sa_spmv_scaled_f32(const sa_csr* A, const float* s, const float* x, double a, double b, float* z, float* y,
                        hipStream_t stream) {
