@@ -366,6 +366,20 @@ int sa_ls_class_pass(const sa_csr* A, int32_t k, int32_t groups, const int32_t* 
 /* e2[c] = sum_i s_ic h_ic from maintained fields (zero e2 first). */
 int sa_ls_energy(int64_t n, int32_t k, int32_t groups, const int8_t* spins, const int32_t* fields, int64_t* e2,
                  hipStream_t stream);
+/* Floating-point couplings (the reference's update_spins_kernel_float32/64 and fp32_energies,
+ * models/spectral_annealing.py:821-976): the same search with the FP32 normalised couplings J / c
+ * (fvals) and FP64 fields. The host takes this path only when every nonzero |J / c| is at least
+ * 2^-30 (fields are then exact sums, so the result does not depend on the order of the atomic
+ * updates and matches the CPU restatement bit for bit). Energies: e2[c] = sum_i s_ic h_ic in FP64,
+ * summed in a fixed order through part (sa_ls_energy_f64_scratch(groups) bytes); E = -c e2 / 2. */
+int sa_ls_init_f64(const sa_csr* A, const float* fvals, int32_t k, int32_t groups, const uint64_t* sb,
+                   int8_t* spins, double* fields, hipStream_t stream);
+int sa_ls_class_pass_f64(const sa_csr* A, const float* fvals, int32_t k, int32_t groups, const int32_t* class_nodes,
+                         int64_t count, uint64_t seed, int64_t pass, double frac, int8_t* spins, double* fields,
+                         int32_t* flags, hipStream_t stream);
+size_t sa_ls_energy_f64_scratch(int32_t groups);
+int sa_ls_energy_f64(int64_t n, int32_t k, int32_t groups, const int8_t* spins, const double* fields, double* part,
+                     double* e2, hipStream_t stream);
 /* out = spins[:, c] as +-1 in T. */
 int sa_ls_extract_f32(int64_t n, int32_t groups, int32_t c, const int8_t* spins, float* out, hipStream_t stream);
 int sa_ls_extract_f64(int64_t n, int32_t groups, int32_t c, const int8_t* spins, double* out, hipStream_t stream);

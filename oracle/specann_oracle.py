@@ -386,7 +386,12 @@ def local_search(Jint: sp.csr_matrix, cand: np.ndarray, seed: int, colours: np.n
     C = int(colours.max()) + 1
     classes = [np.flatnonzero(colours == q) for q in range(C)]
     S = cand.T.astype(np.int64).copy()            # n x k
-    H = np.asarray(Jint @ S, dtype=np.int64)      # fields
+    if Jint.dtype.kind not in "iu":
+        # floating-point couplings (update_spins_kernel_float32/64, :821-919): FP32 values, FP64 fields
+        Jint = sp.csr_matrix((Jint.data.astype(np.float32).astype(np.float64), Jint.indices, Jint.indptr),
+                             shape=Jint.shape)
+        S = S.astype(np.float64)
+    H = np.asarray(Jint @ S)                      # fields (int64, or float64)
     e2_start = (S * H).sum(axis=0)
     q_counter = [0]
 
@@ -413,7 +418,7 @@ def local_search(Jint: sp.csr_matrix, cand: np.ndarray, seed: int, colours: np.n
             delta = np.where(fl, 2 * snew, 0)                    # |nodes| x k
             R = Jint[nodes]                                       # rows of the flipped nodes
             rr = np.repeat(np.arange(nodes.size), np.diff(R.indptr))
-            np.add.at(H, R.indices, R.data[:, None].astype(np.int64) * delta[rr])
+            np.add.at(H, R.indices, R.data[:, None].astype(H.dtype) * delta[rr])
         return flips
 
     active = np.ones(k, dtype=bool)
@@ -432,7 +437,7 @@ def local_search(Jint: sp.csr_matrix, cand: np.ndarray, seed: int, colours: np.n
                 fixed_point = final_sweep_fraction >= 1.0
                 break
     e2_final = (S * H).sum(axis=0)
-    return LocalSearchResult(S.T.astype(np.int8).copy(), passes, e2_start, e2_final, q_counter[0])
+    return LocalSearchResult(np.rint(S.T).astype(np.int8).copy(), passes, e2_start, e2_final, q_counter[0])
 
 
 def local_search_scalar(Jint: sp.csr_matrix, cand: np.ndarray, seed: int, colours: np.ndarray, **kw):
@@ -443,11 +448,14 @@ def local_search_scalar(Jint: sp.csr_matrix, cand: np.ndarray, seed: int, colour
     k = cand.shape[0]
     C = int(colours.max()) + 1
     S = [list(map(int, row)) for row in cand]
-    dense = Jint.toarray().astype(np.int64)
+    if Jint.dtype.kind in "iu":
+        dense = Jint.toarray().astype(np.int64)
+    else:   # FP32 couplings, FP64 field sums
+        dense = Jint.toarray().astype(np.float32).astype(np.float64)
     q = 0
 
     def field(c, i):
-        return int(sum(dense[i, j] * S[c][j] for j in range(n)))
+        return sum(dense[i, j] * S[c][j] for j in range(n))
 
     def run_pass(sel_id, frac):
         nonlocal q

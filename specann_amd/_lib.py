@@ -233,6 +233,11 @@ _SIGNATURES = {
     "sa_ls_class_pass": (c_int32, [POINTER(SaCsr), c_int32, c_int32, _P, c_int64, c_uint64, c_int64, c_double, _P,
                                    _P, _P, _P]),
     "sa_ls_energy": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P, _P]),
+    "sa_ls_init_f64": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, _P, _P, _P, _P]),
+    "sa_ls_class_pass_f64": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, _P, c_int64, c_uint64, c_int64, c_double,
+                                       _P, _P, _P, _P]),
+    "sa_ls_energy_f64_scratch": (c_size_t, [c_int32]),
+    "sa_ls_energy_f64": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P, _P, _P]),
     "sa_ls_extract_f32": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P]),
     "sa_ls_extract_f64": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P]),
     "sa_colour_start": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P]),

@@ -17,6 +17,13 @@ namespace sa {
 
 __device__ inline int32_t uniform(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// A wave-uniform 4-byte coupling (int32 or float) read as a scalar.
+template <typename Src>
+__device__ inline Src uniform_val(Src v) {
+  static_assert(sizeof(Src) == 4, "4-byte couplings");
+  return __builtin_bit_cast(Src, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int32_t, v)));
+}
+
 // ---------------------------------------------------------------------------------------------
 // Exact energies of candidate bit vectors: e2[c] = sum_i s_i sum_j J_ij s_j.
 // ---------------------------------------------------------------------------------------------
@@ -66,11 +73,16 @@ __global__ __launch_bounds__(SA_BLOCK) void bits_transpose_kernel(int64_t n, int
 // ---------------------------------------------------------------------------------------------
 // Fields and raw energies of the start candidates. blockIdx.y = group.
 // ---------------------------------------------------------------------------------------------
+// Src: the couplings the search adds (int32 J, or FP32 J / c); F: the fields (int32, or FP64, exact for
+// FP32 couplings whose nonzero magnitudes are at least 2^-30 of the largest absolute degree, which the
+// host checks: every partial sum is then a multiple of the smallest coupling's last bit below 2^53 of
+// them). e2 may be NULL (the FP64 energies are summed in a fixed order by ls_energy_f64 instead).
+template <typename Src, typename F>
 __global__ __launch_bounds__(SA_BLOCK) void ls_init_kernel(int64_t n, const int32_t* __restrict__ indptr,
                                                            const int32_t* __restrict__ indices,
-                                                           const int32_t* __restrict__ jint, int32_t k,
+                                                           const Src* __restrict__ jval, int32_t k,
                                                            int32_t groups, const uint64_t* __restrict__ sb,
-                                                           int8_t* __restrict__ spins, int32_t* __restrict__ fields,
+                                                           int8_t* __restrict__ spins, F* __restrict__ fields,
                                                            int64_t* __restrict__ e2) {
   __shared__ int64_t red[SA_WAVES_PER_BLOCK][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -81,34 +93,36 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_init_kernel(int64_t n, const int3
   int64_t acc = 0;
   for (int64_t i = (int64_t)blockIdx.x * SA_WAVES_PER_BLOCK + wave; i < n; i += nw) {
     const int32_t e0 = uniform(indptr[i]), e1 = uniform(indptr[i + 1]);
-    int32_t h = 0;
+    F h = 0;
     int32_t e = e0;
-    // eight neighbours' spin words in flight at a time (integer sums: the order does not matter)
+    // eight neighbours' spin words in flight at a time (exact sums: the order does not matter)
     for (; e + 8 <= e1; e += 8) {
-      int32_t j[8], jv[8];
+      int32_t j[8];
+      Src jv[8];
       uint64_t w[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         j[u] = uniform(indices[e + u]);
-        jv[u] = uniform(jint[e + u]);
+        jv[u] = uniform_val(jval[e + u]);
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) w[u] = sb[(int64_t)j[u] * groups + g];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) h += ((w[u] >> lane) & 1ull) ? jv[u] : -jv[u];
+      for (int u = 0; u < 8; ++u) h += ((w[u] >> lane) & 1ull) ? (F)jv[u] : -(F)jv[u];
     }
     for (; e < e1; ++e) {
       const int32_t j = uniform(indices[e]);
-      const int32_t jv = uniform(jint[e]);
+      const Src jv = uniform_val(jval[e]);
       const uint64_t w = sb[(int64_t)j * groups + g];
-      h += ((w >> lane) & 1ull) ? jv : -jv;
+      h += ((w >> lane) & 1ull) ? (F)jv : -(F)jv;
     }
     const uint64_t wi = sb[i * groups + g];
     const int32_t si = ((wi >> lane) & 1ull) ? 1 : -1;
     spins[i * kp + c] = (int8_t)si;
     fields[i * kp + c] = h;
-    acc += (int64_t)si * h;
+    if constexpr (sizeof(F) == 4) acc += (int64_t)si * (int64_t)h;
   }
+  if (!e2) return;
   red[wave][lane] = acc;
   __syncthreads();
   if (wave == 0 && c < k) {
@@ -120,13 +134,14 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_init_kernel(int64_t n, const int3
 // ---------------------------------------------------------------------------------------------
 // One colour class of one pass. blockIdx.y = group; a wave per class node.
 // ---------------------------------------------------------------------------------------------
+template <typename Src, typename F>
 __global__ __launch_bounds__(SA_BLOCK) void ls_class_kernel(const int32_t* __restrict__ indptr,
                                                             const int32_t* __restrict__ indices,
-                                                            const int32_t* __restrict__ jint, int32_t k,
+                                                            const Src* __restrict__ jint, int32_t k,
                                                             int32_t groups, const int32_t* __restrict__ nodes,
                                                             int64_t count, uint64_t seed, uint64_t pass,
                                                             uint64_t thresh, int32_t all, int8_t* __restrict__ spins,
-                                                            int32_t* __restrict__ fields, int32_t* __restrict__ flags) {
+                                                            F* __restrict__ fields, int32_t* __restrict__ flags) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = blockIdx.y;
   const int c = g * 64 + lane;
@@ -137,7 +152,8 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_class_kernel(const int32_t* __res
   // adjacent and the field updates are integer atomics, so the order of the nodes does not matter.
   constexpr int U = 4;
   for (int64_t u0 = (int64_t)blockIdx.x * SA_WAVES_PER_BLOCK + wave; u0 < count; u0 += nw * U) {
-    int32_t node[U], s[U], h[U];
+    int32_t node[U], s[U];
+    F h[U];
     bool sel[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
@@ -153,7 +169,7 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_class_kernel(const int32_t* __res
     }
 #pragma unroll
     for (int q = 0; q < U; ++q) {
-      const bool flip = sel[q] && (c < k) && (s[q] * h[q] < 0);
+      const bool flip = sel[q] && (c < k) && ((s[q] > 0) ? (h[q] < (F)0) : (h[q] > (F)0));
       if (!__any(flip)) continue;
       const int32_t i = node[q];
       const int32_t snew = -s[q];
@@ -162,21 +178,22 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_class_kernel(const int32_t* __res
       const int32_t e0 = uniform(indptr[i]), e1 = uniform(indptr[i + 1]);
       int32_t e = e0;
       for (; e + 8 <= e1; e += 8) {
-        int32_t j[8], jv[8];
+        int32_t j[8];
+        Src jv[8];
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
           j[v] = uniform(indices[e + v]);
-          jv[v] = uniform(jint[e + v]);
+          jv[v] = uniform_val(jint[e + v]);
         }
         if (flip) {
 #pragma unroll
-          for (int v = 0; v < 8; ++v) atomicAdd(&fields[(int64_t)j[v] * kp + c], 2 * jv[v] * snew);
+          for (int v = 0; v < 8; ++v) atomicAdd(&fields[(int64_t)j[v] * kp + c], (F)2 * (F)jv[v] * (F)snew);
         }
       }
       for (; e < e1; ++e) {
         const int32_t j = uniform(indices[e]);
-        const int32_t jv = uniform(jint[e]);
-        if (flip) atomicAdd(&fields[(int64_t)j * kp + c], 2 * jv * snew);
+        const Src jv = uniform_val(jint[e]);
+        if (flip) atomicAdd(&fields[(int64_t)j * kp + c], (F)2 * (F)jv * (F)snew);
       }
     }
   }
@@ -202,6 +219,38 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_energy_kernel(int64_t n, int32_t 
     const int64_t v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
     if (v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&e2[c]), (unsigned long long)v);
   }
+}
+
+// Sum over the nodes of s_ic h_ic in FP64, in a fixed order: workgroup b sums its strided rows (waves in
+// a fixed order) into part[b * kp + c]; ls_energy_f64_finish adds the kEnergyGrid partials in order.
+constexpr int kEnergyGrid = 512;
+
+__global__ __launch_bounds__(SA_BLOCK) void ls_energy_f64_kernel(int64_t n, int32_t groups,
+                                                                 const int8_t* __restrict__ spins,
+                                                                 const double* __restrict__ fields,
+                                                                 double* __restrict__ part) {
+  __shared__ double red[SA_WAVES_PER_BLOCK][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = blockIdx.y;
+  const int c = g * 64 + lane;
+  const int64_t kp = (int64_t)groups * 64;
+  const int64_t nw = (int64_t)gridDim.x * SA_WAVES_PER_BLOCK;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * SA_WAVES_PER_BLOCK + wave; i < n; i += nw)
+    acc += (double)spins[i * kp + c] * fields[i * kp + c];
+  red[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0) part[(int64_t)blockIdx.x * kp + c] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+}
+
+__global__ __launch_bounds__(SA_BLOCK) void ls_energy_f64_finish(int32_t k, int64_t kp, int32_t nparts,
+                                                                 const double* __restrict__ part,
+                                                                 double* __restrict__ e2) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  double acc = 0.0;
+  for (int b = 0; b < nparts; ++b) acc += part[(int64_t)b * kp + c];
+  e2[c] = acc;
 }
 
 template <typename T>
@@ -433,8 +482,8 @@ int sa_ls_init(const sa_csr* A, int32_t k, int32_t groups, const uint64_t* sb, i
                int64_t* e2, hipStream_t stream) {
   SA_REQUIRE(A && A->n > 0 && A->jint && sb && spins && fields && e2, "bad arguments");
   SA_REQUIRE(k >= 1 && groups >= 1 && k <= 64 * groups && groups <= 1024, "bad candidate count");
-  hipLaunchKernelGGL(ls_init_kernel, dim3(grid_for(A->n, SA_WAVES_PER_BLOCK * 8, 4096), groups), dim3(SA_BLOCK), 0,
-                     stream, A->n, A->indptr, A->indices, A->jint, k, groups, sb, spins, fields, e2);
+  hipLaunchKernelGGL((ls_init_kernel<int32_t, int32_t>), dim3(grid_for(A->n, SA_WAVES_PER_BLOCK * 8, 4096), groups),
+                     dim3(SA_BLOCK), 0, stream, A->n, A->indptr, A->indices, A->jint, k, groups, sb, spins, fields, e2);
   return check_launch("ls_init");
 }
 
@@ -447,10 +496,49 @@ int sa_ls_class_pass(const sa_csr* A, int32_t k, int32_t groups, const int32_t* 
   if (count <= 0) return SA_OK;
   const int32_t all = frac >= 1.0 ? 1 : 0;
   const uint64_t thresh = (uint64_t)(frac * 9007199254740992.0);
-  hipLaunchKernelGGL(ls_class_kernel, dim3(grid_for(count, SA_WAVES_PER_BLOCK * 2, 8192), groups), dim3(SA_BLOCK), 0,
-                     stream, A->indptr, A->indices, A->jint, k, groups, class_nodes, count, seed, (uint64_t)pass,
-                     thresh, all, spins, fields, flags);
+  hipLaunchKernelGGL((ls_class_kernel<int32_t, int32_t>), dim3(grid_for(count, SA_WAVES_PER_BLOCK * 2, 8192), groups),
+                     dim3(SA_BLOCK), 0, stream, A->indptr, A->indices, A->jint, k, groups, class_nodes, count, seed,
+                     (uint64_t)pass, thresh, all, spins, fields, flags);
   return check_launch("ls_class_pass");
+}
+
+int sa_ls_init_f64(const sa_csr* A, const float* fvals, int32_t k, int32_t groups, const uint64_t* sb,
+                   int8_t* spins, double* fields, hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && fvals && sb && spins && fields, "bad arguments");
+  SA_REQUIRE(k >= 1 && groups >= 1 && k <= 64 * groups && groups <= 1024, "bad candidate count");
+  hipLaunchKernelGGL((ls_init_kernel<float, double>), dim3(grid_for(A->n, SA_WAVES_PER_BLOCK * 8, 4096), groups),
+                     dim3(SA_BLOCK), 0, stream, A->n, A->indptr, A->indices, fvals, k, groups, sb, spins, fields,
+                     (int64_t*)nullptr);
+  return check_launch("ls_init_f64");
+}
+
+int sa_ls_class_pass_f64(const sa_csr* A, const float* fvals, int32_t k, int32_t groups, const int32_t* class_nodes,
+                         int64_t count, uint64_t seed, int64_t pass, double frac, int8_t* spins, double* fields,
+                         int32_t* flags, hipStream_t stream) {
+  SA_REQUIRE(A && fvals && class_nodes && spins && fields && flags && pass >= 0, "bad arguments");
+  SA_REQUIRE(k >= 1 && groups >= 1 && k <= 64 * groups && groups <= 1024, "bad candidate count");
+  SA_REQUIRE(frac > 0.0 && frac <= 1.0, "frac must be in (0, 1]");
+  if (count <= 0) return SA_OK;
+  const int32_t all = frac >= 1.0 ? 1 : 0;
+  const uint64_t thresh = (uint64_t)(frac * 9007199254740992.0);
+  hipLaunchKernelGGL((ls_class_kernel<float, double>), dim3(grid_for(count, SA_WAVES_PER_BLOCK * 2, 8192), groups),
+                     dim3(SA_BLOCK), 0, stream, A->indptr, A->indices, fvals, k, groups, class_nodes, count, seed,
+                     (uint64_t)pass, thresh, all, spins, fields, flags);
+  return check_launch("ls_class_pass_f64");
+}
+
+size_t sa_ls_energy_f64_scratch(int32_t groups) { return (size_t)kEnergyGrid * (size_t)groups * 64 * sizeof(double); }
+
+int sa_ls_energy_f64(int64_t n, int32_t k, int32_t groups, const int8_t* spins, const double* fields, double* part,
+                     double* e2, hipStream_t stream) {
+  SA_REQUIRE(n > 0 && spins && fields && part && e2 && k >= 1 && groups >= 1 && k <= 64 * groups, "bad arguments");
+  hipLaunchKernelGGL(ls_energy_f64_kernel, dim3(kEnergyGrid, groups), dim3(SA_BLOCK), 0, stream, n, groups, spins,
+                     fields, part);
+  int rc = check_launch("ls_energy_f64");
+  if (rc) return rc;
+  hipLaunchKernelGGL(ls_energy_f64_finish, dim3((k + SA_BLOCK - 1) / SA_BLOCK), dim3(SA_BLOCK), 0, stream, k,
+                     (int64_t)groups * 64, kEnergyGrid, part, e2);
+  return check_launch("ls_energy_f64_finish");
 }
 
 int sa_ls_energy(int64_t n, int32_t k, int32_t groups, const int8_t* spins, const int32_t* fields, int64_t* e2,

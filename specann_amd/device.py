@@ -32,6 +32,16 @@ def exact_int_fields(all_integer: bool, c: float) -> bool:
     return bool(all_integer) and c < INT_FIELD_LIMIT
 
 
+FLOAT_FIELD_MIN = 2.0 ** -30     # smallest |J / c| (FP32) whose FP64 field sums stay exact
+
+
+def float_fields_exact(min_coupling: float) -> bool:
+    """Whether FP64 sums of FP32 couplings J / c with this smallest nonzero magnitude are exact: every
+    value is then a multiple of 2^-53 (at least 2^-30 times 2^-23), and a field's partial sums stay
+    within the largest absolute degree over c, i.e. 1 (+ rounding), below 2^53 such units."""
+    return min_coupling >= FLOAT_FIELD_MIN
+
+
 def require_gpu() -> torch.device:
     """The HIP device the solver runs on; raises when there is none (there is no CPU path)."""
     if not torch.cuda.is_available():
@@ -188,11 +198,27 @@ class DeviceProblem:
         self.row_counts = np.diff(J.indptr).astype(np.int64)   # entries per row = per column (symmetric J)
         # one magnitude for every normalised coupling (J in {-1, +1}, J = -1, ...): the stored |J/c|
         absj = np.abs(J.data)
+        nz = absj[absj > 0]
+        # smallest nonzero |J / c| in FP32: the floating-point local search needs it >= 2^-30
+        self.min_coupling = float(np.float32(nz.min() / self.c)) if nz.size and row_c is None else 0.0
+        self._fvals = None
         self.uniform_magnitude = None
         if nnz and row_c is None and absj.min() == absj.max():
             self.uniform_magnitude = float(torch.abs(self.vals[:1]).item())
         self.npanels = 0
         self.tuning = None
+
+    # -- couplings of the floating-point local search ---------------------------------------------
+    def float_couplings(self) -> torch.Tensor:
+        """J / c in FP32 (the couplings of the floating-point local search and energies). Raises when
+        their dynamic range would make the FP64 fields inexact (order-dependent)."""
+        if not float_fields_exact(self.min_coupling):
+            raise NotImplementedError(
+                f"the floating-point local search needs every nonzero |J| >= 2^-30 of the largest absolute "
+                f"degree (smallest here {self.min_coupling:.3g}); its FP64 fields are exact only then")
+        if self._fvals is None:
+            self._fvals = self.vals if self.dtype == torch.float32 else self.vals.to(torch.float32)
+        return self._fvals
 
     # -- XCD panel layout ----------------------------------------------------------------------
     def panel_bounds(self, npanels: int) -> np.ndarray:

@@ -6,7 +6,11 @@ The reference flips spins in place from many warps at once, so its result depend
 one after another; inside a class every selected node with s_i h_i < 0 flips at once, which is the
 same as flipping them one at a time because no two of them are adjacent. The outcome is therefore
 the sequential single-flip descent in that order, bit for bit, for every candidate, and the energy
-never increases. Fields h = J s are kept exactly in int32 and energies in int64.
+never increases. For integer couplings the fields h = J s are kept exactly in int32 and energies in
+int64. Other couplings (update_spins_kernel_float32/64, :821-919) use the FP32 normalised couplings
+J / c with FP64 fields, which are exact sums when every nonzero |J / c| is at least 2^-30 (checked),
+so the search is again order-independent and bit-exact; their energies -c/2 sum_i s_i h_i are summed in
+FP64 in a fixed order (fp32_energies, :962-976).
 
 The termination schedule is the reference's: main passes over a `subset_fraction` share of the
 nodes until every candidate has had a pass without a flip (at most `max_iterations`), then
@@ -22,6 +26,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
+from . import _lib
 from ._lib import call
 from .device import DeviceProblem, ptr, stream
 
@@ -89,21 +94,31 @@ def colour_graph(prob: DeviceProblem, seed: int) -> Colouring:
 @dataclass
 class LocalSearchState:
     spins: torch.Tensor        # int8 [n][kp]
-    fields: torch.Tensor       # int32 [n][kp]
+    fields: torch.Tensor       # int32 (integer couplings) or float64 [n][kp]
     groups: int
     k: int
     passes: int                # main passes (the reference's local_search_iterations)
     total_passes: int
-    e2_start: np.ndarray       # int64 [k]: sum_i s_i h_i of the start candidates (energy = -e2/2)
+    e2_start: np.ndarray       # sum_i s_i h_i of the start candidates (int64, or float64 in units of c)
     e2_final: np.ndarray
+    scale: float = 1.0         # c for the floating-point path (h in units of J / c), 1 for integers
+
+    @property
+    def exact(self) -> bool:
+        return self.e2_start.dtype.kind == "i"
+
+    def _energy(self, e2: np.ndarray) -> np.ndarray:
+        if self.exact:
+            return -(e2 // 2)
+        return -0.5 * self.scale * e2
 
     @property
     def energies_start(self) -> np.ndarray:
-        return -(self.e2_start // 2)
+        return self._energy(self.e2_start)
 
     @property
     def energies_final(self) -> np.ndarray:
-        return -(self.e2_final // 2)
+        return self._energy(self.e2_final)
 
 
 def candidates_to_node_major(prob: DeviceProblem, cand_bits: torch.Tensor, k: int) -> tuple[torch.Tensor, int]:
@@ -115,17 +130,28 @@ def candidates_to_node_major(prob: DeviceProblem, cand_bits: torch.Tensor, k: in
     return sb, groups
 
 
+def _float_energies(prob: DeviceProblem, spins: torch.Tensor, fields: torch.Tensor, groups: int, k: int) -> np.ndarray:
+    part = torch.empty(int(_lib.load().sa_ls_energy_f64_scratch(groups)) // 8, dtype=torch.float64, device=prob.device)
+    e2 = torch.empty(64 * groups, dtype=torch.float64, device=prob.device)
+    call("sa_ls_energy_f64", prob.n, k, groups, ptr(spins), ptr(fields), ptr(part), ptr(e2), stream())
+    return e2[:k].cpu().numpy().copy()
+
+
 def init_fields(prob: DeviceProblem, cand_bits: torch.Tensor, k: int):
-    """Spins, exact fields and the start energies of k candidates."""
-    if prob.jint is None:
-        raise NotImplementedError("the exact local search needs integer couplings")
+    """Spins, fields and the start sums e2 = sum_i s_i h_i of k candidates: (spins, fields, groups, e2) with
+    e2 an int64 device tensor (integer couplings) or a float64 host array."""
     sb, groups = candidates_to_node_major(prob, cand_bits, k)
     kp = 64 * groups
     spins = torch.empty((prob.n, kp), dtype=torch.int8, device=prob.device)
-    fields = torch.empty((prob.n, kp), dtype=torch.int32, device=prob.device)
-    e2 = torch.zeros(kp, dtype=torch.int64, device=prob.device)
-    call("sa_ls_init", prob.csr_ref, k, groups, ptr(sb), ptr(spins), ptr(fields), ptr(e2), stream())
-    return spins, fields, groups, e2
+    if prob.jint is not None:
+        fields = torch.empty((prob.n, kp), dtype=torch.int32, device=prob.device)
+        e2 = torch.zeros(kp, dtype=torch.int64, device=prob.device)
+        call("sa_ls_init", prob.csr_ref, k, groups, ptr(sb), ptr(spins), ptr(fields), ptr(e2), stream())
+        return spins, fields, groups, e2
+    fv = prob.float_couplings()
+    fields = torch.empty((prob.n, kp), dtype=torch.float64, device=prob.device)
+    call("sa_ls_init_f64", prob.csr_ref, ptr(fv), k, groups, ptr(sb), ptr(spins), ptr(fields), stream())
+    return spins, fields, groups, _float_energies(prob, spins, fields, groups, k)
 
 
 def flip_refinement(prob: DeviceProblem, cand_bits: torch.Tensor, k: int, max_iterations: int = 1000,
@@ -136,7 +162,9 @@ def flip_refinement(prob: DeviceProblem, cand_bits: torch.Tensor, k: int, max_it
     of init_fields(prob, cand_bits, k) when the caller has already computed it."""
     seed = int(seed) & _M
     spins, fields, groups, e2 = init if init is not None else init_fields(prob, cand_bits, k)
-    e2_start = e2[:k].cpu().numpy().copy()
+    exact = fields.dtype == torch.int32
+    e2_start = e2[:k].cpu().numpy().copy() if exact else np.asarray(e2[:k], dtype=np.float64).copy()
+    fv = None if exact else prob.float_couplings()
     col = colouring or colour_graph(prob, seed)
     C = col.num_classes
     flags = torch.zeros(64 * groups, dtype=torch.int32, device=prob.device)
@@ -148,8 +176,13 @@ def flip_refinement(prob: DeviceProblem, cand_bits: torch.Tensor, k: int, max_it
             lo, hi = col.offsets[cls], col.offsets[cls + 1]
             if hi <= lo:
                 continue
-            call("sa_ls_class_pass", prob.csr_ref, k, groups, ctypes.c_void_p(col.class_nodes.data_ptr() + 4 * lo),
-                 hi - lo, seed, int(sel_id), float(frac), ptr(spins), ptr(fields), ptr(flags), stream())
+            nodes = ctypes.c_void_p(col.class_nodes.data_ptr() + 4 * lo)
+            if exact:
+                call("sa_ls_class_pass", prob.csr_ref, k, groups, nodes, hi - lo, seed, int(sel_id), float(frac),
+                     ptr(spins), ptr(fields), ptr(flags), stream())
+            else:
+                call("sa_ls_class_pass_f64", prob.csr_ref, ptr(fv), k, groups, nodes, hi - lo, seed, int(sel_id),
+                     float(frac), ptr(spins), ptr(fields), ptr(flags), stream())
         q[0] += 1
         return flags[:k].cpu().numpy() != 0
 
@@ -169,10 +202,13 @@ def flip_refinement(prob: DeviceProblem, cand_bits: torch.Tensor, k: int, max_it
                 # a pass over every node without a flip is a fixed point of all later passes
                 fixed_point = final_sweep_fraction >= 1.0
                 break
-    e2.zero_()
-    call("sa_ls_energy", prob.n, k, groups, ptr(spins), ptr(fields), ptr(e2), stream())
-    e2_final = e2[:k].cpu().numpy().copy()
-    return LocalSearchState(spins, fields, groups, k, passes, q[0], e2_start, e2_final)
+    if exact:
+        e2.zero_()
+        call("sa_ls_energy", prob.n, k, groups, ptr(spins), ptr(fields), ptr(e2), stream())
+        e2_final = e2[:k].cpu().numpy().copy()
+        return LocalSearchState(spins, fields, groups, k, passes, q[0], e2_start, e2_final)
+    e2_final = _float_energies(prob, spins, fields, groups, k)
+    return LocalSearchState(spins, fields, groups, k, passes, q[0], e2_start, e2_final, scale=prob.c)
 
 
 def extract(prob: DeviceProblem, state: LocalSearchState, c: int) -> torch.Tensor:
@@ -183,9 +219,11 @@ def extract(prob: DeviceProblem, state: LocalSearchState, c: int) -> torch.Tenso
 
 
 def energies_of_bits(prob: DeviceProblem, cand_bits: torch.Tensor, k: int) -> np.ndarray:
-    """Exact energies -1/2 s^T J s of k candidate bit vectors (int64)."""
+    """Energies -1/2 s^T J s of k candidate bit vectors: exact int64 for integer couplings, else FP64 from
+    the FP32 couplings J / c (fp32_energies, models/spectral_annealing.py:962-976)."""
     if prob.jint is None:
-        raise NotImplementedError("exact energies need integer couplings")
+        _, _, _, e2 = init_fields(prob, cand_bits, k)
+        return -0.5 * prob.c * e2
     e2 = torch.zeros(k, dtype=torch.int64, device=prob.device)
     call("sa_energy_bits_i64", prob.csr_ref, k, ptr(cand_bits), cand_bits.shape[1], ptr(e2), stream())
     return -(e2.cpu().numpy() // 2)

@@ -14,7 +14,9 @@ Same constructor (:172-260), same per-alpha logic (:311-421), same post-sweep se
 
 Energies are exact for integer couplings, and the reported energy is that exact value (the
 reference recomputes it in float64 from the input J after the timer, :483-488, which is the same
-number for integer J). ``solve`` returns the best spins as a host numpy array of +-1 in the
+number for integer J). Other couplings take the floating-point local search (FP32 J / c, exact FP64
+fields) and FP64 energies, and the reported energy is recomputed from the input J as the reference
+does. ``solve`` returns the best spins as a host numpy array of +-1 in the
 solver's dtype.
 """
 from __future__ import annotations
@@ -32,6 +34,7 @@ from .device import DeviceProblem, ptr, require_gpu, stream
 from .eigsh import LanczosSolver, use_shadow
 from .failures import NumericalFailure, is_numerical, is_out_of_memory
 from .results import store_alphas, store_general
+from .solution_io import ising_energy
 
 PRED_MODES = ("diagonal", "secant", "prev", "tangent")
 LS_POLICIES = ("default", "fixed_guard", "adaptive_guard", "independent_set", "sequential_oracle")
@@ -207,6 +210,10 @@ class SpectralAnnealing:
             prob.autotune()
         out = self.run_on_device(prob, start_time=start_time)
         total_time = time.time() - start_time
+        if out.best_spins is not None and prob.jint is None:
+            # non-integer couplings: the reported energy is -1/2 s^T J s in FP64 from the input couplings,
+            # as the reference reports it after the timer (:483-488)
+            out.best_energy = ising_energy(J, out.best_spins)
         self._sweep_time = out.sweep_time
         self._post_sweep_time = total_time - out.sweep_time
         self._gpu_pool_bytes = int(torch.cuda.memory_reserved())
@@ -358,11 +365,13 @@ class SpectralAnnealing:
             if self.verbose:
                 print(f"Applying local search to {k} unique solutions (out of {len(alpha_list)} total)")
             state = self._refine(prob, cand_bits[:k], k)
-            raw = [int(v) for v in state.energies_start]
-            refined = [int(v) for v in state.energies_final]
+            as_energy = int if state.exact else float
+            raw = [as_energy(v) for v in state.energies_start]
+            refined = [as_energy(v) for v in state.energies_final]
             out.local_passes = state.passes
         else:
-            raw = [int(raw_known[c]) for c in range(k)]
+            as_energy = int if prob.jint is not None else float
+            raw = [as_energy(raw_known[c]) for c in range(k)]
             refined = raw
         best_e, best_row = float("inf"), None
         for i, r in enumerate(rows):                      # the sweep's own best (:405-407)
@@ -418,7 +427,7 @@ class SpectralAnnealing:
         return LS.LocalSearchState(spins, None, kp // 64, k, max(p.passes for p in parts),
                                    sum(p.total_passes for p in parts),
                                    np.concatenate([p.e2_start for p in parts]),
-                                   np.concatenate([p.e2_final for p in parts]))
+                                   np.concatenate([p.e2_final for p in parts]), scale=parts[0].scale)
 
     # ------------------------------------------------------------------------------------------
     def _eigensolve(self, prob, lz, s, x_rand, alpha_idx, alpha, alpha_prev, y_last, y_prev, lambda1_prev

@@ -334,6 +334,39 @@ def test_local_search_vectorised_equals_scalar(seed):
     assert vec.passes == passes
 
 
+def _real_couplings(n, p, seed):
+    rng = np.random.default_rng(seed)
+    A = np.triu(rng.uniform(-1.5, 1.5, (n, n)) * (rng.random((n, n)) < p), 1)
+    return sp.csr_matrix(A + A.T)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_float_local_search_vectorised_equals_scalar(seed):
+    """The floating-point local search (FP32 J / c, FP64 fields): the vectorised restatement equals the
+    one-node-at-a-time one bit for bit, never raises the energy, ends 1-flip optimal after a full pass,
+    and its FP64 fields are exact (equal to exact rational sums)."""
+    from fractions import Fraction
+    J = _real_couplings(24, 0.3, seed)
+    Jn, d, c = O.normalize(J, np.float32)
+    assert np.abs(Jn.data).min() >= 2.0 ** -30
+    colours = O.greedy_colouring(Jn.indptr, Jn.indices, seed)
+    cands = np.stack([O.round_spins(O.normal_vector(24, seed, i)) for i in range(3)])
+    vec = O.local_search(Jn, cands, seed, colours, max_iterations=50)
+    sc, passes = O.local_search_scalar(Jn, cands, seed, colours, max_iterations=50)
+    np.testing.assert_array_equal(vec.spins, sc)
+    assert vec.passes == passes
+    e0 = O.energies_float(Jn, cands.T, c)
+    e1 = O.energies_float(Jn, vec.spins.T, c)
+    assert np.all(e1 <= e0 + 1e-12)
+    np.testing.assert_allclose(-0.5 * c * vec.e2_final, e1, rtol=1e-12)
+    H = Jn.astype(np.float64) @ vec.spins.T.astype(np.float64)
+    assert np.all(vec.spins.T * H >= 0)
+    dense = Jn.toarray().astype(np.float64)
+    for i in range(24):                                     # the FP64 field is the exact sum
+        exact = sum(Fraction(float(dense[i, j])) * int(vec.spins[0, j]) for j in range(24))
+        assert H[i, 0] == float(exact)
+
+
 def test_local_search_planted_mistake_is_caught():
     """A wrong field update (sign flipped) in a copy of the pass must break 1-flip optimality."""
     J = graphs.erdos_renyi(40, 0.2, seed=4, kind="pm1")
