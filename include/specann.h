@@ -324,6 +324,13 @@ int sa_scale_copy_f64(int64_t n, const double* x, double scale, double* out, hip
 
 /* out = a x + b y + c z (FP64 arithmetic; y or z may be NULL when its coefficient is 0). The vector
  * bookkeeping of the predictors (cf. utils/predictors.py:29-56). */
+/* Diagonal predictor's scaling (utils/predictors.py:19-20): out = max(d, 1e-12)^pexp (.) y in the
+ * vectors' type; the caller normalises. */
+int sa_diag_scale_f32(int64_t n, const float* y, const float* d, double pexp, float* out, hipStream_t stream);
+int sa_diag_scale_f64(int64_t n, const double* y, const double* d, double pexp, double* out, hipStream_t stream);
+/* Candidate-major int8 +-1 spins [k][n] (the reference's spins_list, models/spectral_annealing.py:1058)
+ * -> candidate bits [k][nwords]: bit l of word w is spin 64 w + l > 0. */
+int sa_pack_spins(int64_t n, int32_t k, const int8_t* spins, int64_t nwords, uint64_t* bits, hipStream_t stream);
 int sa_combine3_f32(int64_t n, double a, const float* x, double b, const float* y, double c, const float* z,
                     float* out, hipStream_t stream);
 int sa_combine3_f64(int64_t n, double a, const double* x, double b, const double* y, double c, const double* z,

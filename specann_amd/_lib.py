@@ -242,6 +242,9 @@ _SIGNATURES = {
     "sa_ls_extract_f64": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P]),
     "sa_colour_start": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P]),
     "sa_colour_rounds": (c_int32, [POINTER(SaCsr), c_uint64, _P, _P, _P, c_int32, c_int32, _P]),
+    "sa_diag_scale_f32": (c_int32, [c_int64, _P, _P, c_double, _P, _P]),
+    "sa_diag_scale_f64": (c_int32, [c_int64, _P, _P, c_double, _P, _P]),
+    "sa_pack_spins": (c_int32, [c_int64, c_int32, _P, c_int64, _P, _P]),
     "sa_combine3_f32": (c_int32, [c_int64, c_double, _P, c_double, _P, c_double, _P, _P, _P]),
     "sa_combine3_f64": (c_int32, [c_int64, c_double, _P, c_double, _P, c_double, _P, _P, _P]),
     "sa_minres_state_bytes": (c_size_t, []),

@@ -14,13 +14,15 @@
 //      matrix: largest eigenvalue by 257-section Sturm bisection over the whole workgroup, the last
 //      component s_j of its eigenvector from the derivative of the last LDL^T pivot, and the
 //      residual estimate beta_j |s_j|, the stopping test the reference's eigsh applies per restart
-//      (here it is applied every step). The full eigenvector (inverse iteration with a pivoted
-//      tridiagonal LU) is computed only in the step that stops the solve.
+//      (here it is applied every step). A small or non-finite estimate, and every stop, is decided
+//      instead by the full eigenvector (inverse iteration on the dense projected matrix, pivoted LU
+//      shared by the workgroup), which also gives the Ritz vector's coefficients.
 // Every launch reads the status in the work buffer first and does nothing unless the solve is
 // running, so the host may queue several steps (and the restart pair) without a round trip.
 #include <cstddef>
 
 #include "sa_reduce.hpp"
+#include "sa_ritz.hpp"
 #include "sa_spmv.hpp"
 
 namespace sa {
@@ -711,6 +713,13 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(M map, const T* __res
   if (threadIdx.x == 0) part[sl.blk] = bs;
 }
 
+// The workgroup as the team that shares dense_ritz_vector's factorisation.
+struct BlockTeam {
+  __device__ int rank() const { return threadIdx.x; }
+  __device__ int size() const { return blockDim.x; }
+  __device__ void sync() const { __syncthreads(); }
+};
+
 // Phase 2 of the Gram-Schmidt step: beta_j from the G partial squared norms, then the FP64 Ritz solve
 // of the projected matrix and the stopping test, in one workgroup.
 template <class F>
@@ -721,8 +730,11 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_finish_kernel(F fin) {
   const double beta = sqrt(sum_partials(partials_of(fin.w()), fin.g(), 1, 0));
 
   __shared__ double la[SA_MAX_NCV], lb[SA_MAX_NCV], lb2[SA_MAX_NCV], lar[SA_MAX_NCV], lar2[SA_MAX_NCV];
-  __shared__ double lvec[SA_MAX_NCV], lwork[6 * SA_MAX_NCV], lbounds[4];
+  __shared__ double lvec[SA_MAX_NCV], lbounds[4];
   __shared__ int lpiv[SA_MAX_NCV];
+  __shared__ double ldense[kDenseRitz * kDenseRitz];
+  __shared__ int lstatus, ldecide;
+  __shared__ double lres;
   const int m = j + 1;
   const int k = st->kept;
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
@@ -735,7 +747,6 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_finish_kernel(F fin) {
   const bool finite = isfinite(aj) && isfinite(beta);
   const bool big = fabs(aj) > 10.0 * st->bound || beta > 10.0 * st->bound;
   double theta = 0.0;
-  bool solved = false;
   bool all_finite = finite && !big;
   if (all_finite) {
     for (int i = 0; i < j; ++i) all_finite = all_finite && isfinite(lb[i]) && isfinite(lar[i]);
@@ -744,40 +755,56 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_finish_kernel(F fin) {
   if (all_finite) {
     proj_bounds(P, lb2, lar2, lbounds);
     theta = largest_eigenvalue(P, lbounds);
-    solved = true;
   }
   if (threadIdx.x == 0) {
     st->b[j] = beta;
     st->steps += 1;
-    if (!solved) {
-      st->status = SA_BROKEN;
-    } else {
-      const double res = beta * sqrt(last_component_sq(P, theta, lbounds[2]));
+    int status = SA_BROKEN;
+    double res = 0.0;
+    if (all_finite && isfinite(theta)) {
+      status = SA_RUNNING;
       st->theta = theta;
-      st->res = res;
-      int status = SA_RUNNING;
-      if (!isfinite(theta) || !isfinite(res)) status = SA_BROKEN;
-      if (status == SA_RUNNING && st->gate && j == 0) {
+      res = beta * sqrt(last_component_sq(P, theta, lbounds[2]));
+      if (st->gate && j == 0) {
         st->first_beta = beta;
         st->gate = 0;
         if (beta > st->pred_tol) status = SA_REJECTED;
       }
-      if (status == SA_RUNNING) {
-        if (res <= st->tol) status = SA_CONVERGED;
-        else if (st->steps >= st->maxiter) status = SA_MAXITER;
-        else if (m >= st->ncv) status = SA_RESTART;
-      }
-      if (status == SA_CONVERGED || status == SA_MAXITER) {
-        // the solve stops here: its Ritz vector's coefficients, for the final combination
-        const double tiny = 2.220446049250313e-16 * fmax(lbounds[3], 1e-300);
-        inverse_iteration(P, theta, tiny, 0, lwork, lwork + SA_MAX_NCV, lwork + 2 * SA_MAX_NCV,
-                          lwork + 3 * SA_MAX_NCV, lwork + 4 * SA_MAX_NCV, lwork + 5 * SA_MAX_NCV, lpiv, lvec);
-        for (int i = 0; i < m; ++i) st->ritz[i] = lvec[i];
-      }
+    }
+    lstatus = status;
+    lres = res;
+    // The derivative estimate cancels (to 0, or to inf * 0) once theta equals a pivot to rounding, as
+    // it does when a kept or start vector is already the eigenvector: every small or non-finite
+    // estimate, and every stop, is decided by the pivoted dense eigenvector instead.
+    ldecide = status == SA_RUNNING && (!(res > st->tol) || !isfinite(res) || st->steps >= st->maxiter);
+  }
+  __syncthreads();
+  const bool decide = ldecide != 0;
+  if (decide) {
+    // LDS workspace while it fits; the restart scratch of the work buffer (idle between restarts) beyond
+    double* ws = m <= kDenseRitz ? ldense : restart_of(fin.w());
+    const double tiny = 2.220446049250313e-16 * fmax(lbounds[3], 1e-300);
+    dense_ritz_vector(BlockTeam{}, m, k, la, lb, lar, theta, tiny, ws, lpiv, lvec);
+  }
+  if (threadIdx.x == 0) {
+    int status = lstatus;
+    const double res = decide ? beta * fabs(lvec[m - 1]) : lres;
+    if (all_finite) st->res = res;
+    if (status == SA_RUNNING) {
+      if (!isfinite(res)) status = SA_BROKEN;
+      else if (res <= st->tol) status = SA_CONVERGED;
+      else if (st->steps >= st->maxiter) status = SA_MAXITER;
+      else if (m >= st->ncv) status = SA_RESTART;
+    }
+    if (status == SA_CONVERGED || status == SA_MAXITER) {
+      // the solve stops here (decide was set): its Ritz vector's coefficients, for the final combination
+      for (int i = 0; i < m; ++i) st->ritz[i] = lvec[i];
+    }
+    if (all_finite) {
       st->sigma[j + 1] = beta > 0.0 ? 1.0 / beta : 0.0;
       st->j = m;
-      st->status = status;
     }
+    st->status = status;
   }
 }
 

@@ -216,6 +216,33 @@ __global__ __launch_bounds__(SA_BLOCK) void combine3_kernel(int64_t n, double a,
   }
 }
 
+// out = max(d, 1e-12)^pexp (.) y in T (the diagonal predictor's scaling, utils/predictors.py:19-20).
+template <typename T>
+__global__ __launch_bounds__(SA_BLOCK) void diag_scale_kernel(int64_t n, const T* __restrict__ y,
+                                                              const T* __restrict__ d, T pexp,
+                                                              T* __restrict__ out) {
+  const T clampv = (T)1e-12;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const T dr = d[r] > clampv ? d[r] : clampv;
+    out[r] = pow(dr, pexp) * y[r];
+  }
+}
+
+// Candidate-major int8 +-1 spins [k][n] -> bits [k][nwords] (bit l of word w: spin 64 w + l > 0), one
+// ballot per wave of 64 nodes.
+__global__ __launch_bounds__(SA_BLOCK) void pack_spins_kernel(int64_t n, int64_t nwords, const int8_t* __restrict__ spins,
+                                                              uint64_t* __restrict__ bits) {
+  const int c = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nwords; w += waves) {
+    const int64_t i = w * 64 + lane;
+    const bool up = i < n && spins[(int64_t)c * n + i] > 0;
+    const uint64_t m = __ballot(up);
+    if (lane == 0) bits[(int64_t)c * nwords + w] = m;
+  }
+}
+
 // Box-Muller on two 53-bit uniforms of the stream (seed, stream_id, i); u1 is in (0, 1].
 __host__ __device__ inline double normal_at(uint64_t seed, uint64_t stream_id, uint64_t i) {
   const uint64_t h1 = mix3(seed, 0x5A17C0DE00000000ull ^ stream_id, i);
@@ -451,6 +478,25 @@ int sa_scale_copy_f64(int64_t n, const double* x, double scale, double* out, hip
   SA_REQUIRE(n > 0 && x && out, "bad arguments");
   hipLaunchKernelGGL(scale_copy_kernel<double>, dim3(SA_GRID(n)), dim3(SA_BLOCK), 0, stream, n, x, scale, out);
   return check_launch("scale_copy_f64");
+}
+
+int sa_diag_scale_f32(int64_t n, const float* y, const float* d, double pexp, float* out, hipStream_t stream) {
+  SA_REQUIRE(n > 0 && y && d && out, "bad arguments");
+  hipLaunchKernelGGL(diag_scale_kernel<float>, dim3(SA_GRID(n)), dim3(SA_BLOCK), 0, stream, n, y, d, (float)pexp, out);
+  return check_launch("diag_scale_f32");
+}
+
+int sa_diag_scale_f64(int64_t n, const double* y, const double* d, double pexp, double* out, hipStream_t stream) {
+  SA_REQUIRE(n > 0 && y && d && out, "bad arguments");
+  hipLaunchKernelGGL(diag_scale_kernel<double>, dim3(SA_GRID(n)), dim3(SA_BLOCK), 0, stream, n, y, d, pexp, out);
+  return check_launch("diag_scale_f64");
+}
+
+int sa_pack_spins(int64_t n, int32_t k, const int8_t* spins, int64_t nwords, uint64_t* bits, hipStream_t stream) {
+  SA_REQUIRE(n > 0 && k >= 1 && k <= 65535 && spins && bits && nwords * 64 >= n, "bad arguments");
+  hipLaunchKernelGGL(pack_spins_kernel, dim3(grid_for(nwords, SA_WAVES_PER_BLOCK, 4096), k), dim3(SA_BLOCK), 0, stream,
+                     n, nwords, spins, bits);
+  return check_launch("pack_spins");
 }
 
 int sa_combine3_f32(int64_t n, double a, const float* x, double b, const float* y, double c, const float* z,

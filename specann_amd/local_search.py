@@ -137,18 +137,19 @@ def _float_energies(prob: DeviceProblem, spins: torch.Tensor, fields: torch.Tens
     return e2[:k].cpu().numpy().copy()
 
 
-def init_fields(prob: DeviceProblem, cand_bits: torch.Tensor, k: int):
+def init_fields(prob: DeviceProblem, cand_bits: torch.Tensor, k: int, fvals: torch.Tensor | None = None):
     """Spins, fields and the start sums e2 = sum_i s_i h_i of k candidates: (spins, fields, groups, e2) with
-    e2 an int64 device tensor (integer couplings) or a float64 host array."""
+    e2 an int64 device tensor (integer couplings) or a float64 host array. fvals: FP32 couplings to use
+    instead of the problem's (the floating-point path; each field is then summed in row order)."""
     sb, groups = candidates_to_node_major(prob, cand_bits, k)
     kp = 64 * groups
     spins = torch.empty((prob.n, kp), dtype=torch.int8, device=prob.device)
-    if prob.jint is not None:
+    if prob.jint is not None and fvals is None:
         fields = torch.empty((prob.n, kp), dtype=torch.int32, device=prob.device)
         e2 = torch.zeros(kp, dtype=torch.int64, device=prob.device)
         call("sa_ls_init", prob.csr_ref, k, groups, ptr(sb), ptr(spins), ptr(fields), ptr(e2), stream())
         return spins, fields, groups, e2
-    fv = prob.float_couplings()
+    fv = prob.float_couplings() if fvals is None else fvals
     fields = torch.empty((prob.n, kp), dtype=torch.float64, device=prob.device)
     call("sa_ls_init_f64", prob.csr_ref, ptr(fv), k, groups, ptr(sb), ptr(spins), ptr(fields), stream())
     return spins, fields, groups, _float_energies(prob, spins, fields, groups, k)

@@ -39,6 +39,10 @@ from .solution_io import ising_energy
 PRED_MODES = ("diagonal", "secant", "prev", "tangent")
 LS_POLICIES = ("default", "fixed_guard", "adaptive_guard", "independent_set", "sequential_oracle")
 SCHEDULES = ("sqrt", "linear")
+# eigsh tolerances below this get a true-residual check at convergence (FP64 accuracy asked for), and
+# up to REFINE_ROUNDS continuations from the Ritz vector while that residual is above the tolerance
+REFINE_TOL = 1e-8
+REFINE_ROUNDS = 2
 
 
 def compute_shift_rho(lambda1_prev, gershgorin_bound, eps=1e-3):
@@ -483,18 +487,39 @@ class SpectralAnnealing:
             nrm, dot = self._final_vector(lz, y_last)
         converged = res.status == SA_CONVERGED
         residual = None
+        steps = res.steps
+        if converged and self.eigsh_tol < REFINE_TOL:
+            # tight tolerances: the Lanczos estimate |beta s_j| can undershoot the true residual by a small
+            # factor (rounding in the recurrence and the restarts); check it and continue from the Ritz
+            # vector while it is above tol (at most REFINE_ROUNDS times, within the maxiter budget)
+            residual = self._true_residual(prob, lz, s, res.theta, nrm)
+            for _ in range(REFINE_ROUNDS):
+                if residual <= self.eigsh_tol or steps >= self.eigsh_maxiter:
+                    break
+                lz.begin(self.eigsh_tol, self.eigsh_maxiter - steps, bound)
+                lz.start(s, lz.Y)
+                res = lz.run(s, self.eigsh_maxiter - steps)
+                self._check(res, g)
+                nrm, dot = self._final_vector(lz, y_last)
+                steps += res.steps
+                residual = self._true_residual(prob, lz, s, res.theta, nrm)
         if not converged:
             # at maxiter: the true residual ||N y - theta y|| / ||y|| (:640-647)
-            w = torch.empty_like(lz.Y)
-            prob.matvec(s, lz.Y, 0.0, 1.0, w, lz.W)
-            r, _ = prob.norm_dot(w, lz.Y, 1.0, -res.theta)
-            residual = r / nrm
+            residual = self._true_residual(prob, lz, s, res.theta, nrm)
             converged = residual <= self.eigsh_tol
             if not converged and self.verbose:
                 print(f"Warning: eigsh did not converge for alpha={alpha}: residual {residual:.3g} > tol "
                       f"{self.eigsh_tol:g}; energy bound not certified")
-        return EigenOutcome(res.theta, rho - res.theta, rho, g, degree_scale, res.steps, retry, converged, res.res,
+        return EigenOutcome(res.theta, rho - res.theta, rho, g, degree_scale, steps, retry, converged, res.res,
                             residual, pred_residual, pred_accepted, res.cycles, nrm, dot)
+
+    @staticmethod
+    def _true_residual(prob, lz, s, theta, nrm) -> float:
+        """||N y - theta y|| / ||y|| of the Ritz vector in lz.Y (norm nrm): one operator application."""
+        w = torch.empty_like(lz.Y)
+        prob.matvec(s, lz.Y, 0.0, 1.0, w, lz.W)
+        r, _ = prob.norm_dot(w, lz.Y, 1.0, -theta)
+        return r / nrm
 
     @staticmethod
     def _final_vector(lz, y_last) -> tuple[float, float]:
@@ -539,10 +564,73 @@ class SpectralAnnealing:
             raise EigensolverBreakdown(f"spurious eigenvalue {res.theta:.6g} above the Gershgorin bound {g:.6g}")
 
 
+def _spin_bits_of(prob: DeviceProblem, cols: torch.Tensor) -> torch.Tensor:
+    """Candidate-major int8 spins [k][n] on the device -> candidate bits [k][nwords] (sa_pack_spins)."""
+    k = cols.shape[0]
+    nwords = (prob.n + 63) // 64
+    bits = torch.empty((k, nwords), dtype=torch.int64, device=prob.device)
+    call("sa_pack_spins", prob.n, k, ptr(cols), nwords, ptr(bits), stream())
+    return bits
+
+
+def _problem_of(J_csr, dtype) -> DeviceProblem:
+    if isinstance(J_csr, DeviceProblem):
+        return J_csr
+    tdt = torch.float64 if np.dtype(dtype) == np.float64 else torch.float32
+    return DeviceProblem(J_csr, tdt)
+
+
+def flip_refinement(J_csr, spins_list, max_iterations=1000, final_sweep_fraction=1.0, final_sweep_iters=25,
+                    num_rounds=25, subset_fraction=0.9, dtype=np.float32, seed: int = 0):
+    """The reference's local-search callable (models/spectral_annealing.py:1039-1112): the k candidates
+    of spins_list (+-1 vectors, host or device, not modified) refined on the GPU; returns (n x k int8
+    device tensor, main passes). J_csr: the couplings (scipy sparse, or a DeviceProblem already on the
+    GPU). The search is the deterministic colour-class descent (exact int32 fields for integer
+    couplings, FP32 J / c with exact FP64 fields otherwise), so dtype only sets the precision of the
+    problem this builds; seed fixes the node selection and class order."""
+    prob = _problem_of(J_csr, dtype)
+    cols = torch.stack([t if isinstance(t, torch.Tensor) else torch.from_numpy(np.asarray(t))
+                        for t in spins_list]).to(device=prob.device, dtype=torch.int8).contiguous()
+    k = cols.shape[0]
+    st = LS.flip_refinement(prob, _spin_bits_of(prob, cols), k, max_iterations=max_iterations,
+                            final_sweep_fraction=final_sweep_fraction, final_sweep_iters=final_sweep_iters,
+                            num_rounds=num_rounds, subset_fraction=subset_fraction, seed=seed)
+    return st.spins[:, :k], st.passes
+
+
+def fp32_energies(J_csr, spins, normalization=1.0) -> torch.Tensor:
+    """The reference's energy callable (models/spectral_annealing.py:962-976): E = -1/2 s^T J s *
+    normalization for a +-1 vector or every column of an n x k matrix, with the couplings in FP32 and
+    FP64 accumulation (each row's field in row order, then a fixed-order sum over the rows); a float64
+    device tensor of k energies. J_csr: scipy sparse (its own values), or a DeviceProblem (then its
+    normalised couplings J / c, as the reference calls it with the normalised matrix and c)."""
+    if isinstance(J_csr, DeviceProblem):
+        prob, fv = J_csr, J_csr.vals.to(torch.float32)
+    else:
+        prob = DeviceProblem(J_csr, torch.float32)
+        from .device import upload
+        fv = upload(np.asarray(sp_csr(J_csr).data, dtype=np.float32), prob.device)
+    S = spins if isinstance(spins, torch.Tensor) else torch.from_numpy(np.asarray(spins))
+    S = S.reshape(-1, 1) if S.dim() == 1 else S
+    cols = S.t().to(device=prob.device, dtype=torch.int8).contiguous()
+    k = cols.shape[0]
+    _, _, _, e2 = LS.init_fields(prob, _spin_bits_of(prob, cols), k, fvals=fv)
+    return torch.from_numpy(-0.5 * float(normalization) * e2).to(prob.device)
+
+
+def sp_csr(J):
+    import scipy.sparse as sp
+    J = sp.csr_matrix(J)
+    if not J.has_sorted_indices:
+        J = J.copy()
+        J.sort_indices()
+    return J
+
+
 def unpack_bits(words: np.ndarray, n: int) -> np.ndarray:
     """+-1 spins from uint64 words (bit l of word w = spin 64 w + l)."""
     b = np.unpackbits(words.view(np.uint8), bitorder="little")[:n]
     return np.where(b > 0, 1, -1).astype(np.int8)
 
 
-__all__ = ["SpectralAnnealing", "compute_shift_rho", "unpack_bits", "require_gpu"]
+__all__ = ["SpectralAnnealing", "compute_shift_rho", "flip_refinement", "fp32_energies", "unpack_bits", "require_gpu"]

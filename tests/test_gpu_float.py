@@ -40,7 +40,8 @@ def test_float_local_search_bit_exact(gpu, tdtype, k):
     cands = np.stack([O.round_spins(O.normal_vector(prob.n, 9, i)) for i in range(k)])
     st = LS.flip_refinement(prob, _bits(cands), k, seed=42)
     assert not st.exact
-    Jn, d, c = O.normalize(J, np.float32)
+    # the search's couplings: J / c normalised in the problem's precision, then rounded to FP32
+    Jn, d, c = O.normalize(J, np.float32 if tdtype == torch.float32 else np.float64)
     ref = O.local_search(Jn, cands, 42)
     np.testing.assert_array_equal(st.spins[:, :k].cpu().numpy().T, ref.spins)
     assert st.passes == ref.passes
@@ -74,7 +75,7 @@ def test_float_sweep_end_to_end(gpu, tmp_path, monkeypatch, precision):
     prob = DeviceProblem(J, torch.float32 if precision == 32 else torch.float64)
     out = SpectralAnnealing("float.txt", "synthetic", random_seed=42, precision=precision, num_alphas=12,
                             verbose=False).run_on_device(prob)
-    Jn, d, c = O.normalize(J, np.float32)
+    Jn, d, c = O.normalize(J, np.float32 if precision == 32 else np.float64)
     bits = out.cand_bits.cpu().numpy()
     cands = np.stack([unpack_bits(b, prob.n) for b in bits])
     np.testing.assert_allclose(out.raw_energies, O.energies_float(Jn, cands.T, c), rtol=1e-12)

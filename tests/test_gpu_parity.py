@@ -239,6 +239,8 @@ def test_fp64_eigenpairs_relative_residual_1e10(gpu, monkeypatch, reorth):
         s, _, g = O.alpha_scaling(d, r["alpha"])
         rel = float(np.linalg.norm(O.n_matvec(Jn, s, y) - r["lambda1"] * y) / np.linalg.norm(y))
         assert rel <= 1e-10 * max(1.0, g), (r["alpha"], rel)
+        # the row reports the true residual the solver checked on the GPU
+        assert r["residual"] is not None and r["residual"] == pytest.approx(rel, rel=1e-3, abs=1e-14)
         w, V = np.linalg.eigh(O.dense_n(Jn, s))
         lam, yd, gap = w[-1], V[:, -1], w[-1] - w[-2]
         assert abs(r["lambda1"] - lam) <= 1e-11 * max(1.0, abs(lam))
