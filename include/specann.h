@@ -370,6 +370,21 @@ int sa_ls_init(const sa_csr* A, int32_t k, int32_t groups, const uint64_t* sb, i
 int sa_ls_class_pass(const sa_csr* A, int32_t k, int32_t groups, const int32_t* class_nodes, int64_t count,
                      uint64_t seed, int64_t pass, double frac, int8_t* spins, int32_t* fields, int32_t* flags,
                      hipStream_t stream);
+/* The whole search in one persistent (cooperative) launch: the schedule of flip_refinement
+ * (models/spectral_annealing.py:1039-1112; main passes over a subset_fraction share of the nodes until
+ * every candidate has had a pass without a flip, at most max_iterations, then num_rounds rounds of at
+ * most final_sweep_iters passes over a final_sweep_fraction share, each ending at a pass without flips)
+ * walked on the device, with the colour classes of pass q in the order of ascending
+ * (mix3(seed ^ 0x0DE5C1A55C0FFEE1, q, class), class) and selection ids: main pass t -> t, final
+ * round r -> 2^40 + r; each class as sa_ls_class_pass. class_nodes: the nodes grouped by class,
+ * class r = class_nodes[class_off[r] .. class_off[r + 1]) (1 <= num_classes <= 1024). ctl: a zeroed
+ * int32 workspace of sa_ls_search_ctl_ints(num_classes, groups) entries; afterwards ctl[4] = main
+ * passes, ctl[5] = all passes, ctl[7] != 0 = the search stopped on a grid-barrier timeout. */
+size_t sa_ls_search_ctl_ints(int32_t num_classes, int32_t groups);
+int sa_ls_search(const sa_csr* A, int32_t k, int32_t groups, const int32_t* class_nodes, const int32_t* class_off,
+                 int32_t num_classes, uint64_t seed, int32_t max_iterations, double subset_fraction, int32_t num_rounds,
+                 int32_t final_sweep_iters, double final_sweep_fraction, int8_t* spins, int32_t* fields, int32_t* ctl,
+                 hipStream_t stream);
 /* e2[c] = sum_i s_ic h_ic from maintained fields (zero e2 first). */
 int sa_ls_energy(int64_t n, int32_t k, int32_t groups, const int8_t* spins, const int32_t* fields, int64_t* e2,
                  hipStream_t stream);
@@ -384,6 +399,10 @@ int sa_ls_init_f64(const sa_csr* A, const float* fvals, int32_t k, int32_t group
 int sa_ls_class_pass_f64(const sa_csr* A, const float* fvals, int32_t k, int32_t groups, const int32_t* class_nodes,
                          int64_t count, uint64_t seed, int64_t pass, double frac, int8_t* spins, double* fields,
                          int32_t* flags, hipStream_t stream);
+int sa_ls_search_f64(const sa_csr* A, const float* fvals, int32_t k, int32_t groups, const int32_t* class_nodes,
+                     const int32_t* class_off, int32_t num_classes, uint64_t seed, int32_t max_iterations,
+                     double subset_fraction, int32_t num_rounds, int32_t final_sweep_iters,
+                     double final_sweep_fraction, int8_t* spins, double* fields, int32_t* ctl, hipStream_t stream);
 size_t sa_ls_energy_f64_scratch(int32_t groups);
 int sa_ls_energy_f64(int64_t n, int32_t k, int32_t groups, const int8_t* spins, const double* fields, double* part,
                      double* e2, hipStream_t stream);

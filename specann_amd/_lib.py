@@ -232,6 +232,11 @@ _SIGNATURES = {
     "sa_ls_init": (c_int32, [POINTER(SaCsr), c_int32, c_int32, _P, _P, _P, _P, _P]),
     "sa_ls_class_pass": (c_int32, [POINTER(SaCsr), c_int32, c_int32, _P, c_int64, c_uint64, c_int64, c_double, _P,
                                    _P, _P, _P]),
+    "sa_ls_search_ctl_ints": (c_size_t, [c_int32, c_int32]),
+    "sa_ls_search": (c_int32, [POINTER(SaCsr), c_int32, c_int32, _P, _P, c_int32, c_uint64, c_int32, c_double,
+                               c_int32, c_int32, c_double, _P, _P, _P, _P]),
+    "sa_ls_search_f64": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, _P, _P, c_int32, c_uint64, c_int32,
+                                   c_double, c_int32, c_int32, c_double, _P, _P, _P, _P]),
     "sa_ls_energy": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P, _P]),
     "sa_ls_init_f64": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, _P, _P, _P, _P]),
     "sa_ls_class_pass_f64": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, _P, c_int64, c_uint64, c_int64, c_double,

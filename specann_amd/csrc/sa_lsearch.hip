@@ -132,72 +132,324 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_init_kernel(int64_t n, const int3
 }
 
 // ---------------------------------------------------------------------------------------------
-// One colour class of one pass. blockIdx.y = group; a wave per class node.
+// One colour class of one pass.
 // ---------------------------------------------------------------------------------------------
 template <typename Src, typename F>
-__global__ __launch_bounds__(SA_BLOCK) void ls_class_kernel(const int32_t* __restrict__ indptr,
-                                                            const int32_t* __restrict__ indices,
-                                                            const Src* __restrict__ jint, int32_t k,
-                                                            int32_t groups, const int32_t* __restrict__ nodes,
-                                                            int64_t count, uint64_t seed, uint64_t pass,
-                                                            uint64_t thresh, int32_t all, int8_t* __restrict__ spins,
-                                                            F* __restrict__ fields, int32_t* __restrict__ flags) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int g = blockIdx.y;
+struct LsProblem {
+  const int32_t* __restrict__ indptr;
+  const int32_t* __restrict__ indices;
+  const Src* __restrict__ jval;
+  int32_t k, groups;
+  int8_t* __restrict__ spins;
+  F* __restrict__ fields;
+};
+
+// Spin and field accesses of the search: plain, or (kCoherent, for the persistent kernel, whose
+// workgroups exchange them without a kernel boundary) relaxed atomics at device scope, which are
+// coherent across the XCDs' L2 caches without the L2 write-back and invalidation a device-scope fence
+// costs.
+template <bool kCoherent, typename T>
+__device__ inline T ls_load(const T* p) {
+  if constexpr (kCoherent) return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return *p;
+}
+template <bool kCoherent, typename T>
+__device__ inline void ls_store(T* p, T v) {
+  if constexpr (kCoherent) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+
+// U nodes of one class for candidate group g, nodes[u0 + q * stride] for q < U (those below count),
+// their loads issued together: a selected node whose spin disagrees with the sign of its field flips
+// in the lanes c < k where it does, and its neighbours' fields take the change. No two nodes of a
+// class are adjacent and the field updates are exact, so the order of the nodes does not matter.
+// True in the lanes that flipped a node.
+template <bool kCoherent, typename Src, typename F, int U>
+__device__ inline bool class_step(const LsProblem<Src, F>& P, const int32_t* __restrict__ nodes, int64_t count,
+                                  int64_t u0, int64_t stride, int g, uint64_t seed, uint64_t sel_id, uint64_t thresh,
+                                  int32_t all, int lane) {
   const int c = g * 64 + lane;
-  const int64_t kp = (int64_t)groups * 64;
-  const int64_t nw = (int64_t)gridDim.x * SA_WAVES_PER_BLOCK;
+  const int64_t kp = (int64_t)P.groups * 64;
   bool any_flip = false;
-  // U class nodes per wave and iteration, their loads issued together. No two nodes of a class are
-  // adjacent and the field updates are integer atomics, so the order of the nodes does not matter.
-  constexpr int U = 4;
-  for (int64_t u0 = (int64_t)blockIdx.x * SA_WAVES_PER_BLOCK + wave; u0 < count; u0 += nw * U) {
-    int32_t node[U], s[U];
-    F h[U];
-    bool sel[U];
+  int32_t node[U], s[U];
+  F h[U];
+  bool sel[U];
 #pragma unroll
-    for (int q = 0; q < U; ++q) {
-      const int64_t u = u0 + q * nw;
-      node[q] = uniform(nodes[u < count ? u : count - 1]);
-      sel[q] = u < count && (all || (mix3(seed, pass, (uint64_t)node[q]) >> 11) < thresh);
-    }
+  for (int q = 0; q < U; ++q) {
+    const int64_t u = u0 + q * stride;
+    node[q] = uniform(nodes[u < count ? u : count - 1]);
+    sel[q] = u < count && (all || (mix3(seed, sel_id, (uint64_t)node[q]) >> 11) < thresh);
+  }
 #pragma unroll
-    for (int q = 0; q < U; ++q) {
-      const int64_t off = (int64_t)node[q] * kp + c;
-      s[q] = spins[off];
-      h[q] = fields[off];
-    }
+  for (int q = 0; q < U; ++q) {
+    const int64_t off = (int64_t)node[q] * kp + c;
+    s[q] = ls_load<kCoherent>(&P.spins[off]);
+    h[q] = ls_load<kCoherent>(&P.fields[off]);
+  }
 #pragma unroll
-    for (int q = 0; q < U; ++q) {
-      const bool flip = sel[q] && (c < k) && ((s[q] > 0) ? (h[q] < (F)0) : (h[q] > (F)0));
-      if (!__any(flip)) continue;
-      const int32_t i = node[q];
-      const int32_t snew = -s[q];
-      if (flip) spins[(int64_t)i * kp + c] = (int8_t)snew;
-      any_flip |= flip;
-      const int32_t e0 = uniform(indptr[i]), e1 = uniform(indptr[i + 1]);
-      int32_t e = e0;
-      for (; e + 8 <= e1; e += 8) {
-        int32_t j[8];
-        Src jv[8];
+  for (int q = 0; q < U; ++q) {
+    const bool flip = sel[q] && (c < P.k) && ((s[q] > 0) ? (h[q] < (F)0) : (h[q] > (F)0));
+    if (!__any(flip)) continue;
+    const int32_t i = node[q];
+    const int32_t snew = -s[q];
+    if (flip) ls_store<kCoherent>(&P.spins[(int64_t)i * kp + c], (int8_t)snew);
+    any_flip |= flip;
+    const int32_t e0 = uniform(P.indptr[i]), e1 = uniform(P.indptr[i + 1]);
+    int32_t e = e0;
+    for (; e + 8 <= e1; e += 8) {
+      int32_t j[8];
+      Src jv[8];
 #pragma unroll
-        for (int v = 0; v < 8; ++v) {
-          j[v] = uniform(indices[e + v]);
-          jv[v] = uniform_val(jint[e + v]);
-        }
-        if (flip) {
-#pragma unroll
-          for (int v = 0; v < 8; ++v) atomicAdd(&fields[(int64_t)j[v] * kp + c], (F)2 * (F)jv[v] * (F)snew);
-        }
+      for (int v = 0; v < 8; ++v) {
+        j[v] = uniform(P.indices[e + v]);
+        jv[v] = uniform_val(P.jval[e + v]);
       }
-      for (; e < e1; ++e) {
-        const int32_t j = uniform(indices[e]);
-        const Src jv = uniform_val(jint[e]);
-        if (flip) atomicAdd(&fields[(int64_t)j * kp + c], (F)2 * (F)jv * (F)snew);
+      if (flip) {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) atomicAdd(&P.fields[(int64_t)j[v] * kp + c], (F)2 * (F)jv[v] * (F)snew);
       }
+    }
+    for (; e < e1; ++e) {
+      const int32_t j = uniform(P.indices[e]);
+      const Src jv = uniform_val(P.jval[e]);
+      if (flip) atomicAdd(&P.fields[(int64_t)j * kp + c], (F)2 * (F)jv * (F)snew);
     }
   }
-  if (any_flip) flags[c] = 1;
+  return any_flip;
+}
+
+constexpr int kClassU = 4;  // class nodes per wave and step
+
+// blockIdx.y = group; the class's nodes are spread over the waves of the grid.
+template <typename Src, typename F>
+__global__ __launch_bounds__(SA_BLOCK) void ls_class_kernel(LsProblem<Src, F> P, const int32_t* __restrict__ nodes,
+                                                            int64_t count, uint64_t seed, uint64_t pass,
+                                                            uint64_t thresh, int32_t all, int32_t* __restrict__ flags) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = blockIdx.y;
+  const int64_t nw = (int64_t)gridDim.x * SA_WAVES_PER_BLOCK;
+  bool any_flip = false;
+  for (int64_t u0 = (int64_t)blockIdx.x * SA_WAVES_PER_BLOCK + wave; u0 < count; u0 += nw * kClassU)
+    any_flip |= class_step<false, Src, F, kClassU>(P, nodes, count, u0, nw, g, seed, pass, thresh, all, lane);
+  if (any_flip) flags[g * 64 + lane] = 1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// The whole search in one cooperative launch. The grid (one 1024-thread workgroup per CU) walks the
+// passes of the schedule (main passes, then the final rounds) and, in each, the colour classes in the
+// pass's order, with a grid barrier after every class; between passes workgroup 0 reads the
+// per-candidate flip flags, advances the schedule and writes the next pass's class order. Every
+// workgroup reads the schedule after a barrier, so all of them take the same branches and meet at the
+// same barriers. The barrier's wait is bounded in time: past the bound it records an error and every
+// workgroup returns.
+//
+// Everything the workgroups exchange (spins, fields, flags, schedule, order) is accessed with device-
+// scope atomics (the field updates already are), so a barrier needs only each wave's own operations
+// complete (s_waitcnt) before it arrives: no cache write-back or invalidation.
+//
+// ctl (int32, zeroed by the caller; 128-byte lines of 32 entries): line 0 the schedule ([0] phase,
+// [1] main pass index, [2] final round, [3] passes in the round, [4] main passes, [5] all passes,
+// [6] fixed point reached, [7] error: 1 another workgroup's, 2 barrier timeout); line 1 the barrier's
+// top arrival count, line 2 its generation, lines 3..10 the arrival counts of the eight workgroup
+// groups (b mod 8); from kCtlHead the class order, then the candidates still active in the main phase
+// (kp), then the flip flags of the pass (kp).
+// ---------------------------------------------------------------------------------------------
+constexpr int kCtlTop = 32, kCtlGen = 64, kCtlSub = 96;
+constexpr int kCtlHead = kCtlSub + 8 * 32;
+constexpr int kSearchThreads = 1024;
+constexpr int kSearchWaves = kSearchThreads / 64;
+constexpr int kMaxClasses = 1024;  // class-order keys are ranked in LDS (8 KB)
+constexpr int32_t kPhaseMain = 0, kPhaseFinal = 1, kPhaseDone = 2;
+constexpr uint64_t kOrderSalt = 0x0DE5C1A55C0FFEE1ull;
+constexpr uint64_t kFinalPassBase = 1ull << 40;
+
+struct LsSchedule {
+  const int32_t* __restrict__ class_nodes;
+  const int32_t* __restrict__ class_off;  // C + 1 offsets into class_nodes
+  int32_t C;
+  uint64_t seed;
+  int32_t max_iterations, num_rounds, final_iters;
+  uint64_t thresh_main, thresh_final;
+  int32_t all_main, all_final;
+  uint64_t timeout_ticks;  // of wall_clock64
+};
+
+__device__ inline int32_t ctl_load(const int32_t* p) { return ls_load<true>(p); }
+__device__ inline void ctl_store(int32_t* p, int32_t v) { ls_store<true>(p, v); }
+__device__ inline int32_t ctl_add(int32_t* p, int32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// This wave's memory operations complete, and the compiler keeps them on their side.
+__device__ inline void ls_drain() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Grid barrier over gridDim.x workgroups (all resident: cooperative launch), two-level: a workgroup
+// arrives at its group's count; the last of a group arrives at the top count; the last there opens
+// the next generation. False when the search stopped on an error (then the caller returns).
+__device__ bool ls_grid_sync(int32_t* ctl, uint64_t timeout_ticks) {
+  __shared__ int32_t err;
+  ls_drain();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int32_t g = ctl_load(ctl + kCtlGen);
+    const int grp = (int)(blockIdx.x & 7);
+    const int32_t members = (int32_t)((gridDim.x - grp + 7) / 8);
+    const int32_t ngroups = gridDim.x < 8 ? (int32_t)gridDim.x : 8;
+    int32_t* sub = ctl + kCtlSub + 32 * grp;
+    bool opened = false;
+    if (ctl_add(sub, 1) + 1 == members) {
+      ctl_store(sub, 0);
+      ls_drain();
+      if (ctl_add(ctl + kCtlTop, 1) + 1 == ngroups) {
+        ctl_store(ctl + kCtlTop, 0);
+        ls_drain();
+        ctl_store(ctl + kCtlGen, g + 1);
+        opened = true;
+      }
+    }
+    if (!opened) {
+      const uint64_t t0 = wall_clock64();
+      while (ctl_load(ctl + kCtlGen) == g) {
+        if (ctl_load(ctl + 7) != 0) break;
+        if (wall_clock64() - t0 > timeout_ticks) {
+          ctl_store(ctl + 7, 2);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+      }
+    }
+    err = ctl_load(ctl + 7);
+  }
+  __syncthreads();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  return err == 0;
+}
+
+// Position of every class in pass q's order (ascending (key, class)), by workgroup 0.
+__device__ void ls_write_order(const LsSchedule& S, uint64_t q, int32_t* order) {
+  __shared__ uint64_t keys[kMaxClasses];
+  for (int c = threadIdx.x; c < S.C; c += blockDim.x) keys[c] = mix3(S.seed ^ kOrderSalt, q, (uint64_t)c);
+  __syncthreads();
+  for (int c = threadIdx.x; c < S.C; c += blockDim.x) {
+    const uint64_t kc = keys[c];
+    int rank = 0;
+    for (int d = 0; d < S.C; ++d) {
+      const uint64_t kd = keys[d];
+      rank += (kd < kc || (kd == kc && d < c)) ? 1 : 0;
+    }
+    ctl_store(&order[rank], c);
+  }
+  __syncthreads();
+}
+
+// End of a pass (workgroup 0): the schedule step of the host loop it replaces, the flags cleared, and
+// the next pass's class order.
+__device__ void ls_control(const LsSchedule& S, int32_t k, int32_t kp, int32_t* ctl) {
+  int32_t* order = ctl + kCtlHead;
+  int32_t* active = order + S.C;
+  int32_t* flags = active + kp;
+  const int32_t phase = ctl_load(ctl + 0);
+  int anyf = 0, anya = 0;
+  for (int c = threadIdx.x; c < k; c += blockDim.x) {
+    const int32_t f = ctl_load(&flags[c]) != 0;
+    anyf |= f;
+    if (phase == kPhaseMain) {
+      const int32_t a = active[c] & f;
+      active[c] = a;
+      anya |= a;
+    }
+  }
+  anyf = __syncthreads_or(anyf);
+  anya = __syncthreads_or(anya);
+  for (int c = threadIdx.x; c < kp; c += blockDim.x) ctl_store(&flags[c], 0);
+  __shared__ int32_t next;
+  if (threadIdx.x == 0) {
+    int32_t ph = phase, it = ctl_load(ctl + 1), r = ctl_load(ctl + 2), iters = ctl_load(ctl + 3);
+    int32_t passes = ctl_load(ctl + 4), total = ctl_load(ctl + 5), fixed = ctl_load(ctl + 6);
+    const bool finals = S.num_rounds > 0 && S.final_iters > 0;
+    if (ph == kPhaseMain) {
+      ++passes;
+      ++it;
+      if (!anya || it >= S.max_iterations) {
+        ph = finals ? kPhaseFinal : kPhaseDone;
+        r = 0;
+        iters = 0;
+      }
+    } else {
+      ++iters;
+      bool end = false;
+      if (!anyf) {
+        fixed = S.all_final;  // a pass over every node without a flip is a fixed point of all later passes
+        end = true;
+      } else if (iters >= S.final_iters) {
+        end = true;
+      }
+      if (end) {
+        ++r;
+        iters = 0;
+        if (r >= S.num_rounds || fixed) ph = kPhaseDone;
+      }
+    }
+    ++total;
+    ctl_store(ctl + 0, ph);
+    ctl_store(ctl + 1, it);
+    ctl_store(ctl + 2, r);
+    ctl_store(ctl + 3, iters);
+    ctl_store(ctl + 4, passes);
+    ctl_store(ctl + 5, total);
+    ctl_store(ctl + 6, fixed);
+    next = ph == kPhaseDone ? -1 : total;
+  }
+  __syncthreads();
+  if (next >= 0) ls_write_order(S, (uint64_t)next, order);
+}
+
+template <typename Src, typename F>
+__global__ __launch_bounds__(kSearchThreads) void ls_search_kernel(LsProblem<Src, F> P, LsSchedule S, int32_t* ctl) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int32_t kp = P.groups * 64;
+  int32_t* order = ctl + kCtlHead;
+  int32_t* active = order + S.C;
+  int32_t* flags = active + kp;
+  if (blockIdx.x == 0) {
+    for (int c = threadIdx.x; c < kp; c += blockDim.x) active[c] = c < P.k ? 1 : 0;
+    if (threadIdx.x == 0)
+      ctl_store(ctl + 0, S.max_iterations > 0 ? kPhaseMain
+                                              : (S.num_rounds > 0 && S.final_iters > 0 ? kPhaseFinal : kPhaseDone));
+    ls_write_order(S, 0, order);
+  }
+  if (!ls_grid_sync(ctl, S.timeout_ticks)) return;
+  const int64_t nw = (int64_t)gridDim.x * kSearchWaves;
+  const int64_t gw = (int64_t)blockIdx.x * kSearchWaves + wave;
+  const int64_t max_passes = (int64_t)S.max_iterations + (int64_t)S.num_rounds * S.final_iters;
+  for (int64_t guard = 0; guard <= max_passes; ++guard) {
+    const int32_t phase = ctl_load(ctl + 0);
+    if (phase == kPhaseDone) break;
+    const bool main = phase == kPhaseMain;
+    const uint64_t sel_id = main ? (uint64_t)ctl_load(ctl + 1) : kFinalPassBase + (uint64_t)ctl_load(ctl + 2);
+    const uint64_t thresh = main ? S.thresh_main : S.thresh_final;
+    const int32_t all = main ? S.all_main : S.all_final;
+    // a barrier after every class, empty or not: the schedule and the order read above are rewritten
+    // only after the first of them
+    for (int pos = 0; pos < S.C; ++pos) {
+      const int32_t cls = ctl_load(&order[pos]);
+      const int32_t lo = S.class_off[cls], count = S.class_off[cls + 1] - lo;
+      const int32_t* nodes = S.class_nodes + lo;
+      for (int g = 0; g < P.groups; ++g) {
+        bool any_flip = false;
+        for (int64_t u0 = gw; u0 < count; u0 += nw * kClassU)
+          any_flip |= class_step<true, Src, F, kClassU>(P, nodes, count, u0, nw, g, S.seed, sel_id, thresh, all, lane);
+        if (any_flip) ctl_store(&flags[g * 64 + lane], 1);
+      }
+      if (!ls_grid_sync(ctl, S.timeout_ticks)) return;
+    }
+    if (blockIdx.x == 0) ls_control(S, P.k, kp, ctl);
+    if (!ls_grid_sync(ctl, S.timeout_ticks)) return;
+  }
 }
 
 __global__ __launch_bounds__(SA_BLOCK) void ls_energy_kernel(int64_t n, int32_t k, int32_t groups,
@@ -454,6 +706,60 @@ __global__ __launch_bounds__(SA_BLOCK) void colour_list_kernel(const int32_t* __
   }
 }
 
+constexpr uint64_t kSearchTimeoutSec = 30;  // longest wait at one grid barrier before the search gives up
+
+// The persistent search: one cooperative launch (every workgroup resident), one 1024-thread workgroup
+// per CU.
+template <typename Src, typename F>
+static int ls_search_launch(const LsProblem<Src, F>& P, const int32_t* class_nodes, const int32_t* class_off,
+                            int32_t num_classes, uint64_t seed, int32_t max_iterations, double subset_fraction,
+                            int32_t num_rounds, int32_t final_sweep_iters, double final_sweep_fraction, int32_t* ctl,
+                            hipStream_t stream, const char* what) {
+  SA_REQUIRE(class_nodes && class_off && ctl, "bad arguments");
+  SA_REQUIRE(P.k >= 1 && P.groups >= 1 && P.k <= 64 * P.groups && P.groups <= 1024, "bad candidate count");
+  SA_REQUIRE(num_classes >= 1 && num_classes <= kMaxClasses, "num_classes must be in [1, 1024]");
+  SA_REQUIRE(max_iterations >= 0 && num_rounds >= 0 && final_sweep_iters >= 0, "negative pass counts");
+  SA_REQUIRE(subset_fraction > 0.0 && subset_fraction <= 1.0 && final_sweep_fraction > 0.0 &&
+                 final_sweep_fraction <= 1.0,
+             "fractions must be in (0, 1]");
+  int dev = 0, coop = 0, cus = 0, rate = 0, per = 0;
+  const void* kf = reinterpret_cast<const void*>(&ls_search_kernel<Src, F>);
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ||
+      hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev) ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kf, kSearchThreads, 0)) {
+    set_error(std::string(what) + ": device query failed");
+    return SA_E_LAUNCH;
+  }
+  if (!coop || cus < 1 || rate <= 0 || per < 1) {
+    set_error(std::string(what) + ": the device cannot run the cooperative search kernel");
+    return SA_E_LAUNCH;
+  }
+  LsSchedule S{};
+  S.class_nodes = class_nodes;
+  S.class_off = class_off;
+  S.C = num_classes;
+  S.seed = seed;
+  S.max_iterations = max_iterations;
+  S.num_rounds = num_rounds;
+  S.final_iters = final_sweep_iters;
+  S.all_main = subset_fraction >= 1.0 ? 1 : 0;
+  S.all_final = final_sweep_fraction >= 1.0 ? 1 : 0;
+  S.thresh_main = (uint64_t)(subset_fraction * 9007199254740992.0);
+  S.thresh_final = (uint64_t)(final_sweep_fraction * 9007199254740992.0);
+  S.timeout_ticks = (uint64_t)rate * 1000ull * kSearchTimeoutSec;
+  LsProblem<Src, F> Pa = P;
+  void* args[] = {reinterpret_cast<void*>(&Pa), reinterpret_cast<void*>(&S), reinterpret_cast<void*>(&ctl)};
+  // one workgroup per CU (16 of its 32 wave slots, 8 KB of LDS): every workgroup stays resident while
+  // the others wait at a barrier
+  const hipError_t e = hipLaunchCooperativeKernel(kf, dim3(cus), dim3(kSearchThreads), args, 0, stream);
+  if (e != hipSuccess) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return SA_E_LAUNCH;
+  }
+  return check_launch(what);
+}
+
 }  // namespace sa
 
 using namespace sa;
@@ -496,9 +802,9 @@ int sa_ls_class_pass(const sa_csr* A, int32_t k, int32_t groups, const int32_t* 
   if (count <= 0) return SA_OK;
   const int32_t all = frac >= 1.0 ? 1 : 0;
   const uint64_t thresh = (uint64_t)(frac * 9007199254740992.0);
+  const LsProblem<int32_t, int32_t> P{A->indptr, A->indices, A->jint, k, groups, spins, fields};
   hipLaunchKernelGGL((ls_class_kernel<int32_t, int32_t>), dim3(grid_for(count, SA_WAVES_PER_BLOCK * 2, 8192), groups),
-                     dim3(SA_BLOCK), 0, stream, A->indptr, A->indices, A->jint, k, groups, class_nodes, count, seed,
-                     (uint64_t)pass, thresh, all, spins, fields, flags);
+                     dim3(SA_BLOCK), 0, stream, P, class_nodes, count, seed, (uint64_t)pass, thresh, all, flags);
   return check_launch("ls_class_pass");
 }
 
@@ -521,10 +827,35 @@ int sa_ls_class_pass_f64(const sa_csr* A, const float* fvals, int32_t k, int32_t
   if (count <= 0) return SA_OK;
   const int32_t all = frac >= 1.0 ? 1 : 0;
   const uint64_t thresh = (uint64_t)(frac * 9007199254740992.0);
+  const LsProblem<float, double> P{A->indptr, A->indices, fvals, k, groups, spins, fields};
   hipLaunchKernelGGL((ls_class_kernel<float, double>), dim3(grid_for(count, SA_WAVES_PER_BLOCK * 2, 8192), groups),
-                     dim3(SA_BLOCK), 0, stream, A->indptr, A->indices, fvals, k, groups, class_nodes, count, seed,
-                     (uint64_t)pass, thresh, all, spins, fields, flags);
+                     dim3(SA_BLOCK), 0, stream, P, class_nodes, count, seed, (uint64_t)pass, thresh, all, flags);
   return check_launch("ls_class_pass_f64");
+}
+
+size_t sa_ls_search_ctl_ints(int32_t num_classes, int32_t groups) {
+  if (num_classes < 1 || groups < 1) return 0;
+  return (size_t)kCtlHead + (size_t)num_classes + 2 * (size_t)groups * 64;
+}
+
+int sa_ls_search(const sa_csr* A, int32_t k, int32_t groups, const int32_t* class_nodes, const int32_t* class_off,
+                 int32_t num_classes, uint64_t seed, int32_t max_iterations, double subset_fraction, int32_t num_rounds,
+                 int32_t final_sweep_iters, double final_sweep_fraction, int8_t* spins, int32_t* fields, int32_t* ctl,
+                 hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->jint && spins && fields, "bad arguments");
+  const LsProblem<int32_t, int32_t> P{A->indptr, A->indices, A->jint, k, groups, spins, fields};
+  return ls_search_launch(P, class_nodes, class_off, num_classes, seed, max_iterations, subset_fraction, num_rounds,
+                          final_sweep_iters, final_sweep_fraction, ctl, stream, "ls_search");
+}
+
+int sa_ls_search_f64(const sa_csr* A, const float* fvals, int32_t k, int32_t groups, const int32_t* class_nodes,
+                     const int32_t* class_off, int32_t num_classes, uint64_t seed, int32_t max_iterations,
+                     double subset_fraction, int32_t num_rounds, int32_t final_sweep_iters,
+                     double final_sweep_fraction, int8_t* spins, double* fields, int32_t* ctl, hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && fvals && spins && fields, "bad arguments");
+  const LsProblem<float, double> P{A->indptr, A->indices, fvals, k, groups, spins, fields};
+  return ls_search_launch(P, class_nodes, class_off, num_classes, seed, max_iterations, subset_fraction, num_rounds,
+                          final_sweep_iters, final_sweep_fraction, ctl, stream, "ls_search_f64");
 }
 
 size_t sa_ls_energy_f64_scratch(int32_t groups) { return (size_t)kEnergyGrid * (size_t)groups * 64 * sizeof(double); }
