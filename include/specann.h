@@ -421,6 +421,59 @@ int sa_ls_extract_f64(int64_t n, int32_t groups, int32_t c, const int8_t* spins,
 int sa_colour_start(const sa_csr* A, int32_t* colour, int32_t* lists, int32_t* counts, hipStream_t stream);
 int sa_colour_rounds(const sa_csr* A, uint64_t seed, int32_t* colour, int32_t* lists, int32_t* counts,
                      int32_t first_round, int32_t rounds, hipStream_t stream);
+/* The same rounds in largest-first order: degree, then the smaller index
+ * (greedy_largest_first_coloring, models/local_search_guards/common.py:136-159). */
+int sa_colour_rounds_lf(const sa_csr* A, int32_t* colour, int32_t* lists, int32_t* counts, int32_t first_round,
+                        int32_t rounds, hipStream_t stream);
+
+/* ---- Guarded local-search policies (ls_policy, models/local_search_guards/policies.py:13-52) -------
+ * Candidates in the local search's layout (spins int8 [n][kp], exact fields [n][kp], kp = 64 groups);
+ * integer couplings (J, int32 fields, int64 energies) or, in the _f64 forms, the FP32 couplings J / c
+ * (fvals) with FP64 fields and energies in units of J / c.
+ *
+ * Fixed / Adaptive Guard (safe_flip_refinement, models/spectral_annealing_safe.py:1035-1335; one round
+ * of _run_guarded_phase, :223-797). sa_guard_propose: node i is selected for candidate c iff
+ * (mix3(seed, round, i) >> 11) < thresh[c] (kp uint64; 0 = inactive); a selected node whose spin
+ * disagrees with the sign of its field flips (Jacobi: judged on the current spins). mask[i * groups + g]
+ * bit l = candidate 64 g + l flips i; counts[c] += proposed flips; delta[c] += the exact energy change
+ * of the proposal, sum_F 2 s_i h_i - 2 sum_F s_i sum_{j in F, j ~ i} J_ij s_j (zero counts and delta
+ * first). sa_guard_commit: the proposal's flips for the candidates of accepted[g] (bit l), with the
+ * field updates. */
+int sa_guard_propose(const sa_csr* A, int32_t k, int32_t groups, const int8_t* spins, const int32_t* fields,
+                     uint64_t seed, uint64_t round, const uint64_t* thresh, uint64_t* mask, int32_t* counts,
+                     int64_t* delta, hipStream_t stream);
+int sa_guard_propose_f64(const sa_csr* A, const float* fvals, int32_t k, int32_t groups, const int8_t* spins,
+                         const double* fields, uint64_t seed, uint64_t round, const uint64_t* thresh, uint64_t* mask,
+                         int32_t* counts, double* delta, hipStream_t stream);
+int sa_guard_commit(const sa_csr* A, int32_t groups, int8_t* spins, int32_t* fields, const uint64_t* mask,
+                    const uint64_t* accepted, hipStream_t stream);
+int sa_guard_commit_f64(const sa_csr* A, const float* fvals, int32_t groups, int8_t* spins, double* fields,
+                        const uint64_t* mask, const uint64_t* accepted, hipStream_t stream);
+/* Independent-Set (conflict_free.py:82-296): one colour class; node i flips candidate c iff c is
+ * active (active[g] bit l) and s h < -thr[c]; flips[c] and delta[c] (the change sum 2 s h) accumulate.
+ * sa_guard_class_prep, before each step: energy[c] += delta[c], sweep[c] += flips[c], both cleared, and
+ * thr[c] from the tolerance abs_tol + rel_tol max(1, |E / scale|) of the new energy
+ * (safe_update_policy.py:4-27): floor(tol scale / 2) for integer fields (scale = c), tol / 2 in FP64. */
+int sa_guard_class_step(const sa_csr* A, int32_t groups, const int32_t* class_nodes, int64_t count,
+                        const uint64_t* active, const int32_t* thr, int8_t* spins, int32_t* fields, int32_t* flips,
+                        int64_t* delta, hipStream_t stream);
+int sa_guard_class_step_f64(const sa_csr* A, const float* fvals, int32_t groups, const int32_t* class_nodes,
+                            int64_t count, const uint64_t* active, const double* thr, int8_t* spins, double* fields,
+                            int32_t* flips, double* delta, hipStream_t stream);
+int sa_guard_class_prep(int32_t groups, double scale, double abs_tol, double rel_tol, int64_t* energy,
+                        int64_t* delta, int32_t* flips, int32_t* sweep, int32_t* thr, hipStream_t stream);
+int sa_guard_class_prep_f64(int32_t groups, double abs_tol, double rel_tol, double* energy, double* delta,
+                            int32_t* flips, int32_t* sweep, double* thr, hipStream_t stream);
+/* Sequential-Oracle (sequential_oracle.py:22-121): one 1024-thread workgroup per candidate group runs
+ * exact one-flip steepest descent (each step, every candidate flips its node of smallest s h, the first
+ * on ties, while 2 s h < -tol of its energy). energy[c] (-sum s h / 2, kept in place), steps[g] = steps
+ * with a flip, or -1 if improving flips remain after max_steps; flips[c] = its flips. */
+int sa_guard_sequential(const sa_csr* A, int32_t k, int32_t groups, int8_t* spins, int32_t* fields, int64_t* energy,
+                        double scale, double abs_tol, double rel_tol, int64_t max_steps, int64_t* steps,
+                        int32_t* flips, hipStream_t stream);
+int sa_guard_sequential_f64(const sa_csr* A, const float* fvals, int32_t k, int32_t groups, int8_t* spins,
+                            double* fields, double* energy, double abs_tol, double rel_tol, int64_t max_steps,
+                            int64_t* steps, int32_t* flips, hipStream_t stream);
 
 /* ---- Eigenvector tangent and its adjoint: deflated MINRES -----------------------------------
  * No counterpart in the reference (a new predictor beside utils/predictors.py:6-56). For the unit

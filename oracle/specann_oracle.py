@@ -650,3 +650,208 @@ def cpu_sweep(J: sp.spmatrix, *, precision: int = 32, num_alphas: int = 128, sch
                 best_e, best_s = r["energy"], cands[r["candidate"]][0]
     t_end = time.time()
     return SweepResult(best_e, best_s, rows, cands, t_sweep - t0, t_end - t_sweep, matvecs[0])
+
+
+# ---------------------------------------------------------------------------------------------
+# Guarded local-search policies (models/local_search_guards/*, models/spectral_annealing_safe.py)
+# ---------------------------------------------------------------------------------------------
+GUARD_SALT = 0x6A09E667F3BCC908
+
+# local_search_guards/policies.py:21-37 (the settings the policies run with)
+GUARD_SETTINGS = dict(subset_fraction=0.9, final_sweep_fraction=1.0, final_sweep_iters=25, final_sweep_rounds=25,
+                      absolute_tolerance=1e-10, relative_tolerance=1e-12, safe_patience=3,
+                      minimum_subset_fraction=0.01, batch_shrink_factor=0.5, batch_growth_factor=1.25,
+                      oracle_max_steps_multiplier=100)
+
+
+@dataclass
+class GuardResult:
+    spins: np.ndarray          # k x n int8
+    iterations: int            # the policy's reported count (policies.py:47-49)
+    energies: np.ndarray       # final energies: int64 (integer couplings) or float64 (units of J / c)
+    energies_start: np.ndarray
+    extra: dict = field(default_factory=dict)
+
+
+def _guard_setup(J: sp.spmatrix, cand: np.ndarray):
+    """(couplings, S, H, E, exact): integer couplings exactly, others as FP32 values in FP64; S k x n,
+    H = fields k x n, E = -sum S H / 2 per candidate."""
+    J = sp.csr_matrix(J)
+    exact = J.dtype.kind in "iu" or bool(np.all(J.data == np.rint(J.data)))
+    if exact:
+        Jc = sp.csr_matrix((J.data.astype(np.int64), J.indices, J.indptr), shape=J.shape)
+        S = np.atleast_2d(cand).astype(np.int64).copy()
+    else:
+        Jc = sp.csr_matrix((J.data.astype(np.float32).astype(np.float64), J.indices, J.indptr), shape=J.shape)
+        S = np.atleast_2d(cand).astype(np.float64).copy()
+    H = np.asarray((Jc @ S.T).T)
+    e2 = (S * H).sum(axis=1)
+    E = -(e2 // 2) if exact else -0.5 * e2
+    return Jc, S, H, E, exact
+
+
+def guard_tolerance(E: np.ndarray, scale: float, abs_tol: float, rel_tol: float) -> np.ndarray:
+    """models/safe_update_policy.py:19-23 on the normalised energy E / scale."""
+    En = np.asarray(E, dtype=np.float64) / scale
+    return abs_tol + rel_tol * np.maximum(1.0, np.abs(En))
+
+
+def _flip_threshold(E, scale, abs_tol, rel_tol, exact):
+    """A flip (change 2 s h) passes iff s h < -thr: thr = tol scale / 2, floored for integer fields."""
+    x = guard_tolerance(E, scale, abs_tol, rel_tol) * scale * 0.5
+    return np.minimum(np.floor(x), 2147483647.0) if exact else x
+
+
+def guard_safe(J: sp.spmatrix, cand: np.ndarray, seed: int, *, adaptive: bool, scale: float = 1.0,
+               max_iterations: int = 1000, subset_fraction: float = 0.9, final_sweep_fraction: float = 1.0,
+               final_sweep_iters: int = 25, final_sweep_rounds: int = 25, absolute_tolerance: float = 1e-10,
+               relative_tolerance: float = 1e-12, patience: int = 3, minimum_subset_fraction: float = 0.01,
+               batch_shrink_factor: float = 0.5, batch_growth_factor: float = 1.25) -> GuardResult:
+    """Fixed / Adaptive Guard (safe_flip_refinement, models/spectral_annealing_safe.py:1035-1335; phases
+    :223-797; acceptance and adaptation models/safe_update_policy.py:4-95 with the per-candidate scope
+    safe_common.py:16-44 uses). Each round, candidate c proposes the Jacobi flips of the nodes selected by
+    (mix3(seed ^ GUARD_SALT, round, i) >> 11) < floor(f_c 2^53) (a hashed share f_c of the nodes in place
+    of the reference's random permutation prefix) whose spin disagrees with the sign of its field; the
+    proposal is kept iff its exact energy change is below minus the tolerance of the current energy.
+    Main phase: max_iterations rounds at subset_fraction; final phase: final_sweep_iters *
+    final_sweep_rounds rounds at final_sweep_fraction; each phase starts with every candidate active
+    and leaves when none is (patience). scale: c, the normalisation of the reference's energies."""
+    Jc, S, H, E, exact = _guard_setup(J, cand)
+    E0 = E.copy()
+    k, n = S.shape
+    gseed = u64(int(seed) ^ GUARD_SALT)
+    nodes = np.arange(n, dtype=np.uint64)
+    rid = 0
+    rounds_done = {}
+    for phase, rounds, f0 in (("main", int(max_iterations), float(subset_fraction)),
+                              ("final", int(final_sweep_iters) * int(final_sweep_rounds), float(final_sweep_fraction))):
+        done = 0
+        if rounds > 0:
+            active = np.ones(k, dtype=bool)
+            streak = np.zeros(k, dtype=np.int64)
+            frac = np.full(k, f0)
+            for _ in range(rounds):
+                if not active.any():
+                    break
+                hsel = mix3(gseed, u64(rid), nodes) >> np.uint64(11)
+                thresh = np.array([int(f * TWO53) if a else 0 for f, a in zip(frac, active)], dtype=np.uint64)
+                flip = (hsel[None, :] < thresh[:, None]) & ((S * H) < 0)
+                counts = flip.sum(axis=1)
+                T = np.where(flip, S, 0)
+                inner = np.asarray((Jc @ T.T).T)
+                dE = (2 * T * H).sum(axis=1) - (2 * T * inner).sum(axis=1)
+                has = active & (counts > 0)
+                tol = guard_tolerance(E, scale, absolute_tolerance, relative_tolerance)
+                accepted = has & (np.asarray(dE, dtype=np.float64) / scale < -tol)
+                rejected = has & ~accepted
+                no_flips = active & ~has
+                if accepted.any():
+                    S[accepted] = np.where(flip[accepted], -S[accepted], S[accepted])
+                    H = np.asarray((Jc @ S.T).T)
+                    E = E + np.where(accepted, dE, 0)
+                if adaptive:
+                    newf = np.where(rejected, np.maximum(minimum_subset_fraction, frac * batch_shrink_factor), frac)
+                    newf = np.where(accepted, np.minimum(f0, frac * batch_growth_factor), newf)
+                    shrunk = rejected & (newf < frac)
+                    stalled = no_flips | (rejected & ~shrunk)
+                    frac = newf
+                else:
+                    stalled = active & ~accepted
+                streak = np.where(active, np.where(accepted, 0, np.where(stalled, streak + 1, 0)), streak)
+                active = active & (streak < patience)
+                rid += 1
+                done += 1
+        rounds_done[phase] = done
+    return GuardResult(np.rint(S).astype(np.int8), rounds_done["main"], E, E0, dict(final_rounds=rounds_done["final"]))
+
+
+def largest_first_colouring(indptr: np.ndarray, indices: np.ndarray) -> np.ndarray:
+    """greedy_largest_first_coloring (local_search_guards/common.py:136-159): nodes by descending degree,
+    then ascending index; each takes the smallest colour no coloured neighbour has."""
+    n = indptr.size - 1
+    deg = np.diff(indptr)
+    order = np.lexsort((np.arange(n), -deg))
+    colours = np.full(n, -1, dtype=np.int64)
+    for v in order:
+        nb = indices[indptr[v]:indptr[v + 1]]
+        used = set(int(u) for u in colours[nb] if u >= 0)
+        c = 0
+        while c in used:
+            c += 1
+        colours[v] = c
+    return colours.astype(np.int32)
+
+
+def guard_independent_set(J: sp.spmatrix, cand: np.ndarray, seed: int, *, scale: float = 1.0,
+                          max_updates: int = 1000 + 625, absolute_tolerance: float = 1e-10,
+                          relative_tolerance: float = 1e-12) -> GuardResult:
+    """Independent-Set (conflict_free.py:82-296): colour classes of the largest-first colouring visited
+    in np.random.default_rng(seed).permutation order, a new permutation per sweep; in a step every
+    active candidate flips the class's nodes whose single-flip change 2 s h is below minus the
+    tolerance of its current energy; after a sweep without a flip a candidate is certified (inactive).
+    At most max_updates steps; iterations = steps run (kernel_updates)."""
+    Jc, S, H, E, exact = _guard_setup(J, cand)
+    E0 = E.copy()
+    k, n = S.shape
+    J = sp.csr_matrix(J)
+    colours = largest_first_colouring(J.indptr, J.indices)
+    C = int(colours.max()) + 1
+    classes = [np.flatnonzero(colours == q) for q in range(C)]
+    rng = np.random.default_rng(int(seed))
+    order = rng.permutation(C)
+    pos = 0
+    active = np.ones(k, dtype=bool)
+    sweep = np.zeros(k, dtype=np.int64)
+    steps = 0
+    for _ in range(int(max_updates)):
+        if not active.any():
+            break
+        nodes = classes[int(order[pos])]
+        thr = _flip_threshold(E, scale, absolute_tolerance, relative_tolerance, exact)
+        V = S[:, nodes] * H[:, nodes]
+        flip = active[:, None] & (V < -thr[:, None])
+        E = E + (2 * V * flip).sum(axis=1)
+        sweep += flip.sum(axis=1)
+        if flip.any():
+            S[:, nodes] = np.where(flip, -S[:, nodes], S[:, nodes])
+            H = np.asarray((Jc @ S.T).T)
+        steps += 1
+        pos += 1
+        if pos == C:
+            active &= ~(sweep == 0)
+            sweep[:] = 0
+            order = rng.permutation(C)
+            pos = 0
+    return GuardResult(np.rint(S).astype(np.int8), steps, E, E0, dict(colours=colours))
+
+
+def guard_sequential(J: sp.spmatrix, cand: np.ndarray, *, scale: float = 1.0, max_steps: int | None = None,
+                     absolute_tolerance: float = 1e-10, relative_tolerance: float = 1e-12) -> GuardResult:
+    """Sequential-Oracle (sequential_oracle.py:22-121): each step every candidate flips its node of
+    smallest s_i h_i (first on ties) while 2 s h is below minus the tolerance of its energy, until none
+    improves (a one-flip local optimum). iterations = steps with a flip; converged False if flips still
+    improve after max_steps (default 100 n) steps."""
+    Jc, S, H, E, exact = _guard_setup(J, cand)
+    E0 = E.copy()
+    k, n = S.shape
+    max_steps = 100 * n if max_steps is None else int(max_steps)
+    steps = 0
+    converged = False
+    rows = np.arange(k)
+    while True:
+        V = S * H
+        best = np.argmin(V, axis=1)
+        bv = V[rows, best]
+        thr = _flip_threshold(E, scale, absolute_tolerance, relative_tolerance, exact)
+        improving = bv < -thr
+        if not improving.any():
+            converged = True
+            break
+        if steps == max_steps:
+            break
+        for c in np.flatnonzero(improving):
+            S[c, best[c]] = -S[c, best[c]]
+            E[c] = E[c] + 2 * bv[c]
+        H = np.asarray((Jc @ S.T).T)
+        steps += 1
+    return GuardResult(np.rint(S).astype(np.int8), steps, E, E0, dict(converged=converged))

@@ -247,6 +247,21 @@ _SIGNATURES = {
     "sa_ls_extract_f64": (c_int32, [c_int64, c_int32, c_int32, _P, _P, _P]),
     "sa_colour_start": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P]),
     "sa_colour_rounds": (c_int32, [POINTER(SaCsr), c_uint64, _P, _P, _P, c_int32, c_int32, _P]),
+    "sa_colour_rounds_lf": (c_int32, [POINTER(SaCsr), _P, _P, _P, c_int32, c_int32, _P]),
+    "sa_guard_propose": (c_int32, [POINTER(SaCsr), c_int32, c_int32, _P, _P, c_uint64, c_uint64, _P, _P, _P, _P,
+                                   _P]),
+    "sa_guard_propose_f64": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, _P, _P, c_uint64, c_uint64, _P, _P,
+                                       _P, _P, _P]),
+    "sa_guard_commit": (c_int32, [POINTER(SaCsr), c_int32, _P, _P, _P, _P, _P]),
+    "sa_guard_commit_f64": (c_int32, [POINTER(SaCsr), _P, c_int32, _P, _P, _P, _P, _P]),
+    "sa_guard_class_step": (c_int32, [POINTER(SaCsr), c_int32, _P, c_int64, _P, _P, _P, _P, _P, _P, _P]),
+    "sa_guard_class_step_f64": (c_int32, [POINTER(SaCsr), _P, c_int32, _P, c_int64, _P, _P, _P, _P, _P, _P, _P]),
+    "sa_guard_class_prep": (c_int32, [c_int32, c_double, c_double, c_double, _P, _P, _P, _P, _P, _P]),
+    "sa_guard_class_prep_f64": (c_int32, [c_int32, c_double, c_double, _P, _P, _P, _P, _P, _P]),
+    "sa_guard_sequential": (c_int32, [POINTER(SaCsr), c_int32, c_int32, _P, _P, _P, c_double, c_double, c_double,
+                                      c_int64, _P, _P, _P]),
+    "sa_guard_sequential_f64": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, _P, _P, _P, c_double, c_double,
+                                          c_int64, _P, _P, _P]),
     "sa_diag_scale_f32": (c_int32, [c_int64, _P, _P, c_double, _P, _P]),
     "sa_diag_scale_f64": (c_int32, [c_int64, _P, _P, c_double, _P, _P]),
     "sa_pack_spins": (c_int32, [c_int64, c_int32, _P, c_int64, _P, _P]),

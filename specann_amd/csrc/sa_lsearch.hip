@@ -517,10 +517,13 @@ __global__ __launch_bounds__(SA_BLOCK) void ls_extract_kernel(int64_t n, int32_t
 // Greedy colouring over a worklist (see the header comment of sa_colour_start). Priority order:
 // degree, then hash(seed, i), then i, all descending; the hash is computed only on degree ties.
 // ---------------------------------------------------------------------------------------------
-// hi caches hash(seed, i): computed on the first degree tie only (hi_ready).
+// hi caches hash(seed, i): computed on the first degree tie only (hi_ready). kIndexTies: ties of degree
+// go to the smaller index instead (largest-first order, local_search_guards/common.py:136-159).
+template <bool kIndexTies>
 __device__ inline bool outranks(uint64_t seed, int32_t j, int32_t dj, int32_t i, int32_t di, uint64_t& hi,
                                 bool& hi_ready) {
   if (dj != di) return dj > di;
+  if constexpr (kIndexTies) return j < i;
   if (!hi_ready) {
     hi = mix3(seed, 0xC0102ull, (uint64_t)i);
     hi_ready = true;
@@ -564,6 +567,7 @@ __device__ inline uint64_t wave_or(uint64_t v) {
 // neighbour from resume[i] on that blocks it (uncoloured and of higher priority) becomes resume[i] and
 // the node waits (true); otherwise it takes the smallest colour none of its neighbours has (false).
 // The same outcome as the one-lane scan, with 64 neighbours per step.
+template <bool kIndexTies>
 __device__ inline bool colour_node_wave(const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
                                         uint64_t seed, int32_t* colour, int32_t* __restrict__ resume, int32_t i,
                                         int32_t r, int lane) {
@@ -591,7 +595,8 @@ __device__ inline bool colour_node_wave(const int32_t* __restrict__ indptr, cons
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int32_t e = b + u * 64 + lane;
-      const bool blk = e < e1 && j[u] != i && cj[u] < 0 && outranks(seed, j[u], dj[u], i, di, hi, hi_ready);
+      const bool blk =
+          e < e1 && j[u] != i && cj[u] < 0 && outranks<kIndexTies>(seed, j[u], dj[u], i, di, hi, hi_ready);
       const uint64_t bm = __ballot(blk);
       if (bm) {
         if (lane == 0) resume[i] = b + u * 64 + (int32_t)__builtin_ctzll(bm);
@@ -620,6 +625,7 @@ __device__ inline bool colour_node_wave(const int32_t* __restrict__ indptr, cons
   return false;
 }
 
+template <bool kIndexTies>
 __global__ __launch_bounds__(SA_BLOCK) void colour_list_kernel(const int32_t* __restrict__ indptr,
                                                                const int32_t* __restrict__ indices, uint64_t seed,
                                                                int32_t* colour, const int32_t* __restrict__ in,
@@ -659,7 +665,8 @@ __global__ __launch_bounds__(SA_BLOCK) void colour_list_kernel(const int32_t* __
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            if (!pending && e + u < e1 && j[u] != i && cj[u] < 0 && outranks(seed, j[u], dj[u], i, di, hi, hi_ready)) {
+            if (!pending && e + u < e1 && j[u] != i && cj[u] < 0 &&
+                outranks<kIndexTies>(seed, j[u], dj[u], i, di, hi, hi_ready)) {
               pending = true;
               resume[i] = e + u;
             }
@@ -693,7 +700,7 @@ __global__ __launch_bounds__(SA_BLOCK) void colour_list_kernel(const int32_t* __
       const int l = __builtin_ctzll(hm);
       hm &= hm - 1;
       const int32_t hn = __shfl(i, l);
-      const bool hp = colour_node_wave(indptr, indices, seed, colour, resume, hn, r, lane);
+      const bool hp = colour_node_wave<kIndexTies>(indptr, indices, seed, colour, resume, hn, r, lane);
       if (lane == l) pending = hp;
     }
     const uint64_t m = __ballot(pending);
@@ -704,6 +711,24 @@ __global__ __launch_bounds__(SA_BLOCK) void colour_list_kernel(const int32_t* __
       if (pending) out[at + (int32_t)__popcll(m & ((1ull << lane) - 1))] = i;
     }
   }
+}
+
+template <bool kIndexTies>
+static int colour_rounds(const sa_csr* A, uint64_t seed, int32_t* colour, int32_t* lists, int32_t* counts,
+                         int32_t first_round, int32_t rounds, hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && colour && lists && counts, "bad arguments");
+  SA_REQUIRE(first_round >= 0 && rounds >= 0, "bad round numbers");
+  const int64_t n = A->n;
+  const int G = grid_for(n, SA_BLOCK * 4, SA_RED_GRID);
+  for (int32_t q = 0; q < rounds; ++q) {
+    const int32_t r = first_round + q;
+    hipLaunchKernelGGL(colour_list_kernel<kIndexTies>, dim3(G), dim3(SA_BLOCK), 0, stream, A->indptr, A->indices,
+                       seed, colour, (const int32_t*)(lists + (r & 1) * n), lists + ((r + 1) & 1) * n,
+                       lists + 2 * n, counts, r);
+    const int rc = check_launch("colour_rounds");
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 constexpr uint64_t kSearchTimeoutSec = 30;  // longest wait at one grid barrier before the search gives up
@@ -903,18 +928,12 @@ int sa_colour_start(const sa_csr* A, int32_t* colour, int32_t* lists, int32_t* c
 
 int sa_colour_rounds(const sa_csr* A, uint64_t seed, int32_t* colour, int32_t* lists, int32_t* counts,
                      int32_t first_round, int32_t rounds, hipStream_t stream) {
-  SA_REQUIRE(A && A->n > 0 && A->indptr && A->indices && colour && lists && counts, "bad arguments");
-  SA_REQUIRE(first_round >= 0 && rounds >= 0, "bad round numbers");
-  const int64_t n = A->n;
-  const int G = grid_for(n, SA_BLOCK * 4, SA_RED_GRID);
-  for (int32_t q = 0; q < rounds; ++q) {
-    const int32_t r = first_round + q;
-    hipLaunchKernelGGL(colour_list_kernel, dim3(G), dim3(SA_BLOCK), 0, stream, A->indptr, A->indices, seed, colour,
-                       (const int32_t*)(lists + (r & 1) * n), lists + ((r + 1) & 1) * n, lists + 2 * n, counts, r);
-    const int rc = check_launch("colour_rounds");
-    if (rc) return rc;
-  }
-  return 0;
+  return colour_rounds<false>(A, seed, colour, lists, counts, first_round, rounds, stream);
+}
+
+int sa_colour_rounds_lf(const sa_csr* A, int32_t* colour, int32_t* lists, int32_t* counts, int32_t first_round,
+                        int32_t rounds, hipStream_t stream) {
+  return colour_rounds<true>(A, 0, colour, lists, counts, first_round, rounds, stream);
 }
 
 }  // extern "C"

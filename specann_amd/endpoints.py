@@ -16,6 +16,7 @@ from datetime import datetime
 import numpy as np
 import torch
 
+from . import guards
 from . import local_search as LS
 from ._lib import SA_BROKEN, SA_CONVERGED, EigensolverBreakdown, call
 from .device import DeviceProblem, ptr, stream
@@ -60,8 +61,6 @@ class _Endpoint:
 
     def solve(self, J):
         """Solve the instance with couplings J. Returns (energy, spins, time in seconds)."""
-        if self.ls_policy != "default":
-            raise NotImplementedError(f"ls_policy {self.ls_policy!r}: the guarded policies are not part of this build")
         start = time.time()
         prob = DeviceProblem(J, self.tdtype)
         n = prob.n
@@ -97,7 +96,11 @@ class _Endpoint:
         spins = None
         local_iterations = 0
         if self.use_local_search:
-            st = LS.flip_refinement(prob, bits, 1, max_iterations=self.max_local_iterations, seed=self._seed())
+            if self.ls_policy == "default":
+                st = LS.flip_refinement(prob, bits, 1, max_iterations=self.max_local_iterations, seed=self._seed())
+            else:  # refine_candidates (models/spec_adj.py:125-133)
+                st = guards.refine(self.ls_policy, prob, bits, 1, max_iterations=self.max_local_iterations,
+                                   seed=self._seed())
             local_iterations = st.passes
             refined = float(st.energies_final[0])
             if refined < energy:

@@ -74,9 +74,10 @@ class Colouring:
 COLOUR_BATCH = 8  # rounds queued between checks of the worklist count
 
 
-def colour_graph(prob: DeviceProblem, seed: int) -> Colouring:
-    """Greedy colouring on the GPU (descending degree, then hash, then index) in rounds over a
-    shrinking worklist of uncoloured nodes; rounds past the last one do nothing."""
+def colour_graph(prob: DeviceProblem, seed: int, largest_first: bool = False) -> Colouring:
+    """Greedy colouring on the GPU (descending degree, then hash, then index; largest_first: degree,
+    then the smaller index, as local_search_guards/common.py:136-159) in rounds over a shrinking
+    worklist of uncoloured nodes; rounds past the last one do nothing."""
     n = prob.n
     colours = torch.empty(n, dtype=torch.int32, device=prob.device)
     lists = torch.empty(3 * n, dtype=torch.int32, device=prob.device)  # two worklists, resume points
@@ -84,8 +85,12 @@ def colour_graph(prob: DeviceProblem, seed: int) -> Colouring:
     call("sa_colour_start", prob.csr_ref, ptr(colours), ptr(lists), ptr(counts), stream())
     rounds = 0
     while True:
-        call("sa_colour_rounds", prob.csr_ref, int(seed) & _M, ptr(colours), ptr(lists), ptr(counts), rounds,
-             COLOUR_BATCH, stream())
+        if largest_first:
+            call("sa_colour_rounds_lf", prob.csr_ref, ptr(colours), ptr(lists), ptr(counts), rounds, COLOUR_BATCH,
+                 stream())
+        else:
+            call("sa_colour_rounds", prob.csr_ref, int(seed) & _M, ptr(colours), ptr(lists), ptr(counts), rounds,
+                 COLOUR_BATCH, stream())
         rounds += COLOUR_BATCH
         if int(counts[rounds % 3].item()) == 0:
             break

@@ -28,6 +28,7 @@ from datetime import datetime
 import numpy as np
 import torch
 
+from . import guards
 from . import local_search as LS
 from ._lib import SA_BROKEN, SA_CONVERGED, SA_MAXITER, SA_REJECTED, EigensolverBreakdown, call
 from .device import DeviceProblem, ptr, require_gpu, stream
@@ -204,10 +205,6 @@ class SpectralAnnealing:
         Returns (best energy, best spins, time in seconds)."""
         if self.alpha is not None:
             raise ValueError("Alpha is not set")  # the reference's message for a fixed alpha (:527-528)
-        if self.ls_policy != "default":
-            raise NotImplementedError(
-                f"ls_policy {self.ls_policy!r}: the guarded local-search policies are not part of this build; "
-                "use ls_policy 'default' (the deterministic batched local search)")
         start_time = time.time()
         prob = DeviceProblem(J, self.tdtype, tile_nnz=self.tile_nnz, lanes_per_row=self.lanes_per_row)
         if self.autotune and prob.n >= (1 << 17):
@@ -416,8 +413,15 @@ class SpectralAnnealing:
         out.orth_full_steps, out.orth_selective_steps = self._orth_steps
         return out
 
-    def _refine(self, prob: DeviceProblem, cand_bits: torch.Tensor, k: int) -> LS.LocalSearchState:
+    def _refine(self, prob: DeviceProblem, cand_bits: torch.Tensor, k: int):
+        """Local search of the distinct candidates (_refine_spins, :530-547): the default search, or the
+        guarded policy ls_policy names (specann_amd.guards)."""
         seed = self._seed()
+        if self.ls_policy != "default":
+            st = guards.refine(self.ls_policy, prob, cand_bits, k, max_iterations=self.max_local_iterations,
+                               seed=seed)
+            self._last_ls_summary = st.summary
+            return st
         col = LS.colour_graph(prob, seed)
         if self.ls_batch:
             return LS.flip_refinement(prob, cand_bits, k, max_iterations=self.max_local_iterations, seed=seed,

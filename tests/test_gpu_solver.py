@@ -209,8 +209,12 @@ def test_cli_on_bundled_instances(tmp_path, monkeypatch):
     for run in ("SpecAdj", "SpecSignedLap", "SpecAnn-64"):
         assert main(["--solver", run, "--dataset", "barabasi_albert_3ins_m10_tiny", "--datasets-path", inst,
                      "--instance", "ba_25_1_1.txt", "--seed", "1"]) == 0
+    # the guarded policy's run name (configs/default.yml: ls_policy adaptive_guard)
     assert main(["--solver", "SpecAnn-guard-adaptive", "--dataset", "barabasi_albert_3ins_m10_tiny",
-                 "--datasets-path", inst, "--instance", "ba_25_1_1.txt", "--seed", "1"]) == 2
+                 "--datasets-path", inst, "--instance", "ba_25_1_1.txt", "--seed", "1"]) == 0
+    g = pd.read_csv(tmp_path / "results/barabasi_albert_3ins_m10_tiny/SpecAnn-guard-adaptive/ba_25_1_1.csv")
+    assert g.ls_policy.iloc[0] == "adaptive_guard"
+    assert float(g.energy.iloc[0]) < 0 and int(g.local_search_iterations.iloc[0]) >= 1
 
 
 def test_breakdown_detection_from_the_host(gpu):

@@ -95,11 +95,25 @@ def test_constructor_validation():
         a._get_alpha_values(1)
 
 
-def test_guard_policy_fails_with_clear_error():
+def test_guard_policies_registry_and_bit_packing():
+    """ls_policy values (models/local_search_guards/policies.py:13-18, :55) and the candidate masks
+    the guard kernels read (bit l of word g = candidate 64 g + l)."""
+    from specann_amd import guards
+    from specann_amd.spectral_annealing import LS_POLICIES
+    assert guards.LS_POLICIES == LS_POLICIES
+    assert set(guards.POLICIES) == {"fixed_guard", "adaptive_guard", "independent_set", "sequential_oracle"}
     sa = get_solver_class("spectral_annealing")
-    s = sa("i", "d", ls_policy="adaptive_guard", verbose=False)
-    with pytest.raises(NotImplementedError, match="guarded"):
-        s.solve(graphs.erdos_renyi(20, 0.3, seed=1))
+    assert sa("i", "d", ls_policy="adaptive_guard", verbose=False).ls_policy == "adaptive_guard"
+    with pytest.raises(ValueError, match="ls_batch"):
+        sa("i", "d", ls_policy="fixed_guard", ls_batch=False, verbose=False)
+    rng = np.random.default_rng(0)
+    for k in (1, 63, 64, 65, 130):
+        m = rng.random(k) < 0.5
+        groups = (k + 63) // 64
+        words = guards._bits_per_group(m, groups)
+        assert words.dtype == np.uint64 and words.size == groups
+        back = [(int(words[c // 64]) >> (c % 64)) & 1 for c in range(k)]
+        assert back == m.astype(int).tolist()
 
 
 def test_failure_tagging():
