@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -352,8 +353,19 @@ def main(argv=None) -> int:
                                  eigsh_ncv=args.ncv, reorth=args.reorth, eigsh_keep=args.keep,
                                  lanczos_graphs=args.lanczos_graphs)
 
+    # the FP32 eigenpairs of the (untimed) first warm-up sweep: true residuals now, vectors kept for the
+    # angle to the FP64 sweep's eigenvectors at the same alphas
+    pairs = {"res32": {}, "vec32": {}, "vec64": {}}
+
+    def keep32(ai, alpha, prob, lz, s, scale, eo):
+        pairs["res32"][ai] = SpectralAnnealing._true_residual(prob, lz, s, eo.lambda1, 1.0 / abs(scale))
+        pairs["vec32"][ai] = lz.Y * scale
+
     for i in range(args.warmup):
-        out = solver(32, args.pred).run_on_device(prob32)
+        sw = solver(32, args.pred)
+        if i == 0 and not args.skip_fp64:
+            sw.eigvec_hook = keep32
+        out = sw.run_on_device(prob32)
         log(f"warmup {i}: {out.sweep_time + out.post_sweep_time:.2f}s")
     torch.cuda.synchronize()
     times, outs = [], []
@@ -446,7 +458,10 @@ def main(argv=None) -> int:
         w64.run_on_device(prob64)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        out64 = solver(64).run_on_device(prob64)
+        s64 = solver(64)
+        if pairs["vec32"]:
+            s64.eigvec_hook = lambda ai, alpha, prob, lz, s, scale, eo: pairs["vec64"].__setitem__(ai, lz.Y * scale)
+        out64 = s64.run_on_device(prob64)
         torch.cuda.synchronize()
         sec64 = time.perf_counter() - t
         e64 = out64.best_energy
@@ -456,6 +471,28 @@ def main(argv=None) -> int:
 
     def gap(a, b):
         return abs(a - b) / abs(b) if (a is not None and b) else None
+
+    eigenpairs = None
+    if pairs["vec32"] and pairs["vec64"]:
+        # angle between the FP32 and FP64 unit eigenvectors of each alpha: 2 asin(|u -+ v| / 2)
+        angles = {}
+        for ai, v64 in pairs["vec64"].items():
+            if ai in pairs["vec32"]:
+                u = pairs["vec32"][ai].double()
+                d = min(torch.linalg.vector_norm(u - v64).item(), torch.linalg.vector_norm(u + v64).item())
+                angles[ai] = 2.0 * math.asin(min(1.0, d / 2.0))
+        res = np.array(list(pairs["res32"].values()))
+        ang = np.array(list(angles.values()))
+        worst = max(angles, key=angles.get) if angles else None
+        eigenpairs = {"alphas_compared": len(angles), "fp32_true_residual_max": float(res.max()),
+                      "fp32_true_residual_median": float(np.median(res)),
+                      "fp32_residual_tol": 1e-6, "angle_to_fp64_max": float(ang.max()) if ang.size else None,
+                      "angle_to_fp64_median": float(np.median(ang)) if ang.size else None,
+                      "alpha_index_of_max_angle": worst,
+                      "angles_above_1e-3": int((ang > 1e-3).sum()),
+                      "note": "FP32 tangent sweep (untimed warm-up) against the FP64 diagonal sweep, same alphas; "
+                              "residual = ||N y - lambda y|| / ||y|| of the FP32 eigenpair"}
+        pairs.clear()
 
     line = {
         "metric": METRIC, "value": sec32, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -486,6 +523,7 @@ def main(argv=None) -> int:
                      "kernel": spmv_kernel_name(tune32["choice"], "float"),
                      "bytes_per_launch": spmv_bytes, "us_per_launch": spmv_sec * 1e6},
         "cpu_baseline": cpu,
+        "eigenpairs": eigenpairs,
     }
     print(json.dumps(line), flush=True)
     return 0

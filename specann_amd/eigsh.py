@@ -60,11 +60,16 @@ DEFAULT_KEEP = 12
 # this bound therefore run without the shadow (measured on Barabasi-Albert n = 1500 at tol 1e-10: true
 # residuals up to 2.4e-9 with the FP32 shadow, against estimates below 1e-10).
 SHADOW_MIN_TOL = {torch.float32: 0.0, torch.float64: 1e-8}
+# The rounding of a delocalised unit row in FP16 moves a projection by about 2.8e-4 ||W|| / sqrt(n): below
+# 1.1e-6 ||W|| from n = 2^16 on, but 8.7e-6 ||W|| at n = 1024, where FP32 solves then ended with true
+# residuals up to 1.6e-5 against estimates below their 1e-6 tolerance. Small problems gain nothing
+# from the narrower reads either, so they keep the full-precision basis only.
+SHADOW_MIN_N = 1 << 16
 
 
-def use_shadow(dtype: torch.dtype, tol: float) -> bool:
-    """Whether solves of this precision and tolerance read the half-width shadow basis."""
-    return tol >= SHADOW_MIN_TOL[dtype]
+def use_shadow(dtype: torch.dtype, tol: float, n: int | None = None) -> bool:
+    """Whether solves of this precision, tolerance and size read the half-width shadow basis."""
+    return tol >= SHADOW_MIN_TOL[dtype] and (n is None or n >= SHADOW_MIN_N)
 
 
 def default_orth_tau(dtype: torch.dtype) -> float:
@@ -242,7 +247,7 @@ def eigsh(A: ScaledOperator, k: int = 1, which: str = "SA", tol: float = 0.0, ma
     maxiter = int(maxiter) if maxiter is not None else 10 * prob.n
     tol = float(tol) if tol and tol > 0 else float(np.finfo(np.float32 if prob.dtype == torch.float32
                                                             else np.float64).eps)
-    solver = LanczosSolver(prob, ncv, shadow=use_shadow(prob.dtype, tol))
+    solver = LanczosSolver(prob, ncv, shadow=use_shadow(prob.dtype, tol, prob.n))
     x = v0 if v0 is not None else prob.fill_normal(seed, 0, torch.empty(prob.n, dtype=prob.dtype,
                                                                         device=prob.device))
     rho = A.rho if A.rho is not None else 0.0

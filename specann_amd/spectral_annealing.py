@@ -167,6 +167,9 @@ class SpectralAnnealing:
         # pred_mode "tangent" (no reference counterpart): the diagonal predictor plus the eigenvector
         # tangent's correction, from `tangent_steps` deflated MINRES iterations (specann_amd.tangent)
         self.lanczos_graphs = lanczos_graphs      # None: HIP-graph replay of Lanczos chunks for n <= 2^18
+        # called after each alpha's eigensolve as hook(alpha_idx, alpha, prob, lz, s, scale, outcome), where
+        # lz.Y * scale is the sign-aligned unit eigenvector (None: no call)
+        self.eigvec_hook = None
         if reorth not in ("selective", "full"):
             raise ValueError("reorth must be 'selective' or 'full'")
         self.reorth = reorth                      # full: the reference eigsh's reorthogonalisation every step
@@ -254,7 +257,7 @@ class SpectralAnnealing:
         nwords = (n + 63) // 64
         lz = LanczosSolver(prob, self.eigsh_ncv, graphs=self.lanczos_graphs, keep=self.eigsh_keep,
                            orth_tau=0.0 if self.reorth == "full" else None,
-                           shadow=use_shadow(prob.dtype, self.eigsh_tol))
+                           shadow=use_shadow(prob.dtype, self.eigsh_tol, prob.n))
         s = torch.empty(n, dtype=tdt, device=dev)
         ybuf = [torch.empty(n, dtype=tdt, device=dev) for _ in range(3)]
         x_rand = torch.empty(n, dtype=tdt, device=dev)
@@ -295,6 +298,9 @@ class SpectralAnnealing:
                 slot = n_cand
                 call(f"sa_round_{prob.sfx}", prob.work.ptr, n, ptr(lz.Y), sign / nrm, ptr(free),
                      ptr(cand_bits[slot]), stream())
+                if self.eigvec_hook is not None:
+                    # the aligned unit eigenvector is lz.Y * (sign / nrm), for measurements beside the sweep
+                    self.eigvec_hook(alpha_idx, alpha, prob, lz, s, sign / nrm, eo)
                 sig = int(prob.work.head().hash)
                 alpha_before = alpha_prev
                 if not self.disable_warm_start:

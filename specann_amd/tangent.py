@@ -163,7 +163,7 @@ class EigenContext:
         self.prob = prob
         eps = np.finfo(np.float32 if prob.dtype == torch.float32 else np.float64).eps
         self.tol = float(tol) if tol else float(np.sqrt(eps) * 1e-2 if prob.dtype == torch.float32 else 1e-12)
-        self.lz = LanczosSolver(prob, ncv, shadow=use_shadow(prob.dtype, self.tol))
+        self.lz = LanczosSolver(prob, ncv, shadow=use_shadow(prob.dtype, self.tol, prob.n))
         self.s = torch.empty(prob.n, dtype=prob.dtype, device=prob.device)
         self.x0 = torch.empty_like(self.s)
         self.tangent = TangentPredictor(prob, adjoint_steps, adjoint_tol)

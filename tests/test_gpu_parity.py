@@ -276,3 +276,32 @@ def test_rounding_agrees_above_the_eigen_tolerance(gpu, monkeypatch):
             worst = max(worst, float(np.max(np.abs(y[differ]))))
     print(f"largest |y_i| at a node whose rounding differs: {worst:.3g}")
     assert worst <= 1e-5
+
+
+@pytest.mark.parametrize("pred", ["tangent", "diagonal"])
+def test_fp32_eigenpairs_relative_residual_1e5(gpu, pred):
+    """FP32 sweeps at the reference's tol 1e-6: every converged alpha's eigenpair has a true relative
+    residual ||N y - lambda y|| / ||y|| <= 1e-5 (the FP32 bar), recomputed here in FP64 on the host from
+    the sweep's own unit vectors, and lambda within 1e-5 of the dense eigenvalue."""
+    J = graphs.erdos_renyi(1024, 40 / 1023, seed=42, kind="neg")
+    Jn, d, c = O.normalize(J, np.float32)
+    got = {}
+
+    def hook(ai, alpha, prob, lz, s, scale, eo):
+        got[ai] = (alpha, (lz.Y * scale).double().cpu().numpy(), eo.lambda1, eo.converged)
+
+    sa = SpectralAnnealing("t", "synthetic", random_seed=42, num_alphas=128, pred_mode=pred, verbose=False)
+    sa.eigvec_hook = hook
+    sa.run_on_device(DeviceProblem(J, torch.float32))
+    assert len(got) == 128
+    worst = 0.0
+    for ai, (alpha, y, lam, conv) in got.items():
+        if not conv:
+            continue
+        s, _, _ = O.alpha_scaling(d, alpha)
+        rel = float(np.linalg.norm(O.n_matvec(Jn, s, y) - lam * y) / np.linalg.norm(y))
+        worst = max(worst, rel)
+        if ai % 16 == 0:
+            w = np.linalg.eigvalsh(O.dense_n(Jn, s))
+            assert abs(lam - w[-1]) <= 1e-5
+    assert worst <= 1e-5, worst
