@@ -97,30 +97,61 @@ class BatchProblem:
         self.word_off = torch.from_numpy(self.word_off_h.astype(np.int64)).to(dev)
         self.stride = int(_lib.load().sa_work_bytes(0))
         self.works = torch.zeros(self.B * self.stride, dtype=torch.uint8, device=dev)
-        # SpMV tiles cut per instance (no tile crosses two), so a step skips stopped instances' tiles
-        tile = self.prob.tile_nnz
-        indptr = Jp.indptr.astype(np.int64)
+        self._indptr = Jp.indptr.astype(np.int64)
+        self.desc = SaBatch(B=self.B, rows_per_block=self.rpb, row_off=self.row_off.data_ptr(),
+                            row_end=self.row_end.data_ptr(), blk_off=self.blk_off.data_ptr(),
+                            works=self.works.data_ptr(), stride=self.stride, nblocks=int(blk_off[-1]),
+                            tile_inst=0)
+        self.set_tiling(self.prob.tile_nnz, self.prob.csr.lanes_per_row)
+        self.tuning = None
+        self._head = torch.empty((self.B, _lib.STATE_HEAD_BYTES), dtype=torch.uint8, pin_memory=True)
+
+    def set_tiling(self, tile_nnz: int, lanes_per_row: int = 0) -> None:
+        """Cut the SpMV tiles per instance (no tile crosses two, so a step skips the tiles of stopped
+        instances): about tile_nnz entries each; lanes_per_row 0 picks the lanes per row from each tile's
+        mean row length, 1..64 fixes them."""
+        if tile_nnz not in (512, 1024, 2048, 4096) or not 0 <= int(lanes_per_row) <= 64:
+            raise ValueError("tile_nnz must be 512, 1024, 2048 or 4096 and lanes_per_row in [0, 64]")
+        indptr, row_off = self._indptr, self.row_off_h
         starts, inst = [], []
         for b in range(self.B):
             r0, r1 = int(row_off[b]), int(row_off[b + 1])
             e0, e1 = indptr[r0], indptr[r1]
-            targets = e0 + np.arange(0, max(int(e1 - e0), 1), tile, dtype=np.int64)
+            targets = e0 + np.arange(0, max(int(e1 - e0), 1), tile_nnz, dtype=np.int64)
             rows = r0 + np.searchsorted(indptr[r0:r1 + 1], targets, side="left")
             rows[0] = r0
             starts.append(np.minimum(rows, r1))
             inst.append(np.full(rows.size, b, dtype=np.int32))
         tile_rows = np.concatenate(starts + [np.array([self.n], dtype=np.int64)]).astype(np.int32)
+        dev = self.device
         self.tile_rows = torch.from_numpy(tile_rows).to(dev)
         self.tile_inst = torch.from_numpy(np.concatenate(inst)).to(dev)
-        self.prob.tile_rows = self.tile_rows
-        self.prob.ntiles = int(tile_rows.size - 1)
-        self.prob.csr.tile_rows = self.tile_rows.data_ptr()
-        self.prob.csr.ntiles = self.prob.ntiles
-        self.desc = SaBatch(B=self.B, rows_per_block=self.rpb, row_off=self.row_off.data_ptr(),
-                            row_end=self.row_end.data_ptr(), blk_off=self.blk_off.data_ptr(),
-                            works=self.works.data_ptr(), stride=self.stride, nblocks=int(blk_off[-1]),
-                            tile_inst=self.tile_inst.data_ptr())
-        self._head = torch.empty((self.B, _lib.STATE_HEAD_BYTES), dtype=torch.uint8, pin_memory=True)
+        prob = self.prob
+        prob.tile_nnz = int(tile_nnz)
+        prob.tile_rows = self.tile_rows
+        prob.ntiles = int(tile_rows.size - 1)
+        prob.csr.tile_rows = self.tile_rows.data_ptr()
+        prob.csr.ntiles = prob.ntiles
+        prob.csr.tile_nnz = int(tile_nnz)
+        prob.csr.lanes_per_row = int(lanes_per_row)
+        self.desc.tile_inst = self.tile_inst.data_ptr()
+
+    def autotune(self, reps: int = 4, verbose: bool = False) -> dict:
+        """Pick the tile size and the lanes per row (row-split across a wavefront) of the batched SpMV by
+        timing the packed matrix's SpMV on each (the batch keeps the plain per-instance tiles: its
+        instances' vectors are small enough to stay in L2)."""
+        results = []
+        for tile in (512, 1024, 2048, 4096):
+            for lanes in (0, 4, 8, 16, 32, 64):
+                self.set_tiling(tile, lanes)
+                sec = self.prob.time_spmv(reps)
+                results.append({"tile_nnz": tile, "lanes_per_row": lanes, "seconds": sec})
+                if verbose:
+                    print(f"[batch autotune] tile {tile} lanes {lanes}: {sec * 1e6:.1f} us", flush=True)
+        best = min(results, key=lambda r: r["seconds"])
+        self.set_tiling(best["tile_nnz"], best["lanes_per_row"])
+        self.tuning = {"choice": best, "tried": results}
+        return self.tuning
 
     @property
     def ref(self):
