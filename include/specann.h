@@ -117,6 +117,16 @@ typedef struct sa_batch {
   /* NULL, or the instance of every SpMV tile of the packed CSR, whose tiles then must not cross
    * instances: the batched step skips the tiles of instances that are not running. */
   const int32_t* tile_inst;
+  /* NULL, or B bytes on the device: the per-instance entry points (alpha scaling, begin, start, Ritz
+   * vector, rounding, row copy) then act only on the instances whose byte is nonzero and leave the
+   * others' rows and state untouched; the step, the restart and the energy entry points refuse a
+   * mask (they select instances by status). */
+  const uint8_t* inst_mask;
+  /* NULL, or ntile_list SpMV tiles (device, each in [0, A->ntiles)): the batched step's SpMV then
+   * launches only these, e.g. the tiles of the instances still sweeping; a tile left out is skipped
+   * without a workgroup. */
+  const int32_t* tile_list;
+  int64_t ntile_list;
 } sa_batch;
 
 /* Device-side state of one eigensolve; lives at the start of the work buffer. */
@@ -537,12 +547,22 @@ int sa_minres_end_f64(void* work, int64_t n, const double* y, const sa_minres_bu
 /* ---- Batched sweep: many instances packed block-diagonally (sa_batch) ------------------------
  * No counterpart in the reference (it solves one instance per call); these replace, for a batch,
  * the calls above that cite it. Each instance runs its own eigensolve in its own work buffer (its
- * own j, status, Ritz solve and restarts); every launch covers all instances and is skipped per
- * workgroup by instances not in the status it needs. The batched step uses the plain CSR layout. */
+ * own j, status, Ritz solve and restarts); every launch covers all instances (or those of the
+ * descriptor's inst_mask) and is skipped per workgroup by instances not in the status it needs. The
+ * batched step uses the plain CSR layout. */
 int sa_batch_alpha_scaling_f32(const sa_batch* Bt, const float* d, double alpha, double clamp, float* s,
                                hipStream_t stream);
 int sa_batch_alpha_scaling_f64(const sa_batch* Bt, const double* d, double alpha, double clamp, double* s,
                                hipStream_t stream);
+/* The same with instance b at its own alpha[b] (device array of B): instances that stop at different
+ * steps move on to their next alpha without waiting for the rest of the batch. */
+int sa_batch_alpha_scaling_each_f32(const sa_batch* Bt, const float* d, const double* alpha, double clamp, float* s,
+                                    hipStream_t stream);
+int sa_batch_alpha_scaling_each_f64(const sa_batch* Bt, const double* d, const double* alpha, double clamp,
+                                    double* s, hipStream_t stream);
+/* dst = src on the real rows of every (unmasked) instance. */
+int sa_batch_copy_f32(const sa_batch* Bt, const float* src, float* dst, hipStream_t stream);
+int sa_batch_copy_f64(const sa_batch* Bt, const double* src, double* dst, hipStream_t stream);
 /* bounds: device array of B breakdown bounds (rho_b + g_b). */
 int sa_batch_lanczos_begin(const sa_batch* Bt, int32_t ncv, int32_t maxiter, double tol, const double* bounds,
                            double pred_tol, int32_t keep, double orth_tau, hipStream_t stream);
@@ -552,6 +572,14 @@ int sa_batch_lanczos_start_f32(const sa_batch* Bt, const float* s, const float* 
 int sa_batch_lanczos_start_f64(const sa_batch* Bt, const double* s, const double* x, const double* d, double pexp,
                                double cx, const double* y, double cy, double* V, int64_t ldv, double* Z,
                                int32_t gate, int32_t only_if, hipStream_t stream);
+/* The diagonal-predictor start x0 = d^pexp[b] (.) x with instance b's own exponent pexp[b] (device
+ * array of B; cx = 1, no second vector, every unmasked instance). */
+int sa_batch_lanczos_start_each_f32(const sa_batch* Bt, const float* s, const float* x, const float* d,
+                                    const double* pexp, float* V, int64_t ldv, float* Z, int32_t gate,
+                                    hipStream_t stream);
+int sa_batch_lanczos_start_each_f64(const sa_batch* Bt, const double* s, const double* x, const double* d,
+                                    const double* pexp, double* V, int64_t ldv, double* Z, int32_t gate,
+                                    hipStream_t stream);
 int sa_batch_lanczos_step_f32(const sa_csr* A, const sa_batch* Bt, const float* s, float* V, int64_t ldv, float* Z,
                               float* W, hipStream_t stream);
 int sa_batch_lanczos_step_f64(const sa_csr* A, const sa_batch* Bt, const double* s, double* V, int64_t ldv,

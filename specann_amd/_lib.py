@@ -137,6 +137,9 @@ class SaBatch(ctypes.Structure):
         ("stride", c_int64),
         ("nblocks", c_int64),
         ("tile_inst", c_void_p),
+        ("inst_mask", c_void_p),
+        ("tile_list", c_void_p),
+        ("ntile_list", c_int64),
     ]
 
 
@@ -274,12 +277,18 @@ _SIGNATURES = {
     "sa_normalize_rows_f64": (c_int32, [POINTER(SaCsr), _P, _P, _P, _P, _P]),
     "sa_batch_alpha_scaling_f32": (c_int32, [POINTER(SaBatch), _P, c_double, c_double, _P, _P]),
     "sa_batch_alpha_scaling_f64": (c_int32, [POINTER(SaBatch), _P, c_double, c_double, _P, _P]),
+    "sa_batch_alpha_scaling_each_f32": (c_int32, [POINTER(SaBatch), _P, _P, c_double, _P, _P]),
+    "sa_batch_alpha_scaling_each_f64": (c_int32, [POINTER(SaBatch), _P, _P, c_double, _P, _P]),
+    "sa_batch_copy_f32": (c_int32, [POINTER(SaBatch), _P, _P, _P]),
+    "sa_batch_copy_f64": (c_int32, [POINTER(SaBatch), _P, _P, _P]),
     "sa_batch_lanczos_begin": (c_int32, [POINTER(SaBatch), c_int32, c_int32, c_double, _P, c_double, c_int32, c_double,
                                          _P]),
     "sa_batch_lanczos_start_f32": (c_int32, [POINTER(SaBatch), _P, _P, _P, c_double, c_double, _P, c_double, _P,
                                              c_int64, _P, c_int32, c_int32, _P]),
     "sa_batch_lanczos_start_f64": (c_int32, [POINTER(SaBatch), _P, _P, _P, c_double, c_double, _P, c_double, _P,
                                              c_int64, _P, c_int32, c_int32, _P]),
+    "sa_batch_lanczos_start_each_f32": (c_int32, [POINTER(SaBatch), _P, _P, _P, _P, _P, c_int64, _P, c_int32, _P]),
+    "sa_batch_lanczos_start_each_f64": (c_int32, [POINTER(SaBatch), _P, _P, _P, _P, _P, c_int64, _P, c_int32, _P]),
     "sa_batch_lanczos_step_f32": (c_int32, [POINTER(SaCsr), POINTER(SaBatch), _P, _P, c_int64, _P, _P, _P]),
     "sa_batch_lanczos_step_f64": (c_int32, [POINTER(SaCsr), POINTER(SaBatch), _P, _P, c_int64, _P, _P, _P]),
     "sa_batch_lanczos_thick_restart_f32": (c_int32, [POINTER(SaBatch), _P, _P, c_int64, _P, _P]),

@@ -10,11 +10,12 @@ its own work buffer with its own j, status and residual test; see sa_batch_* in 
 Per alpha and instance the control flow is the one-instance sweep's (specann_amd.spectral_annealing,
 after models/spectral_annealing.py:311-421): Gershgorin bound and rho from the instance's last
 lambda_1, the diagonal (or secant) predictor gated by its step-0 residual, a fall-back start from
-y_last when the gate rejects it, sign alignment, rounding and first-appearance deduplication. Two
-differences: an instance whose Lanczos breaks down skips the alpha (the one-instance sweep retries it
-from a random vector once), and the local search runs on the packed graph with column c holding every
-instance's c-th distinct candidate (an instance with fewer repeats its last one), so its pass
-count is the batch's; per-instance energies are exact int64 sums over the instance's rows.
+y_last when the gate rejects it, a random start in place of a zero predicted vector, one cold retry
+of a failed solve, sign alignment, rounding and first-appearance deduplication. Instances move from one
+alpha to the next independently (BatchSweep). The difference: the local search runs on the packed graph
+with column c holding every instance's c-th distinct candidate (an instance with fewer repeats its last
+one), so its pass count is the batch's; per-instance energies are exact int64 sums over the instance's
+rows.
 """
 from __future__ import annotations
 
@@ -36,6 +37,17 @@ from .spectral_annealing import compute_shift_rho
 
 TERMINAL = (SA_CONVERGED, SA_MAXITER, SA_BROKEN)
 ALIGN = 64
+
+
+def _head_dtype() -> np.dtype:
+    """numpy form of the fields of SaState that lie in its first STATE_HEAD_BYTES bytes."""
+    full = np.dtype(SaState)
+    names = [k for k in full.names if full.fields[k][1] < _lib.STATE_HEAD_BYTES]
+    return np.dtype({"names": names, "formats": [full.fields[k][0] for k in names],
+                     "offsets": [full.fields[k][1] for k in names], "itemsize": _lib.STATE_HEAD_BYTES})
+
+
+HEAD_DTYPE = _head_dtype()
 
 
 def pack(Js: list[sp.spmatrix]) -> tuple[sp.csr_matrix, np.ndarray, np.ndarray]:
@@ -123,6 +135,7 @@ class BatchProblem:
             starts.append(np.minimum(rows, r1))
             inst.append(np.full(rows.size, b, dtype=np.int32))
         tile_rows = np.concatenate(starts + [np.array([self.n], dtype=np.int64)]).astype(np.int32)
+        self.tile_off_h = np.concatenate([[0], np.cumsum([t.size for t in starts])]).astype(np.int64)
         dev = self.device
         self.tile_rows = torch.from_numpy(tile_rows).to(dev)
         self.tile_inst = torch.from_numpy(np.concatenate(inst)).to(dev)
@@ -157,19 +170,14 @@ class BatchProblem:
     def ref(self):
         return ctypes.byref(self.desc)
 
-    def heads(self) -> list[SaState]:
-        """The scalar heads of every instance's state (one copy, one synchronisation)."""
+    def heads(self) -> np.ndarray:
+        """The scalar heads of every instance's state, B records of HEAD_DTYPE (one copy, one
+        synchronisation of the current stream)."""
         hb = _lib.STATE_HEAD_BYTES
         view = self.works.view(self.B, self.stride)[:, :hb]
         self._head.copy_(view, non_blocking=True)
         torch.cuda.current_stream().synchronize()
-        out = []
-        base = self._head.data_ptr()
-        for b in range(self.B):
-            st = SaState()
-            ctypes.memmove(ctypes.addressof(st), base + b * hb, hb)
-            out.append(st)
-        return out
+        return self._head.numpy().reshape(-1).view(HEAD_DTYPE).copy()
 
     def rows_of(self, b: int) -> slice:
         return slice(int(self.row_off_h[b]), int(self.row_off_h[b] + self.sizes[b]))
@@ -201,14 +209,44 @@ class BatchOutcome:
     ls_state: object = None
 
 
+class _Buffers:
+    """Device buffers of one batched sweep."""
+
+    def __init__(self, bp: BatchProblem, ncv_arg, num_alphas: int):
+        dev, dt, n = bp.device, bp.dtype, bp.n
+        self.ncv = default_ncv(int(bp.sizes.min()), ncv_arg)
+        self.keep = max(1, min(_lib.SA_MAX_KEEP, self.ncv // 2, self.ncv - 1))
+        self.ldv = _lib.padded_ld(n, 4 if dt == torch.float32 else 8)
+        self.V = torch.zeros((self.ncv + 1, self.ldv), dtype=dt, device=dev)
+        self.Z, self.W, self.Y, self.s, self.x_rand = (torch.zeros(n, dtype=dt, device=dev) for _ in range(5))
+        self.nwords = (n + 63) // 64
+        self.bounds = torch.empty(bp.B, dtype=torch.float64, device=dev)
+
+
 class BatchSweep:
-    """The alpha sweep of SpectralAnnealing (diagonal, secant or prev predictor) over a batch."""
+    """The alpha sweep of SpectralAnnealing (diagonal, secant or prev predictor) over a batch.
+
+    Two schedules give every instance the same arithmetic. ``lockstep=True`` moves the whole batch from
+    one alpha to the next together, so every alpha waits for its slowest instance. The default moves
+    each instance on as soon as its own solve stops: at each check (a host read of the states every
+    ``chunk`` batched steps) the stopped instances take their Ritz vector, rounding and next start
+    together (those whose predicted start the gate rejected restart from y_last), as one masked set of
+    launches with
+    per-instance alphas and predictor exponents, while the rest keep stepping. The batched step count
+    then follows the instance with the most Lanczos steps over the whole sweep instead of the sum over
+    alphas of the slowest instance per alpha.
+
+    The default schedule also retries a failed solve once from a random vector (breakdown, a
+    Ritz value above the Gershgorin bound, a zero or non-finite Ritz vector) and replaces a zero
+    predicted start by a random gated one, as the one-instance sweep does
+    (models/spectral_annealing.py:568-669); the lockstep schedule skips the alpha instead.
+    """
 
     def __init__(self, random_seed: int = 0, num_alphas: int = 128, schedule: str = "sqrt",
                  schedule_interval=(0.0, 1.0), pred_mode: str = "diagonal", pred_tol: float = 1e-3,
                  shift_eps: float = 1e-3, eigsh_tol: float = 1e-6, eigsh_maxiter: int = 1000,
                  eigsh_ncv: int | None = None, max_local_iterations: int = 1000, use_local_search: bool = True,
-                 chunk: int = 8):
+                 chunk: int = 8, lockstep: bool = False):
         if pred_mode not in ("diagonal", "secant", "prev"):
             raise ValueError("the batched sweep supports pred_mode diagonal, secant or prev")
         self.seed = 0 if random_seed is None else int(random_seed)
@@ -221,24 +259,42 @@ class BatchSweep:
         self.max_local_iterations = int(max_local_iterations)
         self.use_local_search = bool(use_local_search)
         self.chunk = int(chunk)
+        if self.chunk < 1:
+            raise ValueError("chunk must be >= 1")
+        self.lockstep = bool(lockstep)
+        self.retries = 0           # cold retries of the last run (default schedule)
+        self.fallbacks = 0         # zero predicted starts replaced by random ones (default schedule)
 
     def run(self, bp: BatchProblem) -> BatchOutcome:
+        return self._run_lockstep(bp) if self.lockstep else self._run_async(bp)
+
+    def _ok(self, st, theta, g, ai):
+        """The one-instance sweep's acceptance of a stopped solve (:679-691): no breakdown, a finite Ritz
+        value within the Gershgorin bound (arrays over the batch; ai: each instance's alpha index)."""
+        return (st != SA_BROKEN) & np.isfinite(theta) & (theta <= g * (1.0 + max(0.5 * self.shift_eps, 1e-5)))
+
+    def _row(self, bp, b, ai, hs, g, rho, dsum, ok, pred_res):
+        theta = float(hs['theta'][b])
+        return {'alpha': float(self.alphas[ai]), 'alpha_index': int(ai), 'eigsh_iterations': int(hs['steps'][b]),
+                'eigsh_converged': bool(hs['status'][b] == SA_CONVERGED and ok), 'lambda1': theta,
+                'rho': float(rho), 'gershgorin': float(g),
+                'energy_bound': float(-0.5 * theta * dsum * bp.c_h[b]) if ok else float('nan'),
+                'pred_residual': float(pred_res),
+                'pred_accepted': (None if np.isnan(pred_res) else bool(not pred_res > self.pred_tol)),
+                'residual_estimate': float(hs['res'][b]), 'lanczos_restarts': int(hs['cycles'][b]),
+                'failed': not bool(ok), 'retry': False, 'candidate': None}
+
+    # ---- every instance at the same alpha ------------------------------------------------------
+    def _run_lockstep(self, bp: BatchProblem) -> BatchOutcome:
         t_start = time.time()
-        prob, B, n, sfx = bp.prob, bp.B, bp.n, bp.prob.sfx
-        dev, dt = bp.device, bp.dtype
-        ncv = default_ncv(int(bp.sizes.min()), self.ncv_arg)
-        keep = max(1, min(_lib.SA_MAX_KEEP, ncv // 2, ncv - 1))
-        ldv = _lib.padded_ld(n, 4 if dt == torch.float32 else 8)
-        Vbuf = torch.zeros((ncv + 1, ldv), dtype=dt, device=dev)
-        V = Vbuf[:, :n]
-        Z, W, Y = (torch.zeros(n, dtype=dt, device=dev) for _ in range(3))
-        s = torch.zeros(n, dtype=dt, device=dev)
-        x_rand = torch.zeros(n, dtype=dt, device=dev)
+        prob, B, sfx = bp.prob, bp.B, bp.prob.sfx
+        dev, dt, n = bp.device, bp.dtype, bp.n
+        buf = _Buffers(bp, self.ncv_arg, len(self.alphas))
+        ncv, keep, ldv, V = buf.ncv, buf.keep, buf.ldv, buf.V
+        Z, W, Y, s, x_rand, bounds, nwords = buf.Z, buf.W, buf.Y, buf.s, buf.x_rand, buf.bounds, buf.nwords
         ybuf = [torch.zeros(n, dtype=dt, device=dev) for _ in range(3)]
-        nwords = (n + 63) // 64
         A = len(self.alphas)
         cand_bits = torch.zeros((A, nwords), dtype=torch.int64, device=dev)
-        bounds = torch.empty(B, dtype=torch.float64, device=dev)
         R = bp.ref
 
         def step_all():
@@ -267,14 +323,12 @@ class BatchSweep:
                     steps += 1
                 hs = bp.heads()
                 if first is None:
-                    first = np.array([h.first_beta for h in hs])
-                stats = np.array([h.status for h in hs])
+                    first = hs['first_beta'].copy()
+                stats = hs['status']
                 if np.any(stats == SA_REJECTED):
-                    begin_rej = stats == SA_REJECTED
                     start(y_last, gate=0, only_if=SA_REJECTED)
-                    self._rejected |= begin_rej
                     hs = bp.heads()
-                    stats = np.array([h.status for h in hs])
+                    stats = hs['status']
                 if np.all(np.isin(stats, TERMINAL)):
                     return hs, first, steps
                 if steps > limit:
@@ -292,12 +346,11 @@ class BatchSweep:
         for ai, alpha in enumerate(self.alphas):
             call(f"sa_batch_alpha_scaling_{sfx}", R, ptr(prob.d), float(alpha), 1e-12, ptr(s), stream())
             hs = bp.heads()
-            dsum = np.array([h.r[1] for h in hs])
-            g = np.array([h.r[2] for h in hs])
+            dsum = hs['r'][:, 1].copy()
+            g = hs['r'][:, 2].copy()
             rho = np.array([compute_shift_rho(None if np.isnan(lam_prev[b]) else lam_prev[b], g[b], self.shift_eps)
                             for b in range(B)])
             bounds.copy_(torch.from_numpy(rho + g))
-            self._rejected = np.zeros(B, dtype=bool)
             cold = ai == 0 or y_last is None
             if cold:
                 begin(np.inf)
@@ -320,28 +373,18 @@ class BatchSweep:
                     hs, first, steps = run_until_done(1)
                     pred_res = first
             total_steps += steps
-            stats = np.array([h.status for h in hs])
-            theta = np.array([h.theta for h in hs])
-            isteps = np.array([h.steps for h in hs])
-            total_matvecs += int(isteps.sum())
-            ok = (stats != SA_BROKEN) & np.isfinite(theta) & (theta <= g * (1.0 + max(0.5 * self.shift_eps, 1e-5)))
+            stats, theta = hs['status'], hs['theta']
+            total_matvecs += int(hs['steps'].sum())
+            ok = self._ok(stats, theta, g, np.full(B, ai))
             call(f"sa_batch_lanczos_ritz_{sfx}", R, ptr(V), ldv, ptr(y_last), ptr(Y), -1, stream())
             free = next(b for b in ybuf if b is not y_last and b is not y_prev)
             call(f"sa_batch_round_{sfx}", R, ptr(Y), ptr(free), ptr(cand_bits[ai]), int(y_last is not None), stream())
-            hs2 = bp.heads()
             if y_last is not None and not ok.all():
                 for b in np.nonzero(~ok)[0]:          # a failed instance keeps its previous vector
                     free[bp.rows_of(b)].copy_(y_last[bp.rows_of(b)])
             bits_h = cand_bits[ai].cpu().numpy()
             for b in range(B):
-                row = {'alpha': float(alpha), 'alpha_index': ai, 'eigsh_iterations': int(isteps[b]),
-                       'eigsh_converged': bool(stats[b] == SA_CONVERGED and ok[b]), 'lambda1': float(theta[b]),
-                       'rho': float(rho[b]), 'gershgorin': float(g[b]),
-                       'energy_bound': float(-0.5 * theta[b] * dsum[b] * bp.c_h[b]) if ok[b] else float('nan'),
-                       'pred_residual': float(pred_res[b]), 'pred_accepted':
-                           (None if np.isnan(pred_res[b]) else bool(not pred_res[b] > self.pred_tol)),
-                       'residual_estimate': float(hs[b].res), 'lanczos_restarts': int(hs[b].cycles),
-                       'failed': not bool(ok[b]), 'candidate': None}
+                row = self._row(bp, b, ai, hs, g[b], rho[b], dsum[b], ok[b], pred_res[b])
                 if ok[b]:
                     key = bits_h[bp.words_of(b)].tobytes()
                     c = by_hash[b].get(key)
@@ -350,16 +393,213 @@ class BatchSweep:
                         by_hash[b][key] = c
                         cand_alpha[b].append(ai)
                     row['candidate'] = c
+                    lam_prev[b] = theta[b]
                 results[b].rows.append(row)
                 cand_of_row[b].append(row['candidate'])
-                if ok[b]:
-                    lam_prev[b] = theta[b]
             y_prev, y_last = y_last, free
             alpha_prev = alpha
         torch.cuda.synchronize()
-        sweep_time = time.time() - t_start
+        return self._post_sweep(bp, results, cand_alpha, cand_of_row, cand_bits, time.time() - t_start,
+                                total_steps, total_matvecs)
 
-        # ---- post-sweep: packed candidates, exact energies, local search, selection ------------
+    # ---- every instance at its own alpha ---------------------------------------------------------
+    def _run_async(self, bp: BatchProblem) -> BatchOutcome:
+        t_start = time.time()
+        prob, B, sfx = bp.prob, bp.B, bp.prob.sfx
+        dev, dt, n = bp.device, bp.dtype, bp.n
+        buf = _Buffers(bp, self.ncv_arg, len(self.alphas))
+        ncv, keep, ldv, V = buf.ncv, buf.keep, buf.ldv, buf.V
+        Z, W, Y, s, x_rand, nwords = buf.Z, buf.W, buf.Y, buf.s, buf.x_rand, buf.nwords
+        Ylast, Yprev, Ynew = (torch.zeros(n, dtype=dt, device=dev) for _ in range(3))
+        bits = torch.zeros(nwords, dtype=torch.int64, device=dev)
+        bits_h = torch.empty(nwords, dtype=torch.int64, pin_memory=True)
+        alphas, A = self.alphas, len(self.alphas)
+        cand_h = np.zeros((A, nwords), dtype=np.int64)
+        orth_tau = default_orth_tau(bp.dtype)
+        R = bp.ref
+        keep_alive = []                                   # device arrays of the launches queued this check
+
+        def dev_array(values, dtype):
+            t = torch.from_numpy(np.ascontiguousarray(values, dtype=dtype)).to(dev)
+            keep_alive.append(t)
+            return t.data_ptr()
+
+        def masked(sel):
+            d = SaBatch.from_buffer_copy(bp.desc)
+            d.inst_mask = dev_array(sel, np.uint8)
+            return ctypes.byref(d)
+
+        ai = np.zeros(B, dtype=np.int64)          # alpha index each instance is solving (A: finished)
+        gated = np.zeros(B, dtype=bool)           # a gated start whose first_beta is still to be read
+        predicted = np.zeros(B, dtype=bool)       # the current solve started from a predictor
+        retried = np.zeros(B, dtype=bool)
+        fallback = np.zeros(B, dtype=bool)
+        pred_res = np.full(B, np.nan)
+        lam_prev = np.full(B, np.nan)
+        dsum, g, rho = np.zeros(B), np.zeros(B), np.zeros(B)
+        started = np.zeros(B, dtype=np.int64)     # batched step count at the current solve's start
+        matvecs = np.zeros(B, dtype=np.int64)
+        results = [InstanceResult(float("inf"), None, None) for _ in range(B)]
+        by_hash = [dict() for _ in range(B)]
+        cand_alpha = [[] for _ in range(B)]
+        cand_of_row = [[] for _ in range(B)]
+        self.retries = self.fallbacks = 0
+        steps = 0
+        limit = self.maxiter + 8 * ncv + 64
+
+        step_desc = SaBatch.from_buffer_copy(bp.desc)
+        step_ref = ctypes.byref(step_desc)
+        tile_list = None
+
+        def step_all():
+            call(f"sa_batch_lanczos_step_{sfx}", prob.csr_ref, step_ref, ptr(s), ptr(V), ldv, ptr(Z), ptr(W), stream())
+            call(f"sa_batch_lanczos_thick_restart_{sfx}", R, ptr(s), ptr(V), ldv, ptr(Z), stream())
+
+        def begin(sel, pred_tol):
+            call("sa_batch_lanczos_begin", masked(sel), ncv, self.maxiter, self.tol, dev_array(rho + g, np.float64),
+                 float(pred_tol), keep, orth_tau, stream())
+            started[sel] = steps
+
+        def start(sel, x, cx=1.0, y=None, cy=0.0, gate=0):
+            call(f"sa_batch_lanczos_start_{sfx}", masked(sel), ptr(s), ptr(x), None, 0.0, float(cx), ptr(y),
+                 float(cy), ptr(V), ldv, ptr(Z), int(gate), -1, stream())
+
+        def scale(sel):
+            """s at each selected instance's alpha, then its degree sum, Gershgorin bound and shift."""
+            call(f"sa_batch_alpha_scaling_each_{sfx}", masked(sel), ptr(prob.d),
+                 dev_array(alphas[np.minimum(ai, A - 1)], np.float64), 1e-12, ptr(s), stream())
+            hs = bp.heads()
+            keep_alive.clear()
+            dsum[sel] = hs['r'][sel, 1]
+            g[sel] = hs['r'][sel, 2]
+            for b in np.nonzero(sel)[0]:
+                rho[b] = compute_shift_rho(None if np.isnan(lam_prev[b]) else lam_prev[b], g[b], self.shift_eps)
+
+        def random_start(sel, offset, gate):
+            """Random start vectors (stream alpha_index * 4 + offset, as the one-instance sweep draws)."""
+            for a in np.unique(ai[sel]):
+                grp = sel & (ai == a)
+                prob.fill_normal(self.seed, int(a) * 4 + offset, x_rand)
+                begin(grp, self.pred_tol if gate else np.inf)
+                start(grp, x_rand, gate=gate)
+            gated[sel] = bool(gate)
+
+        def predicted_start(sel):
+            if self.pred_mode == "prev":
+                begin(sel, np.inf)
+                start(sel, Ylast)
+                return
+            begin(sel, self.pred_tol)
+            diag = sel & ((ai == 1) if self.pred_mode == "secant" else np.ones(B, dtype=bool))
+            if diag.any():
+                pexp = 0.5 * (alphas[np.minimum(ai, A - 1)] - alphas[np.maximum(ai - 1, 0)])
+                call(f"sa_batch_lanczos_start_each_{sfx}", masked(diag), ptr(s), ptr(Ylast), ptr(prob.d),
+                     dev_array(pexp, np.float64), ptr(V), ldv, ptr(Z), 1, stream())
+            sec = sel & ~diag
+            if sec.any():
+                start(sec, Ylast, cx=2.0, y=Yprev, cy=-1.0, gate=1)
+            gated[sel] = True
+            predicted[sel] = True
+
+        every = np.ones(B, dtype=bool)
+        scale(every)
+        random_start(every, 0, 0)
+        interval = self.chunk
+        sweeping = B
+        while np.any(ai < A):
+            if int(np.count_nonzero(ai < A)) != sweeping:
+                # the SpMV launches only the tiles of the instances still sweeping
+                sweeping = int(np.count_nonzero(ai < A))
+                to = bp.tile_off_h
+                tiles = np.concatenate([np.arange(to[b], to[b + 1], dtype=np.int32) for b in np.nonzero(ai < A)[0]])
+                tile_list = torch.from_numpy(tiles).to(dev)
+                step_desc.tile_list = tile_list.data_ptr()
+                step_desc.ntile_list = int(tiles.size)
+            for _ in range(interval):
+                step_all()
+                steps += 1
+            hs = bp.heads()
+            keep_alive.clear()
+            st = hs['status']
+            live = ai < A
+            rd = gated & live
+            pred_res[rd] = hs['first_beta'][rd]       # kept in the state until the next begin
+            gated[rd] = False
+            over = live & (st == SA_RUNNING) & (steps - started > limit)
+            if over.any():
+                raise RuntimeError(f"batched Lanczos: instance {int(np.argmax(over))} did not stop after "
+                                   f"{limit} steps")
+            if np.any(live & (st == SA_REJECTED)):
+                call(f"sa_batch_lanczos_start_{sfx}", R, ptr(s), ptr(Ylast), None, 0.0, 1.0, None, 0.0, ptr(V), ldv,
+                     ptr(Z), 0, SA_REJECTED, stream())
+            term = live & np.isin(st, TERMINAL)
+            if term.any():
+                theta = hs['theta']
+                zero = term & (st == SA_BROKEN) & predicted & ~fallback & ~(hs['r'][:, 3] >= 1e-12)
+                term &= ~zero
+                ok = term & self._ok(st, theta, g, ai)
+                vec_ok = np.zeros(B, dtype=bool)
+                if ok.any():
+                    m_ok = masked(ok)
+                    call(f"sa_batch_lanczos_ritz_{sfx}", m_ok, ptr(V), ldv, ptr(Ylast), ptr(Y), -1, stream())
+                    call(f"sa_batch_round_{sfx}", m_ok, ptr(Y), ptr(Ynew), ptr(bits), 1, stream())
+                    bits_h.copy_(bits, non_blocking=True)
+                    h2 = bp.heads()
+                    keep_alive.clear()
+                    nrm, dot = h2['r'][:, 4], h2['r'][:, 5]
+                    vec_ok = ok & (nrm > 0) & np.isfinite(nrm) & np.isfinite(dot)
+                    words = bits_h.numpy()
+                retry = term & ~vec_ok & ~retried
+                moved = term & ~retry
+                matvecs += np.where(term, hs['steps'], 0)
+                for b in np.nonzero(moved)[0]:
+                    row = self._row(bp, b, ai[b], hs, g[b], rho[b], dsum[b], vec_ok[b], pred_res[b])
+                    row['retry'] = bool(retried[b])
+                    if vec_ok[b]:
+                        ws = bp.words_of(b)
+                        key = words[ws].tobytes()
+                        c = by_hash[b].get(key)
+                        if c is None:
+                            c = len(cand_alpha[b])
+                            by_hash[b][key] = c
+                            cand_alpha[b].append(int(ai[b]))
+                            cand_h[ai[b], ws] = words[ws]
+                        row['candidate'] = c
+                        lam_prev[b] = theta[b]
+                    results[b].rows.append(row)
+                    cand_of_row[b].append(row['candidate'])
+                if moved.any():                          # y_prev <- y_last; y_last <- the new vector
+                    call(f"sa_batch_copy_{sfx}", masked(moved), ptr(Ylast), ptr(Yprev), stream())
+                if vec_ok.any():
+                    call(f"sa_batch_copy_{sfx}", masked(vec_ok), ptr(Ynew), ptr(Ylast), stream())
+                if zero.any():
+                    self.fallbacks += int(zero.sum())
+                    fallback |= zero
+                    predicted[zero] = False
+                    random_start(zero, 1, 1)
+                if retry.any():
+                    self.retries += int(retry.sum())
+                    retried |= retry
+                    predicted[retry] = False
+                    random_start(retry, 2, 0)
+                ai[moved] += 1
+                retried[moved] = fallback[moved] = predicted[moved] = False
+                pred_res[moved] = np.nan
+                nxt = moved & (ai < A)
+                if nxt.any():
+                    scale(nxt)
+                    predicted_start(nxt)
+            interval = self.chunk
+        torch.cuda.synchronize()
+        sweep_time = time.time() - t_start
+        cand_bits = torch.from_numpy(cand_h).to(dev)
+        return self._post_sweep(bp, results, cand_alpha, cand_of_row, cand_bits, sweep_time, steps,
+                                int(matvecs.sum()))
+
+    def _post_sweep(self, bp, results, cand_alpha, cand_of_row, cand_bits, sweep_time, total_steps, total_matvecs):
+        prob, B, dev, R = bp.prob, bp.B, bp.device, bp.ref
+        nwords = cand_bits.shape[1]
+        # packed candidates, exact energies, local search, selection
         t_post = time.time()
         K = max(len(c) for c in cand_alpha)
         passes = 0

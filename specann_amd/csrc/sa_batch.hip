@@ -10,12 +10,21 @@ namespace sa {
 
 // s = max(d^alpha, clamp)^(-1/2); per instance: sum of d^alpha (column 0) and max of d^(1-alpha)
 // (column 1) over its real rows.
+// alpha_each (NULL, or B values): instance b's own alpha, for a batch whose instances sit at
+// different points of the sweep.
 template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void batch_alpha_scaling_kernel(BatchMap map, const T* __restrict__ d,
-                                                                       T alpha, T one_minus_alpha, T clamp,
-                                                                       T* __restrict__ s) {
+                                                                       T alpha, T one_minus_alpha,
+                                                                       const double* __restrict__ alpha_each,
+                                                                       T clamp, T* __restrict__ s) {
   __shared__ double red[SA_WAVES_PER_BLOCK];
   const Slice sl = map.slice();
+  if (!sl.live) return;
+  if (alpha_each) {
+    const double a = alpha_each[sl.inst];
+    alpha = (T)a;
+    one_minus_alpha = (T)(1.0 - a);
+  }
   double sum = 0.0, mx = -INFINITY;
   for (int64_t r = sl.lo + threadIdx.x; r < sl.hi; r += blockDim.x) {
     const T dr = d[r];
@@ -48,6 +57,7 @@ __global__ __launch_bounds__(SA_BLOCK) void batch_round_kernel(BatchMap map, con
                                                                T* __restrict__ y_last, uint64_t* __restrict__ bits,
                                                                int32_t align) {
   const Slice sl = map.slice();
+  if (!sl.live) return;
   const sa_state* st = state_of(sl.work);
   const double nrm = st->r[4];
   const double sign = (align && st->r[5] < 0.0) ? -1.0 : 1.0;
@@ -70,6 +80,15 @@ __global__ __launch_bounds__(SA_BLOCK) void batch_round_kernel(BatchMap map, con
     }
   }
   block_partials_u64(hsum, sl.work, sl.blk);
+}
+
+// dst = src on the real rows of the live instances.
+template <typename T>
+__global__ __launch_bounds__(SA_BLOCK) void batch_copy_kernel(BatchMap map, const T* __restrict__ src,
+                                                              T* __restrict__ dst) {
+  const Slice sl = map.slice();
+  if (!sl.live) return;
+  for (int64_t r = sl.lo + threadIdx.x; r < sl.hi; r += blockDim.x) dst[r] = src[r];
 }
 
 // e2[b][c] = sum_{i in b} s_i sum_j J_ij s_j for candidate bit vectors (exact, int64).
@@ -140,10 +159,11 @@ __global__ __launch_bounds__(SA_BLOCK) void bits_gather_kernel(int64_t nwords, i
 }
 
 template <typename T>
-static int batch_alpha_scaling(const sa_batch* Bt, const T* d, double alpha, double clamp, T* s, hipStream_t stream) {
+static int batch_alpha_scaling(const sa_batch* Bt, const T* d, double alpha, const double* alpha_each, double clamp,
+                               T* s, hipStream_t stream) {
   SA_REQUIRE(batch_ok(Bt) && d && s, "bad arguments");
   hipLaunchKernelGGL(batch_alpha_scaling_kernel<T>, dim3((unsigned)Bt->nblocks), dim3(SA_BLOCK), 0, stream, bmap(Bt),
-                     d, (T)alpha, (T)(1.0 - alpha), (T)clamp, s);
+                     d, (T)alpha, (T)(1.0 - alpha), alpha_each, (T)clamp, s);
   int rc = check_launch("batch_alpha_scaling");
   if (rc) return rc;
   FinishSpec f{};
@@ -172,6 +192,13 @@ static int batch_round(const sa_batch* Bt, const T* y, T* y_last, uint64_t* bits
   return launch_finish_batch(Bt, f, stream);
 }
 
+template <typename T>
+static int batch_copy(const sa_batch* Bt, const T* src, T* dst, hipStream_t stream) {
+  SA_REQUIRE(batch_ok(Bt) && src && dst, "bad arguments");
+  hipLaunchKernelGGL(batch_copy_kernel<T>, dim3((unsigned)Bt->nblocks), dim3(SA_BLOCK), 0, stream, bmap(Bt), src, dst);
+  return check_launch("batch_copy");
+}
+
 }  // namespace sa
 
 using namespace sa;
@@ -180,12 +207,32 @@ extern "C" {
 
 int sa_batch_alpha_scaling_f32(const sa_batch* Bt, const float* d, double alpha, double clamp, float* s,
                                hipStream_t stream) {
-  return batch_alpha_scaling<float>(Bt, d, alpha, clamp, s, stream);
+  return batch_alpha_scaling<float>(Bt, d, alpha, nullptr, clamp, s, stream);
 }
 
 int sa_batch_alpha_scaling_f64(const sa_batch* Bt, const double* d, double alpha, double clamp, double* s,
                                hipStream_t stream) {
-  return batch_alpha_scaling<double>(Bt, d, alpha, clamp, s, stream);
+  return batch_alpha_scaling<double>(Bt, d, alpha, nullptr, clamp, s, stream);
+}
+
+int sa_batch_alpha_scaling_each_f32(const sa_batch* Bt, const float* d, const double* alpha, double clamp, float* s,
+                                    hipStream_t stream) {
+  SA_REQUIRE(alpha, "bad arguments");
+  return batch_alpha_scaling<float>(Bt, d, 0.0, alpha, clamp, s, stream);
+}
+
+int sa_batch_alpha_scaling_each_f64(const sa_batch* Bt, const double* d, const double* alpha, double clamp,
+                                    double* s, hipStream_t stream) {
+  SA_REQUIRE(alpha, "bad arguments");
+  return batch_alpha_scaling<double>(Bt, d, 0.0, alpha, clamp, s, stream);
+}
+
+int sa_batch_copy_f32(const sa_batch* Bt, const float* src, float* dst, hipStream_t stream) {
+  return batch_copy<float>(Bt, src, dst, stream);
+}
+
+int sa_batch_copy_f64(const sa_batch* Bt, const double* src, double* dst, hipStream_t stream) {
+  return batch_copy<double>(Bt, src, dst, stream);
 }
 
 int sa_batch_round_f32(const sa_batch* Bt, const float* y, float* y_last, uint64_t* bits, int32_t align,
@@ -201,6 +248,7 @@ int sa_batch_round_f64(const sa_batch* Bt, const double* y, double* y_last, uint
 int sa_batch_energy_bits_i64(const sa_csr* A, const sa_batch* Bt, int32_t k, const uint64_t* cand_bits,
                              int64_t nwords, int64_t* e2, hipStream_t stream) {
   SA_REQUIRE(A && A->jint && batch_ok(Bt) && cand_bits && e2 && k >= 1 && k <= 65535, "bad arguments");
+  SA_REQUIRE(!Bt->inst_mask, "the batched energies take every instance (no mask)");
   SA_REQUIRE(nwords * 64 >= A->n, "nwords too small for n");
   hipLaunchKernelGGL(batch_energy_bits_kernel, dim3((unsigned)Bt->nblocks, k), dim3(SA_BLOCK), 0, stream, bmap(Bt),
                      A->indptr, A->indices, A->jint, cand_bits, nwords, k, e2);
@@ -210,6 +258,7 @@ int sa_batch_energy_bits_i64(const sa_csr* A, const sa_batch* Bt, int32_t k, con
 int sa_batch_ls_energy(const sa_batch* Bt, int32_t k, int32_t groups, const int8_t* spins, const int32_t* fields,
                        int64_t* e2, hipStream_t stream) {
   SA_REQUIRE(batch_ok(Bt) && spins && fields && e2 && k >= 1 && groups >= 1 && k <= 64 * groups, "bad arguments");
+  SA_REQUIRE(!Bt->inst_mask, "the batched energies take every instance (no mask)");
   hipLaunchKernelGGL(batch_ls_energy_kernel, dim3((unsigned)Bt->nblocks, groups), dim3(SA_BLOCK), 0, stream, bmap(Bt),
                      k, groups, spins, fields, e2);
   return check_launch("batch_ls_energy");

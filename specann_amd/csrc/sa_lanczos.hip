@@ -53,7 +53,7 @@ using SH = typename Shadow<T>::type;
 template <class F>
 __global__ void lz_begin_kernel(F fin, int32_t ncv, int32_t maxiter, double tol, double bound, const double* bounds,
                                 double pred_tol, int32_t keep, double orth_tau) {
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x != 0 || !fin.live()) return;
   sa_state* st = state_of(fin.w());
   if (bounds) bound = bounds[fin.idx()];
   st->status = SA_RUNNING;
@@ -79,12 +79,15 @@ __global__ void lz_begin_kernel(F fin, int32_t ncv, int32_t maxiter, double tol,
 
 template <typename T, class M>
 __global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(M map, const T* __restrict__ s, const T* __restrict__ x,
-                                                            const T* __restrict__ d, T pexp, double cx,
+                                                            const T* __restrict__ d, T pexp,
+                                                            const double* __restrict__ pexp_each, double cx,
                                                             const T* __restrict__ y, double cy, T* __restrict__ V,
                                                             T* __restrict__ Z, int32_t gate, int32_t only_if) {
   const Slice sl = map.slice();
+  if (!sl.live) return;
   const sa_state* st = state_of(sl.work);
   if (only_if >= 0 && st->status != only_if) return;
+  if (pexp_each) pexp = (T)pexp_each[sl.inst];  // a batch whose instances sit at different alphas
   double acc[1] = {0.0};
   const T clampv = (T)1e-12;
   for (int64_t r = sl.lo + threadIdx.x; r < sl.hi; r += blockDim.x) {
@@ -109,6 +112,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_start_kernel(M map, const T* __re
 // ||x0||, sigma_0 and the fresh state of a started solve (phase 2 of the norm of lz_start).
 template <class F>
 __global__ __launch_bounds__(SA_BLOCK) void lz_start_finish_kernel(F fin, int32_t gate, int32_t only_if) {
+  if (!fin.live()) return;
   sa_state* st = state_of(fin.w());
   if (only_if >= 0 && st->status != only_if) return;
   const double tot = sum_partials(partials_of(fin.w()), fin.g(), 1, 0);
@@ -1049,6 +1053,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_ritz_kernel(M map, const T* __res
                                                            int32_t only_if) {
   __shared__ double coef[SA_MAX_NCV];
   const Slice sl = map.slice();
+  if (!sl.live) return;
   const sa_state* st = state_of(sl.work);
   if (only_if >= 0 && st->status != only_if) return;
   const int m = st->j;
@@ -1086,7 +1091,7 @@ static int lanczos_start(void* work, int64_t n, const T* s, const T* x, const T*
   SA_REQUIRE(work && n > 0 && x && V && Z && ldv >= n, "bad arguments");
   const int G = red_grid(n);
   hipLaunchKernelGGL((lz_start_kernel<T, OneMap>), dim3(G), dim3(SA_BLOCK), 0, stream, OneMap{work, n}, s, x, d,
-                     (T)pexp, cx, y, cy, V, Z, gate, only_if);
+                     (T)pexp, (const double*)nullptr, cx, y, cy, V, Z, gate, only_if);
   int rc = check_launch("lanczos_start");
   if (rc) return rc;
   hipLaunchKernelGGL(lz_start_finish_kernel<OneFin>, dim3(1), dim3(SA_BLOCK), 0, stream, OneFin{work, G}, gate,
@@ -1200,11 +1205,11 @@ bool batch_ok(const sa_batch* Bt) {
 
 BatchMap bmap(const sa_batch* Bt) {
   return BatchMap{reinterpret_cast<char*>(Bt->works), Bt->stride, Bt->row_off, Bt->row_end, Bt->blk_off, Bt->B,
-                  Bt->rows_per_block};
+                  Bt->rows_per_block, Bt->inst_mask};
 }
 
 BatchFin bfin(const sa_batch* Bt) {
-  return BatchFin{reinterpret_cast<char*>(Bt->works), Bt->stride, Bt->blk_off};
+  return BatchFin{reinterpret_cast<char*>(Bt->works), Bt->stride, Bt->blk_off, Bt->inst_mask};
 }
 
 static int batch_begin(const sa_batch* Bt, int32_t ncv, int32_t maxiter, double tol, const double* bounds,
@@ -1219,11 +1224,12 @@ static int batch_begin(const sa_batch* Bt, int32_t ncv, int32_t maxiter, double 
 }
 
 template <typename T>
-static int batch_start(const sa_batch* Bt, const T* s, const T* x, const T* d, double pexp, double cx, const T* y,
-                       double cy, T* V, int64_t ldv, T* Z, int32_t gate, int32_t only_if, hipStream_t stream) {
+static int batch_start(const sa_batch* Bt, const T* s, const T* x, const T* d, double pexp, const double* pexp_each,
+                       double cx, const T* y, double cy, T* V, int64_t ldv, T* Z, int32_t gate, int32_t only_if,
+                       hipStream_t stream) {
   SA_REQUIRE(batch_ok(Bt) && x && V && Z, "bad arguments");
   hipLaunchKernelGGL((lz_start_kernel<T, BatchMap>), dim3((unsigned)Bt->nblocks), dim3(SA_BLOCK), 0, stream, bmap(Bt),
-                     s, x, d, (T)pexp, cx, y, cy, V, Z, gate, only_if);
+                     s, x, d, (T)pexp, pexp_each, cx, y, cy, V, Z, gate, only_if);
   int rc = check_launch("batch_lanczos_start");
   if (rc) return rc;
   hipLaunchKernelGGL(lz_start_finish_kernel<BatchFin>, dim3(Bt->B), dim3(SA_BLOCK), 0, stream, bfin(Bt), gate,
@@ -1237,6 +1243,7 @@ template <typename T>
 static int batch_step(const sa_csr* A, const sa_batch* Bt, const T* s, T* V, int64_t ldv, T* Z, T* W,
                       hipStream_t stream) {
   SA_REQUIRE(A && batch_ok(Bt) && V && Z && W && A->tile_rows && ldv >= A->n, "bad arguments");
+  SA_REQUIRE(!Bt->inst_mask, "the batched step selects instances by status, not by a mask");
   SA_REQUIRE(A->npanels == 0, "the batched step uses the plain CSR layout");
   SA_REQUIRE(aligned16(V) && aligned16(W) && aligned16(Z) && (!s || aligned16(s)) && ldv % vec16<T>::L == 0,
              "V, W, Z and s must be 16-byte aligned and ldv a multiple of 16 bytes");
@@ -1249,6 +1256,8 @@ static int batch_step(const sa_csr* A, const sa_batch* Bt, const T* s, T* V, int
     p.gate_base = reinterpret_cast<const char*>(Bt->works) + offsetof(sa_state, status);
     p.gate_stride = Bt->stride;
   }
+  p.tile_list = Bt->tile_list;
+  p.ntile_list = Bt->ntile_list;
   int rc = launch_spmv<T>(p, A->tile_nnz, stream);
   if (rc) return rc;
   const dim3 grid((unsigned)Bt->nblocks), fgrid(Bt->B);
@@ -1270,6 +1279,7 @@ static int batch_step(const sa_csr* A, const sa_batch* Bt, const T* s, T* V, int
 template <typename T>
 static int batch_thick_restart(const sa_batch* Bt, const T* s, T* V, int64_t ldv, T* Z, hipStream_t stream) {
   SA_REQUIRE(batch_ok(Bt) && V && Z, "bad arguments");
+  SA_REQUIRE(!Bt->inst_mask, "the batched restart selects instances by status, not by a mask");
   SA_REQUIRE(aligned16(V) && aligned16(Z) && aligned16(s) && ldv % vec16<T>::L == 0,
              "V, Z and s must be 16-byte aligned and ldv a multiple of 16 bytes");
   hipLaunchKernelGGL(lz_thick_prepare_kernel<BatchFin>, dim3(Bt->B), dim3(SA_BLOCK), 0, stream, bfin(Bt));
@@ -1366,13 +1376,27 @@ int sa_batch_lanczos_begin(const sa_batch* Bt, int32_t ncv, int32_t maxiter, dou
 int sa_batch_lanczos_start_f32(const sa_batch* Bt, const float* s, const float* x, const float* d, double pexp,
                                double cx, const float* y, double cy, float* V, int64_t ldv, float* Z, int32_t gate,
                                int32_t only_if, hipStream_t stream) {
-  return batch_start<float>(Bt, s, x, d, pexp, cx, y, cy, V, ldv, Z, gate, only_if, stream);
+  return batch_start<float>(Bt, s, x, d, pexp, nullptr, cx, y, cy, V, ldv, Z, gate, only_if, stream);
+}
+
+int sa_batch_lanczos_start_each_f32(const sa_batch* Bt, const float* s, const float* x, const float* d,
+                                    const double* pexp, float* V, int64_t ldv, float* Z, int32_t gate,
+                                    hipStream_t stream) {
+  SA_REQUIRE(pexp && d, "bad arguments");
+  return batch_start<float>(Bt, s, x, d, 0.0, pexp, 1.0, nullptr, 0.0, V, ldv, Z, gate, -1, stream);
 }
 
 int sa_batch_lanczos_start_f64(const sa_batch* Bt, const double* s, const double* x, const double* d, double pexp,
                                double cx, const double* y, double cy, double* V, int64_t ldv, double* Z,
                                int32_t gate, int32_t only_if, hipStream_t stream) {
-  return batch_start<double>(Bt, s, x, d, pexp, cx, y, cy, V, ldv, Z, gate, only_if, stream);
+  return batch_start<double>(Bt, s, x, d, pexp, nullptr, cx, y, cy, V, ldv, Z, gate, only_if, stream);
+}
+
+int sa_batch_lanczos_start_each_f64(const sa_batch* Bt, const double* s, const double* x, const double* d,
+                                    const double* pexp, double* V, int64_t ldv, double* Z, int32_t gate,
+                                    hipStream_t stream) {
+  SA_REQUIRE(pexp && d, "bad arguments");
+  return batch_start<double>(Bt, s, x, d, 0.0, pexp, 1.0, nullptr, 0.0, V, ldv, Z, gate, -1, stream);
 }
 
 int sa_batch_lanczos_step_f32(const sa_csr* A, const sa_batch* Bt, const float* s, float* V, int64_t ldv, float* Z,

@@ -3,7 +3,8 @@
 // buffer) and a batch of instances packed block-diagonally (BatchMap: instance b owns rows
 // [row_off[b], row_off[b+1]) and workgroups [blk_off[b], blk_off[b+1]), and its own work buffer).
 // The one-workgroup finishers take the matching Fin type: OneFin runs one workgroup on one work
-// buffer; BatchFin runs workgroup b on instance b's.
+// buffer; BatchFin runs workgroup b on instance b's. A batch launch may carry an instance mask: the
+// workgroups of masked-out instances return at once, leaving their rows and state untouched.
 #pragma once
 
 #include "sa_common.hpp"
@@ -17,6 +18,7 @@ struct Slice {
   int G;
   int inst;        // instance index (0 for one problem)
   int64_t base;    // first row of the instance
+  bool live;       // false: the launch leaves this instance alone (a masked batch)
 };
 
 struct OneMap {
@@ -33,6 +35,7 @@ struct OneMap {
     s.G = gridDim.x;
     s.inst = 0;
     s.base = 0;
+    s.live = true;
     return s;
   }
 };
@@ -45,6 +48,7 @@ struct BatchMap {
   const int32_t* blk_off;  // B + 1
   int32_t B;
   int32_t rows_per_block;  // a multiple of 64
+  const uint8_t* mask;     // NULL, or B bytes: only instances with a nonzero byte are live
   static constexpr bool kScaleW = true;
   __device__ inline Slice slice() const {
     int lo = 0, hi = B - 1;  // the instance b with blk_off[b] <= blockIdx.x < blk_off[b + 1]
@@ -61,6 +65,7 @@ struct BatchMap {
     s.hi = s.lo + rows_per_block < end ? s.lo + rows_per_block : end;
     s.inst = lo;
     s.base = row_off[lo];
+    s.live = !mask || mask[lo] != 0;
     return s;
   }
 };
@@ -72,16 +77,19 @@ struct OneFin {
   __device__ inline void* w() const { return work; }
   __device__ inline int g() const { return G; }
   __device__ inline int idx() const { return 0; }
+  __device__ inline bool live() const { return true; }
 };
 
 struct BatchFin {
   char* works;
   int64_t stride;
   const int32_t* blk_off;
+  const uint8_t* mask;  // as BatchMap::mask
   static constexpr bool kScaleW = true;
   __device__ inline void* w() const { return works + (int64_t)blockIdx.x * stride; }
   __device__ inline int g() const { return blk_off[blockIdx.x + 1] - blk_off[blockIdx.x]; }
   __device__ inline int idx() const { return blockIdx.x; }
+  __device__ inline bool live() const { return !mask || mask[blockIdx.x] != 0; }
 };
 
 }  // namespace sa

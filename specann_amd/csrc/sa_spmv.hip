@@ -152,7 +152,7 @@ __global__ __launch_bounds__(SA_BLOCK) void spmv_tile_kernel(SpmvParams<T> p) {
   double bb = p.b;
   if (p.gate && *p.gate != SA_RUNNING) return;
   if (p.lz) bb *= p.lz->sigma[p.lz->j];
-  const int64_t t = blockIdx.x;
+  const int64_t t = p.tile_list ? (int64_t)p.tile_list[blockIdx.x] : (int64_t)blockIdx.x;
   if (p.tile_inst &&
       *reinterpret_cast<const int32_t*>(p.gate_base + (int64_t)p.tile_inst[t] * p.gate_stride) != SA_RUNNING)
     return;
@@ -313,6 +313,8 @@ SpmvParams<T> params_from(const sa_csr* A) {
   p.tile_inst = nullptr;
   p.gate_base = nullptr;
   p.gate_stride = 0;
+  p.tile_list = nullptr;
+  p.ntile_list = 0;
   p.y = nullptr;
   p.npanels = A->npanels;
   p.pindptr = A->pindptr;
@@ -373,11 +375,12 @@ int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream) {
 #undef SA_PANEL_CASE
     return check_launch("spmv_panel");
   }
-  if (p.ntiles < 1 || p.ntiles >= (1ll << 31)) {
+  if (p.ntiles < 1 || p.ntiles >= (1ll << 31) || (p.tile_list && (p.ntile_list < 0 || p.ntile_list > p.ntiles))) {
     set_error("spmv: ntiles out of range");
     return SA_E_ARG;
   }
-  const dim3 grid((unsigned)p.ntiles);
+  if (p.tile_list && p.ntile_list == 0) return SA_OK;  // no tile to run
+  const dim3 grid((unsigned)(p.tile_list ? p.ntile_list : p.ntiles));
   switch (tile_nnz) {
     case 512: hipLaunchKernelGGL((spmv_tile_kernel<T, 512>), grid, block, 0, stream, p); break;
     case 1024: hipLaunchKernelGGL((spmv_tile_kernel<T, 1024>), grid, block, 0, stream, p); break;

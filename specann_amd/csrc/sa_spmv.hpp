@@ -30,6 +30,10 @@ struct SpmvParams {
   const int32_t* tile_inst;
   const char* gate_base;
   int64_t gate_stride;
+  // NULL, or the plain-layout tiles to run (ntile_list of them, each in [0, ntiles)): workgroup w runs
+  // tile tile_list[w], so tiles left out cost no launch.
+  const int32_t* tile_list;
+  int64_t ntile_list;
   T* y;
   // XCD panel layout (npanels > 0): partial row sums per panel go to partial[p * n + r].
   int32_t npanels;

@@ -28,6 +28,7 @@ template <class F>
 __global__ __launch_bounds__(SA_BLOCK) void finish_kernel(F fin, FinishSpec spec) {
   __shared__ double red[SA_WAVES_PER_BLOCK];
   __shared__ uint64_t redu[SA_WAVES_PER_BLOCK];
+  if (!fin.live()) return;
   void* work = fin.w();
   const int G = fin.g();
   sa_state* st = state_of(work);
@@ -67,8 +68,7 @@ int launch_finish(void* work, int G, const FinishSpec& spec, hipStream_t stream)
 }
 
 int launch_finish_batch(const sa_batch* Bt, const FinishSpec& spec, hipStream_t stream) {
-  hipLaunchKernelGGL(finish_kernel<BatchFin>, dim3(Bt->B), dim3(SA_BLOCK), 0, stream,
-                     BatchFin{reinterpret_cast<char*>(Bt->works), Bt->stride, Bt->blk_off}, spec);
+  hipLaunchKernelGGL(finish_kernel<BatchFin>, dim3(Bt->B), dim3(SA_BLOCK), 0, stream, bfin(Bt), spec);
   return check_launch("finish_batch");
 }
 

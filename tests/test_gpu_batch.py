@@ -5,7 +5,7 @@ import torch
 
 from oracle import specann_oracle as O
 from specann_amd import graphs
-from specann_amd.batch import BatchProblem, BatchSweep, pack
+from specann_amd.batch import TERMINAL, BatchProblem, BatchSweep, pack
 from specann_amd.spectral_annealing import SpectralAnnealing
 from specann_amd.device import DeviceProblem
 
@@ -128,3 +128,133 @@ def test_batch_edge_cases(gpu):
                 best = min(O.energy_int64(Jint, np.array([1 if (m >> i) & 1 else -1 for i in range(J.shape[0])]))
                            for m in range(1 << J.shape[0]))
                 assert res.best_energy >= best
+
+
+def _rows_equal(r1, r2):
+    keys = ("alpha_index", "eigsh_iterations", "eigsh_converged", "lambda1", "rho", "gershgorin", "energy_bound",
+            "residual_estimate", "lanczos_restarts", "failed", "candidate", "energy", "raw_energy")
+    for k in keys:
+        a, b = r1.get(k), r2.get(k)
+        if isinstance(a, float) and isinstance(b, float) and np.isnan(a) and np.isnan(b):
+            continue
+        assert a == b, (k, a, b)
+    a, b = r1["pred_residual"], r2["pred_residual"]
+    assert (np.isnan(a) and np.isnan(b)) or a == b, ("pred_residual", a, b)
+
+
+@pytest.mark.parametrize("tdtype,pred", [(torch.float32, "diagonal"), (torch.float32, "secant"),
+                                         (torch.float32, "prev"), (torch.float64, "secant")])
+def test_async_schedule_gives_the_lockstep_bits(gpu, tdtype, pred):
+    """Each instance's arithmetic does not depend on when the others move on: moving every instance to
+    its next alpha as soon as its own solve stops gives the lockstep schedule's eigenvalues, step
+    counts, candidates and energies bit for bit."""
+    Js = _instances()
+    print(f"async vs lockstep dtype={tdtype} pred={pred}", flush=True)
+    bp = BatchProblem(Js, tdtype, rows_per_block=256)
+    tol = 1e-6 if tdtype == torch.float32 else 1e-10
+    kw = dict(random_seed=11, num_alphas=12, pred_mode=pred, eigsh_tol=tol)
+    sweep = BatchSweep(**kw)
+    fast = sweep.run(bp)
+    slow = BatchSweep(lockstep=True, **kw).run(bp)
+    assert sweep.retries == 0 and sweep.fallbacks == 0
+    assert fast.matvecs == slow.matvecs
+    for b in range(len(Js)):
+        f, s = fast.results[b], slow.results[b]
+        assert len(f.rows) == len(s.rows) == 12
+        for r1, r2 in zip(f.rows, s.rows):
+            _rows_equal(r1, r2)
+        assert f.raw_energies == s.raw_energies
+        assert f.refined_energies == s.refined_energies
+        assert f.best_energy == s.best_energy and f.best_alpha == s.best_alpha
+        assert np.array_equal(f.best_spins, s.best_spins)
+
+
+def test_async_schedule_retries_a_rejected_solve_cold(gpu, monkeypatch):
+    """A solve the acceptance test refuses is solved again from a random vector at the same alpha (the
+    one-instance sweep's retry); the retried solve converges to the same eigenvalue and the other
+    instances are untouched."""
+    Js = _instances()[:3]
+    bp = BatchProblem(Js, torch.float32, rows_per_block=256)
+    kw = dict(random_seed=5, num_alphas=6, pred_mode="secant")
+    ref = BatchSweep(lockstep=True, **kw).run(bp)
+    seen = []
+    orig = BatchSweep._ok
+
+    def ok(self, st, theta, g, ai):
+        res = orig(self, st, theta, g, ai)
+        if ai[1] == 3 and st[1] in TERMINAL and not seen and res[1]:   # refuse instance 1 at alpha 3, once
+            res = res.copy()
+            res[1] = False
+            seen.append(1)
+        return res
+
+    monkeypatch.setattr(BatchSweep, "_ok", ok)
+    sweep = BatchSweep(**kw)
+    out = sweep.run(bp)
+    assert seen == [1] and sweep.retries == 1
+    row = out.results[1].rows[3]
+    assert row["retry"] and not row["failed"] and row["eigsh_converged"]
+    assert row["lambda1"] == pytest.approx(ref.results[1].rows[3]["lambda1"], rel=2e-5)
+    assert sum(r["retry"] for r in out.results[1].rows) == 1
+    for b in (0, 2):
+        for r1, r2 in zip(out.results[b].rows, ref.results[b].rows):
+            _rows_equal(r1, r2)
+
+
+def test_masked_batch_entry_points(gpu):
+    """A mask limits the per-instance launches to its instances; the per-instance alpha and predictor
+    exponent forms give the scalar forms' bits; the step refuses a mask."""
+    import ctypes
+    from specann_amd import _lib
+    from specann_amd._lib import SaBatch, call
+    from specann_amd.device import ptr, stream
+    Js = _instances()
+    bp = BatchProblem(Js, torch.float32, rows_per_block=256)
+    prob, B, n, dev = bp.prob, bp.B, bp.n, bp.device
+    mask = np.array([1, 0, 1, 0, 1], dtype=np.uint8)
+    mdev = torch.from_numpy(mask).to(dev)
+    desc = SaBatch.from_buffer_copy(bp.desc)
+    desc.inst_mask = mdev.data_ptr()
+    M = ctypes.byref(desc)
+    alphas = np.array([0.3, 0.6, 0.45, 0.9, 0.7])
+    s_ref = torch.full((n,), 7.0, device=dev)
+    s_got = torch.full((n,), 7.0, device=dev)
+    for b in range(B):                       # scalar form, one alpha at a time, kept on instance b's rows
+        tmp = torch.empty(n, device=dev)
+        call("sa_batch_alpha_scaling_f32", bp.ref, ptr(prob.d), float(alphas[b]), 1e-12, ptr(tmp), stream())
+        if mask[b]:
+            s_ref[bp.rows_of(b)] = tmp[bp.rows_of(b)]
+    a_dev = torch.from_numpy(alphas).to(dev)
+    call("sa_batch_alpha_scaling_each_f32", M, ptr(prob.d), ptr(a_dev), 1e-12, ptr(s_got), stream())
+    assert torch.equal(s_got, s_ref)
+    x = torch.randn(n, device=dev)
+    y = torch.full((n,), -1.0, device=dev)
+    call("sa_batch_copy_f32", M, ptr(x), ptr(y), stream())
+    for b in range(B):
+        r = bp.rows_of(b)
+        assert torch.equal(y[r], x[r] if mask[b] else torch.full_like(x[r], -1.0))
+    # per-instance predictor exponent against the scalar one
+    ldv = _lib.padded_ld(n, 4)
+    V1 = torch.zeros((2, ldv), device=dev)
+    V2 = torch.zeros((2, ldv), device=dev)
+    Z1, Z2 = torch.zeros(n, device=dev), torch.zeros(n, device=dev)
+    bounds = torch.full((B,), 10.0, dtype=torch.float64, device=dev)
+    call("sa_batch_lanczos_begin", bp.ref, 8, 100, 1e-6, ptr(bounds), 1e-3, 2, 0.0, stream())
+    pexp = np.array([0.05, 0.0, 0.11, 0.3, 0.02])
+    for b in range(B):
+        if mask[b]:
+            d1 = SaBatch.from_buffer_copy(bp.desc)
+            one = torch.from_numpy((np.arange(B) == b).astype(np.uint8)).to(dev)
+            d1.inst_mask = one.data_ptr()
+            call("sa_batch_lanczos_start_f32", ctypes.byref(d1), ptr(s_got), ptr(x), ptr(prob.d), float(pexp[b]),
+                 1.0, None, 0.0, ptr(V1), ldv, ptr(Z1), 1, -1, stream())
+    p_dev = torch.from_numpy(pexp).to(dev)
+    call("sa_batch_lanczos_start_each_f32", M, ptr(s_got), ptr(x), ptr(prob.d), ptr(p_dev), ptr(V2), ldv, ptr(Z2), 1,
+         stream())
+    assert torch.equal(V1, V2) and torch.equal(Z1, Z2)
+    heads = bp.heads()
+    assert np.all(heads["status"][mask == 0] == _lib.SA_RUNNING)
+    assert np.all(heads["r"][mask == 0, 3] == 0.0) and np.all(heads["r"][mask == 1, 3] > 0.0)  # start norms
+    W = torch.zeros(n, device=dev)
+    with pytest.raises(RuntimeError, match="mask"):
+        call("sa_batch_lanczos_step_f32", prob.csr_ref, M, ptr(s_got), ptr(V2), ldv, ptr(Z2), ptr(W), stream())
