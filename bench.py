@@ -80,25 +80,33 @@ def run_batch(args, w):
 
     tunes = {}
 
-    def sweep(dtype):
-        bp = BatchProblem(Js, dtype)
-        tunes[str(dtype)] = bp.autotune()["choice"]
-        log(f"batched SpMV tiles ({dtype}): {tunes[str(dtype)]}")
+    def sweep(dtype, bp=None, lockstep=False):
+        if bp is None:
+            bp = BatchProblem(Js, dtype)
+            tunes[str(dtype)] = bp.autotune()["choice"]
+            log(f"batched SpMV tiles ({dtype}): {tunes[str(dtype)]}")
         torch.cuda.synchronize()
         t = time.perf_counter()
-        out = BatchSweep(random_seed=42, num_alphas=args.alphas, pred_mode=pred).run(bp)
+        sw = BatchSweep(random_seed=42, num_alphas=args.alphas, pred_mode=pred, lockstep=lockstep)
+        out = sw.run(bp)
         torch.cuda.synchronize()
-        return time.perf_counter() - t, out, bp
+        return time.perf_counter() - t, out, bp, sw
 
-    sec32, out32, bp32 = sweep(torch.float32)
+    sec32, out32, bp32, sw32 = sweep(torch.float32)
     log(f"fp32 batched sweep: {sec32:.3f}s ({out32.steps} batched steps, {out32.matvecs} instance operator "
-        f"applications, local search {out32.post_sweep_time:.3f}s, {out32.local_passes} passes)")
+        f"applications, local search {out32.post_sweep_time:.3f}s, {out32.local_passes} passes, "
+        f"{sw32.retries} cold retries)")
+    lock32 = None
+    if not args.skip_lockstep:
+        lock32, outl, _, _ = sweep(torch.float32, bp32, lockstep=True)
+        same = all(a.best_energy == b.best_energy for a, b in zip(out32.results, outl.results))
+        log(f"fp32 lockstep batched sweep: {lock32:.3f}s ({outl.steps} batched steps; best energies equal: {same})")
     spmv_sec, spmv_bytes = spmv_roofline(bp32.prob)
     achieved = spmv_bytes / spmv_sec / 1e9
     del bp32
     sec64 = out64 = None
     if not args.skip_fp64:
-        sec64, out64, _ = sweep(torch.float64)
+        sec64, out64, _, _ = sweep(torch.float64)
         log(f"fp64 batched sweep: {sec64:.3f}s")
     sample = min(args.c3_sample, len(Js))
     solo = None
@@ -114,26 +122,56 @@ def run_batch(args, w):
     gaps = None
     if out64 is not None:
         e64 = [r.best_energy for r in out64.results]
-        gaps = [abs(a - b) / abs(b) for a, b in zip(e32, e64) if b]
+        gaps = [(a - b) / abs(b) for a, b in zip(e32, e64) if b]
     conv = float(np.mean([r["eigsh_converged"] for res in out32.results for r in res.rows]))
+    cpu = None
+    if not args.skip_cpu:
+        # the CPU path (scipy ARPACK on S_rho, numpy local search) on the instances of up to 2^13 nodes
+        # among every 16th one, extrapolated to the batch by stored entries
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle import specann_oracle as O
+        picks = [i for i in range(0, len(Js), 16) if Js[i].shape[0] <= (1 << 13)]
+        t = time.perf_counter()
+        for i in picks:
+            O.cpu_sweep(Js[i], precision=32, num_alphas=args.alphas, pred_mode="diagonal", seed=42)
+        csec = time.perf_counter() - t
+        snnz = sum(Js[i].nnz for i in picks)
+        cores = len(os.sched_getaffinity(0))
+        cpu = {"value": csec * nnz / snnz if snnz else None, "unit": "s (extrapolated by stored entries)",
+               "cores": 1, "kind": "port",
+               "sample": (f"the full {args.alphas}-alpha CPU path (scipy ARPACK eigsh on S_rho, diagonal predictor, "
+                          f"numpy local search) on {len(picks)} of the instances (every 16th with n <= 2^13: n = "
+                          f"{[int(Js[i].shape[0]) for i in picks]}), measured {csec:.1f} s for {snnz} stored "
+                          f"entries; value = that x {nnz} / {snnz} (extrapolated). Single-threaded; {cores} cores "
+                          f"visible")}
+        log(f"cpu sample: {csec:.3f}s on {len(picks)} instances")
     line = {
         "metric": METRIC, "value": sec32, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": sec32 * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": w["text"], "alphas": args.alphas, "predictor": pred, "instances": len(Js),
                    "n_total": N, "nnz_total": nnz,
-                   "layout": "packed block-diagonal CSR, per-instance tiles; tile size and lanes per row autotuned",
+                   "layout": ("packed block-diagonal CSR; the autotuner picks plain per-instance tiles (tile size, lanes "
+                              "per row) or the instance-staged layout (rows per workgroup, lanes per row)"),
                    "tiles": tunes},
         "fp64_sweep_s": sec64, "fp32_vs_fp64_speedup": (sec64 / sec32) if sec64 else None,
-        "energy_gap_max": max(gaps) if gaps else None, "converged_fraction": conv,
+        # (E32 - E64) / |E64| per instance: negative = FP32 found the lower energy
+        "energy_gap_max": max(abs(x) for x in gaps) if gaps else None,
+        "energy_gap_median": float(np.median(np.abs(gaps))) if gaps else None,
+        "energy_gap_fp32_worse_max": max([0.0] + gaps) if gaps else None,
+        "instances_gap_above_1e-3": int(sum(abs(x) > 1e-3 for x in gaps)) if gaps else None,
+        "converged_fraction": conv, "cold_retries": sw32.retries,
+        "lockstep_sweep_s": lock32,
         "batched_steps": out32.steps, "operator_applications": out32.matvecs,
         "one_at_a_time_sample": {"instances": sample, "seconds": solo,
                                  "extrapolated_all_s": solo * len(Js) / sample if solo else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "spmv_tile_kernel<float> (fused scaled SpMV over the packed CSR)",
+                     "kernel": ("spmv_inst_kernel<float> (instance slices of the gathered vector staged in LDS)"
+                                if tunes[str(torch.float32)]["kind"] == "instance-staged" else
+                                "spmv_tile_kernel<float> (fused scaled SpMV over the packed CSR)"),
                      "bytes_per_launch": spmv_bytes, "us_per_launch": spmv_sec * 1e6},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -307,6 +345,8 @@ def main(argv=None) -> int:
                     help="only the SpMV timing loop and a copy kernel of known size (for counter profiling)")
     ap.add_argument("--c3-sample", type=int, default=16,
                     help="batch workload: instances also swept one at a time, for comparison")
+    ap.add_argument("--skip-lockstep", action="store_true",
+                    help="batch workload: skip the lockstep schedule's sweep (every instance at the same alpha)")
     ap.add_argument("--layout", default=None, metavar="NPANELS,TILE",
                     help="fixed SpMV layout instead of the autotuner's choice, e.g. 8,1024, or blocked (LDS-blocked "
                          "layout; blocked:BROWS,BCOLS for a given geometry)")

@@ -98,6 +98,30 @@ typedef struct sa_csr {
   const uint32_t* bent;     /* 64 * bchunk[8 * bnr * bnc] entries */
   const void* bvals;        /* entry values (T; 0 for padding), or NULL when psigned */
   const int32_t* rbound;    /* NULL, or bnr + 1 row-block boundaries (see above) */
+  /* Optional instance-staged layout of a block-diagonal batch (imode = 1, npanels = 0; no counterpart
+   * in the reference). Instance i couples only rows and columns [ibase[i], ireal[i]) (rows up to
+   * iend[i] are padding without entries). Workgroup w owns rows [irow[w], min(irow[w] + irows,
+   * iend[iinst[w]])) of instance iinst[w]; it stages the instance's slice of the gathered vector in
+   * LDS, sa_inst_columns_per_pass(sizeof(T)) columns per pass (ipasses at most 4 over the batch),
+   * and sums each row from LDS with ilanes lanes (4, 8, 16 or 32), reading the row's entries of pass
+   * c in [isplit[(c-1) n + r], isplit[c n + r]) of the CSR (indptr bounds the first and last pass).
+   * isigned = 1: every coupling of instance i has the magnitude imag[i] (T) and ient holds
+   * column | sign << 31 for each entry; else indices and vals are read. */
+  int32_t imode;
+  int32_t irows;            /* rows per workgroup, 64..2048 */
+  int32_t ilanes;
+  int32_t ipasses;
+  int32_t isigned;
+  int32_t pad4;
+  int64_t inwg;             /* workgroups */
+  const int32_t* iinst;     /* inwg */
+  const int32_t* irow;      /* inwg */
+  const int64_t* ibase;     /* B */
+  const int64_t* ireal;     /* B */
+  const int64_t* iend;      /* B */
+  const int32_t* isplit;    /* (ipasses - 1) * n, or NULL when ipasses = 1 */
+  const int32_t* ient;      /* nnz when isigned */
+  const void* imag;         /* B values of T when isigned */
 } sa_csr;
 
 /* A batch of independent instances packed block-diagonally into one CSR (no counterpart in the
@@ -330,6 +354,8 @@ int sa_norm_dot_f64(void* work, int64_t n, const double* x, const double* y, dou
                     hipStream_t stream);
 /* out_i = scale * x_i (in T). */
 int sa_scale_copy_f32(int64_t n, const float* x, double scale, float* out, hipStream_t stream);
+/* Columns the instance-staged layout stages per pass for elements of elem_bytes (4 or 8; -1 else). */
+int sa_inst_columns_per_pass(int32_t elem_bytes);
 int sa_scale_copy_f64(int64_t n, const double* x, double scale, double* out, hipStream_t stream);
 
 /* out = a x + b y + c z (FP64 arithmetic; y or z may be NULL when its coefficient is 0). The vector

@@ -87,6 +87,21 @@ class SaCsr(ctypes.Structure):
         ("bent", c_void_p),
         ("bvals", c_void_p),
         ("rbound", c_void_p),
+        ("imode", c_int32),
+        ("irows", c_int32),
+        ("ilanes", c_int32),
+        ("ipasses", c_int32),
+        ("isigned", c_int32),
+        ("pad4", c_int32),
+        ("inwg", c_int64),
+        ("iinst", c_void_p),
+        ("irow", c_void_p),
+        ("ibase", c_void_p),
+        ("ireal", c_void_p),
+        ("iend", c_void_p),
+        ("isplit", c_void_p),
+        ("ient", c_void_p),
+        ("imag", c_void_p),
     ]
 
 
@@ -185,6 +200,7 @@ _SIGNATURES = {
     "sa_version": (c_int32, []),
     "sa_last_error": (c_int32, [ctypes.c_char_p, c_size_t]),
     "sa_work_bytes": (c_size_t, [c_int64]),
+    "sa_inst_columns_per_pass": (c_int32, [c_int32]),
     "sa_state_bytes": (c_size_t, []),
     "sa_prepare_values_f32": (c_int32, [c_int64, _P, _P, _P, _P, _P]),
     "sa_prepare_values_f64": (c_int32, [c_int64, _P, _P, _P, _P, _P]),

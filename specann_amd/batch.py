@@ -68,6 +68,42 @@ def pack(Js: list[sp.spmatrix]) -> tuple[sp.csr_matrix, np.ndarray, np.ndarray]:
     return Jp, row_off, sizes
 
 
+def inst_arrays(indptr, indices, vals, jint, row_off, sizes, kc: int) -> dict:
+    """The per-instance and per-row tables of the instance-staged layout, built with torch on the
+    tensors' device: base / real / end (int64 per instance: first row, end of the real columns, end of the
+    rows with padding), split ((passes - 1) * n int32: split[(c - 1) n + r] is the first entry of row r
+    whose column lies at or past base + c * kc, so pass c reads [split[c - 1][r], split[c][r]) with
+    indptr at both ends), and, when every coupling of an instance has one magnitude (signed), ent
+    (column | sign << 31 per entry) and mag (the magnitude per instance, vals' type)."""
+    dev = indptr.device
+    ro = np.asarray(row_off, dtype=np.int64)
+    B, n = len(sizes), int(ro[-1])
+    passes = int(-(-int(np.max(sizes)) // kc))
+    out = {"base": torch.from_numpy(ro[:-1].copy()).to(dev),
+           "real": torch.from_numpy(ro[:-1] + np.asarray(sizes, dtype=np.int64)).to(dev),
+           "end": torch.from_numpy(ro[1:].copy()).to(dev)}
+    deg = torch.diff(indptr.to(torch.int64))
+    inst_row = torch.from_numpy(np.repeat(np.arange(B, dtype=np.int64), np.diff(ro))).to(dev)
+    row_e = torch.repeat_interleave(torch.arange(n, device=dev), deg)
+    inst_e = inst_row[row_e]
+    rel = indices.to(torch.int64) - out["base"][inst_e]    # the column's offset in its instance
+    split = []
+    for c in range(1, passes):
+        below = torch.zeros(n, dtype=torch.int64, device=dev)
+        below.index_add_(0, row_e, (rel < c * kc).to(torch.int64))
+        split.append((indptr[:-1].to(torch.int64) + below).to(torch.int32))
+    out["split"] = torch.cat(split) if split else None
+    out["passes"] = passes
+    mags = torch.abs(jint.to(torch.int64))
+    mx = torch.zeros(B, dtype=torch.int64, device=dev).scatter_reduce_(0, inst_e, mags, "amax")
+    out["signed"] = bool(torch.all(mags == mx[inst_e]).item())
+    if out["signed"]:
+        out["ent"] = torch.bitwise_or(indices, (vals < 0).to(torch.int32) << 31)
+        first = indptr[torch.from_numpy(ro[:-1].copy()).to(dev)].to(torch.int64)
+        out["mag"] = torch.abs(vals[first.clamp(max=max(int(indices.numel()) - 1, 0))]).contiguous()
+    return out
+
+
 class BatchProblem:
     """The packed couplings on the GPU plus the batch descriptor (row and workgroup tables and one
     work buffer per instance)."""
@@ -136,6 +172,9 @@ class BatchProblem:
             inst.append(np.full(rows.size, b, dtype=np.int32))
         tile_rows = np.concatenate(starts + [np.array([self.n], dtype=np.int64)]).astype(np.int32)
         self.tile_off_h = np.concatenate([[0], np.cumsum([t.size for t in starts])]).astype(np.int64)
+        self.unit_start, self.unit_count = self.tile_off_h[:-1], np.diff(self.tile_off_h)
+        self.layout = {"kind": "tiles", "tile_nnz": int(tile_nnz), "lanes_per_row": int(lanes_per_row)}
+        self.prob.csr.imode = 0
         dev = self.device
         self.tile_rows = torch.from_numpy(tile_rows).to(dev)
         self.tile_inst = torch.from_numpy(np.concatenate(inst)).to(dev)
@@ -149,20 +188,78 @@ class BatchProblem:
         prob.csr.lanes_per_row = int(lanes_per_row)
         self.desc.tile_inst = self.tile_inst.data_ptr()
 
+    def inst_eligible(self) -> bool:
+        """Whether every instance fits the instance-staged layout (at most 4 passes of staged columns)."""
+        kc = int(_lib.load().sa_inst_columns_per_pass(self.prob.vals.element_size()))
+        return int(self.sizes.max()) <= 4 * kc
+
+    def set_inst_layout(self, rows: int = 2048, lanes: int = 8) -> None:
+        """Switch the SpMV to the instance-staged layout (sa_csr imode 1): workgroups of `rows` rows of
+        one instance, the biggest instances' first, each staging its instance's slice of the gathered
+        vector in LDS and summing every row with `lanes` lanes."""
+        if not 64 <= rows <= 2048 or rows % 64 or lanes not in (4, 8, 16, 32):
+            raise ValueError("rows must be a multiple of 64 in [64, 2048] and lanes 4, 8, 16 or 32")
+        if not self.inst_eligible():
+            raise ValueError("an instance is too large for the instance-staged layout")
+        prob, dev = self.prob, self.device
+        kc = int(_lib.load().sa_inst_columns_per_pass(prob.vals.element_size()))
+        ro = self.row_off_h
+        order = np.argsort(-self.sizes, kind="stable")
+        wg_inst, wg_row = [], []
+        start = np.zeros(self.B, dtype=np.int64)
+        count = np.zeros(self.B, dtype=np.int64)
+        for b in order:
+            rs = np.arange(ro[b], ro[b + 1], rows, dtype=np.int64)
+            start[b] = len(wg_row)
+            count[b] = rs.size
+            wg_row.extend(rs.tolist())
+            wg_inst.extend([int(b)] * rs.size)
+        passes = int(-(-int(self.sizes.max()) // kc))
+        cache = self._inst_cache = getattr(self, "_inst_cache", None) or {}
+        if "base" not in cache:
+            cache.update(inst_arrays(prob.indptr, prob.indices, prob.vals, prob.jint, ro, self.sizes, kc))
+        cache["iinst"] = torch.tensor(wg_inst, dtype=torch.int32, device=dev)
+        cache["irow"] = torch.tensor(wg_row, dtype=torch.int32, device=dev)
+        csr = prob.csr
+        csr.imode = 1
+        csr.irows, csr.ilanes, csr.ipasses = int(rows), int(lanes), passes
+        csr.isigned = int(cache["signed"])
+        csr.inwg = len(wg_row)
+        csr.iinst, csr.irow = cache["iinst"].data_ptr(), cache["irow"].data_ptr()
+        csr.ibase, csr.ireal, csr.iend = cache["base"].data_ptr(), cache["real"].data_ptr(), cache["end"].data_ptr()
+        csr.isplit = cache["split"].data_ptr() if cache["split"] is not None else None
+        csr.ient = cache["ent"].data_ptr() if cache["signed"] else None
+        csr.imag = cache["mag"].data_ptr() if cache["signed"] else None
+        self.unit_start, self.unit_count = start, count
+        self.layout = {"kind": "instance-staged", "rows": int(rows), "lanes": int(lanes), "passes": passes,
+                       "signed": cache["signed"]}
+
     def autotune(self, reps: int = 4, verbose: bool = False) -> dict:
-        """Pick the tile size and the lanes per row (row-split across a wavefront) of the batched SpMV by
-        timing the packed matrix's SpMV on each (the batch keeps the plain per-instance tiles: its
-        instances' vectors are small enough to stay in L2)."""
+        """Pick the batched SpMV's layout by timing the packed matrix's SpMV on each: plain per-instance
+        tiles (tile size, lanes per row) and, when every instance fits, the instance-staged layout
+        (rows per workgroup, lanes per row)."""
         results = []
         for tile in (512, 1024, 2048, 4096):
             for lanes in (0, 4, 8, 16, 32, 64):
                 self.set_tiling(tile, lanes)
                 sec = self.prob.time_spmv(reps)
-                results.append({"tile_nnz": tile, "lanes_per_row": lanes, "seconds": sec})
+                results.append({"kind": "tiles", "tile_nnz": tile, "lanes_per_row": lanes, "seconds": sec})
                 if verbose:
                     print(f"[batch autotune] tile {tile} lanes {lanes}: {sec * 1e6:.1f} us", flush=True)
+        if self.inst_eligible():
+            for rows in (1024, 2048):
+                for lanes in (4, 8, 16):
+                    self.set_inst_layout(rows, lanes)
+                    sec = self.prob.time_spmv(reps)
+                    results.append({"kind": "instance-staged", "rows": rows, "lanes": lanes, "seconds": sec})
+                    if verbose:
+                        print(f"[batch autotune] instance-staged rows {rows} lanes {lanes}: {sec * 1e6:.1f} us",
+                              flush=True)
         best = min(results, key=lambda r: r["seconds"])
-        self.set_tiling(best["tile_nnz"], best["lanes_per_row"])
+        if best["kind"] == "tiles":
+            self.set_tiling(best["tile_nnz"], best["lanes_per_row"])
+        else:
+            self.set_inst_layout(best["rows"], best["lanes"])
         self.tuning = {"choice": best, "tried": results}
         return self.tuning
 
@@ -510,8 +607,8 @@ class BatchSweep:
             if int(np.count_nonzero(ai < A)) != sweeping:
                 # the SpMV launches only the tiles of the instances still sweeping
                 sweeping = int(np.count_nonzero(ai < A))
-                to = bp.tile_off_h
-                tiles = np.concatenate([np.arange(to[b], to[b + 1], dtype=np.int32) for b in np.nonzero(ai < A)[0]])
+                us, uc = bp.unit_start, bp.unit_count
+                tiles = np.concatenate([np.arange(us[b], us[b] + uc[b], dtype=np.int32) for b in np.nonzero(ai < A)[0]])
                 tile_list = torch.from_numpy(tiles).to(dev)
                 step_desc.tile_list = tile_list.data_ptr()
                 step_desc.ntile_list = int(tiles.size)

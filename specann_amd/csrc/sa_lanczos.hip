@@ -1251,11 +1251,10 @@ static int batch_step(const sa_csr* A, const sa_batch* Bt, const T* s, T* V, int
   p.g = Z;
   p.s = s;
   p.y = W;
-  if (Bt->tile_inst) {  // skip the tiles of instances that are not running
-    p.tile_inst = Bt->tile_inst;
-    p.gate_base = reinterpret_cast<const char*>(Bt->works) + offsetof(sa_state, status);
-    p.gate_stride = Bt->stride;
-  }
+  // the SpMV skips the tiles (or instance-staged workgroups) of instances that are not running
+  p.tile_inst = Bt->tile_inst;
+  p.gate_base = reinterpret_cast<const char*>(Bt->works) + offsetof(sa_state, status);
+  p.gate_stride = Bt->stride;
   p.tile_list = Bt->tile_list;
   p.ntile_list = Bt->ntile_list;
   int rc = launch_spmv<T>(p, A->tile_nnz, stream);

@@ -337,12 +337,31 @@ SpmvParams<T> params_from(const sa_csr* A) {
   p.bent = A->bent;
   p.bvals = A->bvals;
   p.rbound = A->rbound;
+  p.imode = A->imode;
+  p.irows = A->irows;
+  p.ilanes = A->ilanes;
+  p.ipasses = A->ipasses;
+  p.isigned = A->isigned;
+  p.inwg = A->inwg;
+  p.iinst = A->iinst;
+  p.irow = A->irow;
+  p.ibase = A->ibase;
+  p.ireal = A->ireal;
+  p.iend = A->iend;
+  p.isplit = A->isplit;
+  p.ient = A->ient;
+  p.imag = (const T*)A->imag;
   return p;
 }
 
 template <typename T>
 int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream) {
   const dim3 block(SA_BLOCK);
+  if (p.imode == 1 && p.npanels == 0) return launch_inst_spmv<T>(p, stream);
+  if (p.imode != 0 && p.imode != 1) {
+    set_error("spmv: unknown instance layout mode");
+    return SA_E_ARG;
+  }
   if (p.bmode == 1) return launch_block_spmv<T>(p, stream);
   if (p.bmode != 0) {
     set_error("spmv: unknown layout mode");

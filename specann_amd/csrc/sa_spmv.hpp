@@ -54,6 +54,17 @@ struct SpmvParams {
   const uint32_t* bent;
   const void* bvals;
   const int32_t* rbound;  // NULL (row block rb = rows [rb * brows, ...)) or bnr + 1 boundaries
+  // Instance-staged layout of a block-diagonal batch (imode = 1; see sa_inst.hip).
+  int32_t imode, irows, ilanes, ipasses, isigned;
+  int64_t inwg;
+  const int32_t* iinst;
+  const int32_t* irow;
+  const int64_t* ibase;
+  const int64_t* ireal;
+  const int64_t* iend;
+  const int32_t* isplit;
+  const int32_t* ient;
+  const T* imag;
 };
 
 template <typename T>
@@ -64,6 +75,9 @@ int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream);
 
 template <typename T>
 int launch_block_spmv(const SpmvParams<T>& p, hipStream_t stream);
+
+template <typename T>
+int launch_inst_spmv(const SpmvParams<T>& p, hipStream_t stream);
 
 template <typename T>
 int spmv_apply(const sa_csr* A, const T* s, const T* g, double a, const T* x, double b, T* y, const int32_t* gate,

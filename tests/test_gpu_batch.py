@@ -1,5 +1,6 @@
 """The batched (block-diagonal) sweep against one-instance sweeps and the CPU oracle (GPU)."""
 import numpy as np
+import scipy.sparse as sp
 import pytest
 import torch
 
@@ -258,3 +259,70 @@ def test_masked_batch_entry_points(gpu):
     W = torch.zeros(n, device=dev)
     with pytest.raises(RuntimeError, match="mask"):
         call("sa_batch_lanczos_step_f32", prob.csr_ref, M, ptr(s_got), ptr(V2), ldv, ptr(Z2), ptr(W), stream())
+
+
+@pytest.mark.parametrize("tdtype,mixed", [(torch.float32, False), (torch.float64, False), (torch.float32, True)])
+def test_instance_staged_spmv_matches_the_csr_product(gpu, tdtype, mixed):
+    """The instance-staged layout (instance slices of the gathered vector staged in LDS, one or more
+    passes) gives the fused product y = a x + b s (.) (J/c) g of an FP64 host reference, on instances of
+    one to three passes, for several workgroup heights and lanes per row; padding rows get a x."""
+    from specann_amd._lib import call
+    from specann_amd.device import ptr, stream
+    big = graphs.erdos_renyi(40000, 8 / 39999, seed=11, kind="pm1")
+    Js = [big] + _instances()[:3]
+    if mixed:   # one coupling of magnitude 3: entries keep their value stream (no sign packing)
+        J = Js[2].tolil()
+        i, j = sp.triu(J, 1).nonzero()
+        J[i[0], j[0]] = J[j[0], i[0]] = 3
+        Js[2] = J.tocsr()
+    print(f"instance-staged spmv dtype={tdtype} sizes={[J.shape[0] for J in Js]}", flush=True)
+    bp = BatchProblem(Js, tdtype, rows_per_block=256)
+    prob, n = bp.prob, bp.n
+    Jp, row_off, sizes = pack(Js)
+    c = np.zeros(n)
+    for b, J in enumerate(Js):
+        c[row_off[b]:row_off[b] + sizes[b]] = abs(J).sum(axis=1).max()
+    Jn = sp.diags(1.0 / np.where(c > 0, c, 1.0)) @ Jp
+    rng = np.random.default_rng(3)
+    s_h, g_h, x_h = rng.uniform(0.5, 1.5, n), rng.standard_normal(n), rng.standard_normal(n)
+    dev = bp.device
+    s, g, x = (torch.from_numpy(v).to(dev, tdtype) for v in (s_h, g_h, x_h))
+    s64, g64, x64 = (v.double().cpu().numpy() for v in (s, g, x))
+    want = 0.25 * x64 + -1.5 * s64 * (Jn @ g64)
+    scale = 0.25 * np.abs(x64) + 1.5 * s64 * (abs(Jn) @ np.abs(g64))
+    tol = 1e-6 if tdtype == torch.float32 else 1e-13
+    sfx = "f32" if tdtype == torch.float32 else "f64"
+    assert bp.inst_eligible()
+    for rows, lanes in ((2048, 8), (1024, 4), (64, 32), (512, 16)):
+        bp.set_inst_layout(rows, lanes)
+        assert bp.layout["passes"] == (2 if tdtype == torch.float32 else 3)
+        assert bp.layout["signed"] == (not mixed)
+        y = torch.full((n,), 9.0, dtype=tdtype, device=dev)
+        call(f"sa_spmv_gather_{sfx}", prob.csr_ref, ptr(s), ptr(g), 0.25, ptr(x), -1.5, ptr(y), stream())
+        got = y.double().cpu().numpy()
+        assert np.all(np.abs(got - want) <= tol * (scale + 1e-30) + 1e-300), (rows, lanes)
+    bp.set_tiling(1024, 0)
+    y = torch.full((n,), 9.0, dtype=tdtype, device=dev)
+    call(f"sa_spmv_gather_{sfx}", prob.csr_ref, ptr(s), ptr(g), 0.25, ptr(x), -1.5, ptr(y), stream())
+    assert np.all(np.abs(y.double().cpu().numpy() - want) <= tol * (scale + 1e-30) + 1e-300)
+
+
+@pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
+def test_batched_sweep_on_the_instance_staged_layout(gpu, tdtype):
+    """The sweep on the instance-staged layout finds the plain-tile layout's eigenvalues at every alpha
+    (the SpMV differs only in the order of its FP64 row sums)."""
+    Js = _instances()
+    tol = 1e-6 if tdtype == torch.float32 else 1e-10
+    kw = dict(random_seed=7, num_alphas=8, pred_mode="secant", eigsh_tol=tol)
+    bp = BatchProblem(Js, tdtype, rows_per_block=256)
+    ref = BatchSweep(**kw).run(bp)
+    bp.set_inst_layout(1024, 8)
+    out = BatchSweep(**kw).run(bp)
+    rel = 2e-5 if tdtype == torch.float32 else 1e-8
+    for b, J in enumerate(Js):
+        for r1, r2 in zip(out.results[b].rows, ref.results[b].rows):
+            assert r1["eigsh_converged"] and r2["eigsh_converged"]
+            assert r1["lambda1"] == pytest.approx(r2["lambda1"], rel=rel)
+        Jint = O.integer_couplings(J)
+        res = out.results[b]
+        assert res.best_energy == O.energy_int64(Jint, res.best_spins.astype(np.int64))
