@@ -122,6 +122,15 @@ typedef struct sa_csr {
   const int32_t* isplit;    /* (ipasses - 1) * n, or NULL when ipasses = 1 */
   const int32_t* ient;      /* nnz when isigned */
   const void* imag;         /* B values of T when isigned */
+  /* Optional column-streamed layout (bmode = 2, FP32 only; needs npanels = 8, pbound, partial, pld,
+   * and psigned / pmag as above; no counterpart in the reference). Workgroup w = rb * 8 + p sums the
+   * entries of row block rb (brows rows, or [rbound[rb], rbound[rb + 1])) in panel p, sorted by
+   * column then row, from the 256-entry chunks [bchunk[w], bchunk[w + 1]); chunk k's entries are
+   * stored lane-major (sorted entry 64 j + l at bent[256 k + 4 l + j]) and its columns are
+   * pbound[p] + bcbase[k] + delta. An entry is sign << 31 | local row << 16 | delta (16 bits);
+   * padding entries have local row = sa_cstream_pad_row() and delta 0. bvals (float, same order)
+   * holds the couplings unless psigned. bcols and bnc are not used. */
+  const int32_t* bcbase;    /* bchunk[8 * bnr] chunk base columns, relative to the panel */
 } sa_csr;
 
 /* A batch of independent instances packed block-diagonally into one CSR (no counterpart in the
@@ -260,6 +269,27 @@ int sa_block_fill_f32(const sa_csr* A, const int32_t* pbound, int32_t brows, int
 int sa_block_fill_f64(const sa_csr* A, const int32_t* pbound, int32_t brows, int32_t bcols, int32_t bnr, int32_t bnc,
                       int32_t* runs, const int32_t* bchunk, int64_t nent, uint32_t* bent, double* bvals, double mag,
                       int32_t* flag, hipStream_t stream);
+
+/* Column-streamed SpMV layout (bmode = 2, see sa_csr; no counterpart in the reference). Built in
+ * three steps on the caller's buffers:
+ *   sa_cstream_plan: one 64-bit key per stored entry (workgroup << 40 | column in the panel << 15 |
+ *     local row) and its entry index, radix-sorted (keys, idx -> keys_out, idx_out; tmp of
+ *     sa_cstream_sort_bytes bytes), then start[w] = the first sorted entry of workgroup w
+ *     (8 * bnr + 1 values). *flag bit 1: a row block taller than brows (or than the kernel's LDS
+ *     holds), bit 2: a panel wider than 2^25 columns.
+ *   the caller turns start into bchunk (chunks of 256 entries per workgroup, exclusive scan);
+ *   sa_cstream_fill_f32: writes the nchunks * 256 entries, bcbase and (mag = 0) the values; *flag
+ *     bit 4: a chunk spans more than 65536 columns, bit 8: a coupling's magnitude is not mag. */
+int sa_cstream_sort_bytes(int64_t nnz, size_t* bytes);
+int sa_cstream_max_rows(void);
+int sa_cstream_pad_row(void);
+int sa_cstream_plan(const sa_csr* A, const int32_t* pbound, int32_t brows, int32_t bnr, uint64_t* keys, int32_t* idx,
+                    uint64_t* keys_out, int32_t* idx_out, void* tmp, size_t tmp_bytes, int32_t* start, int32_t* flag,
+                    hipStream_t stream);
+int sa_cstream_fill_f32(const sa_csr* A, const int32_t* pbound, int32_t brows, int32_t bnr, const uint64_t* keys,
+                        const int32_t* idx, const int32_t* start, const int32_t* bchunk, int64_t nchunks,
+                        uint32_t* bent, float* bvals, int32_t* bcbase, double mag, int32_t* flag,
+                        hipStream_t stream);
 
 /* ---- Per-alpha scaling (models/spectral_annealing.py:316, :34) ------------------------------ */
 /* s_i = 1 / sqrt(max(d_i^alpha, clamp)); state.r[1] = sum_i d_i^alpha (degree_scale, :326);

@@ -102,6 +102,7 @@ class SaCsr(ctypes.Structure):
         ("isplit", c_void_p),
         ("ient", c_void_p),
         ("imag", c_void_p),
+        ("bcbase", c_void_p),
     ]
 
 
@@ -219,6 +220,12 @@ _SIGNATURES = {
                                     c_double, _P, _P]),
     "sa_block_fill_f64": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, c_int32, c_int32, _P, _P, c_int64, _P, _P,
                                     c_double, _P, _P]),
+    "sa_cstream_sort_bytes": (c_int32, [c_int64, POINTER(c_size_t)]),
+    "sa_cstream_max_rows": (c_int32, []),
+    "sa_cstream_pad_row": (c_int32, []),
+    "sa_cstream_plan": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, _P, _P, _P, _P, _P, c_size_t, _P, _P, _P]),
+    "sa_cstream_fill_f32": (c_int32, [POINTER(SaCsr), _P, c_int32, c_int32, _P, _P, _P, _P, c_int64, _P, _P, _P,
+                                      c_double, _P, _P]),
     "sa_alpha_scaling_f32": (c_int32, [c_int64, _P, c_double, c_double, _P, _P, _P]),
     "sa_alpha_scaling_f64": (c_int32, [c_int64, _P, c_double, c_double, _P, _P, _P]),
     "sa_spmv_scaled_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, c_double, _P, _P, _P]),

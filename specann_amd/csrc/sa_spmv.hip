@@ -351,6 +351,7 @@ SpmvParams<T> params_from(const sa_csr* A) {
   p.isplit = A->isplit;
   p.ient = A->ient;
   p.imag = (const T*)A->imag;
+  p.bcbase = A->bcbase;
   return p;
 }
 
@@ -363,6 +364,14 @@ int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream) {
     return SA_E_ARG;
   }
   if (p.bmode == 1) return launch_block_spmv<T>(p, stream);
+  if (p.bmode == 2) {
+    if constexpr (std::is_same<T, float>::value) {
+      return launch_cstream_spmv(p, stream);
+    } else {
+      set_error("spmv: the column-streamed layout is FP32 only");
+      return SA_E_ARG;
+    }
+  }
   if (p.bmode != 0) {
     set_error("spmv: unknown layout mode");
     return SA_E_ARG;

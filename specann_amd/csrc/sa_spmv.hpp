@@ -65,6 +65,9 @@ struct SpmvParams {
   const int32_t* isplit;
   const int32_t* ient;
   const T* imag;
+  // Column-streamed layout (bmode = 2): chunk base columns (bchunk, bent, bvals, pbound, rbound,
+  // brows and bnr as above).
+  const int32_t* bcbase;
 };
 
 template <typename T>
@@ -78,6 +81,8 @@ int launch_block_spmv(const SpmvParams<T>& p, hipStream_t stream);
 
 template <typename T>
 int launch_inst_spmv(const SpmvParams<T>& p, hipStream_t stream);
+
+int launch_cstream_spmv(const SpmvParams<float>& p, hipStream_t stream);
 
 template <typename T>
 int spmv_apply(const sa_csr* A, const T* s, const T* g, double a, const T* x, double b, T* y, const int32_t* gate,

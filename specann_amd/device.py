@@ -345,6 +345,24 @@ class DeviceProblem:
             out += [a + i * step for i in range(1, k)] + [b]
         return np.asarray(out, dtype=np.int64)
 
+    def whole_round_row_bounds(self, nblocks: int, cap: int) -> np.ndarray:
+        """balanced_row_bounds for about nblocks blocks of at most cap rows, in a block count that is a
+        whole number of rounds of one workgroup per CU of an XCD if a nearby target gives one."""
+        n = self.n
+        ip = self.indptr.cpu().numpy().astype(np.int64)
+        bounds_r = self.balanced_row_bounds(nblocks, cap, ip)
+        for extra in range(1, 2 * CUS_PER_XCD):
+            if (len(bounds_r) - 1) % CUS_PER_XCD == 0:
+                break
+            cand = self.balanced_row_bounds(nblocks + extra, cap, ip)
+            if (len(cand) - 1) % CUS_PER_XCD == 0:
+                bounds_r = cand
+                break
+        if bounds_r[-1] != n or np.any(np.diff(bounds_r) <= 0) or np.any(np.diff(bounds_r) > cap) or \
+                np.any(bounds_r[:-1] % 64):
+            raise RuntimeError("balanced row blocks out of shape")
+        return bounds_r
+
     def capped_panel_bounds(self, wmax: int) -> np.ndarray | None:
         """Eight column panels (boundaries multiples of 64, the last = n), none wider than wmax columns,
         whose largest entry share is as small as a greedy cover allows: a binary search on that share,
@@ -400,19 +418,7 @@ class DeviceProblem:
         rbound = None
         if balanced:
             cap = int(brows or BLOCK_ROWS[es])
-            # a block count in whole rounds of one workgroup per CU of an XCD, if a nearby target gives one
-            ip = self.indptr.cpu().numpy().astype(np.int64)
-            bounds_r = self.balanced_row_bounds(bnr, cap, ip)
-            for extra in range(1, 2 * CUS_PER_XCD):
-                if (len(bounds_r) - 1) % CUS_PER_XCD == 0:
-                    break
-                cand = self.balanced_row_bounds(bnr + extra, cap, ip)
-                if (len(cand) - 1) % CUS_PER_XCD == 0:
-                    bounds_r = cand
-                    break
-            if bounds_r[-1] != n or np.any(np.diff(bounds_r) <= 0) or np.any(np.diff(bounds_r) > cap) or \
-                    np.any(bounds_r[:-1] % 64):
-                raise RuntimeError("balanced row blocks out of shape")
+            bounds_r = self.whole_round_row_bounds(bnr, cap)
             rb, bnr = cap, len(bounds_r) - 1
             rbound = torch.from_numpy(bounds_r.astype(np.int32)).to(dev)
         if not (64 <= rb <= BLOCK_ROWS[es] and rb % 64 == 0 and cb % 64 == 0 and 64 <= cb <= BLOCK_COLS[es]):
@@ -469,11 +475,107 @@ class DeviceProblem:
         c.rbound = rbound.data_ptr() if rbound is not None else None  # cleared by build_panels(0) above
         return True
 
+    # -- column-streamed layout -------------------------------------------------------------------
+    def cstream_geometry(self) -> tuple[int, int]:
+        """(brows, bnr) of the column-streamed layout: the fewest row blocks, in whole rounds of one
+        workgroup per CU of an XCD, whose FP64 row sums fit the workgroup's LDS."""
+        cap = int(_lib.load().sa_cstream_max_rows())
+        bnr = CUS_PER_XCD
+        while True:
+            rb = -(-(-(-self.n // bnr)) // 64) * 64
+            if rb <= cap:
+                return rb, bnr
+            bnr += CUS_PER_XCD
+
+    def build_cstream(self, brows: int | None = None, balanced: bool = False, signed: bool | None = None) -> bool:
+        """Regroup the couplings into the column-streamed layout (FP32 only): per row block and column
+        panel, the entries sorted by column in 256-entry chunks (sa_cstream_plan, a radix sort on the
+        GPU, then sa_cstream_fill_f32). False (layout unchanged) for FP64 problems or when a chunk would
+        span more columns than its 16-bit offsets reach.
+        brows: rows per row block (at most sa_cstream_max_rows); balanced: row blocks of variable
+        height holding about equal entry counts (hubs); signed: sign-packed entries (None: whenever
+        every coupling has one magnitude; False: values stored beside the entries)."""
+        if self.dtype != torch.float32:
+            return False
+        n, nnz, dev, s = self.n, self.nnz, self.device, stream()
+        cap = int(_lib.load().sa_cstream_max_rows())
+        rb, bnr = self.cstream_geometry()
+        if brows is not None:
+            rb = int(brows)
+            if not (64 <= rb <= cap):
+                raise ValueError(f"column-streamed row blocks must hold 64..{cap} rows")
+            bnr = -(-n // rb)
+        rbound = None
+        if balanced:
+            bounds_r = self.whole_round_row_bounds(bnr, rb)
+            bnr = len(bounds_r) - 1
+            rbound = torch.from_numpy(bounds_r.astype(np.int32)).to(dev)
+        bounds = self.panel_bounds(8)
+        bounds[1:-1] = (bounds[1:-1] // 64) * 64
+        bounds = np.maximum.accumulate(bounds)
+        pbound = torch.from_numpy(bounds.astype(np.int32)).to(dev)
+        nwg = 8 * bnr
+        keys = torch.empty(max(nnz, 1), dtype=torch.int64, device=dev)
+        idx = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+        keys_out, idx_out = torch.empty_like(keys), torch.empty_like(idx)
+        tmp_bytes = ctypes.c_size_t()
+        call("sa_cstream_sort_bytes", nnz, ctypes.byref(tmp_bytes))
+        tmp = torch.empty(max(int(tmp_bytes.value), 16), dtype=torch.uint8, device=dev)
+        start = torch.empty(nwg + 1, dtype=torch.int32, device=dev)
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        old_rbound = self.csr.rbound
+        self.csr.rbound = rbound.data_ptr() if rbound is not None else None  # the plan reads it
+        try:
+            call("sa_cstream_plan", self.csr_ref, ptr(pbound), rb, bnr, ptr(keys), ptr(idx), ptr(keys_out),
+                 ptr(idx_out), ptr(tmp), tmp.numel(), ptr(start), ptr(flag), s)
+        finally:
+            self.csr.rbound = old_rbound
+        del keys, idx, tmp
+        if int(flag.item()) != 0:
+            return False
+        counts = (start[1:] - start[:-1]).to(torch.int64)
+        chunks = (counts + 255) // 256
+        bchunk = torch.zeros(nwg + 1, dtype=torch.int64, device=dev)
+        bchunk[1:] = torch.cumsum(chunks, 0)
+        nchunks = int(bchunk[-1].item())
+        if nchunks >= 2**31:
+            return False
+        bchunk = bchunk.to(torch.int32)
+        if signed is None:
+            signed = self.uniform_magnitude is not None
+        signed = bool(signed and self.uniform_magnitude is not None)
+        mag = self.uniform_magnitude if signed else 0.0
+        bent = torch.empty(max(256 * nchunks, 256), dtype=torch.int32, device=dev)
+        bvals = None if signed else torch.empty(max(256 * nchunks, 256), dtype=torch.float32, device=dev)
+        bcbase = torch.zeros(max(nchunks, 1), dtype=torch.int32, device=dev)
+        call("sa_cstream_fill_f32", self.csr_ref, ptr(pbound), rb, bnr, ptr(keys_out), ptr(idx_out), ptr(start),
+             ptr(bchunk), nchunks, ptr(bent), ptr(bvals), ptr(bcbase), mag, ptr(flag), s)
+        del keys_out, idx_out, start
+        if int(flag.item()) != 0:
+            return False
+        self.build_panels(0)
+        self.pbound, self.bchunk, self.bent, self.bvals, self.rbound, self.bcbase = \
+            pbound, bchunk, bent, bvals, rbound, bcbase
+        self.pld = _lib.padded_ld(n, 4)
+        self.partial = torch.empty((8, self.pld), dtype=self.dtype, device=dev)
+        self.npanels = 8
+        self.block = {"layout": "cstream", "brows": rb, "bnr": bnr, "chunks": nchunks,
+                      "balanced": rbound is not None}
+        c = self.csr
+        c.npanels, c.partial, c.pld = 8, self.partial.data_ptr(), self.pld
+        c.psigned, c.pmag = (1, mag) if signed else (0, 0.0)
+        c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 2, rb, 0, bnr, 0
+        c.pbound, c.bchunk, c.bent = pbound.data_ptr(), bchunk.data_ptr(), bent.data_ptr()
+        c.bvals = bvals.data_ptr() if bvals is not None else None
+        c.rbound = rbound.data_ptr() if rbound is not None else None
+        c.bcbase = bcbase.data_ptr()
+        return True
+
     def _clear_blocks(self) -> None:
         c = self.csr
         c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 0, 0, 0, 0, 0
-        c.pbound = c.bchunk = c.bent = c.bvals = c.rbound = None
-        for name in ("bchunk", "bent", "bvals", "block", "rbound"):
+        c.pbound = c.bchunk = c.bent = c.bvals = c.rbound = c.bcbase = None
+        for name in ("bchunk", "bent", "bvals", "block", "rbound", "bcbase"):
             if hasattr(self, name):
                 delattr(self, name)
 
@@ -495,8 +597,8 @@ class DeviceProblem:
         return e0.elapsed_time(e1) / 1e3 / reps
 
     def autotune(self, reps: int = 4, verbose: bool = False, blocks: bool = True) -> dict:
-        """Pick the SpMV layout (plain tiles, XCD panels or LDS blocks) and tile size by timing them on
-        this matrix; small problems keep the plain layout."""
+        """Pick the SpMV layout (plain tiles, XCD panels, LDS blocks or column-streamed chunks) and tile
+        size by timing them on this matrix; small problems keep the plain layout."""
         configs = [(0, t) for t in (1024, 2048, 4096)]
         P = self.auto_panels() if self.n >= (1 << 17) else 0
         if P:
@@ -507,25 +609,34 @@ class DeviceProblem:
             # rule above is only a starting point: time the other panel counts too
             configs += [(p, 1024) for p in (8, 16, 32, 64) if p != P]
         if self.n >= (1 << 16) and blocks:
-            # the LDS-blocked layout (its tile size does not matter; the plain tiles stay for other uses),
-            # with row blocks balanced by entries too where the uniform ones are not (hubs)
+            # the LDS-blocked and (FP32) column-streamed layouts (their tile size does not matter; the
+            # plain tiles stay for other uses), with row blocks balanced by entries too where the
+            # uniform ones are not (hubs)
             configs.append((-1, self.tile_nnz))
             if self.row_block_imbalance(self.block_geometry()[0]) > 1.25:
                 configs.append((-2, self.tile_nnz))
+            if self.dtype == torch.float32:
+                configs.append((-3, self.tile_nnz))
+                if self.row_block_imbalance(self.cstream_geometry()[0]) > 1.25:
+                    configs.append((-4, self.tile_nnz))
         best, results = None, []
         for panels, tile in configs:
             self.set_tiling(tile, self.csr.lanes_per_row)
-            if panels < 0:
+            if panels in (-1, -2):
                 if not self.build_blocks(balanced=panels == -2):
+                    continue
+            elif panels in (-3, -4):
+                if not self.build_cstream(balanced=panels == -4):
                     continue
             else:
                 self.build_panels(panels) if panels else self.build_panels(0)
             sec = self.time_spmv(reps)
-            layout = "blocked" if panels < 0 else "panels" if panels else "csr"
+            layout = {-1: "blocked", -2: "blocked", -3: "cstream", -4: "cstream"}.get(
+                panels, "panels" if panels else "csr")
             results.append({"layout": layout, "npanels": 8 if panels < 0 else panels, "tile_nnz": tile,
                             "seconds": sec})
             if panels < 0:
-                results[-1].update({k: v for k, v in self.block.items() if k != "entries"})
+                results[-1].update({k: v for k, v in self.block.items() if k not in ("entries", "layout", "chunks")})
             if verbose:
                 print(f"[autotune] {layout} npanels={results[-1]['npanels']} tile_nnz={tile}: {sec * 1e6:.1f} us",
                       flush=True)
@@ -533,8 +644,12 @@ class DeviceProblem:
                 best = results[-1]
         self.set_tiling(best["tile_nnz"], self.csr.lanes_per_row)
         if best["layout"] == "blocked":
-            if not self.csr.bmode or self.block.get("balanced") != best.get("balanced"):
+            if self.csr.bmode != 1 or self.block.get("balanced") != best.get("balanced"):
                 self.build_blocks(balanced=bool(best.get("balanced")))
+        elif best["layout"] == "cstream":
+            if self.csr.bmode != 2 or self.block.get("balanced") != best.get("balanced"):
+                if not self.build_cstream(balanced=bool(best.get("balanced"))):
+                    raise RuntimeError("column-streamed layout could not be rebuilt")
         elif best["npanels"] != self.npanels or best["npanels"] or self.csr.bmode:
             self.build_panels(best["npanels"])
         self._free_unused_panels()

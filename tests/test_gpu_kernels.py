@@ -9,6 +9,7 @@ import torch
 from oracle import specann_oracle as O
 from specann_amd import graphs
 from specann_amd.device import DeviceProblem, ptr, stream
+from specann_amd import _lib
 from specann_amd._lib import call
 
 pytestmark = pytest.mark.gpu
@@ -184,6 +185,99 @@ def _check_blocked(gpu, J, case, tdtype, geom, signed, balanced):
     m = int(np.max(np.diff(sp.csr_matrix(J).indptr)))
     err = np.abs(y1.astype(np.float64) - ref)
     assert np.all(err <= (10 + m) * np.finfo(dt).eps * mag + 1e-300), f"max excess {np.max(err / (mag + 1e-300))}"
+
+
+def _valued(J):
+    """J with couplings of several magnitudes (the column-streamed layout then stores values)."""
+    J = sp.csr_matrix(J, dtype=np.float64, copy=True)
+    rows = np.repeat(np.arange(J.shape[0]), np.diff(J.indptr))
+    J.data = J.data * (1.0 + ((rows + J.indices) % 3))  # symmetric in (row, column)
+    return J
+
+
+@pytest.mark.parametrize("case", sorted(CASES) + ["er_100k", "er_100003", "ba_200k", "er_100k_valued"])
+@pytest.mark.parametrize("brows", [None, 1024])
+@pytest.mark.parametrize("balanced", [False, True])
+def test_cstream_spmv_matches_oracle(gpu, case, brows, balanced):
+    """The column-streamed layout (FP32): the same operator as the oracle within the FP summation
+    bound, every entry placed once, bitwise repeatable across applications (FP64 row sums of FP32
+    addends), for sign-packed and valued entries, hubs and ragged sizes."""
+    if case.startswith("er_100k"):
+        J = graphs.erdos_renyi(100000, 40 / 99999, seed=12, kind="pm1")
+    elif case == "er_100003":
+        J = graphs.erdos_renyi(100003, 30 / 100002, seed=13, kind="pm1")
+    elif case == "ba_200k":
+        J = graphs.barabasi_albert(200000, 20, seed=5, kind="neg")
+    else:
+        J = CASES[case]()
+    if case.endswith("valued"):
+        J = _valued(J)
+    prob = DeviceProblem(J, torch.float32)
+    print(f"cstream spmv case={case} brows={brows} balanced={balanced} n={prob.n} nnz={prob.nnz}", flush=True)
+    assert prob.build_cstream(brows=brows, balanced=balanced)
+    c = prob.csr
+    assert c.bmode == 2 and c.npanels == 8 and c.psigned == int(prob.uniform_magnitude is not None)
+    # every stored coupling sits in exactly one chunk slot; the rest are padding
+    bent = prob.bent.cpu().numpy().view(np.uint32)
+    pad = (bent >> 16) & 0x7FFF == _lib.load().sa_cstream_pad_row()
+    assert int((~pad).sum()) == prob.nnz
+    bch = prob.bchunk.cpu().numpy()
+    assert bch[0] == 0 and np.all(np.diff(bch) >= 0) and 256 * bch[-1] == bent.size
+    if balanced:
+        rbound = prob.rbound.cpu().numpy()
+        assert rbound[0] == 0 and rbound[-1] == prob.n and np.all(np.diff(rbound) > 0)
+        assert np.all(np.diff(rbound) <= c.brows) and len(rbound) == c.bnr + 1
+    Jn, d, _ = O.normalize(J, np.float32)
+    s = torch.empty(prob.n, dtype=torch.float32, device=gpu)
+    prob.alpha_scaling(0.41, 1e-12, s)
+    x = O.normal_vector(prob.n, 7, 0).astype(np.float32)
+    xt = torch.from_numpy(x).to(gpu)
+    y = torch.empty_like(xt)
+    z = torch.empty_like(xt)
+    rho = 0.9
+    prob.matvec(s, xt, rho, -1.0, y, z)
+    y1 = y.cpu().numpy()
+    for _ in range(3):
+        prob.matvec(s, xt, rho, -1.0, y, z)
+        np.testing.assert_array_equal(y.cpu().numpy(), y1)
+    s_dev = s.cpu().numpy()
+    ref = O.s_matvec(Jn, s_dev, x, rho)
+    mag = rho * np.abs(x) + s_dev.astype(np.float64) * (abs(Jn).astype(np.float64) @ np.abs(s_dev * x))
+    # exact FP64 row sums of FP32 addends (each valued product rounded once), one rounding per panel
+    # partial, the panel combine and the epilogue
+    err = np.abs(y1.astype(np.float64) - ref)
+    assert np.all(err <= 16 * F32 * mag + 1e-300), f"max excess {np.max(err / (mag + 1e-300))}"
+
+
+def test_cstream_signed_and_valued_agree(gpu):
+    """Sign-packed and valued entries of one matrix give the same products within rounding."""
+    J = graphs.erdos_renyi(60000, 30 / 59999, seed=21, kind="pm1")
+    outs = []
+    for signed in (True, False):
+        prob = DeviceProblem(J, torch.float32)
+        assert prob.build_cstream(signed=signed)
+        assert prob.csr.psigned == int(signed)
+        s = torch.empty(prob.n, dtype=torch.float32, device=gpu)
+        prob.alpha_scaling(0.6, 1e-12, s)
+        xt = torch.from_numpy(O.normal_vector(prob.n, 2, 0).astype(np.float32)).to(gpu)
+        y = torch.empty_like(xt)
+        prob.matvec(s, xt, 0.25, -1.0, y, torch.empty_like(xt))
+        outs.append(y.cpu().numpy().astype(np.float64))
+    np.testing.assert_allclose(outs[0], outs[1], rtol=0, atol=8 * F32 * np.abs(outs[0]).max())
+
+
+def test_cstream_refuses_wide_chunks_and_fp64(gpu):
+    """A chunk whose columns span more than its 16-bit offsets reach (a sparse matching cut into
+    short row blocks) makes the build refuse and leave the layout as it was; FP64 problems refuse."""
+    n = 1 << 21
+    perm = np.random.default_rng(3).permutation(n)
+    a, b = perm[: n // 2], perm[n // 2:]
+    J = graphs.symmetric_csr(n, np.minimum(a, b), np.maximum(a, b), np.ones(n // 2))
+    prob = DeviceProblem(J, torch.float32)
+    prob.build_panels(8)
+    assert not prob.build_cstream(brows=1024)
+    assert prob.csr.bmode == 0 and prob.csr.npanels == 8
+    assert not DeviceProblem(CASES["er_2k_pm1"](), torch.float64).build_cstream()
 
 
 @pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
