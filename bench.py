@@ -227,6 +227,8 @@ def spmv_kernel_name(choice, tname):
         return f"spmv_block_kernel<{tname}> + combine (LDS-blocked fused scaled SpMV)"
     if choice.get("layout") == "cstream":
         return "spmv_cstream_kernel + combine (column-streamed fused scaled SpMV)"
+    if choice.get("layout") == "pblock":
+        return "pb_expand_kernel + pb_reduce_kernel (two-pass fused scaled SpMV)"
     if choice.get("npanels"):
         return f"spmv_panel_kernel<{tname}> + combine (XCD-panel fused scaled SpMV)"
     return f"spmv_tile_kernel<{tname}> (fused scaled CSR SpMV)"
@@ -244,8 +246,8 @@ def measured_traffic(workload, dtype, layout):
     want = {"npanels": layout["npanels"], "tile_nnz": layout["tile_nnz"]}
     if layout.get("layout") == "blocked":
         want = {"layout": "blocked", **{k: layout[k] for k in ("brows", "bcols", "bnr", "bnc")}}
-    if layout.get("layout") == "cstream":
-        want = {"layout": "cstream", **{k: layout[k] for k in ("brows", "bnr")}}
+    if layout.get("layout") in ("cstream", "pblock"):
+        want = {"layout": layout["layout"], **{k: layout[k] for k in ("brows", "bnr")}}
     if rec.get("layout") != want:
         return None
     return float(rec["traffic"])
@@ -353,7 +355,7 @@ def main(argv=None) -> int:
                     help="batch workload: skip the lockstep schedule's sweep (every instance at the same alpha)")
     ap.add_argument("--layout", default=None, metavar="NPANELS,TILE",
                     help="fixed SpMV layout instead of the autotuner's choice, e.g. 8,1024, blocked (LDS-blocked "
-                         "layout; blocked:BROWS,BCOLS for a given geometry) or cstream (column-streamed)")
+                         "layout; blocked:BROWS,BCOLS for a given geometry), cstream (column-streamed) or pblock (two-pass)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -371,11 +373,13 @@ def main(argv=None) -> int:
     J = make_graph(w)
     log(f"graph {args.workload}: n={J.shape[0]} nnz={J.nnz} generated in {time.time() - t0:.1f}s")
     prob32 = DeviceProblem(J, torch.float32)
-    if args.layout == "cstream":
-        if not prob32.build_cstream():
-            raise SystemExit("the column-streamed layout cannot hold this matrix")
-        tune32 = {"choice": {"layout": "cstream", "npanels": 8, "tile_nnz": prob32.tile_nnz, "seconds": None,
-                             **{k: v for k, v in prob32.block.items() if k not in ("layout", "chunks")}}, "tried": []}
+    if args.layout in ("cstream", "pblock"):
+        if not (prob32.build_cstream() if args.layout == "cstream" else prob32.build_pblock()):
+            raise SystemExit(f"the {args.layout} layout cannot hold this matrix")
+        tune32 = {"choice": {"layout": args.layout, "npanels": 8 if args.layout == "cstream" else 0,
+                             "tile_nnz": prob32.tile_nnz, "seconds": None,
+                             **{k: v for k, v in prob32.block.items() if k not in ("layout", "chunks", "column_blocks")}},
+                  "tried": []}
     elif args.layout and args.layout.startswith("blocked"):
         geom = [int(v) for v in args.layout.split(":")[1].split(",")] if ":" in args.layout else [None, None]
         if not prob32.build_blocks(brows=geom[0], bcols=geom[1]):

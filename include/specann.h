@@ -131,6 +131,22 @@ typedef struct sa_csr {
    * padding entries have local row = sa_cstream_pad_row() and delta 0. bvals (float, same order)
    * holds the couplings unless psigned. bcols and bnc are not used. */
   const int32_t* bcbase;    /* bchunk[8 * bnr] chunk base columns, relative to the panel */
+  /* Optional two-pass layout (bmode = 3, FP32 only, npanels = 0; no counterpart in the reference),
+   * for vectors far larger than an XCD's L2. Columns are cut into pbnc blocks [pbound[c],
+   * pbound[c + 1]) of at most sa_pblock_cols() columns, rows into bnr blocks of brows rows (or
+   * rbound). The entries are stored
+   * sorted by tile t = column block * bnr + row block, then column, then row; tile t holds
+   * [pbtoff[t], pbtoff[t + 1]). Pass one: workgroup C stages its column block of g in LDS and writes
+   * the product of every entry of column block C, in stored order, to pbcontrib. Pass two:
+   * workgroup R adds the products of its tiles into FP64 row sums in LDS and applies the epilogue.
+   * pbent: local column | sign << 15 (uint16, 8 padding entries at the end); pbrow: local row
+   * (uint16); bvals (float, same order) holds the couplings unless psigned. */
+  const uint16_t* pbent;    /* nnz + 8 */
+  const uint16_t* pbrow;    /* nnz */
+  const int32_t* pbtoff;    /* pbnc * bnr + 1 tile offsets */
+  void* pbcontrib;          /* nnz + 8 floats: the products between the two passes */
+  int32_t pbnc;             /* column blocks */
+  int32_t pad5;
 } sa_csr;
 
 /* A batch of independent instances packed block-diagonally into one CSR (no counterpart in the
@@ -281,6 +297,21 @@ int sa_block_fill_f64(const sa_csr* A, const int32_t* pbound, int32_t brows, int
  *   sa_cstream_fill_f32: writes the nchunks * 256 entries, bcbase and (mag = 0) the values; *flag
  *     bit 4: a chunk spans more than 65536 columns, bit 8: a coupling's magnitude is not mag. */
 int sa_cstream_sort_bytes(int64_t nnz, size_t* bytes);
+/* Two-pass layout (bmode = 3, see sa_csr; no counterpart in the reference):
+ *   sa_pblock_plan: one key per stored entry (tile << 30 | column in the block << 15 | local row) and
+ *     its entry index, radix-sorted on the caller's buffers (tmp of sa_cstream_sort_bytes bytes),
+ *     then pbtoff (ncb * bnr + 1). Column blocks: cbound (ncb + 1 boundaries, multiples of 64 except
+ *     cbound[ncb] = n, each block at most sa_pblock_cols() wide; the layout's pbound) and cgran
+ *     (ceil(n / 64) entries: the block of columns [64 g, 64 g + 64)). *flag bit 1: a row block taller
+ *     than brows, bit 2: a column block wider than sa_pblock_cols().
+ *   sa_pblock_fill_f32: pbent, pbrow and (mag = 0) the values, in sorted order; *flag bit 8: a
+ *     coupling's magnitude is not mag. */
+int sa_pblock_cols(void);
+int sa_pblock_plan(const sa_csr* A, int32_t brows, int32_t bnr, const int32_t* cbound, int32_t ncb,
+                   const int32_t* cgran, uint64_t* keys, int32_t* idx, uint64_t* keys_out, int32_t* idx_out, void* tmp,
+                   size_t tmp_bytes, int32_t* toff, int32_t* flag, hipStream_t stream);
+int sa_pblock_fill_f32(const sa_csr* A, int32_t brows, int32_t bnr, const uint64_t* keys, const int32_t* idx,
+                       uint16_t* pbent, uint16_t* pbrow, float* bvals, double mag, int32_t* flag, hipStream_t stream);
 int sa_cstream_max_rows(void);
 int sa_cstream_pad_row(void);
 int sa_cstream_plan(const sa_csr* A, const int32_t* pbound, int32_t brows, int32_t bnr, uint64_t* keys, int32_t* idx,

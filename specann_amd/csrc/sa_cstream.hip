@@ -164,15 +164,16 @@ __global__ __launch_bounds__(SA_BLOCK) void cs_key_kernel(sa_csr A, const int32_
   }
 }
 
-// start[w] = the first sorted entry of workgroup w; start[nwg] = nnz.
-__global__ __launch_bounds__(SA_BLOCK) void cs_start_kernel(int64_t nnz, const uint64_t* __restrict__ keys,
-                                                            int64_t nwg, int32_t* __restrict__ start) {
+// start[w] = the first of the sorted keys whose segment (key >> shift) is w; start[nseg] = nnz.
+__global__ __launch_bounds__(SA_BLOCK) void segment_start_kernel(int64_t nnz, const uint64_t* __restrict__ keys,
+                                                                 int shift, int64_t nseg,
+                                                                 int32_t* __restrict__ start) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t w = (int64_t)(keys[i] >> kCsKeyWg);
-    const int64_t wp = i ? (int64_t)(keys[i - 1] >> kCsKeyWg) : -1;
-    for (int64_t q = wp + 1; q <= w && q <= nwg; ++q) start[q] = (int32_t)i;
+    const int64_t w = (int64_t)(keys[i] >> shift);
+    const int64_t wp = i ? (int64_t)(keys[i - 1] >> shift) : -1;
+    for (int64_t q = wp + 1; q <= w && q <= nseg; ++q) start[q] = (int32_t)i;
     if (i == nnz - 1)
-      for (int64_t q = w + 1; q <= nwg; ++q) start[q] = (int32_t)nnz;
+      for (int64_t q = w + 1; q <= nseg; ++q) start[q] = (int32_t)nnz;
   }
 }
 
@@ -248,22 +249,49 @@ __global__ __launch_bounds__(SA_BLOCK) void cs_fill_kernel(int64_t nnz, const ui
   }
 }
 
-static int cs_bits(int64_t nwg) {
+int key_bits(int64_t count) {
   int b = 1;
-  while ((1ll << b) < nwg) ++b;
+  while ((1ll << b) < count) ++b;
   return b;
 }
 
-static int cs_sort_bytes(int64_t nnz, int end_bit, size_t& bytes) {
+static int sort_bytes(int64_t nnz, int end_bit, size_t& bytes) {
   bytes = 0;
   const hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
                                                  (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)nnz, 0u,
                                                  (unsigned)end_bit, (hipStream_t)0);
   if (e != hipSuccess) {
-    set_error(std::string("column-streamed layout: radix sort size query failed: ") + hipGetErrorString(e));
+    set_error(std::string("layout build: radix sort size query failed: ") + hipGetErrorString(e));
     return SA_E_LAUNCH;
   }
   return SA_OK;
+}
+
+int sort_entry_keys(const uint64_t* keys, const int32_t* idx, uint64_t* keys_out, int32_t* idx_out, int64_t nnz,
+                    int end_bit, void* tmp, size_t tmp_bytes, int shift, int64_t nseg, int32_t* start,
+                    hipStream_t stream) {
+  size_t need = 0;
+  int rc = sort_bytes(nnz, end_bit, need);
+  if (rc) return rc;
+  if (!tmp || tmp_bytes < need) {
+    set_error("layout build: sort scratch smaller than sa_cstream_sort_bytes");
+    return SA_E_ARG;
+  }
+  if (nnz == 0) {
+    hipLaunchKernelGGL(cs_fill_value_kernel, dim3(grid_for(nseg + 1, SA_BLOCK, 1024)), dim3(SA_BLOCK), 0, stream,
+                       nseg + 1, 0, start);
+    return check_launch("segment_start");
+  }
+  size_t bytes = tmp_bytes;
+  const hipError_t e = rocprim::radix_sort_pairs(tmp, bytes, keys, keys_out, idx, idx_out, (size_t)nnz, 0u,
+                                                 (unsigned)end_bit, stream);
+  if (e != hipSuccess) {
+    set_error(std::string("layout build: radix sort failed: ") + hipGetErrorString(e));
+    return SA_E_LAUNCH;
+  }
+  hipLaunchKernelGGL(segment_start_kernel, dim3(grid_for(nnz, SA_BLOCK, 8192)), dim3(SA_BLOCK), 0, stream, nnz,
+                     (const uint64_t*)keys_out, shift, nseg, start);
+  return check_launch("segment_start");
 }
 
 }  // namespace sa
@@ -277,7 +305,7 @@ int sa_cstream_pad_row(void) { return sa::kCsPad; }
 int sa_cstream_sort_bytes(int64_t nnz, size_t* bytes) {
   SA_REQUIRE(bytes && nnz >= 0 && nnz < (1ll << 31), "bad arguments");
   size_t b = 0;
-  const int rc = sa::cs_sort_bytes(nnz, 64, b);
+  const int rc = sa::sort_bytes(nnz, 64, b);
   if (rc) return rc;
   *bytes = b;
   return SA_OK;
@@ -292,30 +320,14 @@ int sa_cstream_plan(const sa_csr* A, const int32_t* pbound, int32_t brows, int32
              "bnr row blocks must cover the rows");
   SA_REQUIRE(keys && idx && keys_out && idx_out && start && flag, "bad arguments");
   const int64_t nwg = (int64_t)8 * bnr;
-  const int end_bit = sa::kCsKeyWg + sa::cs_bits(nwg + 1);
-  size_t need = 0;
-  int rc = sa::cs_sort_bytes(A->nnz, end_bit, need);
-  if (rc) return rc;
-  SA_REQUIRE(tmp && tmp_bytes >= need, "sort scratch smaller than sa_cstream_sort_bytes");
-  if (A->nnz == 0) {
-    hipLaunchKernelGGL(sa::cs_fill_value_kernel, dim3(sa::grid_for(nwg + 1, SA_BLOCK, 1024)), dim3(SA_BLOCK), 0,
-                       stream, nwg + 1, 0, start);
-    return sa::check_launch("cstream_start");
+  if (A->nnz > 0) {
+    hipLaunchKernelGGL(sa::cs_key_kernel, dim3(sa::grid_for(A->n, SA_BLOCK, 8192)), dim3(SA_BLOCK), 0, stream, *A,
+                       pbound, brows, bnr, keys, idx, flag);
+    const int rc = sa::check_launch("cstream_keys");
+    if (rc) return rc;
   }
-  hipLaunchKernelGGL(sa::cs_key_kernel, dim3(sa::grid_for(A->n, SA_BLOCK, 8192)), dim3(SA_BLOCK), 0, stream, *A,
-                     pbound, brows, bnr, keys, idx, flag);
-  rc = sa::check_launch("cstream_keys");
-  if (rc) return rc;
-  size_t bytes = tmp_bytes;
-  const hipError_t e = rocprim::radix_sort_pairs(tmp, bytes, (const uint64_t*)keys, keys_out, (const int32_t*)idx,
-                                                 idx_out, (size_t)A->nnz, 0u, (unsigned)end_bit, stream);
-  if (e != hipSuccess) {
-    sa::set_error(std::string("column-streamed layout: radix sort failed: ") + hipGetErrorString(e));
-    return SA_E_LAUNCH;
-  }
-  hipLaunchKernelGGL(sa::cs_start_kernel, dim3(sa::grid_for(A->nnz, SA_BLOCK, 8192)), dim3(SA_BLOCK), 0, stream,
-                     A->nnz, keys_out, nwg, start);
-  return sa::check_launch("cstream_start");
+  return sa::sort_entry_keys(keys, idx, keys_out, idx_out, A->nnz, sa::kCsKeyWg + sa::key_bits(nwg + 1), tmp,
+                             tmp_bytes, sa::kCsKeyWg, nwg, start, stream);
 }
 
 int sa_cstream_fill_f32(const sa_csr* A, const int32_t* pbound, int32_t brows, int32_t bnr, const uint64_t* keys,

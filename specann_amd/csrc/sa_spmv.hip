@@ -352,6 +352,11 @@ SpmvParams<T> params_from(const sa_csr* A) {
   p.ient = A->ient;
   p.imag = (const T*)A->imag;
   p.bcbase = A->bcbase;
+  p.pbent = A->pbent;
+  p.pbrow = A->pbrow;
+  p.pbtoff = A->pbtoff;
+  p.pbcontrib = (float*)A->pbcontrib;
+  p.pbnc = A->pbnc;
   return p;
 }
 
@@ -364,11 +369,11 @@ int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream) {
     return SA_E_ARG;
   }
   if (p.bmode == 1) return launch_block_spmv<T>(p, stream);
-  if (p.bmode == 2) {
+  if (p.bmode == 2 || p.bmode == 3) {
     if constexpr (std::is_same<T, float>::value) {
-      return launch_cstream_spmv(p, stream);
+      return p.bmode == 2 ? launch_cstream_spmv(p, stream) : launch_pblock_spmv(p, stream);
     } else {
-      set_error("spmv: the column-streamed layout is FP32 only");
+      set_error("spmv: the column-streamed and two-pass layouts are FP32 only");
       return SA_E_ARG;
     }
   }

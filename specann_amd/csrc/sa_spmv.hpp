@@ -68,6 +68,12 @@ struct SpmvParams {
   // Column-streamed layout (bmode = 2): chunk base columns (bchunk, bent, bvals, pbound, rbound,
   // brows and bnr as above).
   const int32_t* bcbase;
+  // Two-pass layout (bmode = 3): see sa_csr.
+  const uint16_t* pbent;
+  const uint16_t* pbrow;
+  const int32_t* pbtoff;
+  float* pbcontrib;
+  int32_t pbnc;
 };
 
 template <typename T>
@@ -83,6 +89,15 @@ template <typename T>
 int launch_inst_spmv(const SpmvParams<T>& p, hipStream_t stream);
 
 int launch_cstream_spmv(const SpmvParams<float>& p, hipStream_t stream);
+
+int launch_pblock_spmv(const SpmvParams<float>& p, hipStream_t stream);
+
+// Layout builds: radix-sort (key, entry index) pairs (on the caller's scratch) and set start[q] to the
+// first sorted key whose segment (key >> shift) is q, for q in [0, nseg]; bits needed to count to count.
+int sort_entry_keys(const uint64_t* keys, const int32_t* idx, uint64_t* keys_out, int32_t* idx_out, int64_t nnz,
+                    int end_bit, void* tmp, size_t tmp_bytes, int shift, int64_t nseg, int32_t* start,
+                    hipStream_t stream);
+int key_bits(int64_t count);
 
 template <typename T>
 int spmv_apply(const sa_csr* A, const T* s, const T* g, double a, const T* x, double b, T* y, const int32_t* gate,

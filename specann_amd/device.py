@@ -571,11 +571,92 @@ class DeviceProblem:
         c.bcbase = bcbase.data_ptr()
         return True
 
+    # -- two-pass layout ----------------------------------------------------------------------------
+    def build_pblock(self, brows: int | None = None, balanced: bool = False, signed: bool | None = None) -> bool:
+        """Regroup the couplings into the two-pass layout (FP32 only; for vectors far larger than an
+        XCD's L2): entries sorted by (column block, row block, column, row) with sa_pblock_plan (a radix
+        sort on the GPU) and sa_pblock_fill_f32, plus the products buffer between the passes. False
+        (layout unchanged) for FP64 problems. brows, balanced, signed: as for build_cstream."""
+        if self.dtype != torch.float32:
+            return False
+        n, nnz, dev, s = self.n, self.nnz, self.device, stream()
+        lib = _lib.load()
+        cap = int(lib.sa_cstream_max_rows())
+        cols = int(lib.sa_pblock_cols())
+        rb, bnr = self.cstream_geometry()
+        if brows is not None:
+            rb = int(brows)
+            if not (64 <= rb <= cap):
+                raise ValueError(f"two-pass row blocks must hold 64..{cap} rows")
+            bnr = -(-n // rb)
+        rbound = None
+        if balanced:
+            bounds_r = self.whole_round_row_bounds(bnr, rb)
+            bnr = len(bounds_r) - 1
+            rbound = torch.from_numpy(bounds_r.astype(np.int32)).to(dev)
+        # column blocks of about equal entry counts (twice as many as uniform blocks of the widest size
+        # the LDS holds), so that a hub's columns do not make one pass-one workgroup the long pole
+        ip = self.indptr.cpu().numpy().astype(np.int64)
+        cb_np = self.balanced_row_bounds(2 * -(-n // cols), cols, ip)
+        if cb_np[-1] != n or np.any(np.diff(cb_np) <= 0) or np.any(np.diff(cb_np) > cols) or np.any(cb_np[:-1] % 64):
+            raise RuntimeError("two-pass column blocks out of shape")
+        nc = len(cb_np) - 1
+        gran = np.searchsorted(cb_np, np.arange(0, n, 64, dtype=np.int64), side="right") - 1
+        cbound = torch.from_numpy(cb_np.astype(np.int32)).to(dev)
+        cgran = torch.from_numpy(gran.astype(np.int32)).to(dev)
+        keys = torch.empty(max(nnz, 1), dtype=torch.int64, device=dev)
+        idx = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+        keys_out, idx_out = torch.empty_like(keys), torch.empty_like(idx)
+        tmp_bytes = ctypes.c_size_t()
+        call("sa_cstream_sort_bytes", nnz, ctypes.byref(tmp_bytes))
+        tmp = torch.empty(max(int(tmp_bytes.value), 16), dtype=torch.uint8, device=dev)
+        toff = torch.empty(nc * bnr + 1, dtype=torch.int32, device=dev)
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        old_rbound = self.csr.rbound
+        self.csr.rbound = rbound.data_ptr() if rbound is not None else None  # the plan reads it
+        try:
+            call("sa_pblock_plan", self.csr_ref, rb, bnr, ptr(cbound), nc, ptr(cgran), ptr(keys), ptr(idx),
+                 ptr(keys_out), ptr(idx_out), ptr(tmp), tmp.numel(), ptr(toff), ptr(flag), s)
+        finally:
+            self.csr.rbound = old_rbound
+        del keys, idx, tmp, cgran
+        if int(flag.item()) != 0:
+            return False
+        if signed is None:
+            signed = self.uniform_magnitude is not None
+        signed = bool(signed and self.uniform_magnitude is not None)
+        mag = self.uniform_magnitude if signed else 0.0
+        pbent = torch.zeros(nnz + 16, dtype=torch.int16, device=dev)
+        pbrow = torch.empty(max(nnz, 1), dtype=torch.int16, device=dev)
+        bvals = None if signed else torch.zeros(nnz + 16, dtype=torch.float32, device=dev)
+        call("sa_pblock_fill_f32", self.csr_ref, rb, bnr, ptr(keys_out), ptr(idx_out), ptr(pbent), ptr(pbrow),
+             ptr(bvals), mag, ptr(flag), s)
+        del keys_out, idx_out
+        if int(flag.item()) != 0:
+            return False
+        contrib = torch.empty(nnz + 16, dtype=torch.float32, device=dev)
+        self.build_panels(0)
+        self.pbent, self.pbrow, self.pbtoff, self.pbcontrib = pbent, pbrow, toff, contrib
+        self.bvals, self.rbound, self.pbound = bvals, rbound, cbound
+        self.block = {"layout": "pblock", "brows": rb, "bnr": bnr, "column_blocks": nc,
+                      "balanced": rbound is not None}
+        c = self.csr
+        c.psigned, c.pmag = (1, mag) if signed else (0, 0.0)
+        c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 3, rb, 0, bnr, 0
+        c.bvals = bvals.data_ptr() if bvals is not None else None
+        c.rbound = rbound.data_ptr() if rbound is not None else None
+        c.pbent, c.pbrow, c.pbtoff, c.pbcontrib, c.pbnc = (pbent.data_ptr(), pbrow.data_ptr(), toff.data_ptr(),
+                                                           contrib.data_ptr(), nc)
+        c.pbound = cbound.data_ptr()
+        return True
+
     def _clear_blocks(self) -> None:
         c = self.csr
         c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 0, 0, 0, 0, 0
         c.pbound = c.bchunk = c.bent = c.bvals = c.rbound = c.bcbase = None
-        for name in ("bchunk", "bent", "bvals", "block", "rbound", "bcbase"):
+        c.pbent = c.pbrow = c.pbtoff = c.pbcontrib = None
+        c.pbnc = 0
+        for name in ("bchunk", "bent", "bvals", "block", "rbound", "bcbase", "pbent", "pbrow", "pbtoff", "pbcontrib"):
             if hasattr(self, name):
                 delattr(self, name)
 
@@ -597,8 +678,9 @@ class DeviceProblem:
         return e0.elapsed_time(e1) / 1e3 / reps
 
     def autotune(self, reps: int = 4, verbose: bool = False, blocks: bool = True) -> dict:
-        """Pick the SpMV layout (plain tiles, XCD panels, LDS blocks or column-streamed chunks) and tile
-        size by timing them on this matrix; small problems keep the plain layout."""
+        """Pick the SpMV layout (plain tiles, XCD panels, LDS blocks, column-streamed chunks or the
+        two-pass layout) and tile size by timing them on this matrix; small problems keep the plain
+        layout."""
         configs = [(0, t) for t in (1024, 2048, 4096)]
         P = self.auto_panels() if self.n >= (1 << 17) else 0
         if P:
@@ -616,9 +698,15 @@ class DeviceProblem:
             if self.row_block_imbalance(self.block_geometry()[0]) > 1.25:
                 configs.append((-2, self.tile_nnz))
             if self.dtype == torch.float32:
+                unbalanced = self.row_block_imbalance(self.cstream_geometry()[0]) > 1.25
                 configs.append((-3, self.tile_nnz))
-                if self.row_block_imbalance(self.cstream_geometry()[0]) > 1.25:
+                if unbalanced:
                     configs.append((-4, self.tile_nnz))
+                if self.n >= (1 << 21):
+                    # the two-pass layout, for vectors far larger than an XCD's L2
+                    configs.append((-5, self.tile_nnz))
+                    if unbalanced:
+                        configs.append((-6, self.tile_nnz))
         best, results = None, []
         for panels, tile in configs:
             self.set_tiling(tile, self.csr.lanes_per_row)
@@ -628,15 +716,19 @@ class DeviceProblem:
             elif panels in (-3, -4):
                 if not self.build_cstream(balanced=panels == -4):
                     continue
+            elif panels in (-5, -6):
+                if not self.build_pblock(balanced=panels == -6):
+                    continue
             else:
                 self.build_panels(panels) if panels else self.build_panels(0)
             sec = self.time_spmv(reps)
-            layout = {-1: "blocked", -2: "blocked", -3: "cstream", -4: "cstream"}.get(
+            layout = {-1: "blocked", -2: "blocked", -3: "cstream", -4: "cstream", -5: "pblock", -6: "pblock"}.get(
                 panels, "panels" if panels else "csr")
-            results.append({"layout": layout, "npanels": 8 if panels < 0 else panels, "tile_nnz": tile,
-                            "seconds": sec})
+            results.append({"layout": layout, "npanels": 8 if panels in (-1, -2, -3, -4) else max(panels, 0),
+                            "tile_nnz": tile, "seconds": sec})
             if panels < 0:
-                results[-1].update({k: v for k, v in self.block.items() if k not in ("entries", "layout", "chunks")})
+                results[-1].update({k: v for k, v in self.block.items()
+                                    if k not in ("entries", "layout", "chunks", "column_blocks")})
             if verbose:
                 print(f"[autotune] {layout} npanels={results[-1]['npanels']} tile_nnz={tile}: {sec * 1e6:.1f} us",
                       flush=True)
@@ -650,6 +742,10 @@ class DeviceProblem:
             if self.csr.bmode != 2 or self.block.get("balanced") != best.get("balanced"):
                 if not self.build_cstream(balanced=bool(best.get("balanced"))):
                     raise RuntimeError("column-streamed layout could not be rebuilt")
+        elif best["layout"] == "pblock":
+            if self.csr.bmode != 3 or self.block.get("balanced") != best.get("balanced"):
+                if not self.build_pblock(balanced=bool(best.get("balanced"))):
+                    raise RuntimeError("two-pass layout could not be rebuilt")
         elif best["npanels"] != self.npanels or best["npanels"] or self.csr.bmode:
             self.build_panels(best["npanels"])
         self._free_unused_panels()

@@ -249,6 +249,63 @@ def test_cstream_spmv_matches_oracle(gpu, case, brows, balanced):
     assert np.all(err <= 16 * F32 * mag + 1e-300), f"max excess {np.max(err / (mag + 1e-300))}"
 
 
+@pytest.mark.parametrize("case", ["er_2k_pm1", "star_9k", "er_100k", "er_100003", "ba_200k", "ba_200k_valued"])
+@pytest.mark.parametrize("brows", [None, 1024])
+@pytest.mark.parametrize("balanced", [False, True])
+def test_pblock_spmv_matches_oracle(gpu, case, brows, balanced):
+    """The two-pass layout (FP32): products per column block, then FP64 row sums per row block; the
+    same operator as the oracle within the FP summation bound, every entry in its tile once, bitwise
+    repeatable, several column blocks (n > 32768), hubs, ragged sizes and valued couplings."""
+    if case == "er_100k":
+        J = graphs.erdos_renyi(100000, 40 / 99999, seed=12, kind="pm1")
+    elif case == "er_100003":
+        J = graphs.erdos_renyi(100003, 30 / 100002, seed=13, kind="pm1")
+    elif case.startswith("ba_200k"):
+        J = graphs.barabasi_albert(200000, 20, seed=5, kind="neg")
+    else:
+        J = CASES[case]()
+    if case.endswith("valued"):
+        J = _valued(J)
+    prob = DeviceProblem(J, torch.float32)
+    print(f"pblock spmv case={case} brows={brows} balanced={balanced} n={prob.n} nnz={prob.nnz}", flush=True)
+    assert prob.build_pblock(brows=brows, balanced=balanced)
+    c = prob.csr
+    cols = _lib.load().sa_pblock_cols()
+    cbound = prob.pbound.cpu().numpy().astype(np.int64)
+    assert c.bmode == 3 and c.npanels == 0 and cbound.size == c.pbnc + 1
+    assert cbound[0] == 0 and cbound[-1] == prob.n and np.all(np.diff(cbound) > 0) and np.all(np.diff(cbound) <= cols)
+    toff = prob.pbtoff.cpu().numpy()
+    assert toff[0] == 0 and toff[-1] == prob.nnz and np.all(np.diff(toff) >= 0) and toff.size == c.pbnc * c.bnr + 1
+    # the stored local rows and columns reproduce the matrix, tile by tile
+    rows_l = prob.pbrow.cpu().numpy().view(np.uint16).astype(np.int64)
+    ent = prob.pbent.cpu().numpy().view(np.uint16)[: prob.nnz].astype(np.int64)
+    t = np.repeat(np.arange(toff.size - 1), np.diff(toff))
+    cb, rb = t // c.bnr, t % c.bnr
+    r0 = prob.rbound.cpu().numpy()[rb] if balanced else rb * c.brows
+    got = sp.csr_matrix((np.ones(prob.nnz), (r0 + rows_l, cbound[cb] + (ent & 0x7FFF))), shape=J.shape)
+    want = sp.csr_matrix(J, copy=True)
+    want.data[:] = 1.0
+    assert (got != want).nnz == 0
+    Jn, d, _ = O.normalize(J, np.float32)
+    s = torch.empty(prob.n, dtype=torch.float32, device=gpu)
+    prob.alpha_scaling(0.41, 1e-12, s)
+    x = O.normal_vector(prob.n, 7, 0).astype(np.float32)
+    xt = torch.from_numpy(x).to(gpu)
+    y = torch.empty_like(xt)
+    z = torch.empty_like(xt)
+    rho = 0.9
+    prob.matvec(s, xt, rho, -1.0, y, z)
+    y1 = y.cpu().numpy()
+    for _ in range(3):
+        prob.matvec(s, xt, rho, -1.0, y, z)
+        np.testing.assert_array_equal(y.cpu().numpy(), y1)
+    s_dev = s.cpu().numpy()
+    ref = O.s_matvec(Jn, s_dev, x, rho)
+    mag = rho * np.abs(x) + s_dev.astype(np.float64) * (abs(Jn).astype(np.float64) @ np.abs(s_dev * x))
+    err = np.abs(y1.astype(np.float64) - ref)
+    assert np.all(err <= 8 * F32 * mag + 1e-300), f"max excess {np.max(err / (mag + 1e-300))}"
+
+
 def test_cstream_signed_and_valued_agree(gpu):
     """Sign-packed and valued entries of one matrix give the same products within rounding."""
     J = graphs.erdos_renyi(60000, 30 / 59999, seed=21, kind="pm1")
@@ -278,6 +335,7 @@ def test_cstream_refuses_wide_chunks_and_fp64(gpu):
     assert not prob.build_cstream(brows=1024)
     assert prob.csr.bmode == 0 and prob.csr.npanels == 8
     assert not DeviceProblem(CASES["er_2k_pm1"](), torch.float64).build_cstream()
+    assert not DeviceProblem(CASES["er_2k_pm1"](), torch.float64).build_pblock()
 
 
 @pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])

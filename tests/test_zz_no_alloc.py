@@ -1,6 +1,6 @@
 """The allocation-free SpMV entry point (sa_spmv_scaled_static_f32) against the allocating one
 (sa_spmv_scaled_f32) through the C ABI: bit-identical outputs on two calls in a row, the second on a
-smaller problem, for the plain, XCD-panel, LDS-blocked and column-streamed layouts; sizes beyond its static scratch
+smaller problem, for the plain, XCD-panel, LDS-blocked, column-streamed and two-pass layouts; sizes beyond its static scratch
 are refused with a message."""
 import ctypes
 
@@ -24,6 +24,8 @@ def _problem(gpu, n, deg, seed, layout):
         assert prob.build_blocks()
     elif layout == "cstream":
         assert prob.build_cstream()
+    elif layout == "pblock":
+        assert prob.build_pblock()
     s = torch.empty(prob.n, dtype=torch.float32, device=gpu)
     prob.alpha_scaling(0.4, 1e-12, s)
     x = torch.from_numpy(np.random.default_rng(seed).standard_normal(prob.n).astype(np.float32)).to(gpu)
@@ -46,7 +48,7 @@ def _static(prob, s, x, a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("layout", ["csr", "panels", "blocked", "cstream"])
+@pytest.mark.parametrize("layout", ["csr", "panels", "blocked", "cstream", "pblock"])
 def test_static_scratch_spmv_equals_allocating_entry_point(gpu, layout):
     big_n = 1 << 17
     print(f"no-alloc SpMV, layout {layout}: n = {big_n} then 5000", flush=True)
