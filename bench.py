@@ -355,7 +355,8 @@ def main(argv=None) -> int:
                     help="batch workload: skip the lockstep schedule's sweep (every instance at the same alpha)")
     ap.add_argument("--layout", default=None, metavar="NPANELS,TILE",
                     help="fixed SpMV layout instead of the autotuner's choice, e.g. 8,1024, blocked (LDS-blocked "
-                         "layout; blocked:BROWS,BCOLS for a given geometry), cstream (column-streamed) or pblock (two-pass)")
+                         "layout; blocked:BROWS,BCOLS for a given geometry), cstream (column-streamed) or pblock (two-pass), "
+                         "each also as NAME-balanced (row blocks balanced by entries)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -373,10 +374,11 @@ def main(argv=None) -> int:
     J = make_graph(w)
     log(f"graph {args.workload}: n={J.shape[0]} nnz={J.nnz} generated in {time.time() - t0:.1f}s")
     prob32 = DeviceProblem(J, torch.float32)
-    if args.layout in ("cstream", "pblock"):
-        if not (prob32.build_cstream() if args.layout == "cstream" else prob32.build_pblock()):
+    if args.layout and args.layout.split("-")[0] in ("cstream", "pblock"):
+        name, bal = args.layout.split("-")[0], args.layout.endswith("-balanced")
+        if not (prob32.build_cstream(balanced=bal) if name == "cstream" else prob32.build_pblock(balanced=bal)):
             raise SystemExit(f"the {args.layout} layout cannot hold this matrix")
-        tune32 = {"choice": {"layout": args.layout, "npanels": 8 if args.layout == "cstream" else 0,
+        tune32 = {"choice": {"layout": name, "npanels": 8 if name == "cstream" else 0,
                              "tile_nnz": prob32.tile_nnz, "seconds": None,
                              **{k: v for k, v in prob32.block.items() if k not in ("layout", "chunks", "column_blocks")}},
                   "tried": []}

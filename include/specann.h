@@ -135,7 +135,7 @@ typedef struct sa_csr {
    * for vectors far larger than an XCD's L2. Columns are cut into pbnc blocks [pbound[c],
    * pbound[c + 1]) of at most sa_pblock_cols() columns, rows into bnr blocks of brows rows (or
    * rbound). The entries are stored
-   * sorted by tile t = column block * bnr + row block, then column, then row; tile t holds
+   * sorted by tile t = column block * bnr + row block, then row, then column; tile t holds
    * [pbtoff[t], pbtoff[t + 1]). Pass one: workgroup C stages its column block of g in LDS and writes
    * the product of every entry of column block C, in stored order, to pbcontrib. Pass two:
    * workgroup R adds the products of its tiles into FP64 row sums in LDS and applies the epilogue.
@@ -298,7 +298,7 @@ int sa_block_fill_f64(const sa_csr* A, const int32_t* pbound, int32_t brows, int
  *     bit 4: a chunk spans more than 65536 columns, bit 8: a coupling's magnitude is not mag. */
 int sa_cstream_sort_bytes(int64_t nnz, size_t* bytes);
 /* Two-pass layout (bmode = 3, see sa_csr; no counterpart in the reference):
- *   sa_pblock_plan: one key per stored entry (tile << 30 | column in the block << 15 | local row) and
+ *   sa_pblock_plan: one key per stored entry (tile << 30 | local row << 15 | column in the block) and
  *     its entry index, radix-sorted on the caller's buffers (tmp of sa_cstream_sort_bytes bytes),
  *     then pbtoff (ncb * bnr + 1). Column blocks: cbound (ncb + 1 boundaries, multiples of 64 except
  *     cbound[ncb] = n, each block at most sa_pblock_cols() wide; the layout's pbound) and cgran

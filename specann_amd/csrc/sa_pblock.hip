@@ -8,8 +8,9 @@
 //     the entries of those columns in stored order, writing each entry's product J_rj g_j to the
 //     next slot of a products array (coalesced: the products have the entries' order);
 //   pass two: workgroup R walks the products of its row block, tile by tile (one tile per column
-//     block, each a contiguous range), adds them into FP64 row sums in LDS with atomic adds and
-//     applies the epilogue y = a x + b s (.) (row sums).
+//     block, each a contiguous range, its entries running by row), sums each row's run in a wave,
+//     adds it into FP64 row sums in LDS with one atomic add and applies the epilogue
+//     y = a x + b s (.) (row sums).
 // The stream is 2 bytes per entry (local column and sign) in pass one, 4 bytes of product written and
 // read back and 2 bytes of local row in pass two: 12 bytes per entry, all sequential, against one
 // random line per entry for the gathers. The row sums are exact FP64 sums of FP32 products, so the
@@ -23,8 +24,8 @@ constexpr int kPbColBits = 15;
 constexpr int kPbCols = 1 << kPbColBits;   // column block: 128 KiB of g in LDS
 constexpr int kPbRows = 20416;             // row block: FP64 row sums + the padding slot in LDS
 constexpr int kPbKeyTile = 30;             // key: tile << 30 | local column << 15 | local row
-constexpr int kPbUnroll = 4;               // 8-entry groups per thread in flight (pass one)
-constexpr int kPbUnroll2 = 4;              // entries per lane in flight (pass two)
+constexpr int kPbUnroll = 8;               // 8-entry groups per thread in flight (pass one)
+constexpr int kPbUnroll2 = 16;             // entries per lane in flight (pass two)
 
 typedef uint16_t pb_u16x8 __attribute__((ext_vector_type(8)));
 typedef float pb_f4 __attribute__((ext_vector_type(4)));
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(kPbThreads) void pb_expand_kernel(SpmvParams<float>
   }
 }
 
-// Pass two: workgroup R; wave w takes the tiles of column blocks w, w + 16, ...
+// Pass two: workgroup R; wave w takes the tiles of column blocks w, w + 16, ..., 64 at a time.
 template <bool kSigned>
 __global__ __launch_bounds__(kPbThreads) void pb_reduce_kernel(SpmvParams<float> p) {
   __shared__ double acc[kPbRows + 1];
@@ -109,22 +110,65 @@ __global__ __launch_bounds__(kPbThreads) void pb_reduce_kernel(SpmvParams<float>
   __syncthreads();
   const float* __restrict__ prod = p.pbcontrib;
   const uint16_t* __restrict__ prow = p.pbrow;
-  for (int C = wave; C < p.pbnc; C += kPbThreads / 64) {
-    const int64_t t = (int64_t)C * p.bnr + R;
-    const int64_t a = p.pbtoff[t], b = p.pbtoff[t + 1];
-    for (int64_t i0 = a + lane; i0 < b; i0 += 64 * kPbUnroll2) {
+  // The tiles are short (a few hundred entries), so a wave packs 64 of them into one index space:
+  // lane l holds tile C = wave + 16 (k + l), its start and the exclusive prefix of the tile sizes, and
+  // slot q of the batch belongs to the last lane whose prefix is <= q (found by a binary search over
+  // the lanes with shuffles).
+  constexpr int kWaves = kPbThreads / 64;
+  for (int k = 0; k * kWaves + wave < p.pbnc; k += 64) {
+    const int C = wave + kWaves * (k + lane);
+    int64_t a = 0;
+    int cnt = 0;
+    if (C < p.pbnc) {
+      const int64_t t = (int64_t)C * p.bnr + R;
+      a = p.pbtoff[t];
+      cnt = (int)(p.pbtoff[t + 1] - a);
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    const int excl = incl - cnt;
+    const int total = __shfl(incl, 63, 64);
+    for (int q0 = 0; q0 < total; q0 += 64 * kPbUnroll2) {
       float v[kPbUnroll2];
       uint32_t r[kPbUnroll2];
 #pragma unroll
       for (int u = 0; u < kPbUnroll2; ++u) {
-        const int64_t i = i0 + 64 * u;
-        const int64_t is = i < b ? i : a;
-        v[u] = __builtin_nontemporal_load(&prod[is]);
-        r[u] = __builtin_nontemporal_load(&prow[is]);
+        const int q = q0 + 64 * u + lane;
+        int lo = 0;  // the last lane with excl <= q (q < total: a nonempty tile)
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+          const int e = __shfl(excl, lo + step, 64);
+          if (e <= q) lo += step;
+        }
+        const int64_t base = __shfl(a, lo, 64);
+        const int off = q - __shfl(excl, lo, 64);
+        const int64_t i = q < total ? base + off : 0;
+        v[u] = __builtin_nontemporal_load(&prod[i]);
+        r[u] = __builtin_nontemporal_load(&prow[i]);
       }
 #pragma unroll
-      for (int u = 0; u < kPbUnroll2; ++u)
-        if (i0 + 64 * u < b) atomicAdd(&acc[r[u]], (double)v[u]);
+      for (int u = 0; u < kPbUnroll2; ++u) {
+        // runs of one row (a tile's entries run by row; a run may continue into the next tile, whose
+        // local rows are the same rows) are summed onto their first lane in FP64, which adds once
+        const bool ok = q0 + 64 * u + lane < total;
+        const uint32_t rn = (uint32_t)__shfl_down((int)r[u], 1, 64);
+        const uint64_t m = __ballot(ok && lane < 63 && q0 + 64 * u + lane + 1 < total && rn == r[u]);
+        double d = ok ? (double)v[u] : 0.0;
+        if (m) {
+          uint64_t c = m;
+          for (int dd = 1; c != 0; dd <<= 1) {
+            const double w = __shfl_down(d, dd, 64);
+            if ((c >> lane) & 1ull) d += w;
+            c &= c >> dd;
+          }
+        }
+        const bool head = ok && (lane == 0 || !((m >> (lane - 1)) & 1ull));
+        if (head) atomicAdd(&acc[r[u]], d);
+      }
     }
   }
   __syncthreads();
@@ -173,7 +217,8 @@ __device__ inline int pb_row_block(const int32_t* rbound, int32_t brows, int32_t
   return lo;
 }
 
-// key = (column block * bnr + row block) << 30 | column in the block << 15 | local row (thread per row).
+// key = (column block * bnr + row block) << 30 | local row << 15 | column in the block (thread per row):
+// inside a tile the entries run by row, so pass two sums a row's run before its one atomic add.
 // Column block of column j: cgran[j / 64] (the block boundaries are multiples of 64).
 __global__ __launch_bounds__(SA_BLOCK) void pb_key_kernel(sa_csr A, int32_t brows, int32_t bnr,
                                                           const int32_t* __restrict__ cbound,
@@ -196,7 +241,7 @@ __global__ __launch_bounds__(SA_BLOCK) void pb_key_kernel(sa_csr A, int32_t brow
         continue;
       }
       const uint64_t t = (uint64_t)cb * (uint64_t)bnr + (uint64_t)rb;
-      keys[e] = (t << kPbKeyTile) | ((uint64_t)lc << 15) | (uint64_t)lr;
+      keys[e] = (t << kPbKeyTile) | ((uint64_t)lr << 15) | (uint64_t)lc;
       idx[e] = e;
     }
   }
@@ -215,7 +260,7 @@ __global__ __launch_bounds__(SA_BLOCK) void pb_fill_kernel(int64_t nnz, const ui
     }
     const uint64_t key = keys[i];
     const float v = vals[idx[i]];
-    uint32_t code = (uint32_t)((key >> 15) & (kPbCols - 1));
+    uint32_t code = (uint32_t)(key & (kPbCols - 1));
     if (mag > 0.f) {
       if ((v < 0.f ? -v : v) != mag) atomicOr(flag, 8);
       if (v < 0.f) code |= 0x8000u;
@@ -223,7 +268,7 @@ __global__ __launch_bounds__(SA_BLOCK) void pb_fill_kernel(int64_t nnz, const ui
       bvals[i] = v;
     }
     pbent[i] = (uint16_t)code;
-    pbrow[i] = (uint16_t)(key & 0x7FFFu);
+    pbrow[i] = (uint16_t)((key >> 15) & 0x7FFFu);
   }
 }
 
