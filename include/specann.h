@@ -137,14 +137,17 @@ typedef struct sa_csr {
    * rbound). The entries are stored
    * sorted by tile t = column block * bnr + row block, then row, then column; tile t holds
    * [pbtoff[t], pbtoff[t + 1]). Pass one: workgroup C stages its column block of g in LDS and writes
-   * the product of every entry of column block C, in stored order, to pbcontrib. Pass two:
-   * workgroup R adds the products of its tiles into FP64 row sums in LDS and applies the epilogue.
-   * pbent: local column | sign << 15 (uint16, 8 padding entries at the end); pbrow: local row
-   * (uint16); bvals (float, same order) holds the couplings unless psigned. */
+   * the product of every entry of column block C to pbcontrib, where the products are ordered by row
+   * block, then column block (tile (c, r) at pbrtoff[r * pbnc + c], in its stored order). Pass two:
+   * workgroup R adds its contiguous products into FP64 row sums in LDS and applies the epilogue.
+   * pbent: local column | sign << 15 (uint16, stored order, 8 padding entries at the end); pbrow:
+   * local row (uint16, products order); bvals (float, stored order) holds the couplings unless
+   * psigned; bnr is at most 3840. */
   const uint16_t* pbent;    /* nnz + 8 */
   const uint16_t* pbrow;    /* nnz */
-  const int32_t* pbtoff;    /* pbnc * bnr + 1 tile offsets */
-  void* pbcontrib;          /* nnz + 8 floats: the products between the two passes */
+  const int32_t* pbtoff;    /* pbnc * bnr + 1 tile offsets (stored order) */
+  const int32_t* pbrtoff;   /* bnr * pbnc + 1: tile (c, r) starts at pbrtoff[r * pbnc + c] in pbcontrib */
+  void* pbcontrib;          /* nnz + 8 floats: the products between the two passes, row-block order */
   int32_t pbnc;             /* column blocks */
   int32_t pad5;
 } sa_csr;
@@ -304,14 +307,17 @@ int sa_cstream_sort_bytes(int64_t nnz, size_t* bytes);
  *     cbound[ncb] = n, each block at most sa_pblock_cols() wide; the layout's pbound) and cgran
  *     (ceil(n / 64) entries: the block of columns [64 g, 64 g + 64)). *flag bit 1: a row block taller
  *     than brows, bit 2: a column block wider than sa_pblock_cols().
- *   sa_pblock_fill_f32: pbent, pbrow and (mag = 0) the values, in sorted order; *flag bit 8: a
- *     coupling's magnitude is not mag. */
+ *   the caller derives pbrtoff (the tiles' sizes pbtoff[t + 1] - pbtoff[t], scanned in row-block
+ *     order);
+ *   sa_pblock_fill_f32: pbent and (mag = 0) the values in stored order, pbrow in products order;
+ *     *flag bit 8: a coupling's magnitude is not mag. */
 int sa_pblock_cols(void);
 int sa_pblock_plan(const sa_csr* A, int32_t brows, int32_t bnr, const int32_t* cbound, int32_t ncb,
                    const int32_t* cgran, uint64_t* keys, int32_t* idx, uint64_t* keys_out, int32_t* idx_out, void* tmp,
                    size_t tmp_bytes, int32_t* toff, int32_t* flag, hipStream_t stream);
-int sa_pblock_fill_f32(const sa_csr* A, int32_t brows, int32_t bnr, const uint64_t* keys, const int32_t* idx,
-                       uint16_t* pbent, uint16_t* pbrow, float* bvals, double mag, int32_t* flag, hipStream_t stream);
+int sa_pblock_fill_f32(const sa_csr* A, int32_t brows, int32_t bnr, int32_t ncb, const uint64_t* keys,
+                       const int32_t* idx, const int32_t* toff, const int32_t* rtoff, uint16_t* pbent,
+                       uint16_t* pbrow, float* bvals, double mag, int32_t* flag, hipStream_t stream);
 int sa_cstream_max_rows(void);
 int sa_cstream_pad_row(void);
 int sa_cstream_plan(const sa_csr* A, const int32_t* pbound, int32_t brows, int32_t bnr, uint64_t* keys, int32_t* idx,

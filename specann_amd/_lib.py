@@ -106,6 +106,7 @@ class SaCsr(ctypes.Structure):
         ("pbent", c_void_p),
         ("pbrow", c_void_p),
         ("pbtoff", c_void_p),
+        ("pbrtoff", c_void_p),
         ("pbcontrib", c_void_p),
         ("pbnc", c_int32),
         ("pad5", c_int32),
@@ -235,7 +236,8 @@ _SIGNATURES = {
     "sa_pblock_cols": (c_int32, []),
     "sa_pblock_plan": (c_int32, [POINTER(SaCsr), c_int32, c_int32, _P, c_int32, _P, _P, _P, _P, _P, _P, c_size_t,
                                  _P, _P, _P]),
-    "sa_pblock_fill_f32": (c_int32, [POINTER(SaCsr), c_int32, c_int32, _P, _P, _P, _P, _P, c_double, _P, _P]),
+    "sa_pblock_fill_f32": (c_int32, [POINTER(SaCsr), c_int32, c_int32, c_int32, _P, _P, _P, _P, _P, _P, _P,
+                                     c_double, _P, _P]),
     "sa_alpha_scaling_f32": (c_int32, [c_int64, _P, c_double, c_double, _P, _P, _P]),
     "sa_alpha_scaling_f64": (c_int32, [c_int64, _P, c_double, c_double, _P, _P, _P]),
     "sa_spmv_scaled_f32": (c_int32, [POINTER(SaCsr), _P, _P, c_double, c_double, _P, _P, _P]),

@@ -5,12 +5,13 @@
 // splits the product in two streaming passes over a fixed intermediate:
 //   pass one: workgroup C copies column block C of g (at most 32768 columns, 128 KiB; the blocks are cut
 //     to hold about equal entry counts, so hub columns sit in narrow ones) into LDS once and walks
-//     the entries of those columns in stored order, writing each entry's product J_rj g_j to the
-//     next slot of a products array (coalesced: the products have the entries' order);
-//   pass two: workgroup R walks the products of its row block, tile by tile (one tile per column
-//     block, each a contiguous range, its entries running by row), sums each row's run in a wave,
-//     adds it into FP64 row sums in LDS with one atomic add and applies the epilogue
-//     y = a x + b s (.) (row sums).
+//     the entries of those columns in stored order, writing each entry's product J_rj g_j to a
+//     products array.
+//     The products are written in row-block order (pass one's tiles each land in one contiguous run
+//     of their row block's range), so that
+//   pass two: workgroup R streams the contiguous products of its row block (column block by column
+//     block, each tile running by row), sums each row's run in a wave, adds it into FP64 row sums in
+//     LDS with one atomic add and applies the epilogue y = a x + b s (.) (row sums).
 // The stream is 2 bytes per entry (local column and sign) in pass one, 4 bytes of product written and
 // read back and 2 bytes of local row in pass two: 12 bytes per entry, all sequential, against one
 // random line per entry for the gathers. The row sums are exact FP64 sums of FP32 products, so the
@@ -24,23 +25,26 @@ constexpr int kPbColBits = 15;
 constexpr int kPbCols = 1 << kPbColBits;   // column block: 128 KiB of g in LDS
 constexpr int kPbRows = 20416;             // row block: FP64 row sums + the padding slot in LDS
 constexpr int kPbKeyTile = 30;             // key: tile << 30 | local column << 15 | local row
-constexpr int kPbUnroll = 8;               // 8-entry groups per thread in flight (pass one)
+constexpr int kPbUnroll = 16;              // entries per lane in flight (pass one)
 constexpr int kPbUnroll2 = 16;             // entries per lane in flight (pass two)
+constexpr int kPbMaxRowBlocks = 3840;      // pass one keeps two int32 per row block in LDS
 
-typedef uint16_t pb_u16x8 __attribute__((ext_vector_type(8)));
 typedef float pb_f4 __attribute__((ext_vector_type(4)));
 
-// Pass one. Groups of eight stored entries (16 bytes of pbent) per thread; the groups at the ends of
-// the column block's range also hold entries of the neighbouring blocks, which are not written.
+// Pass one. The column block's tile starts (stored order) and the tiles' places in the products array
+// (row-block order) sit in LDS beside the block of g.
 template <bool kSigned>
 __global__ __launch_bounds__(kPbThreads) void pb_expand_kernel(SpmvParams<float> p) {
   __shared__ __attribute__((aligned(16))) float xs[kPbCols];
+  __shared__ int32_t ct[kPbMaxRowBlocks + 1];   // tile starts of this column block, stored order
+  __shared__ int32_t cr[kPbMaxRowBlocks];       // the same tiles' starts in the products array
   if (p.gate && *p.gate != SA_RUNNING) return;
   const int C = (int)blockIdx.x;
   const int64_t c0 = p.pbound[C];
   const int64_t width = (int64_t)p.pbound[C + 1] - c0;
   const int cnt = (int)(width < kPbCols ? width : kPbCols);  // at most kPbCols by construction
   const int tid = threadIdx.x;
+  const int nr = p.bnr;  // at most kPbMaxRowBlocks (checked at launch)
   for (int i = tid * 4; i < cnt; i += kPbThreads * 4) {
     if (i + 4 <= cnt) {
       *reinterpret_cast<pb_f4*>(&xs[i]) = *reinterpret_cast<const pb_f4*>(&p.g[c0 + i]);
@@ -48,51 +52,56 @@ __global__ __launch_bounds__(kPbThreads) void pb_expand_kernel(SpmvParams<float>
       for (int q = i; q < cnt; ++q) xs[q] = p.g[c0 + q];
     }
   }
+  for (int r = tid; r <= nr; r += kPbThreads) {
+    ct[r] = p.pbtoff[(int64_t)C * nr + r];
+    if (r < nr) cr[r] = p.pbrtoff[(int64_t)r * p.pbnc + C];
+  }
   __syncthreads();
-  const int64_t e0 = p.pbtoff[(int64_t)C * p.bnr], e1 = p.pbtoff[(int64_t)(C + 1) * p.bnr];
+  const int64_t e0 = ct[0], e1 = ct[nr];
   if (e0 >= e1) return;
-  const int64_t base = e0 & ~(int64_t)7;
-  const pb_u16x8* __restrict__ ent = reinterpret_cast<const pb_u16x8*>(p.pbent);
+  // Wave w takes the windows of 64 x kPbUnroll consecutive entries w, w + 16, ...; lane l the entries
+  // l, l + 64, ... of its window, so that every load and every run of stores is contiguous across the
+  // wave. The tile of a window's first entry comes from a binary search over ct (uniform), and each
+  // lane moves forward from it past the tile starts at or below its entry (a window of a few hundred
+  // entries crosses a few short tiles).
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint16_t* __restrict__ ent = p.pbent;
   const float* __restrict__ vals = (const float*)p.bvals;
   float* __restrict__ out = p.pbcontrib;
-  for (int64_t g0 = base + (int64_t)tid * 8; g0 < e1; g0 += (int64_t)kPbThreads * 8 * kPbUnroll) {
-    pb_u16x8 e[kPbUnroll];
-    pb_f4 w[kPbUnroll][2];
+  constexpr int64_t kWin = 64 * kPbUnroll;
+  for (int64_t w0 = e0 + (int64_t)wave * kWin; w0 < e1; w0 += (int64_t)(kPbThreads / 64) * kWin) {
+    uint32_t e[kPbUnroll];
+    float wv[kPbUnroll];
 #pragma unroll
     for (int u = 0; u < kPbUnroll; ++u) {
-      const int64_t g = g0 + (int64_t)u * kPbThreads * 8;
-      const int64_t gs = g < e1 ? g : base;  // past the end: a group of this block, not written
-      e[u] = __builtin_nontemporal_load(&ent[gs >> 3]);
-      if constexpr (!kSigned) {
-        w[u][0] = __builtin_nontemporal_load(reinterpret_cast<const pb_f4*>(&vals[gs]));
-        w[u][1] = __builtin_nontemporal_load(reinterpret_cast<const pb_f4*>(&vals[gs + 4]));
-      }
+      const int64_t i = w0 + 64 * u + lane;
+      const int64_t is = i < e1 ? i : e0;  // past the end: an entry of this block, not written
+      e[u] = __builtin_nontemporal_load(&ent[is]);
+      if constexpr (!kSigned) wv[u] = __builtin_nontemporal_load(&vals[is]);
     }
+    int lo = 0, hi = nr - 1;  // the last tile starting at or below w0
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (ct[mid] <= w0) lo = mid; else hi = mid - 1;
+    }
+    int R = lo;
 #pragma unroll
     for (int u = 0; u < kPbUnroll; ++u) {
-      const int64_t g = g0 + (int64_t)u * kPbThreads * 8;
-      if (g >= e1) break;
-      float v[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const uint32_t c = e[u][q];
-        const float x = xs[c & (kPbCols - 1)];
-        if constexpr (kSigned) v[q] = (c & 0x8000u) ? -x : x;
-        else v[q] = w[u][q >> 2][q & 3] * x;
-      }
-      if (g >= e0 && g + 8 <= e1) {
-        *reinterpret_cast<pb_f4*>(&out[g]) = pb_f4{v[0], v[1], v[2], v[3]};
-        *reinterpret_cast<pb_f4*>(&out[g + 4]) = pb_f4{v[4], v[5], v[6], v[7]};
-      } else {
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-          if (g + q >= e0 && g + q < e1) out[g + q] = v[q];
+      const int64_t i = w0 + 64 * u + lane;
+      const float x = xs[e[u] & (kPbCols - 1)];
+      float v;
+      if constexpr (kSigned) v = (e[u] & 0x8000u) ? -x : x;
+      else v = wv[u] * x;
+      if (i < e1) {
+        while (i >= ct[R + 1]) ++R;
+        out[(int64_t)cr[R] + (i - ct[R])] = v;
       }
     }
   }
 }
 
-// Pass two: workgroup R; wave w takes the tiles of column blocks w, w + 16, ..., 64 at a time.
+// Pass two: workgroup R streams its products; wave w takes every 16th run of 64 x kPbUnroll2.
 template <bool kSigned>
 __global__ __launch_bounds__(kPbThreads) void pb_reduce_kernel(SpmvParams<float> p) {
   __shared__ double acc[kPbRows + 1];
@@ -110,65 +119,37 @@ __global__ __launch_bounds__(kPbThreads) void pb_reduce_kernel(SpmvParams<float>
   __syncthreads();
   const float* __restrict__ prod = p.pbcontrib;
   const uint16_t* __restrict__ prow = p.pbrow;
-  // The tiles are short (a few hundred entries), so a wave packs 64 of them into one index space:
-  // lane l holds tile C = wave + 16 (k + l), its start and the exclusive prefix of the tile sizes, and
-  // slot q of the batch belongs to the last lane whose prefix is <= q (found by a binary search over
-  // the lanes with shuffles).
-  constexpr int kWaves = kPbThreads / 64;
-  for (int k = 0; k * kWaves + wave < p.pbnc; k += 64) {
-    const int C = wave + kWaves * (k + lane);
-    int64_t a = 0;
-    int cnt = 0;
-    if (C < p.pbnc) {
-      const int64_t t = (int64_t)C * p.bnr + R;
-      a = p.pbtoff[t];
-      cnt = (int)(p.pbtoff[t + 1] - a);
+  // the row block's products are contiguous (column block by column block, each running by row)
+  const int64_t a = p.pbrtoff[(int64_t)R * p.pbnc], b = p.pbrtoff[(int64_t)(R + 1) * p.pbnc];
+  for (int64_t i0 = a + (int64_t)wave * 64 * kPbUnroll2; i0 < b; i0 += (int64_t)kPbThreads * kPbUnroll2) {
+    float v[kPbUnroll2];
+    uint32_t r[kPbUnroll2];
+#pragma unroll
+    for (int u = 0; u < kPbUnroll2; ++u) {
+      const int64_t i = i0 + 64 * u + lane;
+      const int64_t is = i < b ? i : a;
+      v[u] = __builtin_nontemporal_load(&prod[is]);
+      r[u] = __builtin_nontemporal_load(&prow[is]);
     }
-    int incl = cnt;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += y;
-    }
-    const int excl = incl - cnt;
-    const int total = __shfl(incl, 63, 64);
-    for (int q0 = 0; q0 < total; q0 += 64 * kPbUnroll2) {
-      float v[kPbUnroll2];
-      uint32_t r[kPbUnroll2];
-#pragma unroll
-      for (int u = 0; u < kPbUnroll2; ++u) {
-        const int q = q0 + 64 * u + lane;
-        int lo = 0;  // the last lane with excl <= q (q < total: a nonempty tile)
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1) {
-          const int e = __shfl(excl, lo + step, 64);
-          if (e <= q) lo += step;
+    for (int u = 0; u < kPbUnroll2; ++u) {
+      // a row's run (consecutive entries of one row, possibly across two column blocks' tiles) is
+      // summed onto its first lane in FP64, which adds once
+      const int64_t i = i0 + 64 * u + lane;
+      const bool ok = i < b;
+      const uint32_t rn = (uint32_t)__shfl_down((int)r[u], 1, 64);
+      const uint64_t m = __ballot(ok && lane < 63 && i + 1 < b && rn == r[u]);
+      double d = ok ? (double)v[u] : 0.0;
+      if (m) {
+        uint64_t c = m;
+        for (int dd = 1; c != 0; dd <<= 1) {
+          const double w = __shfl_down(d, dd, 64);
+          if ((c >> lane) & 1ull) d += w;
+          c &= c >> dd;
         }
-        const int64_t base = __shfl(a, lo, 64);
-        const int off = q - __shfl(excl, lo, 64);
-        const int64_t i = q < total ? base + off : 0;
-        v[u] = __builtin_nontemporal_load(&prod[i]);
-        r[u] = __builtin_nontemporal_load(&prow[i]);
       }
-#pragma unroll
-      for (int u = 0; u < kPbUnroll2; ++u) {
-        // runs of one row (a tile's entries run by row; a run may continue into the next tile, whose
-        // local rows are the same rows) are summed onto their first lane in FP64, which adds once
-        const bool ok = q0 + 64 * u + lane < total;
-        const uint32_t rn = (uint32_t)__shfl_down((int)r[u], 1, 64);
-        const uint64_t m = __ballot(ok && lane < 63 && q0 + 64 * u + lane + 1 < total && rn == r[u]);
-        double d = ok ? (double)v[u] : 0.0;
-        if (m) {
-          uint64_t c = m;
-          for (int dd = 1; c != 0; dd <<= 1) {
-            const double w = __shfl_down(d, dd, 64);
-            if ((c >> lane) & 1ull) d += w;
-            c &= c >> dd;
-          }
-        }
-        const bool head = ok && (lane == 0 || !((m >> (lane - 1)) & 1ull));
-        if (head) atomicAdd(&acc[r[u]], d);
-      }
+      const bool head = ok && (lane == 0 || !((m >> (lane - 1)) & 1ull));
+      if (head) atomicAdd(&acc[r[u]], d);
     }
   }
   __syncthreads();
@@ -187,8 +168,9 @@ int launch_pblock_spmv(const SpmvParams<float>& p, hipStream_t stream) {
     set_error("spmv: incomplete two-pass layout");
     return SA_E_ARG;
   }
-  if (p.brows < 64 || p.brows > kPbRows || p.bnr < 1 || (!p.rbound && (int64_t)p.bnr * p.brows < p.n) ||
-      p.pbnc < 1 || !p.pbound || (int64_t)p.pbnc * p.bnr >= (1ll << 31)) {
+  if (p.brows < 64 || p.brows > kPbRows || p.bnr < 1 || p.bnr > kPbMaxRowBlocks ||
+      (!p.rbound && (int64_t)p.bnr * p.brows < p.n) || p.pbnc < 1 || !p.pbound || !p.pbrtoff ||
+      (int64_t)p.pbnc * p.bnr >= (1ll << 31)) {
     set_error("spmv: two-pass layout sizes out of range");
     return SA_E_ARG;
   }
@@ -247,9 +229,13 @@ __global__ __launch_bounds__(SA_BLOCK) void pb_key_kernel(sa_csr A, int32_t brow
   }
 }
 
+// Sorted entry i (stored order) of tile t = C * bnr + R, rank k in its tile: pbent[i] (and the value),
+// and its local row at the tile's place in the products array, pbrow[rtoff[R * ncb + C] + k].
 __global__ __launch_bounds__(SA_BLOCK) void pb_fill_kernel(int64_t nnz, const uint64_t* __restrict__ keys,
                                                            const int32_t* __restrict__ idx,
-                                                           const float* __restrict__ vals, float mag,
+                                                           const int32_t* __restrict__ toff,
+                                                           const int32_t* __restrict__ rtoff, int32_t bnr,
+                                                           int32_t ncb, const float* __restrict__ vals, float mag,
                                                            uint16_t* __restrict__ pbent, uint16_t* __restrict__ pbrow,
                                                            float* __restrict__ bvals, int32_t* __restrict__ flag) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz + 8; i += (int64_t)gridDim.x * blockDim.x) {
@@ -268,7 +254,9 @@ __global__ __launch_bounds__(SA_BLOCK) void pb_fill_kernel(int64_t nnz, const ui
       bvals[i] = v;
     }
     pbent[i] = (uint16_t)code;
-    pbrow[i] = (uint16_t)((key >> 15) & 0x7FFFu);
+    const int64_t t = (int64_t)(key >> kPbKeyTile);
+    const int64_t R = t % bnr, C = t / bnr;
+    pbrow[(int64_t)rtoff[R * ncb + C] + (i - toff[t])] = (uint16_t)((key >> 15) & 0x7FFFu);
   }
 }
 
@@ -299,13 +287,14 @@ int sa_pblock_plan(const sa_csr* A, int32_t brows, int32_t bnr, const int32_t* c
                              tmp_bytes, sa::kPbKeyTile, ntiles, toff, stream);
 }
 
-int sa_pblock_fill_f32(const sa_csr* A, int32_t brows, int32_t bnr, const uint64_t* keys, const int32_t* idx,
-                       uint16_t* pbent, uint16_t* pbrow, float* bvals, double mag, int32_t* flag, hipStream_t stream) {
-  SA_REQUIRE(A && A->n > 0 && A->nnz >= 0 && A->vals && brows >= 64 && bnr >= 1, "bad arguments");
-  SA_REQUIRE(keys && idx && pbent && pbrow && flag && (mag > 0.0 || bvals), "bad arguments");
+int sa_pblock_fill_f32(const sa_csr* A, int32_t brows, int32_t bnr, int32_t ncb, const uint64_t* keys,
+                       const int32_t* idx, const int32_t* toff, const int32_t* rtoff, uint16_t* pbent,
+                       uint16_t* pbrow, float* bvals, double mag, int32_t* flag, hipStream_t stream) {
+  SA_REQUIRE(A && A->n > 0 && A->nnz >= 0 && A->vals && brows >= 64 && bnr >= 1 && ncb >= 1, "bad arguments");
+  SA_REQUIRE(keys && idx && toff && rtoff && pbent && pbrow && flag && (mag > 0.0 || bvals), "bad arguments");
   hipLaunchKernelGGL(sa::pb_fill_kernel, dim3(sa::grid_for(A->nnz + 8, SA_BLOCK, 8192)), dim3(SA_BLOCK), 0, stream,
-                     A->nnz, keys, idx, (const float*)A->vals, (float)(mag > 0.0 ? mag : 0.0), pbent, pbrow,
-                     mag > 0.0 ? (float*)nullptr : bvals, flag);
+                     A->nnz, keys, idx, toff, rtoff, bnr, ncb, (const float*)A->vals, (float)(mag > 0.0 ? mag : 0.0),
+                     pbent, pbrow, mag > 0.0 ? (float*)nullptr : bvals, flag);
   return sa::check_launch("pblock_fill");
 }
 

@@ -355,6 +355,7 @@ SpmvParams<T> params_from(const sa_csr* A) {
   p.pbent = A->pbent;
   p.pbrow = A->pbrow;
   p.pbtoff = A->pbtoff;
+  p.pbrtoff = A->pbrtoff;
   p.pbcontrib = (float*)A->pbcontrib;
   p.pbnc = A->pbnc;
   return p;

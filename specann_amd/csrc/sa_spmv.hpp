@@ -72,6 +72,7 @@ struct SpmvParams {
   const uint16_t* pbent;
   const uint16_t* pbrow;
   const int32_t* pbtoff;
+  const int32_t* pbrtoff;
   float* pbcontrib;
   int32_t pbnc;
 };

@@ -23,6 +23,7 @@ BLOCK_ROWS = {4: 26240, 8: 13312}
 BLOCK_COLS = {4: 14336, 8: 6976}
 BLOCK_MAX_TILES = 368       # column blocks per panel the blocked SpMV can walk
 CUS_PER_XCD = 32
+PBLOCK_MAX_ROW_BLOCKS = 3840   # row blocks of the two-pass layout (pass one keeps two int32 each in LDS)
 INT_FIELD_LIMIT = float(1 << 29)   # largest absolute degree the exact int32-field local search takes
 
 
@@ -594,6 +595,8 @@ class DeviceProblem:
             bounds_r = self.whole_round_row_bounds(bnr, rb)
             bnr = len(bounds_r) - 1
             rbound = torch.from_numpy(bounds_r.astype(np.int32)).to(dev)
+        if bnr > PBLOCK_MAX_ROW_BLOCKS:
+            return False
         # column blocks of about equal entry counts (twice as many as uniform blocks of the widest size
         # the LDS holds), so that a hub's columns do not make one pass-one workgroup the long pole
         ip = self.indptr.cpu().numpy().astype(np.int64)
@@ -626,17 +629,22 @@ class DeviceProblem:
             signed = self.uniform_magnitude is not None
         signed = bool(signed and self.uniform_magnitude is not None)
         mag = self.uniform_magnitude if signed else 0.0
+        # tile (c, r)'s place in the products array, ordered by row block then column block
+        sizes = (toff[1:] - toff[:-1]).to(torch.int64).view(nc, bnr).t().reshape(-1)
+        rtoff = torch.zeros(nc * bnr + 1, dtype=torch.int64, device=dev)
+        rtoff[1:] = torch.cumsum(sizes, 0)
+        rtoff = rtoff.to(torch.int32)
         pbent = torch.zeros(nnz + 16, dtype=torch.int16, device=dev)
         pbrow = torch.empty(max(nnz, 1), dtype=torch.int16, device=dev)
         bvals = None if signed else torch.zeros(nnz + 16, dtype=torch.float32, device=dev)
-        call("sa_pblock_fill_f32", self.csr_ref, rb, bnr, ptr(keys_out), ptr(idx_out), ptr(pbent), ptr(pbrow),
-             ptr(bvals), mag, ptr(flag), s)
+        call("sa_pblock_fill_f32", self.csr_ref, rb, bnr, nc, ptr(keys_out), ptr(idx_out), ptr(toff), ptr(rtoff),
+             ptr(pbent), ptr(pbrow), ptr(bvals), mag, ptr(flag), s)
         del keys_out, idx_out
         if int(flag.item()) != 0:
             return False
         contrib = torch.empty(nnz + 16, dtype=torch.float32, device=dev)
         self.build_panels(0)
-        self.pbent, self.pbrow, self.pbtoff, self.pbcontrib = pbent, pbrow, toff, contrib
+        self.pbent, self.pbrow, self.pbtoff, self.pbrtoff, self.pbcontrib = pbent, pbrow, toff, rtoff, contrib
         self.bvals, self.rbound, self.pbound = bvals, rbound, cbound
         self.block = {"layout": "pblock", "brows": rb, "bnr": bnr, "column_blocks": nc,
                       "balanced": rbound is not None}
@@ -647,16 +655,17 @@ class DeviceProblem:
         c.rbound = rbound.data_ptr() if rbound is not None else None
         c.pbent, c.pbrow, c.pbtoff, c.pbcontrib, c.pbnc = (pbent.data_ptr(), pbrow.data_ptr(), toff.data_ptr(),
                                                            contrib.data_ptr(), nc)
-        c.pbound = cbound.data_ptr()
+        c.pbound, c.pbrtoff = cbound.data_ptr(), rtoff.data_ptr()
         return True
 
     def _clear_blocks(self) -> None:
         c = self.csr
         c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 0, 0, 0, 0, 0
         c.pbound = c.bchunk = c.bent = c.bvals = c.rbound = c.bcbase = None
-        c.pbent = c.pbrow = c.pbtoff = c.pbcontrib = None
+        c.pbent = c.pbrow = c.pbtoff = c.pbrtoff = c.pbcontrib = None
         c.pbnc = 0
-        for name in ("bchunk", "bent", "bvals", "block", "rbound", "bcbase", "pbent", "pbrow", "pbtoff", "pbcontrib"):
+        for name in ("bchunk", "bent", "bvals", "block", "rbound", "bcbase", "pbent", "pbrow", "pbtoff", "pbrtoff",
+                     "pbcontrib"):
             if hasattr(self, name):
                 delattr(self, name)
 

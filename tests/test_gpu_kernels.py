@@ -276,11 +276,15 @@ def test_pblock_spmv_matches_oracle(gpu, case, brows, balanced):
     assert cbound[0] == 0 and cbound[-1] == prob.n and np.all(np.diff(cbound) > 0) and np.all(np.diff(cbound) <= cols)
     toff = prob.pbtoff.cpu().numpy()
     assert toff[0] == 0 and toff[-1] == prob.nnz and np.all(np.diff(toff) >= 0) and toff.size == c.pbnc * c.bnr + 1
-    # the stored local rows and columns reproduce the matrix, tile by tile
-    rows_l = prob.pbrow.cpu().numpy().view(np.uint16).astype(np.int64)
+    # the stored local columns (stored order) and rows (products order) reproduce the matrix, tile by tile
+    rtoff = prob.pbrtoff.cpu().numpy().astype(np.int64)
+    sizes = np.diff(toff).reshape(c.pbnc, c.bnr)
+    assert rtoff[0] == 0 and rtoff[-1] == prob.nnz and np.array_equal(np.diff(rtoff), sizes.T.reshape(-1))
     ent = prob.pbent.cpu().numpy().view(np.uint16)[: prob.nnz].astype(np.int64)
     t = np.repeat(np.arange(toff.size - 1), np.diff(toff))
     cb, rb = t // c.bnr, t % c.bnr
+    rank = np.arange(prob.nnz) - toff[t]
+    rows_l = prob.pbrow.cpu().numpy().view(np.uint16).astype(np.int64)[rtoff[rb * c.pbnc + cb] + rank]
     r0 = prob.rbound.cpu().numpy()[rb] if balanced else rb * c.brows
     got = sp.csr_matrix((np.ones(prob.nnz), (r0 + rows_l, cbound[cb] + (ent & 0x7FFF))), shape=J.shape)
     want = sp.csr_matrix(J, copy=True)
