@@ -151,8 +151,10 @@ class LanczosSolver:
 
     def _queue(self, s: torch.Tensor | None, count: int) -> None:
         if self.use_graphs and count == self.chunk:
-            key = (None if s is None else s.data_ptr(), self.prob.csr.tile_rows, self.prob.csr.npanels,
-                   self.prob.csr.tile_nnz)
+            c = self.prob.csr
+            # the captured launches carry the layout's pointers: any layout change needs a new graph
+            key = (None if s is None else s.data_ptr(), c.tile_rows, c.npanels, c.tile_nnz, c.bmode, c.bent,
+                   c.pbent, c.partial)
             if self._graph is None or self._graph_key != key:
                 g = torch.cuda.CUDAGraph()
                 torch.cuda.synchronize()
