@@ -183,6 +183,59 @@ __global__ __launch_bounds__(SA_BLOCK) void spmv_panel_kernel(SpmvParams<T> p) {
                               p.lanes_fixed, prod, red, st);
 }
 
+// The same with 16-byte loads and stores (4 floats or 2 doubles per lane), every row summed in the same
+// order: bit-identical to combine_kernel. Needs 16-byte aligned vectors and pld a multiple of the vector.
+template <typename T>
+__global__ __launch_bounds__(SA_BLOCK) void combine_vec_kernel(int64_t n, int32_t npanels,
+                                                               const T* __restrict__ partial, int64_t pld,
+                                                               const T* __restrict__ s, const T* __restrict__ x,
+                                                               double a, double b, T* __restrict__ y,
+                                                               const int32_t* gate) {
+  typedef typename vec16<T>::type V;
+  constexpr int L = vec16<T>::L;
+  if (gate && *gate != SA_RUNNING) return;
+  const int64_t nv = n / L;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nv; q += stride) {
+    const int64_t r = q * L;
+    double acc[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) acc[k] = 0.0;
+    int p = 0;
+    for (; p + 4 <= npanels; p += 4) {
+      V v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(reinterpret_cast<const V*>(&partial[(int64_t)(p + u) * pld + r]));
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int k = 0; k < L; ++k) acc[k] += (double)v[u][k];
+    }
+    for (; p < npanels; ++p) {
+      const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(&partial[(int64_t)p * pld + r]));
+#pragma unroll
+      for (int k = 0; k < L; ++k) acc[k] += (double)v[k];
+    }
+    V sv, xv, out;
+    if (s) sv = *reinterpret_cast<const V*>(&s[r]);
+    if (x) xv = *reinterpret_cast<const V*>(&x[r]);
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      double t = b * (s ? (double)sv[k] : 1.0) * acc[k];
+      if (x) t += a * (double)xv[k];
+      out[k] = (T)t;
+    }
+    *reinterpret_cast<V*>(&y[r]) = out;
+  }
+  for (int64_t r = nv * L + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += stride) {
+    double acc = 0.0;
+    for (int p = 0; p < npanels; ++p) acc += (double)partial[(int64_t)p * pld + r];
+    double t = b * (s ? (double)s[r] : 1.0) * acc;
+    if (x) t += a * (double)x[r];
+    y[r] = (T)t;
+  }
+}
+
 // y_r = a x_r + b s_r sum_p partial[p][r] (panel order fixed).
 template <typename T>
 __global__ __launch_bounds__(SA_BLOCK) void combine_kernel(int64_t n, int32_t npanels, const T* __restrict__ partial,
@@ -454,8 +507,14 @@ int spmv_apply(const sa_csr* A, const T* s, const T* g, double a, const T* x, do
   p.gate = gate;
   int rc = launch_spmv<T>(p, A->tile_nnz, stream);
   if (rc || A->npanels == 0) return rc;
-  hipLaunchKernelGGL(combine_kernel<T>, dim3(grid_for(A->n, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0, stream, A->n,
-                     A->npanels, (const T*)A->partial, p.pld, s, p.x, a, b, y, gate);
+  constexpr int L = vec16<T>::L;
+  if (aligned16(A->partial) && p.pld % L == 0 && aligned16(y) && (!s || aligned16(s)) && (!p.x || aligned16(p.x))) {
+    hipLaunchKernelGGL(combine_vec_kernel<T>, dim3(grid_for(A->n, SA_BLOCK * L * 2, 4096)), dim3(SA_BLOCK), 0, stream,
+                       A->n, A->npanels, (const T*)A->partial, p.pld, s, p.x, a, b, y, gate);
+  } else {
+    hipLaunchKernelGGL(combine_kernel<T>, dim3(grid_for(A->n, SA_BLOCK * 4, 4096)), dim3(SA_BLOCK), 0, stream, A->n,
+                       A->npanels, (const T*)A->partial, p.pld, s, p.x, a, b, y, gate);
+  }
   return check_launch("spmv_combine");
 }
 
