@@ -107,9 +107,9 @@ __device__ inline void walk_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The walk over the column blocks is software-pipelined one tile deep: while tile c is summed, the
-// column block and the entries of tile c + 1 are loaded into registers; the block goes to LDS between
-// the two barriers of the next tile. The walk is unrolled by two so that the two register sets of
+// The walk over the column blocks is software-pipelined: while tile c is summed, the entries of tile
+// c + 1 are loaded into registers, and after it the column block of tile c + 2 (two register sets of
+// blocks alternate); a block goes to LDS between the two barriers of its tile. The walk is unrolled by two so that the two register sets of
 // entries never move between registers (a move would make their loads complete early). Every load is
 // issued unconditionally (out-of-range lanes read a valid stand-in address and drop the value), so
 // the loads in flight are counted exactly and each wait covers only the loads it needs.
@@ -152,8 +152,8 @@ __global__ __launch_bounds__(kBThreads) void spmv_block_kernel(SpmvParams<T> p) 
     return (int)(rest < CB ? rest : CB);  // <= 0 past the panel
   };
 
-  V xr[kBXv];
-  auto load_block = [&](int c) {
+  V xa[kBXv], xq[kBXv];  // column blocks c and c + 1 in flight, alternating
+  auto load_block = [&](int c, V* xr) {
     const int64_t lo = c0 + (int64_t)c * CB;
     const int cnt = block_count(c);
 #pragma unroll
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kBThreads) void spmv_block_kernel(SpmvParams<T> p) 
       xr[u] = *reinterpret_cast<const V*>(i + L <= cnt ? p.g + lo + i : p.g);
     }
   };
-  auto store_block = [&](int c) {
+  auto store_block = [&](int c, const V* xr) {
     const int64_t lo = c0 + (int64_t)c * CB;
     const int cnt = block_count(c);
 #pragma unroll
@@ -186,15 +186,15 @@ __global__ __launch_bounds__(kBThreads) void spmv_block_kernel(SpmvParams<T> p) 
     }
   };
   // Tile c (summed only if c < nc; a step past the panel keeps the barriers uniform): its entries are
-  // in cur; the block and the entries of tile c + 1 are loaded (into xr and nxt).
-  auto step = [&](int c, const uint32_t* cur, const T* curv, uint32_t* nxt, T* nxtv) {
+  // in cur and its column block in X; the entries of tile c + 1 are loaded into nxt and the column
+  // block of tile c + 2 into X.
+  auto step = [&](int c, V* X, const uint32_t* cur, const T* curv, uint32_t* nxt, T* nxtv) {
     const bool live = c < nc;
     const int64_t k0 = live ? bound(c) : 0, k1 = live ? bound(c + 1) : 0;
     const int64_t f0 = bound(c + 1), f1 = bound(c + 2);
     __syncthreads();  // tile c - 1 is summed: xb is free
-    store_block(c);
+    store_block(c, X);
     __syncthreads();
-    load_block(c + 1);
     fetch(f0, f1, nxt, nxtv);
     T v[kBPre];
     bool on[kBPre];
@@ -212,17 +212,21 @@ __global__ __launch_bounds__(kBThreads) void spmv_block_kernel(SpmvParams<T> p) 
       const bool one[1] = {true};
       block_chunks_add<T, 1>(e, x, one, yacc, lane);
     }
+    // the column block two tiles ahead, issued after the summing: its loads then do not delay this
+    // tile's and have a whole tile to land before their store
+    load_block(c + 2, X);
   };
 
   uint32_t eA[kBPre], eB[kBPre];
   T vA[kBPre], vB[kBPre];
   if (nc > 0) {
-    load_block(0);
+    load_block(0, xa);
     fetch(bound(0), bound(1), eA, vA);
+    load_block(1, xq);
   }
   for (int c = 0; c < nc; c += 2) {
-    step(c, eA, vA, eB, vB);
-    step(c + 1, eB, vB, eA, vA);
+    step(c, xa, eA, vA, eB, vB);
+    step(c + 1, xq, eB, vB, eA, vA);
   }
   __syncthreads();
   const T scale = kSigned ? p.pmag : (T)1;
