@@ -152,7 +152,8 @@ def run_batch(args, w):
         "config": {"workload": w["text"], "alphas": args.alphas, "predictor": pred, "instances": len(Js),
                    "n_total": N, "nnz_total": nnz,
                    "layout": ("packed block-diagonal CSR; the autotuner picks plain per-instance tiles (tile size, lanes "
-                              "per row) or the instance-staged layout (rows per workgroup, lanes per row)"),
+                              "per row), the instance-staged layout (rows per workgroup, lanes per row) or, in FP32, "
+                              "the instance-streamed layout (rows per workgroup)"),
                    "tiles": tunes},
         "fp64_sweep_s": sec64, "fp32_vs_fp64_speedup": (sec64 / sec32) if sec64 else None,
         # (E32 - E64) / |E64| per instance: negative = FP32 found the lower energy
@@ -167,9 +168,12 @@ def run_batch(args, w):
                                  "extrapolated_all_s": solo * len(Js) / sample if solo else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": ("spmv_inst_kernel<float> (instance slices of the gathered vector staged in LDS)"
-                                if tunes[str(torch.float32)]["kind"] == "instance-staged" else
-                                "spmv_tile_kernel<float> (fused scaled SpMV over the packed CSR)"),
+                     "kernel": {"instance-staged": "spmv_inst_kernel<float> (instance slices of the gathered vector "
+                                                   "staged in LDS)",
+                                "instance-streamed": "spmv_istream_kernel (each workgroup's rows of one instance "
+                                                     "streamed by column, FP64 row sums in LDS)"}.get(
+                                    tunes[str(torch.float32)]["kind"],
+                                    "spmv_tile_kernel<float> (fused scaled SpMV over the packed CSR)"),
                      "bytes_per_launch": spmv_bytes, "us_per_launch": spmv_sec * 1e6},
         "cpu_baseline": cpu,
     }

@@ -107,8 +107,13 @@ typedef struct sa_csr {
    * c in [isplit[(c-1) n + r], isplit[c n + r]) of the CSR (indptr bounds the first and last pass).
    * isigned = 1: every coupling of instance i has the magnitude imag[i] (T) and ient holds
    * column | sign << 31 for each entry; else indices and vals are read. */
+  /* imode = 2 (FP32, no counterpart in the reference either): the instance-streamed layout, the
+   * column-streamed chunks (bchunk, bent, bvals, bcbase; values always stored) built with one row
+   * block per workgroup w, rows [irow[w], min(irow[w] + irows, iend[iinst[w]])) with irows up to
+   * sa_cstream_max_rows(), and every column in panel 0: workgroup w's chunks are
+   * [bchunk[8 w], bchunk[8 w + 1]). */
   int32_t imode;
-  int32_t irows;            /* rows per workgroup, 64..2048 */
+  int32_t irows;            /* rows per workgroup, 64..2048 (imode 1) */
   int32_t ilanes;
   int32_t ipasses;
   int32_t isigned;

@@ -234,10 +234,56 @@ class BatchProblem:
         self.layout = {"kind": "instance-staged", "rows": int(rows), "lanes": int(lanes), "passes": passes,
                        "signed": cache["signed"]}
 
+    def istream_eligible(self) -> bool:
+        """Whether the instance-streamed layout applies: FP32, and every instance within its chunks'
+        16-bit column offsets."""
+        return self.dtype == torch.float32 and int(self.sizes.max()) <= 65536
+
+    def set_istream_layout(self, rows: int = 20416) -> bool:
+        """Switch the SpMV to the instance-streamed layout (sa_csr imode 2): workgroups of at most `rows`
+        rows of one instance, each streaming its rows' entries sorted by column (the column-streamed
+        chunks, built with these row blocks and one panel) with FP64 row sums in LDS. False (layout
+        unchanged) when it does not apply."""
+        prob, dev = self.prob, self.device
+        cap = int(_lib.load().sa_cstream_max_rows())
+        if not self.istream_eligible() or not 64 <= rows <= cap or rows % 64:
+            return False
+        cache = self._istream_cache = getattr(self, "_istream_cache", None) or {}
+        if cache.get("rows") != rows:
+            ro = self.row_off_h
+            wg_row, wg_inst, start, count = [], [], np.zeros(self.B, np.int64), np.zeros(self.B, np.int64)
+            for b in range(self.B):
+                rs = np.arange(ro[b], ro[b + 1], rows, dtype=np.int64)
+                start[b], count[b] = len(wg_row), rs.size
+                wg_row.extend(rs.tolist())
+                wg_inst.extend([b] * rs.size)
+            rbound = np.asarray(wg_row + [self.n], dtype=np.int64)
+            bounds = np.array([0] + [self.n] * 8, dtype=np.int64)   # every column in panel 0
+            arr = prob.cstream_arrays(rows, len(wg_row), torch.from_numpy(rbound.astype(np.int32)).to(dev), bounds,
+                                      signed=False)
+            if arr is None:
+                return False
+            cache.clear()
+            cache.update(arr)
+            cache.update(rows=rows, start=start, count=count, nwg=len(wg_row),
+                         iinst=torch.tensor(wg_inst, dtype=torch.int32, device=dev),
+                         irow=torch.tensor(wg_row, dtype=torch.int32, device=dev),
+                         iend=torch.from_numpy(ro[1:].astype(np.int64)).to(dev))
+        csr = prob.csr
+        csr.imode = 2
+        csr.irows, csr.inwg = int(rows), int(cache["nwg"])
+        csr.iinst, csr.irow, csr.iend = (cache["iinst"].data_ptr(), cache["irow"].data_ptr(),
+                                         cache["iend"].data_ptr())
+        csr.bchunk, csr.bent, csr.bvals, csr.bcbase = (cache["bchunk"].data_ptr(), cache["bent"].data_ptr(),
+                                                       cache["bvals"].data_ptr(), cache["bcbase"].data_ptr())
+        self.unit_start, self.unit_count = cache["start"], cache["count"]
+        self.layout = {"kind": "instance-streamed", "rows": int(rows), "chunks": int(cache["nchunks"])}
+        return True
+
     def autotune(self, reps: int = 4, verbose: bool = False) -> dict:
         """Pick the batched SpMV's layout by timing the packed matrix's SpMV on each: plain per-instance
-        tiles (tile size, lanes per row) and, when every instance fits, the instance-staged layout
-        (rows per workgroup, lanes per row)."""
+        tiles (tile size, lanes per row), when every instance fits, the instance-staged layout (rows per
+        workgroup, lanes per row), and in FP32 the instance-streamed layout (rows per workgroup)."""
         results = []
         for tile in (512, 1024, 2048, 4096):
             for lanes in (0, 4, 8, 16, 32, 64):
@@ -255,11 +301,20 @@ class BatchProblem:
                     if verbose:
                         print(f"[batch autotune] instance-staged rows {rows} lanes {lanes}: {sec * 1e6:.1f} us",
                               flush=True)
+        if self.istream_eligible():
+            for rows in (4096, 20416):
+                if self.set_istream_layout(rows):
+                    sec = self.prob.time_spmv(reps)
+                    results.append({"kind": "instance-streamed", "rows": rows, "seconds": sec})
+                    if verbose:
+                        print(f"[batch autotune] instance-streamed rows {rows}: {sec * 1e6:.1f} us", flush=True)
         best = min(results, key=lambda r: r["seconds"])
         if best["kind"] == "tiles":
             self.set_tiling(best["tile_nnz"], best["lanes_per_row"])
-        else:
+        elif best["kind"] == "instance-staged":
             self.set_inst_layout(best["rows"], best["lanes"])
+        else:
+            self.set_istream_layout(best["rows"])
         self.tuning = {"choice": best, "tried": results}
         return self.tuning
 

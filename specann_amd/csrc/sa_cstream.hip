@@ -104,6 +104,102 @@ __global__ __launch_bounds__(kCsThreads) void spmv_cstream_kernel(SpmvParams<flo
   for (int i = tid; i < rows; i += kCsThreads) out[i] = (float)(scale * acc[i]);
 }
 
+// Instance-streamed SpMV of a block-diagonal batch (imode = 2): the same walk per workgroup of one
+// instance's rows. Workgroup w (tile_list[blockIdx.x] when a list is given) owns rows [irow[w],
+// min(irow[w] + irows, iend[iinst[w]])) and the chunks [bchunk[8 w], bchunk[8 w + 1]) of the
+// column-streamed layout built with those row blocks and one panel (every column in panel 0), entries
+// with values. An instance's columns are its rows' (block diagonal), so a row block's columns are
+// dense (tens of entries per column for instances of a few thousand nodes) and a 64-lane gather
+// reads one or two lines. The workgroup does nothing unless its instance is running, and the epilogue
+// is the plain CSR kernel's: y = a x + b s (.) (row sums); padding rows get y = a x.
+__global__ __launch_bounds__(kCsThreads) void spmv_istream_kernel(SpmvParams<float> p) {
+  constexpr int kU = kCsUnrollValued;
+  __shared__ double acc[kCsRows + 1];
+  if (p.gate && *p.gate != SA_RUNNING) return;
+  double bb = p.b;
+  if (p.lz) bb *= p.lz->sigma[p.lz->j];
+  const int64_t w = p.tile_list ? (int64_t)p.tile_list[blockIdx.x] : (int64_t)blockIdx.x;
+  const int inst = p.iinst[w];
+  if (p.gate_base && *reinterpret_cast<const int32_t*>(p.gate_base + (int64_t)inst * p.gate_stride) != SA_RUNNING)
+    return;
+  const int64_t r0 = p.irow[w];
+  const int64_t end = p.iend[inst];
+  const int rows = (int)((r0 + p.irows < end ? r0 + p.irows : end) - r0);  // at most irows <= kCsRows
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < rows; i += kCsThreads) acc[i] = 0.0;
+  __syncthreads();
+  const int64_t k0 = p.bchunk[8 * w], k1 = p.bchunk[8 * w + 1];
+  const float* __restrict__ gp = p.g;
+  const cs_u4* __restrict__ ent = reinterpret_cast<const cs_u4*>(p.bent);
+  const cs_f4* __restrict__ val = reinterpret_cast<const cs_f4*>(p.bvals);
+  cs_u4 e[kU];
+  cs_f4 wv[kU];
+  int32_t cb[kU];
+  auto fetch = [&](int64_t k) {
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t kk = k + (int64_t)u * kCsWaves;
+      const int64_t ks = kk < k1 ? kk : k0;
+      e[u] = __builtin_nontemporal_load(&ent[ks * 64 + lane]);
+      wv[u] = __builtin_nontemporal_load(&val[ks * 64 + lane]);
+      cb[u] = p.bcbase[ks];
+    }
+  };
+  int64_t k = k0 + wave;
+  if (k < k1) fetch(k);
+  for (; k < k1; k += (int64_t)kU * kCsWaves) {
+    float x[kU][4];
+    cs_u4 ec[kU];
+    cs_f4 wc[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      ec[u] = e[u];
+      wc[u] = wv[u];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) x[u][j] = gp[cb[u] + (int)(ec[u][j] & 0xFFFFu)];
+    }
+    if (k + (int64_t)kU * kCsWaves < k1) fetch(k + (int64_t)kU * kCsWaves);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (k + (int64_t)u * kCsWaves < k1) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) atomicAdd(&acc[(ec[u][j] >> 16) & 0x7FFFu], (double)(wc[u][j] * x[u][j]));
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < rows; i += kCsThreads) {
+    const int64_t r = r0 + i;
+    double v = bb * (p.s ? (double)p.s[r] : 1.0) * acc[i];
+    if (p.x) v += p.a * (double)p.x[r];
+    p.y[r] = (float)v;
+  }
+}
+
+int launch_istream_spmv(const SpmvParams<float>& p, hipStream_t stream) {
+  if (!p.iinst || !p.irow || !p.iend || !p.bchunk || !p.bent || !p.bvals || !p.bcbase || !p.y) {
+    set_error("spmv: incomplete instance-streamed layout");
+    return SA_E_ARG;
+  }
+  if (p.irows < 64 || p.irows > kCsRows || p.inwg < 1 || p.inwg >= (1ll << 28)) {
+    set_error("spmv: instance-streamed layout sizes out of range");
+    return SA_E_ARG;
+  }
+  if (!aligned16(p.bent) || !aligned16(p.bvals)) {
+    set_error("spmv: the instance-streamed layout needs 16-byte aligned entries and values");
+    return SA_E_ARG;
+  }
+  if (p.tile_list && (p.ntile_list < 0 || p.ntile_list > p.inwg)) {
+    set_error("spmv: tile list longer than the layout's workgroups");
+    return SA_E_ARG;
+  }
+  const int64_t nwg = p.tile_list ? p.ntile_list : p.inwg;
+  if (nwg == 0) return SA_OK;
+  hipLaunchKernelGGL(spmv_istream_kernel, dim3((unsigned)nwg), dim3(kCsThreads), 0, stream, p);
+  return check_launch("spmv_istream");
+}
+
 int launch_cstream_spmv(const SpmvParams<float>& p, hipStream_t stream) {
   if (p.npanels != 8 || !p.pbound || !p.bchunk || !p.bent || !p.bcbase || (!p.bvals && !p.psigned) || !p.partial) {
     set_error("spmv: incomplete column-streamed layout");

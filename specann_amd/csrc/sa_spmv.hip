@@ -418,7 +418,15 @@ template <typename T>
 int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream) {
   const dim3 block(SA_BLOCK);
   if (p.imode == 1 && p.npanels == 0) return launch_inst_spmv<T>(p, stream);
-  if (p.imode != 0 && p.imode != 1) {
+  if (p.imode == 2 && p.npanels == 0) {
+    if constexpr (std::is_same<T, float>::value) {
+      return launch_istream_spmv(p, stream);
+    } else {
+      set_error("spmv: the instance-streamed layout is FP32 only");
+      return SA_E_ARG;
+    }
+  }
+  if (p.imode != 0 && p.imode != 1 && p.imode != 2) {
     set_error("spmv: unknown instance layout mode");
     return SA_E_ARG;
   }

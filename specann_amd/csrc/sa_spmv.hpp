@@ -93,6 +93,8 @@ int launch_cstream_spmv(const SpmvParams<float>& p, hipStream_t stream);
 
 int launch_pblock_spmv(const SpmvParams<float>& p, hipStream_t stream);
 
+int launch_istream_spmv(const SpmvParams<float>& p, hipStream_t stream);
+
 // Layout builds: radix-sort (key, entry index) pairs (on the caller's scratch) and set start[q] to the
 // first sorted key whose segment (key >> shift) is q, for q in [0, nseg]; bits needed to count to count.
 int sort_entry_keys(const uint64_t* keys, const int32_t* idx, uint64_t* keys_out, int32_t* idx_out, int64_t nnz,

@@ -498,7 +498,7 @@ class DeviceProblem:
         every coupling has one magnitude; False: values stored beside the entries)."""
         if self.dtype != torch.float32:
             return False
-        n, nnz, dev, s = self.n, self.nnz, self.device, stream()
+        n, dev = self.n, self.device
         cap = int(_lib.load().sa_cstream_max_rows())
         rb, bnr = self.cstream_geometry()
         if brows is not None:
@@ -514,7 +514,39 @@ class DeviceProblem:
         bounds = self.panel_bounds(8)
         bounds[1:-1] = (bounds[1:-1] // 64) * 64
         bounds = np.maximum.accumulate(bounds)
-        pbound = torch.from_numpy(bounds.astype(np.int32)).to(dev)
+        if signed is None:
+            signed = self.uniform_magnitude is not None
+        arr = self.cstream_arrays(rb, bnr, rbound, bounds, bool(signed))
+        if arr is None:
+            return False
+        self.build_panels(0)
+        pbound, bchunk, bent, bvals, bcbase, mag = (arr[k] for k in ("pbound", "bchunk", "bent", "bvals", "bcbase",
+                                                                    "mag"))
+        self.pbound, self.bchunk, self.bent, self.bvals, self.rbound, self.bcbase = \
+            pbound, bchunk, bent, bvals, rbound, bcbase
+        self.pld = _lib.padded_ld(n, 4)
+        self.partial = torch.empty((8, self.pld), dtype=self.dtype, device=dev)
+        self.npanels = 8
+        self.block = {"layout": "cstream", "brows": rb, "bnr": bnr, "chunks": arr["nchunks"],
+                      "balanced": rbound is not None}
+        c = self.csr
+        c.npanels, c.partial, c.pld = 8, self.partial.data_ptr(), self.pld
+        c.psigned, c.pmag = (1, mag) if mag > 0.0 else (0, 0.0)
+        c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 2, rb, 0, bnr, 0
+        c.pbound, c.bchunk, c.bent = pbound.data_ptr(), bchunk.data_ptr(), bent.data_ptr()
+        c.bvals = bvals.data_ptr() if bvals is not None else None
+        c.rbound = rbound.data_ptr() if rbound is not None else None
+        c.bcbase = bcbase.data_ptr()
+        return True
+
+    def cstream_arrays(self, rb: int, bnr: int, rbound: torch.Tensor | None, bounds: np.ndarray,
+                       signed: bool) -> dict | None:
+        """The column-streamed layout's arrays for row blocks of at most rb rows (bnr of them, or the
+        bnr + 1 boundaries rbound) and the 8 column panels bounds (9 boundaries): sa_cstream_plan (a radix
+        sort on the GPU) then sa_cstream_fill_f32. None when the layout cannot hold this matrix (a chunk
+        wider than its 16-bit column offsets, a row block taller than rb)."""
+        n, nnz, dev, s = self.n, self.nnz, self.device, stream()
+        pbound = torch.from_numpy(np.asarray(bounds).astype(np.int32)).to(dev)
         nwg = 8 * bnr
         keys = torch.empty(max(nnz, 1), dtype=torch.int64, device=dev)
         idx = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
@@ -533,17 +565,15 @@ class DeviceProblem:
             self.csr.rbound = old_rbound
         del keys, idx, tmp
         if int(flag.item()) != 0:
-            return False
+            return None
         counts = (start[1:] - start[:-1]).to(torch.int64)
         chunks = (counts + 255) // 256
         bchunk = torch.zeros(nwg + 1, dtype=torch.int64, device=dev)
         bchunk[1:] = torch.cumsum(chunks, 0)
         nchunks = int(bchunk[-1].item())
         if nchunks >= 2**31:
-            return False
+            return None
         bchunk = bchunk.to(torch.int32)
-        if signed is None:
-            signed = self.uniform_magnitude is not None
         signed = bool(signed and self.uniform_magnitude is not None)
         mag = self.uniform_magnitude if signed else 0.0
         bent = torch.empty(max(256 * nchunks, 256), dtype=torch.int32, device=dev)
@@ -553,24 +583,9 @@ class DeviceProblem:
              ptr(bchunk), nchunks, ptr(bent), ptr(bvals), ptr(bcbase), mag, ptr(flag), s)
         del keys_out, idx_out, start
         if int(flag.item()) != 0:
-            return False
-        self.build_panels(0)
-        self.pbound, self.bchunk, self.bent, self.bvals, self.rbound, self.bcbase = \
-            pbound, bchunk, bent, bvals, rbound, bcbase
-        self.pld = _lib.padded_ld(n, 4)
-        self.partial = torch.empty((8, self.pld), dtype=self.dtype, device=dev)
-        self.npanels = 8
-        self.block = {"layout": "cstream", "brows": rb, "bnr": bnr, "chunks": nchunks,
-                      "balanced": rbound is not None}
-        c = self.csr
-        c.npanels, c.partial, c.pld = 8, self.partial.data_ptr(), self.pld
-        c.psigned, c.pmag = (1, mag) if signed else (0, 0.0)
-        c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 2, rb, 0, bnr, 0
-        c.pbound, c.bchunk, c.bent = pbound.data_ptr(), bchunk.data_ptr(), bent.data_ptr()
-        c.bvals = bvals.data_ptr() if bvals is not None else None
-        c.rbound = rbound.data_ptr() if rbound is not None else None
-        c.bcbase = bcbase.data_ptr()
-        return True
+            return None
+        return {"pbound": pbound, "bchunk": bchunk, "bent": bent, "bvals": bvals, "bcbase": bcbase,
+                "nchunks": nchunks, "mag": mag}
 
     # -- two-pass layout ----------------------------------------------------------------------------
     def build_pblock(self, brows: int | None = None, balanced: bool = False, signed: bool | None = None) -> bool:

@@ -326,3 +326,66 @@ def test_batched_sweep_on_the_instance_staged_layout(gpu, tdtype):
         Jint = O.integer_couplings(J)
         res = out.results[b]
         assert res.best_energy == O.energy_int64(Jint, res.best_spins.astype(np.int64))
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_instance_streamed_spmv_matches_the_csr_product(gpu, mixed):
+    """The instance-streamed layout (FP32; each workgroup's rows of one instance streamed by column,
+    FP64 row sums): the fused product y = a x + b s (.) (J/c) g of an FP64 host reference for several
+    workgroup heights, padding rows y = a x, bitwise repeatable (the batched sweep's test runs it with
+    tile lists of the instances still sweeping)."""
+    from specann_amd._lib import call
+    from specann_amd.device import ptr, stream
+    big = graphs.erdos_renyi(40000, 8 / 39999, seed=11, kind="pm1")
+    Js = [big] + _instances()[:3]
+    if mixed:
+        J = Js[2].tolil()
+        i, j = sp.triu(J, 1).nonzero()
+        J[i[0], j[0]] = J[j[0], i[0]] = 3
+        Js[2] = J.tocsr()
+    bp = BatchProblem(Js, torch.float32, rows_per_block=256)
+    prob, n = bp.prob, bp.n
+    Jp, row_off, sizes = pack(Js)
+    c = np.zeros(n)
+    for b, J in enumerate(Js):
+        c[row_off[b]:row_off[b] + sizes[b]] = abs(J).sum(axis=1).max()
+    Jn = sp.diags(1.0 / np.where(c > 0, c, 1.0)) @ Jp
+    rng = np.random.default_rng(3)
+    s_h, g_h, x_h = rng.uniform(0.5, 1.5, n), rng.standard_normal(n), rng.standard_normal(n)
+    dev = bp.device
+    s, g, x = (torch.from_numpy(v).to(dev, torch.float32) for v in (s_h, g_h, x_h))
+    s64, g64, x64 = (v.double().cpu().numpy() for v in (s, g, x))
+    want = 0.25 * x64 + -1.5 * s64 * (Jn @ g64)
+    scale = 0.25 * np.abs(x64) + 1.5 * s64 * (abs(Jn) @ np.abs(g64))
+    assert bp.istream_eligible()
+    for rows in (20416, 4096, 1024, 64):
+        print(f"instance-streamed spmv rows={rows} sizes={[J.shape[0] for J in Js]}", flush=True)
+        assert bp.set_istream_layout(rows)
+        assert bp.layout["kind"] == "instance-streamed" and prob.csr.imode == 2
+        ys = []
+        for _ in range(2):
+            y = torch.full((n,), 9.0, dtype=torch.float32, device=dev)
+            call("sa_spmv_gather_f32", prob.csr_ref, ptr(s), ptr(g), 0.25, ptr(x), -1.5, ptr(y), stream())
+            ys.append(y.cpu().numpy())
+        np.testing.assert_array_equal(ys[0], ys[1])
+        got = ys[0].astype(np.float64)
+        assert np.all(np.abs(got - want) <= 1e-6 * (scale + 1e-30) + 1e-300), rows
+
+
+def test_batched_sweep_on_the_instance_streamed_layout(gpu):
+    """The sweep on the instance-streamed layout (instances finish at different steps, so the SpMV
+    runs on tile lists of the instances still sweeping) finds the plain-tile layout's eigenvalues at
+    every alpha and exact energies of its spins."""
+    Js = _instances()
+    kw = dict(random_seed=7, num_alphas=8, pred_mode="secant", eigsh_tol=1e-6)
+    bp = BatchProblem(Js, torch.float32, rows_per_block=256)
+    ref = BatchSweep(**kw).run(bp)
+    assert bp.set_istream_layout(4096)
+    out = BatchSweep(**kw).run(bp)
+    for b, J in enumerate(Js):
+        for r1, r2 in zip(out.results[b].rows, ref.results[b].rows):
+            assert r1["eigsh_converged"] and r2["eigsh_converged"]
+            assert r1["lambda1"] == pytest.approx(r2["lambda1"], rel=2e-5)
+        Jint = O.integer_couplings(J)
+        res = out.results[b]
+        assert res.best_energy == O.energy_int64(Jint, res.best_spins.astype(np.int64))
