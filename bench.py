@@ -231,6 +231,8 @@ def spmv_kernel_name(choice, tname):
         return f"spmv_block_kernel<{tname}> + combine (LDS-blocked fused scaled SpMV)"
     if choice.get("layout") == "cstream":
         return "spmv_cstream_kernel + combine (column-streamed fused scaled SpMV)"
+    if choice.get("layout") == "rstream":
+        return "spmv_cstream_kernel<*, true> (row-owned column-streamed fused scaled SpMV, no combine)"
     if choice.get("layout") == "pblock":
         return "pb_expand_kernel + pb_reduce_kernel (two-pass fused scaled SpMV)"
     if choice.get("npanels"):
@@ -250,7 +252,7 @@ def measured_traffic(workload, dtype, layout):
     want = {"npanels": layout["npanels"], "tile_nnz": layout["tile_nnz"]}
     if layout.get("layout") == "blocked":
         want = {"layout": "blocked", **{k: layout[k] for k in ("brows", "bcols", "bnr", "bnc")}}
-    if layout.get("layout") in ("cstream", "pblock"):
+    if layout.get("layout") in ("cstream", "rstream", "pblock"):
         want = {"layout": layout["layout"], **{k: layout[k] for k in ("brows", "bnr")}}
     if rec.get("layout") != want:
         return None
@@ -359,7 +361,8 @@ def main(argv=None) -> int:
                     help="batch workload: skip the lockstep schedule's sweep (every instance at the same alpha)")
     ap.add_argument("--layout", default=None, metavar="NPANELS,TILE",
                     help="fixed SpMV layout instead of the autotuner's choice, e.g. 8,1024, blocked (LDS-blocked "
-                         "layout; blocked:BROWS,BCOLS for a given geometry), cstream (column-streamed) or pblock (two-pass), "
+                         "layout; blocked:BROWS,BCOLS for a given geometry), cstream (column-streamed), rstream "
+                         "(column-streamed, one workgroup per row block) or pblock (two-pass), "
                          "each also as NAME-balanced (row blocks balanced by entries)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -378,9 +381,10 @@ def main(argv=None) -> int:
     J = make_graph(w)
     log(f"graph {args.workload}: n={J.shape[0]} nnz={J.nnz} generated in {time.time() - t0:.1f}s")
     prob32 = DeviceProblem(J, torch.float32)
-    if args.layout and args.layout.split("-")[0] in ("cstream", "pblock"):
+    if args.layout and args.layout.split("-")[0] in ("cstream", "rstream", "pblock"):
         name, bal = args.layout.split("-")[0], args.layout.endswith("-balanced")
-        if not (prob32.build_cstream(balanced=bal) if name == "cstream" else prob32.build_pblock(balanced=bal)):
+        build = {"cstream": prob32.build_cstream, "rstream": prob32.build_rstream, "pblock": prob32.build_pblock}[name]
+        if not build(balanced=bal):
             raise SystemExit(f"the {args.layout} layout cannot hold this matrix")
         tune32 = {"choice": {"layout": name, "npanels": 8 if name == "cstream" else 0,
                              "tile_nnz": prob32.tile_nnz, "seconds": None,

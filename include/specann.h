@@ -134,7 +134,11 @@ typedef struct sa_csr {
    * stored lane-major (sorted entry 64 j + l at bent[256 k + 4 l + j]) and its columns are
    * pbound[p] + bcbase[k] + delta. An entry is sign << 31 | local row << 16 | delta (16 bits);
    * padding entries have local row = sa_cstream_pad_row() and delta 0. bvals (float, same order)
-   * holds the couplings unless psigned. bcols and bnc are not used. */
+   * holds the couplings unless psigned. bcols and bnc are not used.
+   * Row-owned form (bmode = 4, FP32 only, npanels = 0, no partial): the same arrays built with one
+   * panel (pbound = {0, n, n, n, n, n, n, n, n}); workgroup rb sums every entry of row block rb from
+   * the chunks [bchunk[8 rb], bchunk[8 rb + 1]) (the other seven segments are empty) and writes its
+   * rows of y directly. */
   const int32_t* bcbase;    /* bchunk[8 * bnr] chunk base columns, relative to the panel */
   /* Optional two-pass layout (bmode = 3, FP32 only, npanels = 0; no counterpart in the reference),
    * for vectors far larger than an XCD's L2. Columns are cut into pbnc blocks [pbound[c],
@@ -294,8 +298,8 @@ int sa_block_fill_f64(const sa_csr* A, const int32_t* pbound, int32_t brows, int
                       int32_t* runs, const int32_t* bchunk, int64_t nent, uint32_t* bent, double* bvals, double mag,
                       int32_t* flag, hipStream_t stream);
 
-/* Column-streamed SpMV layout (bmode = 2, see sa_csr; no counterpart in the reference). Built in
- * three steps on the caller's buffers:
+/* Column-streamed SpMV layout (bmode = 2, or its row-owned form bmode = 4 built with the one-panel
+ * pbound; see sa_csr; no counterpart in the reference). Built in three steps on the caller's buffers:
  *   sa_cstream_plan: one 64-bit key per stored entry (workgroup << 40 | column in the panel << 15 |
  *     local row) and its entry index, radix-sorted (keys, idx -> keys_out, idx_out; tmp of
  *     sa_cstream_sort_bytes bytes), then start[w] = the first sorted entry of workgroup w

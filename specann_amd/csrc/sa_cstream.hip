@@ -15,6 +15,14 @@
 // reproducible for every input whose addends stay within that range (beyond it an FP64 sum may differ
 // in its last bit between runs, and its FP32 rounding almost never). The FP32 partials go to
 // partial[p * pld + r] like the other panel layouts', and the same consumers combine the panels.
+//
+// The row-owned form (bmode = 4) is the same walk with one workgroup per row block over every column:
+// the layout is built with a single panel (panel 0 = all columns), the row blocks come in whole rounds
+// of one workgroup per CU, and each workgroup's waves walk its columns left to right, so the 32 CUs of
+// an XCD sweep the gathered vector through their L2 at about the same place. Its row sums are final:
+// the epilogue writes y = a x + b s (.) (row sums) directly, with no panel partials and no combine (the
+// panel form writes and the combine reads back 8 partial vectors; the row-owned form gathers at a
+// lower column density, fewer rows per workgroup, and reads g once per XCD).
 #include <cstring>  // rocprim's texture cache iterator calls memset
 #include <rocprim/rocprim.hpp>
 
@@ -35,13 +43,16 @@ constexpr int kCsColBits = 25;
 typedef uint32_t cs_u4 __attribute__((ext_vector_type(4)));
 typedef float cs_f4 __attribute__((ext_vector_type(4)));
 
-template <bool kSigned>
+// kOwned: workgroup = row block over every column (bmode 4), final rows; else workgroup = (row block,
+// panel) on that panel's XCD (bmode 2), panel partials.
+template <bool kSigned, bool kOwned>
 __global__ __launch_bounds__(kCsThreads) void spmv_cstream_kernel(SpmvParams<float> p) {
   constexpr int kCsUnroll = kSigned ? kCsUnrollSigned : kCsUnrollValued;
   __shared__ double acc[kCsRows + 1];
   if (p.gate && *p.gate != SA_RUNNING) return;
-  const int panel = (int)(blockIdx.x & 7);
-  const int rb = (int)(blockIdx.x >> 3);
+  const int panel = kOwned ? 0 : (int)(blockIdx.x & 7);
+  const int rb = kOwned ? (int)blockIdx.x : (int)(blockIdx.x >> 3);
+  const int64_t seg = kOwned ? (int64_t)8 * blockIdx.x : (int64_t)blockIdx.x;  // its chunks: segment seg
   const int64_t r0 = p.rbound ? (int64_t)p.rbound[rb] : (int64_t)rb * p.brows;
   if (r0 >= p.n) return;
   const int64_t r1 = p.rbound ? (int64_t)p.rbound[rb + 1] : r0 + p.brows;
@@ -51,7 +62,7 @@ __global__ __launch_bounds__(kCsThreads) void spmv_cstream_kernel(SpmvParams<flo
   for (int i = tid; i < rows; i += kCsThreads) acc[i] = 0.0;  // the padding slot is never read
   __syncthreads();
 
-  const int64_t k0 = p.bchunk[blockIdx.x], k1 = p.bchunk[blockIdx.x + 1];
+  const int64_t k0 = p.bchunk[seg], k1 = p.bchunk[seg + 1];
   const float* __restrict__ gp = p.g + p.pbound[panel];
   const cs_u4* __restrict__ ent = reinterpret_cast<const cs_u4*>(p.bent);
   const cs_f4* __restrict__ val = reinterpret_cast<const cs_f4*>(p.bvals);
@@ -100,8 +111,20 @@ __global__ __launch_bounds__(kCsThreads) void spmv_cstream_kernel(SpmvParams<flo
   }
   __syncthreads();
   const double scale = kSigned ? (double)p.pmag : 1.0;
-  float* out = p.partial + (int64_t)panel * p.pld + r0;
-  for (int i = tid; i < rows; i += kCsThreads) out[i] = (float)(scale * acc[i]);
+  if constexpr (kOwned) {
+    double bb = p.b;
+    if (p.lz) bb *= p.lz->sigma[p.lz->j];
+    for (int i = tid; i < rows; i += kCsThreads) {
+      const int64_t r = r0 + i;
+      const double sr = p.s ? (double)p.s[r] : 1.0;
+      double v = bb * sr * (scale * acc[i]);
+      if (p.x) v += p.a * (double)p.x[r];
+      p.y[r] = (float)v;
+    }
+  } else {
+    float* out = p.partial + (int64_t)panel * p.pld + r0;
+    for (int i = tid; i < rows; i += kCsThreads) out[i] = (float)(scale * acc[i]);
+  }
 }
 
 // Instance-streamed SpMV of a block-diagonal batch (imode = 2): the same walk per workgroup of one
@@ -215,9 +238,29 @@ int launch_cstream_spmv(const SpmvParams<float>& p, hipStream_t stream) {
     return SA_E_ARG;
   }
   const dim3 grid((unsigned)(8 * p.bnr)), block(kCsThreads);
-  if (p.psigned) hipLaunchKernelGGL(spmv_cstream_kernel<true>, grid, block, 0, stream, p);
-  else hipLaunchKernelGGL(spmv_cstream_kernel<false>, grid, block, 0, stream, p);
+  if (p.psigned) hipLaunchKernelGGL((spmv_cstream_kernel<true, false>), grid, block, 0, stream, p);
+  else hipLaunchKernelGGL((spmv_cstream_kernel<false, false>), grid, block, 0, stream, p);
   return check_launch("spmv_cstream");
+}
+
+int launch_rstream_spmv(const SpmvParams<float>& p, hipStream_t stream) {
+  if (p.npanels != 0 || !p.pbound || !p.bchunk || !p.bent || !p.bcbase || (!p.bvals && !p.psigned) || !p.y) {
+    set_error("spmv: incomplete row-owned column-streamed layout");
+    return SA_E_ARG;
+  }
+  if (p.brows < 64 || p.brows > kCsRows || p.bnr < 1 || (!p.rbound && (int64_t)p.bnr * p.brows < p.n) ||
+      (int64_t)8 * p.bnr >= (1ll << 31)) {
+    set_error("spmv: row-owned column-streamed layout sizes out of range");
+    return SA_E_ARG;
+  }
+  if (!aligned16(p.bent) || (p.bvals && !aligned16(p.bvals))) {
+    set_error("spmv: the column-streamed layout needs 16-byte aligned entries and values");
+    return SA_E_ARG;
+  }
+  const dim3 grid((unsigned)p.bnr), block(kCsThreads);
+  if (p.psigned) hipLaunchKernelGGL((spmv_cstream_kernel<true, true>), grid, block, 0, stream, p);
+  else hipLaunchKernelGGL((spmv_cstream_kernel<false, true>), grid, block, 0, stream, p);
+  return check_launch("spmv_rstream");
 }
 
 // ---- layout construction ---------------------------------------------------------------------

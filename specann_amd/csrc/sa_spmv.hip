@@ -431,8 +431,9 @@ int launch_spmv(const SpmvParams<T>& p, int32_t tile_nnz, hipStream_t stream) {
     return SA_E_ARG;
   }
   if (p.bmode == 1) return launch_block_spmv<T>(p, stream);
-  if (p.bmode == 2 || p.bmode == 3) {
+  if (p.bmode == 2 || p.bmode == 3 || p.bmode == 4) {
     if constexpr (std::is_same<T, float>::value) {
+      if (p.bmode == 4) return launch_rstream_spmv(p, stream);
       return p.bmode == 2 ? launch_cstream_spmv(p, stream) : launch_pblock_spmv(p, stream);
     } else {
       set_error("spmv: the column-streamed and two-pass layouts are FP32 only");

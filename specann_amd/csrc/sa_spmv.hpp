@@ -91,6 +91,8 @@ int launch_inst_spmv(const SpmvParams<T>& p, hipStream_t stream);
 
 int launch_cstream_spmv(const SpmvParams<float>& p, hipStream_t stream);
 
+int launch_rstream_spmv(const SpmvParams<float>& p, hipStream_t stream);
+
 int launch_pblock_spmv(const SpmvParams<float>& p, hipStream_t stream);
 
 int launch_istream_spmv(const SpmvParams<float>& p, hipStream_t stream);

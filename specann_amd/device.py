@@ -346,17 +346,18 @@ class DeviceProblem:
             out += [a + i * step for i in range(1, k)] + [b]
         return np.asarray(out, dtype=np.int64)
 
-    def whole_round_row_bounds(self, nblocks: int, cap: int) -> np.ndarray:
+    def whole_round_row_bounds(self, nblocks: int, cap: int, round_of: int = CUS_PER_XCD) -> np.ndarray:
         """balanced_row_bounds for about nblocks blocks of at most cap rows, in a block count that is a
-        whole number of rounds of one workgroup per CU of an XCD if a nearby target gives one."""
+        whole number of rounds of round_of workgroups (default one per CU of an XCD) if a nearby target
+        gives one."""
         n = self.n
         ip = self.indptr.cpu().numpy().astype(np.int64)
         bounds_r = self.balanced_row_bounds(nblocks, cap, ip)
-        for extra in range(1, 2 * CUS_PER_XCD):
-            if (len(bounds_r) - 1) % CUS_PER_XCD == 0:
+        for extra in range(1, 2 * round_of):
+            if (len(bounds_r) - 1) % round_of == 0:
                 break
             cand = self.balanced_row_bounds(nblocks + extra, cap, ip)
-            if (len(cand) - 1) % CUS_PER_XCD == 0:
+            if (len(cand) - 1) % round_of == 0:
                 bounds_r = cand
                 break
         if bounds_r[-1] != n or np.any(np.diff(bounds_r) <= 0) or np.any(np.diff(bounds_r) > cap) or \
@@ -539,6 +540,61 @@ class DeviceProblem:
         c.bcbase = bcbase.data_ptr()
         return True
 
+    def rstream_geometry(self) -> tuple[int, int]:
+        """(brows, bnr) of the row-owned column-streamed layout: whole rounds of one workgroup per CU
+        (8 XCDs x 32), each row block as tall as the rounds allow (denser columns per gather) and at
+        most what the workgroup's LDS holds in FP64 row sums."""
+        cap = int(_lib.load().sa_cstream_max_rows())
+        cus = 8 * CUS_PER_XCD
+        rounds = max(1, -(-self.n // (cus * cap)))
+        rb = -(-(-(-self.n // (cus * rounds))) // 64) * 64
+        return rb, -(-self.n // rb)
+
+    def build_rstream(self, brows: int | None = None, balanced: bool = False, signed: bool | None = None) -> bool:
+        """Regroup the couplings into the row-owned column-streamed layout (FP32 only, bmode 4): the
+        column-streamed arrays built with one panel, so that each workgroup sums every entry of its
+        row block (sorted by column, 256-entry chunks) and writes its rows of y with no panel partials.
+        False (layout unchanged) for FP64 problems or when a chunk would span more columns than its
+        16-bit offsets reach. brows: rows per row block (at most sa_cstream_max_rows); balanced: row
+        blocks of variable height holding about equal entry counts, in whole rounds of one workgroup
+        per CU; signed: as for build_cstream."""
+        if self.dtype != torch.float32:
+            return False
+        n, dev = self.n, self.device
+        cap = int(_lib.load().sa_cstream_max_rows())
+        rb, bnr = self.rstream_geometry()
+        if brows is not None:
+            rb = int(brows)
+            if not (64 <= rb <= cap):
+                raise ValueError(f"column-streamed row blocks must hold 64..{cap} rows")
+            bnr = -(-n // rb)
+        rbound = None
+        if balanced:
+            bounds_r = self.whole_round_row_bounds(bnr, rb, 8 * CUS_PER_XCD)
+            bnr = len(bounds_r) - 1
+            rbound = torch.from_numpy(bounds_r.astype(np.int32)).to(dev)
+        bounds = np.array([0] + [n] * 8, dtype=np.int64)  # one panel: every column
+        if signed is None:
+            signed = self.uniform_magnitude is not None
+        arr = self.cstream_arrays(rb, bnr, rbound, bounds, bool(signed))
+        if arr is None:
+            return False
+        self.build_panels(0)
+        pbound, bchunk, bent, bvals, bcbase, mag = (arr[k] for k in ("pbound", "bchunk", "bent", "bvals", "bcbase",
+                                                                    "mag"))
+        self.pbound, self.bchunk, self.bent, self.bvals, self.rbound, self.bcbase = \
+            pbound, bchunk, bent, bvals, rbound, bcbase
+        self.block = {"layout": "rstream", "brows": rb, "bnr": bnr, "chunks": arr["nchunks"],
+                      "balanced": rbound is not None}
+        c = self.csr
+        c.psigned, c.pmag = (1, mag) if mag > 0.0 else (0, 0.0)
+        c.bmode, c.brows, c.bcols, c.bnr, c.bnc = 4, rb, 0, bnr, 0
+        c.pbound, c.bchunk, c.bent = pbound.data_ptr(), bchunk.data_ptr(), bent.data_ptr()
+        c.bvals = bvals.data_ptr() if bvals is not None else None
+        c.rbound = rbound.data_ptr() if rbound is not None else None
+        c.bcbase = bcbase.data_ptr()
+        return True
+
     def cstream_arrays(self, rb: int, bnr: int, rbound: torch.Tensor | None, bounds: np.ndarray,
                        signed: bool) -> dict | None:
         """The column-streamed layout's arrays for row blocks of at most rb rows (bnr of them, or the
@@ -702,9 +758,9 @@ class DeviceProblem:
         return e0.elapsed_time(e1) / 1e3 / reps
 
     def autotune(self, reps: int = 4, verbose: bool = False, blocks: bool = True) -> dict:
-        """Pick the SpMV layout (plain tiles, XCD panels, LDS blocks, column-streamed chunks or the
-        two-pass layout) and tile size by timing them on this matrix; small problems keep the plain
-        layout."""
+        """Pick the SpMV layout (plain tiles, XCD panels, LDS blocks, column-streamed chunks per panel
+        or per row block, or the two-pass layout) and tile size by timing them on this matrix; small
+        problems keep the plain layout."""
         configs = [(0, t) for t in (1024, 2048, 4096)]
         P = self.auto_panels() if self.n >= (1 << 17) else 0
         if P:
@@ -726,6 +782,10 @@ class DeviceProblem:
                 configs.append((-3, self.tile_nnz))
                 if unbalanced:
                     configs.append((-4, self.tile_nnz))
+                # the row-owned form: no panel partials, one workgroup per CU per round
+                configs.append((-7, self.tile_nnz))
+                if self.row_block_imbalance(self.rstream_geometry()[0]) > 1.25:
+                    configs.append((-8, self.tile_nnz))
                 if self.n >= (1 << 21):
                     # the two-pass layout, for vectors far larger than an XCD's L2
                     configs.append((-5, self.tile_nnz))
@@ -743,11 +803,14 @@ class DeviceProblem:
             elif panels in (-5, -6):
                 if not self.build_pblock(balanced=panels == -6):
                     continue
+            elif panels in (-7, -8):
+                if not self.build_rstream(balanced=panels == -8):
+                    continue
             else:
                 self.build_panels(panels) if panels else self.build_panels(0)
             sec = self.time_spmv(reps)
-            layout = {-1: "blocked", -2: "blocked", -3: "cstream", -4: "cstream", -5: "pblock", -6: "pblock"}.get(
-                panels, "panels" if panels else "csr")
+            layout = {-1: "blocked", -2: "blocked", -3: "cstream", -4: "cstream", -5: "pblock", -6: "pblock",
+                      -7: "rstream", -8: "rstream"}.get(panels, "panels" if panels else "csr")
             results.append({"layout": layout, "npanels": 8 if panels in (-1, -2, -3, -4) else max(panels, 0),
                             "tile_nnz": tile, "seconds": sec})
             if panels < 0:
@@ -770,6 +833,10 @@ class DeviceProblem:
             if self.csr.bmode != 3 or self.block.get("balanced") != best.get("balanced"):
                 if not self.build_pblock(balanced=bool(best.get("balanced"))):
                     raise RuntimeError("two-pass layout could not be rebuilt")
+        elif best["layout"] == "rstream":
+            if self.csr.bmode != 4 or self.block.get("balanced") != best.get("balanced"):
+                if not self.build_rstream(balanced=bool(best.get("balanced"))):
+                    raise RuntimeError("row-owned column-streamed layout could not be rebuilt")
         elif best["npanels"] != self.npanels or best["npanels"] or self.csr.bmode:
             self.build_panels(best["npanels"])
         self._free_unused_panels()

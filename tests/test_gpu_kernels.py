@@ -249,6 +249,61 @@ def test_cstream_spmv_matches_oracle(gpu, case, brows, balanced):
     assert np.all(err <= 16 * F32 * mag + 1e-300), f"max excess {np.max(err / (mag + 1e-300))}"
 
 
+@pytest.mark.parametrize("case", sorted(CASES) + ["er_100k", "er_100003", "ba_200k", "er_100k_valued"])
+@pytest.mark.parametrize("brows", [None, 1024])
+@pytest.mark.parametrize("balanced", [False, True])
+def test_rstream_spmv_matches_oracle(gpu, case, brows, balanced):
+    """The row-owned column-streamed layout (FP32): one workgroup per row block over every column,
+    final rows written with no panel partials; the same operator as the oracle within the FP
+    summation bound (one rounding of an exact FP64 row sum, then the epilogue), every entry placed
+    once, bitwise repeatable, for sign-packed and valued entries, hubs and ragged sizes."""
+    if case.startswith("er_100k"):
+        J = graphs.erdos_renyi(100000, 40 / 99999, seed=12, kind="pm1")
+    elif case == "er_100003":
+        J = graphs.erdos_renyi(100003, 30 / 100002, seed=13, kind="pm1")
+    elif case == "ba_200k":
+        J = graphs.barabasi_albert(200000, 20, seed=5, kind="neg")
+    else:
+        J = CASES[case]()
+    if case.endswith("valued"):
+        J = _valued(J)
+    prob = DeviceProblem(J, torch.float32)
+    print(f"rstream spmv case={case} brows={brows} balanced={balanced} n={prob.n} nnz={prob.nnz}", flush=True)
+    assert prob.build_rstream(brows=brows, balanced=balanced)
+    c = prob.csr
+    assert c.bmode == 4 and c.npanels == 0 and c.psigned == int(prob.uniform_magnitude is not None)
+    assert c.brows * c.bnr >= prob.n or balanced
+    bent = prob.bent.cpu().numpy().view(np.uint32)
+    pad = (bent >> 16) & 0x7FFF == _lib.load().sa_cstream_pad_row()
+    assert int((~pad).sum()) == prob.nnz
+    bch = prob.bchunk.cpu().numpy().astype(np.int64)
+    assert bch.size == 8 * c.bnr + 1 and bch[0] == 0 and np.all(np.diff(bch) >= 0) and 256 * bch[-1] == bent.size
+    seg = bch[:-1].reshape(c.bnr, 8)
+    assert np.all(seg[:, 1:] == np.r_[seg[1:, 0], bch[-1]][:, None])  # one panel: segments 1..7 empty
+    if balanced:
+        rbound = prob.rbound.cpu().numpy()
+        assert rbound[0] == 0 and rbound[-1] == prob.n and np.all(np.diff(rbound) > 0)
+        assert np.all(np.diff(rbound) <= c.brows) and len(rbound) == c.bnr + 1
+    Jn, d, _ = O.normalize(J, np.float32)
+    s = torch.empty(prob.n, dtype=torch.float32, device=gpu)
+    prob.alpha_scaling(0.41, 1e-12, s)
+    x = O.normal_vector(prob.n, 7, 0).astype(np.float32)
+    xt = torch.from_numpy(x).to(gpu)
+    y = torch.empty_like(xt)
+    z = torch.empty_like(xt)
+    rho = 0.9
+    prob.matvec(s, xt, rho, -1.0, y, z)
+    y1 = y.cpu().numpy()
+    for _ in range(3):
+        prob.matvec(s, xt, rho, -1.0, y, z)
+        np.testing.assert_array_equal(y.cpu().numpy(), y1)
+    s_dev = s.cpu().numpy()
+    ref = O.s_matvec(Jn, s_dev, x, rho)
+    mag = rho * np.abs(x) + s_dev.astype(np.float64) * (abs(Jn).astype(np.float64) @ np.abs(s_dev * x))
+    err = np.abs(y1.astype(np.float64) - ref)
+    assert np.all(err <= 8 * F32 * mag + 1e-300), f"max excess {np.max(err / (mag + 1e-300))}"
+
+
 @pytest.mark.parametrize("case", ["er_2k_pm1", "star_9k", "er_100k", "er_100003", "ba_200k", "ba_200k_valued"])
 @pytest.mark.parametrize("brows", [None, 1024])
 @pytest.mark.parametrize("balanced", [False, True])
@@ -338,7 +393,10 @@ def test_cstream_refuses_wide_chunks_and_fp64(gpu):
     prob.build_panels(8)
     assert not prob.build_cstream(brows=1024)
     assert prob.csr.bmode == 0 and prob.csr.npanels == 8
+    assert not prob.build_rstream(brows=64)
+    assert prob.csr.bmode == 0 and prob.csr.npanels == 8
     assert not DeviceProblem(CASES["er_2k_pm1"](), torch.float64).build_cstream()
+    assert not DeviceProblem(CASES["er_2k_pm1"](), torch.float64).build_rstream()
     assert not DeviceProblem(CASES["er_2k_pm1"](), torch.float64).build_pblock()
 
 
@@ -417,15 +475,24 @@ def test_local_search_bit_exact(gpu, case, k):
 
 @pytest.mark.parametrize("case", ["er_2k_pm1", "ba_20k_neg"])
 @pytest.mark.parametrize("tdtype", [torch.float32, torch.float64])
-@pytest.mark.parametrize("panels", [0, 8, -1])
+@pytest.mark.parametrize("panels", [0, 8, -1, -3, -5, -7])
 def test_lanczos_leading_eigenpair(gpu, case, tdtype, panels):
-    """panels -1: the LDS-blocked layout (where it holds the matrix)."""
+    """panels -1: the LDS-blocked layout (where it holds the matrix); -3, -5, -7: the column-streamed,
+    two-pass and row-owned column-streamed layouts (FP32 only)."""
     from specann_amd.eigsh import LanczosSolver
     from specann_amd._lib import SA_CONVERGED
+    if panels < -1 and tdtype != torch.float32:
+        pytest.skip("the streamed layouts are FP32 only")
     J = CASES[case]()
     prob = DeviceProblem(J, tdtype)
-    if panels < 0:
+    if panels == -1:
         assert prob.build_blocks(brows=1024, bcols=1024)
+    elif panels == -3:
+        assert prob.build_cstream(brows=1024)
+    elif panels == -5:
+        assert prob.build_pblock(brows=1024)
+    elif panels == -7:
+        assert prob.build_rstream()
     else:
         prob.build_panels(panels)
     dt = _dt(tdtype)

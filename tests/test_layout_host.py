@@ -1,5 +1,5 @@
-"""Host-side planning of the LDS-blocked SpMV layout (no GPU): row blocks balanced by entries and
-column panels capped at the kernel's walk."""
+"""Host-side planning of the LDS-blocked and column-streamed SpMV layouts (no GPU): row blocks balanced
+by entries, column panels capped at the kernel's walk, whole rounds of workgroups."""
 from types import SimpleNamespace
 
 import numpy as np
@@ -82,3 +82,31 @@ def test_capped_panels_refuse_an_impossible_cover():
     J = graphs.erdos_renyi(5000, 10 / 4999, seed=3, kind="pm1")
     h = _host(J)
     assert DeviceProblem.capped_panel_bounds(h, 64 * 8) is None  # eight panels of 512 columns < 5000
+
+
+def test_row_owned_geometry_fills_whole_rounds():
+    """The row-owned column-streamed layout: one round of 256 workgroups (one per CU) while the rows
+    fit, the tallest row blocks the rounds allow, never more rows than the LDS row sums hold."""
+    from specann_amd import _lib
+    cap = int(_lib.load().sa_cstream_max_rows())
+    assert DeviceProblem.rstream_geometry(SimpleNamespace(n=1 << 22)) == (16384, 256)
+    assert DeviceProblem.rstream_geometry(SimpleNamespace(n=1 << 20)) == (4096, 256)
+    for n in (1000, 65536, 100003, 3 << 20, 1 << 23, 1 << 25):
+        rb, bnr = DeviceProblem.rstream_geometry(SimpleNamespace(n=n))
+        rounds = -(-n // (256 * cap))
+        assert rb % 64 == 0 and 64 <= rb <= cap and rb * bnr >= n > rb * (bnr - 1)
+        assert bnr <= 256 * rounds and (rounds == 1 or bnr > 256 * (rounds - 1))
+
+
+def test_row_owned_balanced_blocks_come_in_whole_rounds():
+    """Entry-balanced row blocks for the row-owned layout: a multiple of 256 blocks where a nearby
+    count gives one, every row covered once, none taller than the cap, the hubs spread."""
+    J = graphs.barabasi_albert(400000, 20, seed=5, kind="neg")
+    h = _host(J)
+    h.balanced_row_bounds = lambda nb, cap, ip=None: DeviceProblem.balanced_row_bounds(h, nb, cap, ip)
+    b = DeviceProblem.whole_round_row_bounds(h, 256, 1600, 256)
+    _check_row_bounds(b, h.n, 1600)
+    assert (len(b) - 1) % 256 == 0
+    per = np.diff(J.indptr.astype(np.int64)[b])
+    uniform = DeviceProblem.row_block_imbalance(h, -(-h.n // (len(b) - 1)))
+    assert per.max() / per.mean() < 0.5 * uniform  # 64-row granularity bounds the hub blocks

@@ -1,7 +1,7 @@
 """The allocation-free SpMV entry point (sa_spmv_scaled_static_f32) against the allocating one
 (sa_spmv_scaled_f32) through the C ABI: bit-identical outputs on two calls in a row, the second on a
-smaller problem, for the plain, XCD-panel, LDS-blocked, column-streamed and two-pass layouts; sizes beyond its static scratch
-are refused with a message."""
+smaller problem, for the plain, XCD-panel, LDS-blocked, column-streamed (per panel and per row block)
+and two-pass layouts; sizes beyond its static scratch are refused with a message."""
 import ctypes
 
 import numpy as np
@@ -26,6 +26,8 @@ def _problem(gpu, n, deg, seed, layout):
         assert prob.build_cstream()
     elif layout == "pblock":
         assert prob.build_pblock()
+    elif layout == "rstream":
+        assert prob.build_rstream()
     s = torch.empty(prob.n, dtype=torch.float32, device=gpu)
     prob.alpha_scaling(0.4, 1e-12, s)
     x = torch.from_numpy(np.random.default_rng(seed).standard_normal(prob.n).astype(np.float32)).to(gpu)
@@ -48,7 +50,7 @@ def _static(prob, s, x, a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("layout", ["csr", "panels", "blocked", "cstream", "pblock"])
+@pytest.mark.parametrize("layout", ["csr", "panels", "blocked", "cstream", "rstream", "pblock"])
 def test_static_scratch_spmv_equals_allocating_entry_point(gpu, layout):
     big_n = 1 << 17
     print(f"no-alloc SpMV, layout {layout}: n = {big_n} then 5000", flush=True)
