@@ -7,8 +7,9 @@
 //
 // One step = three launches:
 //   1. the fused scaled SpMV  W = sigma_j s (.) (J Z),  Z = s (.) V[j] written by the previous step;
-//   2. projections h_m = v_m . W for every basis vector of the cycle (FP64 partial sums, 8 vectors
-//      per chunk, last-workgroup reduction);
+//   2. projections h_m = v_m . W for every basis vector of the cycle (FP64 partial sums, up to 12
+//      shadow rows and the two newest full-precision rows per pass over W, then a one-workgroup
+//      finisher that sums the partials in workgroup order);
 //   3. classical Gram-Schmidt W - sum h_m v_m into V[j+1] (raw; sigma_{j+1} = 1/beta_j), Z for the
 //      next step, beta_j = ||.||, then, in the last workgroup, the FP64 Ritz solve of the projected
 //      matrix: largest eigenvalue by 257-section Sturm bisection over the whole workgroup, the last
@@ -149,26 +150,29 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_combine_kernel(void* work, int64_
   }
 }
 
-// h_m = sigma_m v_m . W for every basis vector of the cycle. 16-byte loads (from the shadow too, which
-// then covers twice the entries per lane, so a chunk keeps as many bytes in flight as a full-precision
-// one), chunks of 8 rows per pass over the workgroup's rows: first the rows below mh from the shadow, then
-// the rest in full precision (each chunk's loads are unconditional, rows past the chunk clamped to a
-// valid one and dropped, so all eight stay in flight). Per-workgroup partials are stored
-// column-major (part[m * G + b]); column j + 1 takes ||W||^2. The finisher sums them in workgroup
-// order. The pass over row j also writes its shadow row.
+// h_m = sigma_m v_m . W for every basis vector of the cycle. Each thread takes 2 L consecutive entries
+// per step (one 16-byte shadow load, two 16-byte loads of W and of a full-precision row). The shadow
+// rows below mh go in passes of kDotsShRows rows over the workgroup's rows, the first pass also
+// carrying W's norm and the full-precision rows j - 1 and j, so W is read once per pass and a pass has
+// all its rows' loads in flight; without the shadow the rows go SA_RED_COLS per pass in full precision.
+// Per-workgroup partials are stored column-major (part[m * G + b]); column j + 1 takes ||W||^2. The
+// finisher sums them in workgroup order. The pass over row j also writes its shadow row.
 // With npanels > 0 the workgroup first forms W on its own rows from the SpMV's panel partials (the
 // panel combine: sigma_j s_r times their FP64 sum in panel order, as lz_combine_kernel computes it)
 // and writes it for the update kernel; partial rows and W are then 16-byte aligned.
+// Shadow rows per pass of the projections: twelve 16-byte shadow loads plus W and two full rows per
+// lane keep the kernel at two waves per SIMD without spills (sixteen spill, 32 do not fit).
+constexpr int kDotsShRows = 12;
+static_assert(kDotsShRows >= SA_RED_COLS, "a pass without the shadow takes SA_RED_COLS rows");
+
 template <typename T, class M>
-__global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(M map, const T* __restrict__ V, int64_t ldv,
+__global__ __launch_bounds__(SA_BLOCK, 2) void lz_dots_kernel(M map, const T* __restrict__ V, int64_t ldv,
                                                            SH<T>* __restrict__ Vh, double shscale, T* __restrict__ W,
                                                            const T* __restrict__ partial, int64_t pld, int npanels,
                                                            const T* __restrict__ s) {
   using VT = typename vec16<T>::type;
-  using SV = typename Shadow<T>::vec;
   constexpr int L = vec16<T>::L;
-  __shared__ double red[2][SA_WAVES_PER_BLOCK][SA_RED_COLS];
-  __shared__ double redw[SA_WAVES_PER_BLOCK];
+  __shared__ double red[2][SA_WAVES_PER_BLOCK][kDotsShRows + 3];
   const Slice sl = map.slice();
   const sa_state* st = state_of(sl.work);
   if (st->status != SA_RUNNING) return;
@@ -217,114 +221,155 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_dots_kernel(M map, const T* __res
     }
     __syncthreads();  // the chunks below read rows other threads of this workgroup wrote
   }
-  double ww = 0.0;  // ||W||^2, accumulated in the first pass
-  int ch = 0;
-  for (int m0 = 0; m0 <= j; m0 += SA_RED_COLS, ++ch) {
-    const bool shadow = m0 < mh;
-    const int mend = shadow ? mh : j + 1;  // chunks do not cross mh
-    const int mc = min(SA_RED_COLS, mend - m0);
-    double acc[SA_RED_COLS];
-    T get[SA_RED_COLS];  // shadow row -> raw scale of v_m
+  // One pass over the workgroup's rows per group of basis rows, W loaded once per pass. With the
+  // shadow on (mh > 0) a group is up to kDotsShRows shadow rows below mh, and the first pass also
+  // carries the full-precision rows j - 1 and j (mh = j - 1); without it a group is SA_RED_COLS
+  // full-precision rows. Every load of a group is unconditional (rows past the group clamped to its
+  // last row, their sums dropped), so the whole group's loads are in flight together.
+  constexpr int E = 2 * L;
+  using S16 = typename Shadow<T>::vec16;
+  const int64_t ne = hi > lo ? (hi - lo) / E : 0;
+  const int64_t tail = lo + ne * E;
+  const bool sh = mh > 0;
+  const int gend = sh ? mh : j + 1;  // the grouped rows are [0, gend); with the shadow, rows mh..j ride along
+  double ww = 0.0;                   // ||W||^2, accumulated in the first pass
+  int pass = 0;
+  for (int p0 = 0; p0 < gend; p0 += sh ? kDotsShRows : SA_RED_COLS, ++pass) {
+    const int cnt = min(sh ? kDotsShRows : SA_RED_COLS, gend - p0);
+    const bool extra = sh && pass == 0;
+    double acc[kDotsShRows];
 #pragma unroll
-    for (int c = 0; c < SA_RED_COLS; ++c) {
-      acc[c] = 0.0;
-      get[c] = (shadow && c < mc) ? (T)(1.0 / (st->sigma[m0 + c] * shscale)) : (T)0;
-    }
-    const bool first = ch == 0;
-    int64_t tail = lo + nv * L;  // first row of the scalar tail
-    if (shadow) {
-      // 16-byte shadow loads: E = 2 L entries per lane per row, so that eight rows keep as many bytes
-      // in flight as the full-precision chunks
-      using S16 = typename Shadow<T>::vec16;
-      constexpr int E = 2 * L;
-      const int64_t ne = hi > lo ? (hi - lo) / E : 0;
-      tail = lo + ne * E;
-      for (int64_t g = threadIdx.x; g < ne; g += SA_BLOCK) {
-        const int64_t r = lo + g * E;
-        const VT w0 = *reinterpret_cast<const VT*>(&W[r]);
-        const VT w1 = *reinterpret_cast<const VT*>(&W[r + L]);
-        if (first) {
+    for (int q = 0; q < kDotsShRows; ++q) acc[q] = 0.0;
+    double f0 = 0.0, f1 = 0.0;  // rows j - 1 and j (extra)
+    for (int64_t u = threadIdx.x; u < ne; u += SA_BLOCK) {
+      const int64_t r = lo + u * E;
+      const VT w0 = *reinterpret_cast<const VT*>(&W[r]);
+      const VT w1 = *reinterpret_cast<const VT*>(&W[r + L]);
+      double wd[E];
 #pragma unroll
-          for (int l = 0; l < L; ++l) ww += (double)w0[l] * (double)w0[l] + (double)w1[l] * (double)w1[l];
+      for (int l = 0; l < L; ++l) {
+        wd[l] = (double)w0[l];
+        wd[L + l] = (double)w1[l];
+      }
+      if (pass == 0) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) ww += wd[e] * wd[e];
+      }
+      if (sh) {
+        S16 h[kDotsShRows];
+#pragma unroll
+        for (int q = 0; q < kDotsShRows; ++q)
+          h[q] = *reinterpret_cast<const S16*>(&Vh[(int64_t)(p0 + min(q, cnt - 1)) * ldv + r]);
+        VT a0, a1, b0, b1;
+        if (extra) {
+          a0 = *reinterpret_cast<const VT*>(&V[(int64_t)(j - 1) * ldv + r]);
+          a1 = *reinterpret_cast<const VT*>(&V[(int64_t)(j - 1) * ldv + r + L]);
+          b0 = *reinterpret_cast<const VT*>(&V[(int64_t)j * ldv + r]);
+          b1 = *reinterpret_cast<const VT*>(&V[(int64_t)j * ldv + r + L]);
         }
-        S16 h[SA_RED_COLS];
 #pragma unroll
-        for (int c = 0; c < SA_RED_COLS; ++c)
-          h[c] = *reinterpret_cast<const S16*>(&Vh[(int64_t)(m0 + min(c, mc - 1)) * ldv + r]);
+        for (int q = 0; q < kDotsShRows; ++q) {
+          if (q < cnt) {
 #pragma unroll
-        for (int c = 0; c < SA_RED_COLS; ++c) {
-          if (c < mc) {
+            for (int e = 0; e < E; ++e) acc[q] += (double)h[q][e] * wd[e];
+          }
+        }
+        if (extra) {
+          S16 hj;
 #pragma unroll
-            for (int l = 0; l < L; ++l) {
-              acc[c] += (double)((T)h[c][l] * get[c]) * (double)w0[l];
-              acc[c] += (double)((T)h[c][L + l] * get[c]) * (double)w1[l];
+          for (int l = 0; l < L; ++l) {
+            f0 += (double)a0[l] * wd[l] + (double)a1[l] * wd[L + l];
+            f1 += (double)b0[l] * wd[l] + (double)b1[l] * wd[L + l];
+            hj[l] = (SH<T>)(b0[l] * put);
+            hj[L + l] = (SH<T>)(b1[l] * put);
+          }
+          *reinterpret_cast<S16*>(&Vh[(int64_t)j * ldv + r]) = hj;  // the shadow row of v_j
+        }
+      } else {
+        VT a[SA_RED_COLS][2];
+#pragma unroll
+        for (int q = 0; q < SA_RED_COLS; ++q) {
+          const int64_t at = (int64_t)(p0 + min(q, cnt - 1)) * ldv + r;
+          a[q][0] = *reinterpret_cast<const VT*>(&V[at]);
+          a[q][1] = *reinterpret_cast<const VT*>(&V[at + L]);
+        }
+#pragma unroll
+        for (int q = 0; q < SA_RED_COLS; ++q) {
+          if (q < cnt) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) acc[q] += (double)a[q][0][l] * wd[l] + (double)a[q][1][l] * wd[L + l];
+            if (Vh && p0 + q == j) {
+              S16 hj;
+#pragma unroll
+              for (int l = 0; l < L; ++l) {
+                hj[l] = (SH<T>)(a[q][0][l] * put);
+                hj[L + l] = (SH<T>)(a[q][1][l] * put);
+              }
+              *reinterpret_cast<S16*>(&Vh[(int64_t)j * ldv + r]) = hj;
             }
           }
         }
       }
     }
-    for (int64_t g = threadIdx.x; !shadow && g < nv; g += SA_BLOCK) {
-      const int64_t r = lo + g * L;
-      const VT w = *reinterpret_cast<const VT*>(&W[r]);
-      if (first) {
-#pragma unroll
-        for (int l = 0; l < L; ++l) ww += (double)w[l] * (double)w[l];
-      }
-      VT v[SA_RED_COLS];
-      {
-#pragma unroll
-        for (int c = 0; c < SA_RED_COLS; ++c)
-          v[c] = *reinterpret_cast<const VT*>(&V[(int64_t)(m0 + min(c, mc - 1)) * ldv + r]);
-        if (Vh && m0 + mc - 1 == j) {
-#pragma unroll
-          for (int c = 0; c < SA_RED_COLS; ++c) {
-            if (c == mc - 1) {
-              SV hj;
-#pragma unroll
-              for (int l = 0; l < L; ++l) hj[l] = (SH<T>)(v[c][l] * put);
-              *reinterpret_cast<SV*>(&Vh[(int64_t)j * ldv + r]) = hj;
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < SA_RED_COLS; ++c) {
-        if (c < mc) {
-#pragma unroll
-          for (int l = 0; l < L; ++l) acc[c] += (double)v[c][l] * (double)w[l];
-        }
-      }
-    }
-    for (int64_t r = tail + threadIdx.x; r < hi; r += SA_BLOCK) {
+    for (int64_t r = tail + threadIdx.x; r < hi; r += SA_BLOCK) {  // rows past the last whole unit
       const double w = (double)W[r];
-      if (first) ww += w * w;
+      if (pass == 0) ww += w * w;
 #pragma unroll
-      for (int c = 0; c < SA_RED_COLS; ++c) {
-        if (c >= mc) continue;
-        const int64_t at = (int64_t)(m0 + c) * ldv + r;
-        if (shadow) {
-          acc[c] += (double)((T)Vh[at] * get[c]) * w;
-        } else {
-          acc[c] += (double)V[at] * w;
-          if (Vh && m0 + c == j) Vh[at] = (SH<T>)(V[at] * put);
+      for (int q = 0; q < kDotsShRows; ++q) {
+        if (q < cnt) {
+          const int64_t at = (int64_t)(p0 + q) * ldv + r;
+          acc[q] += (sh ? (double)Vh[at] : (double)V[at]) * w;
+          if (!sh && Vh && p0 + q == j) Vh[at] = (SH<T>)(V[at] * put);
         }
       }
+      if (extra) {
+        f0 += (double)V[(int64_t)(j - 1) * ldv + r] * w;
+        const T vj = V[(int64_t)j * ldv + r];
+        f1 += (double)vj * w;
+        Vh[(int64_t)j * ldv + r] = (SH<T>)(vj * put);
+      }
     }
+    // values: the group's rows, then (extra) rows j - 1 and j, then (first pass) ||W||^2
+    double (*rd)[kDotsShRows + 3] = red[pass & 1];
 #pragma unroll
-    for (int c = 0; c < SA_RED_COLS; ++c) {
-      const double x = wave_sum(acc[c]);
-      if (lane == 0) red[ch & 1][wave][c] = x;
+    for (int q = 0; q < kDotsShRows; ++q) {
+      if (q < cnt) {
+        const double x = wave_sum(acc[q]);
+        if (lane == 0) rd[wave][q] = x;
+      }
+    }
+    if (extra) {
+      const double x0 = wave_sum(f0), x1 = wave_sum(f1);
+      if (lane == 0) {
+        rd[wave][kDotsShRows] = x0;
+        rd[wave][kDotsShRows + 1] = x1;
+      }
+    }
+    if (pass == 0) {
+      const double x = wave_sum(ww);
+      if (lane == 0) rd[wave][kDotsShRows + 2] = x;
     }
     __syncthreads();
-    if ((int)threadIdx.x < mc) {
-      const int c = threadIdx.x;
-      part[(int64_t)(m0 + c) * G + sl.blk] =
-          (red[ch & 1][0][c] + red[ch & 1][1][c]) + (red[ch & 1][2][c] + red[ch & 1][3][c]);
+    const int t = threadIdx.x;
+    if (t < kDotsShRows + 3) {
+      const bool row = t < cnt;
+      const bool ext = extra && (t == kDotsShRows || t == kDotsShRows + 1);
+      const bool wsq = pass == 0 && t == kDotsShRows + 2;
+      if (row || ext || wsq) {
+        double x = (rd[0][t] + rd[1][t]) + (rd[2][t] + rd[3][t]);
+        int64_t col;
+        if (row) {
+          col = p0 + t;
+          if (sh) x *= 1.0 / (st->sigma[p0 + t] * shscale);  // shadow row -> raw scale of v_m
+        } else if (ext) {
+          col = t == kDotsShRows ? j - 1 : j;
+        } else {
+          col = j + 1;
+        }
+        part[col * G + sl.blk] = x;
+      }
     }
-    if (shadow && m0 + mc == mh) m0 = mh - SA_RED_COLS;  // continue with the full-precision rows at mh
   }
-  const double wsum = block_sum(ww, redw);
-  if (threadIdx.x == 0) part[(int64_t)(j + 1) * G + sl.blk] = wsum;
 }
 
 // Phase 2 of the projections: column m summed over the G workgroups by a 32-lane group, eight loads
