@@ -560,7 +560,9 @@ def main(argv=None) -> int:
                       "alpha_index_of_max_angle": worst,
                       "angles_above_1e-3": int((ang > 1e-3).sum()),
                       "note": "FP32 tangent sweep (untimed warm-up) against the FP64 diagonal sweep, same alphas; "
-                              "residual = ||N y - lambda y|| / ||y|| of the FP32 eigenpair"}
+                              "residual = ||N y - lambda y|| / ||y|| of the FP32 eigenpair, N y taken by the FP32 "
+                              "SpMV (its rounding, ~1e-7 of ||y||, comes on top of the Lanczos stopping test's "
+                              "tolerance fp32_residual_tol)"}
         pairs.clear()
 
     line = {
