@@ -108,10 +108,11 @@ typedef struct sa_csr {
    * isigned = 1: every coupling of instance i has the magnitude imag[i] (T) and ient holds
    * column | sign << 31 for each entry; else indices and vals are read. */
   /* imode = 2 (FP32, no counterpart in the reference either): the instance-streamed layout, the
-   * column-streamed chunks (bchunk, bent, bvals, bcbase; values always stored) built with one row
-   * block per workgroup w, rows [irow[w], min(irow[w] + irows, iend[iinst[w]])) with irows up to
-   * sa_cstream_max_rows(), and every column in panel 0: workgroup w's chunks are
-   * [bchunk[8 w], bchunk[8 w + 1]). */
+   * column-streamed chunks (bchunk, bent, bvals, bcbase) built with one row block per workgroup w,
+   * rows [irow[w], min(irow[w] + irows, iend[iinst[w]])) with irows up to sa_cstream_max_rows(), and
+   * every column in panel 0: workgroup w's chunks are [bchunk[8 w], bchunk[8 w + 1]). isigned = 1:
+   * every coupling of instance i has the magnitude imag[i] and the entries carry the sign (bit 31),
+   * bvals unused; else bvals holds the couplings. */
   int32_t imode;
   int32_t irows;            /* rows per workgroup, 64..2048 (imode 1) */
   int32_t ilanes;

@@ -263,10 +263,11 @@ class BatchProblem:
                                       signed=False)
             if arr is None:
                 return False
+            iinst = torch.tensor(wg_inst, dtype=torch.int32, device=dev)
+            mag = self._istream_sign_pack(arr, iinst)
             cache.clear()
             cache.update(arr)
-            cache.update(rows=rows, start=start, count=count, nwg=len(wg_row),
-                         iinst=torch.tensor(wg_inst, dtype=torch.int32, device=dev),
+            cache.update(rows=rows, start=start, count=count, nwg=len(wg_row), iinst=iinst, mag=mag,
                          irow=torch.tensor(wg_row, dtype=torch.int32, device=dev),
                          iend=torch.from_numpy(ro[1:].astype(np.int64)).to(dev))
         csr = prob.csr
@@ -274,11 +275,46 @@ class BatchProblem:
         csr.irows, csr.inwg = int(rows), int(cache["nwg"])
         csr.iinst, csr.irow, csr.iend = (cache["iinst"].data_ptr(), cache["irow"].data_ptr(),
                                          cache["iend"].data_ptr())
-        csr.bchunk, csr.bent, csr.bvals, csr.bcbase = (cache["bchunk"].data_ptr(), cache["bent"].data_ptr(),
-                                                       cache["bvals"].data_ptr(), cache["bcbase"].data_ptr())
+        signed = cache["mag"] is not None
+        csr.isigned = int(signed)
+        csr.imag = cache["mag"].data_ptr() if signed else None
+        csr.bchunk, csr.bent, csr.bcbase = (cache["bchunk"].data_ptr(), cache["bent"].data_ptr(),
+                                            cache["bcbase"].data_ptr())
+        csr.bvals = None if signed else cache["bvals"].data_ptr()
         self.unit_start, self.unit_count = cache["start"], cache["count"]
-        self.layout = {"kind": "instance-streamed", "rows": int(rows), "chunks": int(cache["nchunks"])}
+        self.layout = {"kind": "instance-streamed", "rows": int(rows), "chunks": int(cache["nchunks"]),
+                       "signed": signed}
         return True
+
+    def _istream_sign_pack(self, arr: dict, iinst: torch.Tensor) -> torch.Tensor | None:
+        """When every coupling of each instance has one magnitude, fold the signs of the instance-streamed
+        layout's values into its entries (bit 31) and drop the values: returns the per-instance magnitudes
+        (float32, B) and updates arr in place; None (arr unchanged) otherwise. Padding slots (local row =
+        sa_cstream_pad_row(), value 0) keep a clear sign bit."""
+        pad = int(_lib.load().sa_cstream_pad_row())
+        nch = int(arr["nchunks"])
+        if nch == 0:
+            return None
+        bent = arr["bent"][: 256 * nch]
+        vals = arr["bvals"][: 256 * nch]
+        real = ((bent >> 16) & 0x7FFF) != pad
+        # the instance of each chunk: chunk k lies in segment w of bchunk (8 segments per workgroup)
+        seg = torch.searchsorted(arr["bchunk"][1:].to(torch.int64), torch.arange(nch, device=bent.device),
+                                 right=True)
+        inst_e = iinst[(seg // 8).clamp(max=iinst.numel() - 1)].to(torch.int64).repeat_interleave(256)
+        a = torch.abs(vals)
+        big = torch.full((self.B,), -1.0, dtype=torch.float32, device=bent.device)
+        big.scatter_reduce_(0, inst_e[real], a[real], "amax")
+        small = torch.full((self.B,), float("inf"), dtype=torch.float32, device=bent.device)
+        small.scatter_reduce_(0, inst_e[real], a[real], "amin")
+        seen = big >= 0
+        if not bool(torch.all(~seen | (big == small)).item()) or not bool(torch.all(a[real] > 0).item()):
+            return None
+        mag = torch.where(seen, big, torch.ones_like(big)).contiguous()
+        arr["bent"][: 256 * nch] = torch.where(real & (vals < 0), bent | torch.tensor(-(1 << 31), dtype=torch.int32,
+                                                                                        device=bent.device), bent)
+        arr["bvals"] = None
+        return mag
 
     def autotune(self, reps: int = 4, verbose: bool = False) -> dict:
         """Pick the batched SpMV's layout by timing the packed matrix's SpMV on each: plain per-instance

@@ -134,9 +134,11 @@ __global__ __launch_bounds__(kCsThreads) void spmv_cstream_kernel(SpmvParams<flo
 // with values. An instance's columns are its rows' (block diagonal), so a row block's columns are
 // dense (tens of entries per column for instances of a few thousand nodes) and a 64-lane gather
 // reads one or two lines. The workgroup does nothing unless its instance is running, and the epilogue
-// is the plain CSR kernel's: y = a x + b s (.) (row sums); padding rows get y = a x.
+// is the plain CSR kernel's: y = a x + b s (.) (row sums); padding rows get y = a x. kSigned: every
+// coupling of an instance has one magnitude (imag[instance]); the entries carry the sign bit, no values.
+template <bool kSigned>
 __global__ __launch_bounds__(kCsThreads) void spmv_istream_kernel(SpmvParams<float> p) {
-  constexpr int kU = kCsUnrollValued;
+  constexpr int kU = kSigned ? kCsUnrollSigned : kCsUnrollValued;
   __shared__ double acc[kCsRows + 1];
   if (p.gate && *p.gate != SA_RUNNING) return;
   double bb = p.b;
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(kCsThreads) void spmv_istream_kernel(SpmvParams<flo
       const int64_t kk = k + (int64_t)u * kCsWaves;
       const int64_t ks = kk < k1 ? kk : k0;
       e[u] = __builtin_nontemporal_load(&ent[ks * 64 + lane]);
-      wv[u] = __builtin_nontemporal_load(&val[ks * 64 + lane]);
+      if constexpr (!kSigned) wv[u] = __builtin_nontemporal_load(&val[ks * 64 + lane]);
       cb[u] = p.bcbase[ks];
     }
   };
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(kCsThreads) void spmv_istream_kernel(SpmvParams<flo
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       ec[u] = e[u];
-      wc[u] = wv[u];
+      if constexpr (!kSigned) wc[u] = wv[u];
 #pragma unroll
       for (int j = 0; j < 4; ++j) x[u][j] = gp[cb[u] + (int)(ec[u][j] & 0xFFFFu)];
     }
@@ -187,21 +189,28 @@ __global__ __launch_bounds__(kCsThreads) void spmv_istream_kernel(SpmvParams<flo
     for (int u = 0; u < kU; ++u) {
       if (k + (int64_t)u * kCsWaves < k1) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) atomicAdd(&acc[(ec[u][j] >> 16) & 0x7FFFu], (double)(wc[u][j] * x[u][j]));
+        for (int j = 0; j < 4; ++j) {
+          float v;
+          if constexpr (kSigned) v = (ec[u][j] & 0x80000000u) ? -x[u][j] : x[u][j];
+          else v = wc[u][j] * x[u][j];
+          atomicAdd(&acc[(ec[u][j] >> 16) & 0x7FFFu], (double)v);
+        }
       }
     }
   }
   __syncthreads();
+  const double scale = kSigned ? (double)p.imag[inst] : 1.0;
   for (int i = tid; i < rows; i += kCsThreads) {
     const int64_t r = r0 + i;
-    double v = bb * (p.s ? (double)p.s[r] : 1.0) * acc[i];
+    double v = bb * (p.s ? (double)p.s[r] : 1.0) * (scale * acc[i]);
     if (p.x) v += p.a * (double)p.x[r];
     p.y[r] = (float)v;
   }
 }
 
 int launch_istream_spmv(const SpmvParams<float>& p, hipStream_t stream) {
-  if (!p.iinst || !p.irow || !p.iend || !p.bchunk || !p.bent || !p.bvals || !p.bcbase || !p.y) {
+  if (!p.iinst || !p.irow || !p.iend || !p.bchunk || !p.bent || !p.bcbase || !p.y ||
+      (p.isigned ? !p.imag : !p.bvals)) {
     set_error("spmv: incomplete instance-streamed layout");
     return SA_E_ARG;
   }
@@ -209,7 +218,7 @@ int launch_istream_spmv(const SpmvParams<float>& p, hipStream_t stream) {
     set_error("spmv: instance-streamed layout sizes out of range");
     return SA_E_ARG;
   }
-  if (!aligned16(p.bent) || !aligned16(p.bvals)) {
+  if (!aligned16(p.bent) || (!p.isigned && !aligned16(p.bvals))) {
     set_error("spmv: the instance-streamed layout needs 16-byte aligned entries and values");
     return SA_E_ARG;
   }
@@ -219,7 +228,8 @@ int launch_istream_spmv(const SpmvParams<float>& p, hipStream_t stream) {
   }
   const int64_t nwg = p.tile_list ? p.ntile_list : p.inwg;
   if (nwg == 0) return SA_OK;
-  hipLaunchKernelGGL(spmv_istream_kernel, dim3((unsigned)nwg), dim3(kCsThreads), 0, stream, p);
+  if (p.isigned) hipLaunchKernelGGL(spmv_istream_kernel<true>, dim3((unsigned)nwg), dim3(kCsThreads), 0, stream, p);
+  else hipLaunchKernelGGL(spmv_istream_kernel<false>, dim3((unsigned)nwg), dim3(kCsThreads), 0, stream, p);
   return check_launch("spmv_istream");
 }
 

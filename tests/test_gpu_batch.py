@@ -333,7 +333,8 @@ def test_instance_streamed_spmv_matches_the_csr_product(gpu, mixed):
     """The instance-streamed layout (FP32; each workgroup's rows of one instance streamed by column,
     FP64 row sums): the fused product y = a x + b s (.) (J/c) g of an FP64 host reference for several
     workgroup heights, padding rows y = a x, bitwise repeatable (the batched sweep's test runs it with
-    tile lists of the instances still sweeping)."""
+    tile lists of the instances still sweeping); sign-packed entries with per-instance magnitudes when
+    every instance's couplings have one magnitude, values otherwise."""
     from specann_amd._lib import call
     from specann_amd.device import ptr, stream
     big = graphs.erdos_renyi(40000, 8 / 39999, seed=11, kind="pm1")
@@ -362,6 +363,8 @@ def test_instance_streamed_spmv_matches_the_csr_product(gpu, mixed):
         print(f"instance-streamed spmv rows={rows} sizes={[J.shape[0] for J in Js]}", flush=True)
         assert bp.set_istream_layout(rows)
         assert bp.layout["kind"] == "instance-streamed" and prob.csr.imode == 2
+        # one magnitude per instance (J = +-1 or -1 each): sign-packed entries; the mixed batch keeps values
+        assert bp.layout["signed"] == (not mixed) and prob.csr.isigned == int(not mixed)
         ys = []
         for _ in range(2):
             y = torch.full((n,), 9.0, dtype=torch.float32, device=dev)
