@@ -32,7 +32,10 @@ namespace sa {
 
 constexpr int kCsThreads = 1024;           // 16 waves: one workgroup per CU (the row sums fill LDS)
 constexpr int kCsWaves = kCsThreads / 64;
-constexpr int kCsUnrollSigned = 6;         // chunks each wave has in flight (sign-packed entries)
+#ifndef SA_CS_UNROLL_SIGNED
+#define SA_CS_UNROLL_SIGNED 6
+#endif
+constexpr int kCsUnrollSigned = SA_CS_UNROLL_SIGNED;  // chunks each wave has in flight (sign-packed entries)
 constexpr int kCsUnrollValued = 4;         // with a value per entry (more registers per chunk)
 constexpr int kCsRows = 20416;             // FP64 row sums + the padding slot: 163336 bytes of LDS
 constexpr int kCsPad = kCsRows;            // local row of padding entries

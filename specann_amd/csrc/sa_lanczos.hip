@@ -641,15 +641,48 @@ __device__ void proj_bounds(const Proj& T, double* b2, double* ar2, double* out4
 }
 
 // Largest eigenvalue of T by multisection: every thread evaluates one Sturm count per round.
-__device__ double largest_eigenvalue(const Proj& T, const double* bounds) {
+// `hint` (finite) is the largest eigenvalue of a leading principal submatrix of T (the previous
+// step's Ritz value: by interlacing the answer is at least that large). The first round then places
+// its points geometrically above it, hint + (hi - hint) 2^((t - 255) / 4), so that a Ritz value that
+// moved by d since the previous step is bracketed to about d / 5 at once (late in a cycle d is tiny and
+// two or three uniform rounds finish it instead of six or seven from the Gershgorin interval). If every
+// eigenvalue lies below the lowest point (a hint from rounding above the answer), the search goes on
+// in [lo, that point].
+__device__ double largest_eigenvalue(const Proj& T, const double* bounds, double hint) {
   __shared__ unsigned long long masks[SA_WAVES_PER_BLOCK];
   const int t = threadIdx.x;
   double lo = bounds[0], hi = bounds[1];
   const double pivmin = bounds[2];
   const int nsect = blockDim.x + 1;
+  const bool geo = isfinite(hint) && hint > lo && hint < hi && blockDim.x == 256;
   for (int round = 0; round < 24; ++round) {
     const double width = hi - lo;
     if (width <= 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + 2.0 * pivmin) break;
+    if (geo && round == 0) {
+      const double base = hint - 8.0 * 2.220446049250313e-16 * fabs(hint) - pivmin;
+      const double span = hi - base;
+      auto point = [&](int u) { return u >= 255 ? hi : base + span * exp2((double)(u - 255) * 0.25); };
+      const int c = sturm_count(T, point(t), pivmin);
+      const unsigned long long mk = __ballot(c >= T.m);
+      if ((t & 63) == 0) masks[t >> 6] = mk;
+      __syncthreads();
+      int first = -1;
+      for (int w = 0; w < SA_WAVES_PER_BLOCK; ++w) {
+        if (masks[w]) {
+          first = w * 64 + __builtin_ctzll(masks[w]);
+          break;
+        }
+      }
+      __syncthreads();
+      // point(255) = hi lies above every eigenvalue, so first >= 0
+      if (first == 0) {
+        hi = point(0);
+      } else if (first > 0) {
+        lo = fmax(lo, point(first - 1));
+        hi = point(first);
+      }
+      continue;
+    }
     const double x = lo + width * (double)(t + 1) / (double)nsect;
     const int c = sturm_count(T, x, pivmin);
     const unsigned long long mk = __ballot(c >= T.m);
@@ -698,8 +731,37 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(M map, const T* __res
   const int64_t lo = sl.lo, hi = sl.hi;
   const int64_t nv = hi > lo ? (hi - lo) / L : 0;
   // 16-byte loads: L entries per lane, eight rows per group; the rows left after the last full group
-  // (rows j - 1 and j in a selective step) are loaded as one clamped group, all in flight together
-  for (int64_t g = threadIdx.x; g < nv; g += blockDim.x) {
+  // are loaded as one clamped group, all in flight together. A selective step (rows j - 1 and j) loads
+  // just those two rows and forms the same sums: the clamped group's other six partial sums stay 0.0.
+  const bool two = mlo == j - 1 && j >= 1;
+  if (two) {
+    const double c0 = coef[j - 1], c1 = coef[j];
+    const T* __restrict__ v0 = V + (int64_t)(j - 1) * ldv;
+    const T* __restrict__ v1 = V + (int64_t)j * ldv;
+    for (int64_t g = threadIdx.x; g < nv; g += blockDim.x) {
+      const int64_t r = lo + g * L;
+      const VT w0 = *reinterpret_cast<const VT*>(&W[r]);
+      const VT a0 = *reinterpret_cast<const VT*>(&v0[r]);
+      const VT a1 = *reinterpret_cast<const VT*>(&v1[r]);
+      VT sv;
+      if (s) sv = *reinterpret_cast<const VT*>(&s[r]);
+      VT o, z;
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        double x0 = ws * (double)w0[l];
+        x0 -= c0 * (double)a0[l];
+        double x1 = 0.0;
+        x1 -= c1 * (double)a1[l];
+        const double w = ((x0 + x1) + 0.0) + 0.0;
+        o[l] = (T)w;
+        acc += (double)o[l] * (double)o[l];
+        z[l] = s ? sv[l] * o[l] : o[l];
+      }
+      *reinterpret_cast<VT*>(&vn[r]) = o;
+      *reinterpret_cast<VT*>(&Z[r]) = z;
+    }
+  }
+  for (int64_t g = two ? nv : threadIdx.x; g < nv; g += blockDim.x) {
     const int64_t r = lo + g * L;
     const VT w0 = *reinterpret_cast<const VT*>(&W[r]);
     double wv[8][L];
@@ -803,7 +865,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_finish_kernel(F fin) {
   const Proj P{m, k, la, lb, lar, lb2, lar2};
   if (all_finite) {
     proj_bounds(P, lb2, lar2, lbounds);
-    theta = largest_eigenvalue(P, lbounds);
+    theta = largest_eigenvalue(P, lbounds, j >= 1 ? st->theta : NAN);
   }
   if (threadIdx.x == 0) {
     st->b[j] = beta;
