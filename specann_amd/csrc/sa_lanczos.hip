@@ -1065,6 +1065,22 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(M map, const
       for (int l = 0; l < L; ++l) acc[u][l] = 0.0;
     }
     int p = 0;
+    for (; p + 7 < m; p += 8) {  // eight rows' loads in flight
+      VT v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = *reinterpret_cast<const VT*>(&V[(int64_t)(p + q) * ldv + r]);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+#pragma unroll
+        for (int u = 0; u < SA_MAX_KEEP; ++u) {
+          if (u < kk) {
+            const double c = coef[(p + q) * SA_MAX_KEEP + u];
+#pragma unroll
+            for (int l = 0; l < L; ++l) acc[u][l] += c * (double)v[q][l];
+          }
+        }
+      }
+    }
     for (; p + 3 < m; p += 4) {
       VT v[4];
 #pragma unroll
@@ -1101,15 +1117,19 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_vectors_kernel(M map, const
         for (int l = 0; l < L; ++l) o[l] = (T)acc[u][l];
         *reinterpret_cast<VT*>(&V[(int64_t)u * ldv + r]) = o;
         if (Vh) {
+          typename Shadow<T>::vec hv;  // the L shadow entries in one store
 #pragma unroll
-          for (int l = 0; l < L; ++l) Vh[(int64_t)u * ldv + r + l] = (SH<T>)(o[l] * put_kept);
+          for (int l = 0; l < L; ++l) hv[l] = (SH<T>)(o[l] * put_kept);
+          *reinterpret_cast<typename Shadow<T>::vec*>(&Vh[(int64_t)u * ldv + r]) = hv;
         }
       }
     }
     *reinterpret_cast<VT*>(&V[(int64_t)kk * ldv + r]) = lastv;
     if (Vh) {
+      typename Shadow<T>::vec hv;
 #pragma unroll
-      for (int l = 0; l < L; ++l) Vh[(int64_t)kk * ldv + r + l] = (SH<T>)(lastv[l] * put_last);
+      for (int l = 0; l < L; ++l) hv[l] = (SH<T>)(lastv[l] * put_last);
+      *reinterpret_cast<typename Shadow<T>::vec*>(&Vh[(int64_t)kk * ldv + r]) = hv;
     }
     VT z = lastv;
     if (s) z = *reinterpret_cast<const VT*>(&s[r]) * lastv;
