@@ -163,6 +163,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_combine_kernel(void* work, int64_
 // Shadow rows per pass of the projections: twelve 16-byte shadow loads plus W and two full rows per
 // lane keep the kernel at two waves per SIMD without spills (sixteen spill, 32 do not fit).
 constexpr int kDotsShRows = 12;
+constexpr int kDotsWLds = 1024;  // units of 2 L entries of W kept in LDS between passes (32 KiB)
 static_assert(kDotsShRows >= SA_RED_COLS, "a pass without the shadow takes SA_RED_COLS rows");
 
 template <typename T, class M>
@@ -233,6 +234,10 @@ __global__ __launch_bounds__(SA_BLOCK, 2) void lz_dots_kernel(M map, const T* __
   const bool sh = mh > 0;
   const int gend = sh ? mh : j + 1;  // the grouped rows are [0, gend); with the shadow, rows mh..j ride along
   double ww = 0.0;                   // ||W||^2, accumulated in the first pass
+  // The workgroup's slice of W stays in LDS after the first pass when it fits (every thread reads back
+  // its own units, so no barrier), and later passes stream only basis rows.
+  __shared__ VT wl[kDotsWLds][2];
+  const bool wlds = ne <= kDotsWLds;
   int pass = 0;
   for (int p0 = 0; p0 < gend; p0 += sh ? kDotsShRows : SA_RED_COLS, ++pass) {
     const int cnt = min(sh ? kDotsShRows : SA_RED_COLS, gend - p0);
@@ -243,8 +248,18 @@ __global__ __launch_bounds__(SA_BLOCK, 2) void lz_dots_kernel(M map, const T* __
     double f0 = 0.0, f1 = 0.0;  // rows j - 1 and j (extra)
     for (int64_t u = threadIdx.x; u < ne; u += SA_BLOCK) {
       const int64_t r = lo + u * E;
-      const VT w0 = *reinterpret_cast<const VT*>(&W[r]);
-      const VT w1 = *reinterpret_cast<const VT*>(&W[r + L]);
+      VT w0, w1;
+      if (wlds && pass > 0) {
+        w0 = wl[u][0];
+        w1 = wl[u][1];
+      } else {
+        w0 = *reinterpret_cast<const VT*>(&W[r]);
+        w1 = *reinterpret_cast<const VT*>(&W[r + L]);
+        if (wlds) {
+          wl[u][0] = w0;
+          wl[u][1] = w1;
+        }
+      }
       double wd[E];
 #pragma unroll
       for (int l = 0; l < L; ++l) {
