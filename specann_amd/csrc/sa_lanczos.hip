@@ -839,11 +839,17 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_kernel(M map, const T* __res
   if (threadIdx.x == 0) part[sl.blk] = bs;
 }
 
-// The workgroup as the team that shares dense_ritz_vector's factorisation.
-struct BlockTeam {
-  __device__ int rank() const { return threadIdx.x; }
-  __device__ int size() const { return blockDim.x; }
-  __device__ void sync() const { __syncthreads(); }
+// The first wave of the workgroup as the team that shares dense_ritz_vector's factorisation and
+// substitutions: its meeting points are wave barriers with a workgroup-scope fence (the workspace may
+// be in LDS or in global memory), far cheaper than a workgroup barrier at the ~3m of them per solve.
+struct WaveTeam {
+  __device__ int rank() const { return threadIdx.x & 63; }
+  __device__ int size() const { return 64; }
+  __device__ void sync() const {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
 };
 
 // Phase 2 of the Gram-Schmidt step: beta_j from the G partial squared norms, then the FP64 Ritz solve
@@ -910,7 +916,8 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_finish_kernel(F fin) {
     // LDS workspace while it fits; the restart scratch of the work buffer (idle between restarts) beyond
     double* ws = m <= kDenseRitz ? ldense : restart_of(fin.w());
     const double tiny = 2.220446049250313e-16 * fmax(lbounds[3], 1e-300);
-    dense_ritz_vector(BlockTeam{}, m, k, la, lb, lar, theta, tiny, ws, lpiv, lvec);
+    if (threadIdx.x < 64) dense_ritz_vector(WaveTeam{}, m, k, la, lb, lar, theta, tiny, ws, lpiv, lvec);
+    __syncthreads();  // lvec for thread 0 below
   }
   if (threadIdx.x == 0) {
     int status = lstatus;

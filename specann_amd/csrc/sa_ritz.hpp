@@ -27,7 +27,7 @@ struct SerialTeam {
 
 // x (m entries, unit norm) from two inverse-iteration solves of (P - theta I) x = b starting from the
 // all-ones vector; pivots below tiny are set to +-tiny. M: m * m doubles of workspace, perm: m ints,
-// both visible to the whole team; x is written by rank 0 (and visible to it only when this returns).
+// both visible to the whole team, as is x (m entries), which every rank sees complete on return.
 // Every member of the team calls this with the same arguments. The arithmetic of each entry does not
 // depend on the team's size, so every team computes the same bits.
 template <class Team>
@@ -50,10 +50,16 @@ __host__ __device__ inline void dense_ritz_vector(const Team& team, int m, int k
   // P M = L U with full-row interchanges: the multipliers end in their final rows
   for (int c = 0; c < m; ++c) {
     team.sync();  // the previous column's multipliers and this column's values are in place
-    if (t == 0) {
+    if (t == 0) {  // the first row of largest magnitude (the running maximum kept in a register)
       int p = c;
-      for (int r = c + 1; r < m; ++r)
-        if (fabs(M[r * m + c]) > fabs(M[p * m + c])) p = r;
+      double best = fabs(M[c * m + c]);
+      for (int r = c + 1; r < m; ++r) {
+        const double v = fabs(M[r * m + c]);
+        if (v > best) {
+          best = v;
+          p = r;
+        }
+      }
       perm[c] = p;
     }
     team.sync();
@@ -68,45 +74,60 @@ __host__ __device__ inline void dense_ritz_vector(const Team& team, int m, int k
     double pv = M[c * m + c];
     if (!(fabs(pv) >= tiny)) pv = pv < 0.0 ? -tiny : tiny;
     const double inv = 1.0 / pv;
-    const int w = m - c - 1;  // order of the trailing block
-    for (int e = t; e < w * w; e += nt) {
-      const int r = c + 1 + e / w, cc = c + 1 + e % w;
-      M[r * m + cc] -= M[r * m + c] * inv * M[c * m + cc];
-    }
-    team.sync();  // every update has read column c
+    // each rank eliminates its own rows of the trailing block (a rank of a slower team reading the
+    // pivot after rank 0 stored its clamped value clamps it to the same value)
     if (t == 0) M[c * m + c] = pv;
-    for (int r = c + 1 + t; r < m; r += nt) M[r * m + c] *= inv;
+    for (int r = c + 1 + t; r < m; r += nt) {
+      double* __restrict__ row = M + r * m;
+      const double* __restrict__ piv = M + c * m;
+      const double l = row[c] * inv;
+      for (int cc = c + 1; cc < m; ++cc) row[cc] -= l * piv[cc];
+      row[c] = l;
+    }
   }
-  team.sync();
-  if (t != 0) return;
-  for (int i = 0; i < m; ++i) x[i] = 1.0;
+  // Two inverse-iteration solves. Rank 0 applies the row interchanges and normalises; the
+  // substitutions go column by column, each rank updating its own rows (r = t mod nt), so every x[r]
+  // takes its updates in the same order (columns ascending for L, descending for U) whatever the team.
+  for (int i = t; i < m; i += nt) x[i] = 1.0;
   for (int round = 0; round < 2; ++round) {
-    for (int c = 0; c < m; ++c) {  // x <- P x: the interchanges in order
-      const int p = perm[c];
-      if (p != c) {
-        const double v = x[c];
-        x[c] = x[p];
-        x[p] = v;
+    team.sync();
+    if (t == 0 && round > 0) {  // (the first solve's right-hand side, all ones, is invariant under P)
+      for (int c = 0; c < m; ++c) {  // x <- P x: the interchanges in order
+        const int p = perm[c];
+        if (p != c) {
+          const double v = x[c];
+          x[c] = x[p];
+          x[p] = v;
+        }
       }
     }
-    for (int c = 0; c < m; ++c)  // L y = x (unit lower)
-      for (int r = c + 1; r < m; ++r) x[r] -= M[r * m + c] * x[c];
-    for (int r = m - 1; r >= 0; --r) {  // U x = y
-      double v = x[r];
-      for (int cc = r + 1; cc < m; ++cc) v -= M[r * m + cc] * x[cc];
-      x[r] = v / M[r * m + r];
+    for (int c = 0; c < m; ++c) {  // L y = x (unit lower)
+      team.sync();
+      const double xc = x[c];
+      for (int r = c + 1 + t; r < m; r += nt) x[r] -= M[r * m + c] * xc;
     }
-    double mx = 0.0;  // normalise (by the largest entry first, so that the squares cannot overflow)
-    for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(x[i]));
-    const double s = mx > 0.0 ? 1.0 / mx : 0.0;
-    double nn = 0.0;
-    for (int i = 0; i < m; ++i) {
-      x[i] *= s;
-      nn += x[i] * x[i];
+    for (int c = m - 1; c >= 0; --c) {  // U x = y
+      team.sync();
+      if (t == 0) x[c] = x[c] / M[c * m + c];
+      team.sync();
+      const double xc = x[c];
+      for (int r = t; r < c; r += nt) x[r] -= M[r * m + c] * xc;
     }
-    const double inv = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
-    for (int i = 0; i < m; ++i) x[i] *= inv;
+    team.sync();
+    if (t == 0) {
+      double mx = 0.0;  // normalise (by the largest entry first, so that the squares cannot overflow)
+      for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(x[i]));
+      const double s = mx > 0.0 ? 1.0 / mx : 0.0;
+      double nn = 0.0;
+      for (int i = 0; i < m; ++i) {
+        x[i] *= s;
+        nn += x[i] * x[i];
+      }
+      const double inv = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+      for (int i = 0; i < m; ++i) x[i] *= inv;
+    }
   }
+  team.sync();
 }
 
 }  // namespace sa
