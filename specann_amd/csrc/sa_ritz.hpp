@@ -1,6 +1,6 @@
 // Eigenvector of the projected matrix of the Lanczos solve for one eigenvalue, by inverse iteration on
 // the dense matrix with partial pivoting. Host and device code: the factorisation is shared by a team
-// of threads (a workgroup on the device; one thread, or host threads with a barrier, in the CPU tests
+// of threads (one wavefront on the device; one thread, or host threads with a barrier, in the CPU tests
 // that check it against numpy).
 //
 // The projected matrix of order m: rows 0..k-1 are kept Ritz values (diagonal a[i]) coupled to row k

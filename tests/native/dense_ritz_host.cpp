@@ -1,7 +1,8 @@
 // Host build of the Lanczos solve's dense Ritz-vector routine (specann_amd/csrc/sa_ritz.hpp), for the
 // CPU test that checks it against numpy's eigh. dense_ritz_host runs it on one thread;
-// dense_ritz_team runs it on `threads` host threads that meet at a barrier wherever a workgroup
-// meets at __syncthreads, which checks the work split and the barrier placement of the device form.
+// dense_ritz_team runs it on `threads` host threads that meet at a barrier wherever the device's
+// wavefront meets at a wave barrier, which checks the work split and the barrier placement of the
+// device form (64 threads is the device's team size).
 #include <barrier>
 #include <thread>
 #include <vector>
