@@ -658,7 +658,8 @@ __device__ void proj_bounds(const Proj& T, double* b2, double* ar2, double* out4
 // Largest eigenvalue of T by multisection: every thread evaluates one Sturm count per round.
 // `hint` (finite) is the largest eigenvalue of a leading principal submatrix of T (the previous
 // step's Ritz value: by interlacing the answer is at least that large). The first round then places
-// its points geometrically above it, hint + (hi - hint) 2^((t - 255) / 4), so that a Ritz value that
+// its points geometrically above it, base + (hi - base) 2^((t - 255) / 4) with base a few ulps below
+// the hint (its own rounding), so that a Ritz value that
 // moved by d since the previous step is bracketed to about d / 5 at once (late in a cycle d is tiny and
 // two or three uniform rounds finish it instead of six or seven from the Gershgorin interval). If every
 // eigenvalue lies below the lowest point (a hint from rounding above the answer), the search goes on
