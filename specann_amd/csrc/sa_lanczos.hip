@@ -948,14 +948,18 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_orth_finish_kernel(F fin) {
 // the vectors in order (modified Gram-Schmidt) and writes the new projected matrix: diag(theta)
 // bordered by arrow_u = beta_{m-1} s_u[m-1]. Scratch lives in LDS while m <= kPrepLds.
 constexpr int kPrepLds = 48;
+// Per-thread strides (doubles) of the inverse-iteration scratch and of the vectors in LDS: odd, so the
+// threads that walk their own arrays in step fall on different LDS banks.
+constexpr int kPrepScr = 8 * kPrepLds + 1;
+constexpr int kPrepVec = kPrepLds + 1;
 
 template <class F>
 __global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(F fin) {
   void* work = fin.w();
   __shared__ double la[SA_MAX_NCV], lb[SA_MAX_NCV], lb2[SA_MAX_NCV], lar[SA_MAX_NCV], lar2[SA_MAX_NCV];
   __shared__ double lbounds[4], ltheta[SA_MAX_KEEP];
-  __shared__ double lscr[SA_MAX_KEEP * 8 * kPrepLds];
-  __shared__ double lvec[SA_MAX_KEEP * kPrepLds];
+  __shared__ double lscr[SA_MAX_KEEP * kPrepScr];
+  __shared__ double lvec[SA_MAX_KEEP * kPrepVec];
   sa_state* st = state_of(work);
   if (st->status != SA_RESTART) return;
   const int m = st->j;
@@ -996,10 +1000,10 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(F fin) {
   double* S = restart_of(work);                              // S[u * SA_MAX_NCV + p], read by part 2
   double* gscr = S + (int64_t)SA_MAX_KEEP * SA_MAX_NCV;
   const bool in_lds = m <= kPrepLds;
-  const int vstride = in_lds ? kPrepLds : SA_MAX_NCV;
+  const int vstride = in_lds ? kPrepVec : SA_MAX_NCV;
   double* vecs = in_lds ? lvec : S;
   if (t < keep) {
-    double* base = in_lds ? lscr + t * 8 * kPrepLds : gscr + (int64_t)t * 8 * SA_MAX_NCV;
+    double* base = in_lds ? lscr + t * kPrepScr : gscr + (int64_t)t * 8 * SA_MAX_NCV;
     const int sc = in_lds ? kPrepLds : SA_MAX_NCV;
     const double tiny = 2.220446049250313e-16 * fmax(lbounds[3], 1e-300);
     inverse_iteration(P, ltheta[t], tiny, t, base, base + sc, base + 2 * sc, base + 3 * sc, base + 4 * sc,
@@ -1046,7 +1050,7 @@ __global__ __launch_bounds__(SA_BLOCK) void lz_thick_prepare_kernel(F fin) {
   if (in_lds) {
     for (int q = t; q < keep * m; q += blockDim.x) {
       const int u = q / m, p = q % m;
-      S[(int64_t)u * SA_MAX_NCV + p] = lvec[u * kPrepLds + p];
+      S[(int64_t)u * SA_MAX_NCV + p] = lvec[u * kPrepVec + p];
     }
   }
 }
